@@ -1,0 +1,109 @@
+/*
+ * FP64 CPU restatement of BD_BOX's Ermak-McCammon step with hydrodynamic interactions.
+ *
+ * TEST INFRASTRUCTURE ONLY: this library is the checker for the HIP path.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The product path
+ * (bd_box_amd, libbdbox_hip.so) never calls into it.
+ *
+ * Conventions (the reference's, src/data.h:252-253, src/data.c:85-109):
+ *   coord  : double[4N], bead i at coord[4i..4i+3] = (x, y, z, a)
+ *   LJ     : double[2N], (R_i, eps_i) with R_i half the LJ diameter
+ *   D, L   : column-major n x n (n = 3N) with leading dimension ld >= n,
+ *            element (r, c) at [r + c*ld].
+ *
+ * Each function cites the reference lines it restates; see bdbox_oracle.c.
+ */
+#ifndef BDBOX_ORACLE_H
+#define BDBOX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_params {
+    double dt, T, visc, vfactor;
+    double alpha_lj, cutoff_lj;       /* cutoff_lj <= 0 -> 1.122462 * sigma_ij (calc_func.c:389) */
+    int    lj_6_term;
+    int    elec;
+    double epsilon_c, kappa_c, gamma_c, cutoff_c;
+    double bond_lj_scale, bond_c_scale;
+    int    check_overlap;
+    int    bc;                        /* 0 none, 1 periodic cube */
+    double box;                       /* cube edge for bc == 1 */
+    int    hydro;                     /* 0 none, 1 cholesky, 2 geyer (TEA-HI) */
+    int    ewald;                     /* 1 -> Beenakker-Ewald RPY (bc periodic and hydro) */
+    int    ewald_real;                /* lattice sphere radius (ewald_real == ewald_recip) */
+} orc_params;
+
+/* kB*T/(6 pi eta vfactor) and kB*T/(8 pi eta vfactor)  (diff_decl.c:93-101) */
+void orc_prefactors(const orc_params* p, double* preDii, double* preDij);
+
+/* One free-space RPY 3x3 block, rows for bead j, columns for bead i, as assembled at
+   D(3j.., 3i..) by compute_D (diff_tensor.c:45-92, 425-435); row-major out[9]. */
+void orc_rpy_block(const double* ci, const double* cj, double preDii, double preDij, double* out);
+
+/* Beenakker-Ewald block for pair (i, j), i != j or i == j (self term included)
+   (diff_tensor.c:265-422, init_tensor :546-559 with alpha = sqrt(pi)/L).  out[9] row-major. */
+void orc_beenakker_block(const double* coord, int i, int j, double box, int rr,
+                         double preDii, double preDij, double* out);
+
+/* Full symmetric D (both triangles written), free space or Beenakker (compute_D,
+   diff_tensor.c:472-544). */
+void orc_compute_D(int N, const double* coord, const orc_params* p, double* D, int ld);
+
+/* In-place lower Cholesky D = L L^T, upper triangle left untouched; returns 0 or the
+   1-based index of the first non-positive pivot (LAPACK convention, myblas.c:45-92). */
+int orc_cholesky(int n, double* A, int ld);
+
+/* y = L x with L lower (myblas.c:94-118);  y = D x with D symmetric (diff_equation.c:152-159) */
+void orc_trmv_lower(int n, const double* L, int ld, const double* x, double* y);
+void orc_symv_lower(int n, const double* D, int ld, const double* x, double* y);
+
+/* Bonded + nonbonded forces, brute force over all pairs (calc_func.c:106-228, 370-409,
+   pre_func.c:97-159, bond.c:43-85, LJ.c:37-99, electro.c:36-78).  F[3N] is overwritten;
+   E[2] = {E_LJ, E_coulomb} + E_bond returned in E[2] when E != NULL.  Returns the
+   "repeat" flag (overlap/NaN with check_overlap).  Returns -1 on an overstretched bond. */
+int orc_forces(int N, const double* coord, const double* Q, const double* LJ,
+               int nbonds, const int* bonds, const double* bond_parms,
+               const orc_params* p, double* F, double* E);
+
+/* glibc drand48_r stream (48-bit LCG a=0x5DEECE66D, c=0xB) and the polar Box-Muller
+   pairs of gauss_rand (rand_move.c:53-168). state = 48-bit LCG state. */
+uint64_t orc_srand48(uint32_t seed);
+double   orc_drand48(uint64_t* state);
+void     orc_gauss(uint64_t* state, double* out, int count);
+
+/* Geyer-Winter TEA-HI random displacement from the current configuration
+   (diff_rand.c:86-230), on the fly; X holds the raw Gaussians on entry (overwritten by the
+   scaled ones as in the reference).  Rs[3N] out. */
+void orc_geyer_rs(int N, const double* coord, const orc_params* p, double dt,
+                  double* X, double* Rs);
+
+/* D.F for TEA runs: Rf = (dt/kT) D F with blocks recomputed on the fly (diff_equation.c:55-117) */
+void orc_rf_onfly(int N, const double* coord, const orc_params* p, double dt,
+                  const double* F, double* Rf);
+
+/* Stateful Ermak-McCammon integrator (comp_step_ermak with ermak_const and
+   check_overlap no, diff_comp.c:152-179; compute_hydro_ermak diff_equation.c:216-264).
+   refactor_every = 1 reproduces the reference (factor every step); k > 1 reuses the
+   factor of the last refactor step for k-1 further steps. */
+typedef struct orc_sim orc_sim;
+orc_sim* orc_sim_create(int N, const double* coord, const double* Q, const double* LJ,
+                        int nbonds, const int* bonds, const double* bond_parms,
+                        const orc_params* p, uint32_t seed, int refactor_every);
+/* Advance `steps` steps; when xi != NULL the Gaussians are taken from xi[steps*3N]
+   instead of the internal drand48 stream.  Returns 0, or nonzero on failure. */
+int  orc_sim_run(orc_sim* s, int steps, const double* xi);
+void orc_sim_coords(const orc_sim* s, double* coord_out);
+void orc_sim_forces(const orc_sim* s, double* F_out);
+/* displacement parts of the last step: Rf (deterministic) and Rs (random) */
+void orc_sim_last(const orc_sim* s, double* Rf, double* Rs);
+uint64_t orc_sim_rng_state(const orc_sim* s);
+void orc_sim_destroy(orc_sim* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,129 @@
+"""Generate the golden fixtures in this folder from the compiled reference.
+
+Runs `oracle/_ref/ref_harness` (the reference's own CPU path -- its object files built
+from the reference sources by `make -C oracle ref` -- plus a main() that dumps values
+with 17 significant digits) on small seeded systems, and stores inputs and outputs as
+compressed `.npz` files.  Only the reference's outputs are stored, never its code.
+
+    python tests/golden/make_golden.py            # rewrites tests/golden/*.npz
+
+Every run uses OMP_NUM_THREADS=1 so the reference's per-thread force and displacement
+buffers are reduced in one fixed order.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from bd_box_amd import workloads as wl  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def parse(text: str):
+    out = {}
+    lines = text.splitlines()
+    i = 0
+    while i < len(lines):
+        t = lines[i].split()
+        i += 1
+        if not t:
+            continue
+        if len(t) == 2 and t[0] in ("D", "L"):
+            n = int(t[1])
+            out.setdefault(t[0], []).append(
+                np.array([[float(x) for x in lines[i + r].split()] for r in range(n)]))
+            i += n
+        elif len(t) == 2 and t[0] not in ("size",) and t[1].lstrip("-").isdigit() and \
+                t[0] in ("coord", "Q", "LJ", "F", "E", "g", "X", "Rf", "Rs", "F0"):
+            n = int(t[1])
+            out.setdefault(t[0], []).append(np.array([float(x) for x in lines[i:i + n]]))
+            i += n
+        elif len(t) == 2:
+            out.setdefault(t[0], []).append(float(t[1]))
+    return {k: (np.array(v[0]) if len(v) == 1 else np.array(v)) for k, v in out.items()}
+
+
+def run(mode_args, sysm, params, tmp):
+    sp = os.path.join(tmp, "in.str")
+    pp = os.path.join(tmp, "in.prm")
+    wl.write_str(sp, sysm)
+    wl.write_prm(pp, params, sp, tmp)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([HARNESS, *mode_args, "--", pp], capture_output=True, text=True,
+                       env=env, cwd=tmp)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness {mode_args} failed: {r.stderr}\n{r.stdout[-2000:]}")
+    return parse(r.stdout)
+
+
+def case(name, sysm, params, modes):
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for m in modes:
+            out = run(m, sysm, params, tmp)
+            tag = m[0]
+            for k, v in out.items():
+                res[f"{tag}.{k}"] = v
+    p = {k: np.array(str(v)) for k, v in params.items()}
+    np.savez_compressed(
+        os.path.join(HERE, name + ".npz"),
+        in_coord=sysm.coord, in_Q=sysm.Q, in_LJ=sysm.LJ, in_bonds=sysm.bonds,
+        in_bond_parms=sysm.bond_parms,
+        **{"prm." + k: v for k, v in p.items()}, **res)
+    print(name, sorted(res.keys()))
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference first: make -C oracle ref")
+    # 1. bead-spring chain, free-space RPY + Cholesky (the N=256/N=3000 recipe, small N)
+    s16 = wl.chain(16, seed=3)
+    prm = wl.chain_params(rand_seed=4242)
+    case("chain16", s16, prm,
+         [["params"], ["D"], ["forces"], ["chol"], ["step1"], ["traj", "6"], ["gauss", "9", "3"]])
+    s40 = wl.chain(40, seed=11)
+    case("chain40", s40, wl.chain_params(rand_seed=77),
+         [["D"], ["forces"], ["traj", "4"]])
+    # 2. beads with unequal radii, some overlapping (RPY overlap branch, diff_tensor.c:56-72)
+    rng = np.random.default_rng(5)
+    N = 10
+    P = rng.uniform(-6, 6, size=(N, 3))
+    a = rng.uniform(1.0, 3.0, size=N)
+    ov = wl.System(np.concatenate([P, a[:, None]], 1), np.zeros(N), np.tile([0.5, 0.1], (N, 1)),
+                   np.zeros((0, 2), np.int32), np.zeros((0, 3)))
+    case("overlap10", ov, wl.chain_params(elec="no", alpha_lj=0.0), [["D"], ["chol"]])
+    # 3. TEA-HI (Geyer-Winter) noise on a chain, free space (diff_rand.c:86-230)
+    s24 = wl.chain(24, seed=21)
+    case("geyer24", s24, wl.chain_params(hydro="geyer", rand_seed=99),
+         [["step1"], ["traj", "3"]])
+    # 4. periodic Beenakker-Ewald RPY (diff_tensor.c:265-422) on a few hard spheres.
+    #    TEA runs use the stored-D variant (geyer_on_the_fly no, diff_rand.c:136-164): the
+    #    on-the-fly variant accumulates Ewald blocks of one row into the same tD[9]
+    #    (diff_rand.c:97-120 zeroes tD once per bead i while diff_tensor.c:265-400 adds into
+    #    it), so its TEA sums are not those of the tensor; the stored-D path is.
+    hs, L = wl.spheres(12, seed=8, radius=25.0, phi=0.05)
+    pe = wl.chain_params(bc="periodic", xbox=L, ybox=L, zbox=L, hydro="geyer",
+                         geyer_on_the_fly="no",
+                         ewald_method="beenakker", ewald_real=2, ewald_recip=2, elec="no",
+                         alpha_lj=4.0, cutoff_lj=-1, lj_6_term="yes", dt=1.0, rand_seed=5)
+    hs.LJ[:, 1] = 1.0
+    case("beenakker12", hs, pe, [["params"], ["D"], ["step1"]])
+    pc = dict(pe, hydro="cholesky")
+    case("beenakker12_chol", hs, pc, [["D"], ["step1"]])
+    # 5. the Gaussian stream alone, several seeds and an odd table length
+    for seed in (1, 12345):
+        case(f"gauss_seed{seed}", wl.chain(4, seed=1), wl.chain_params(rand_seed=seed),
+             [["gauss", "15", "4"]])
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+"""Pin the FP64 oracle (oracle/bdbox_oracle.c) before it is trusted as the checker.
+
+Three kinds of pins:
+  * golden vectors written by the compiled reference (tests/golden/*.npz, produced by
+    tests/golden/make_golden.py through oracle/_ref/ref_harness);
+  * closed forms and invariants of the mathematics (self mobility, Oseen limit, continuity
+    of the two RPY branches, symmetry/SPD, LL^T = D, Newton's third law, energy gradients);
+  * a library routine for the random stream (glibc drand48_r itself, through ctypes).
+"""
+import ctypes
+import ctypes.util
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from bd_box_amd import workloads as wl
+
+KB = 0.001986248532617
+
+
+def test_prefactors_match_reference(golden):
+    g, sysm, P, p = golden("chain16")
+    dii, dij = po.prefactors(p)
+    assert dii == g["params.preDii"]
+    assert dij == g["params.preDij"]
+    # closed form: kB T / (6 pi eta vfactor) with the reference's float-rounded inputs
+    assert abs(dii - KB * P.T / (6 * np.pi * P.visc * P.vfactor)) < 1e-12 * dii
+    assert P.dt == g["params.dt"] and P.T == g["params.T"]
+
+
+@pytest.mark.parametrize("case", ["chain16", "chain40", "overlap10"])
+def test_D_bitwise_vs_reference(golden, case):
+    g, sysm, P, p = golden(case)
+    D = po.compute_D(sysm.coord, p)
+    Dref = g["D.D"]
+    assert D.shape == Dref.shape
+    # same expression order as the reference -> identical doubles
+    assert np.array_equal(D, Dref), np.max(np.abs(D - Dref))
+
+
+@pytest.mark.parametrize("case", ["beenakker12", "beenakker12_chol"])
+def test_beenakker_D_vs_reference(golden, case):
+    g, sysm, P, p = golden(case)
+    coord = g["params.coord"].reshape(-1, 4) if "params.coord" in g else None
+    if coord is None:
+        coord = golden("beenakker12")[0]["params.coord"].reshape(-1, 4)
+    D = po.compute_D(coord, p)
+    Dref = g["D.D"]
+    err = np.max(np.abs(D - Dref)) / np.max(np.abs(Dref))
+    assert err < 1e-13, err
+    assert np.allclose(D, D.T, rtol=0, atol=1e-15 * np.abs(D).max())
+
+
+def test_rpy_closed_forms():
+    p = po.OrcParams(T=293.0, visc=np.float32(0.01), vfactor=np.float32(14.4))
+    dii, dij = po.prefactors(p)
+    a = 1.9
+    # Oseen limit: r >> a -> (kT/8 pi eta r)(I + e e^T)
+    r = 1e5
+    B = po.rpy_block([0, 0, 0, a], [r, 0, 0, a], dii, dij)
+    oseen = dij / r * np.diag([2.0, 1.0, 1.0])
+    assert np.allclose(B, oseen, rtol=1e-9)
+    # continuity at r = a_i + a_j (equal radii): both branches give
+    # (kT / pi eta a)(7/96 I + 3/96 e e^T) = preDii/a * (1 - 9/16, ...)-form
+    e = np.array([1.0, 2.0, -0.5]); e /= np.linalg.norm(e)
+    r0 = 2 * a
+    Bin = po.rpy_block([0, 0, 0, a], list(e * (r0 * (1 - 1e-12))) + [a], dii, dij)
+    Bout = po.rpy_block([0, 0, 0, a], list(e * (r0 * (1 + 1e-12))) + [a], dii, dij)
+    assert np.allclose(Bin, Bout, rtol=1e-9, atol=0)
+    closed = dii / a * (7.0 / 16 * np.eye(3) + 3.0 / 16 * np.outer(e, e))
+    assert np.allclose(Bout, closed, rtol=1e-9)
+    # a plausible slip (swapping the 1 + s/3 and 1 - s terms) would break this:
+    swapped = dij / r0 * ((1 - 2 * a * a / r0 ** 2) * np.eye(3) + (1 + 2 * a * a / (3 * r0 ** 2)) * np.outer(e, e))
+    assert not np.allclose(Bout, swapped, rtol=1e-3)
+
+
+def test_D_spd_and_self_mobility(golden):
+    g, sysm, P, p = golden("chain40")
+    D = po.compute_D(sysm.coord, p)
+    dii, _ = po.prefactors(p)
+    assert np.allclose(np.diag(D), dii / 1.9, rtol=1e-15)
+    assert np.linalg.eigvalsh(D).min() > 0
+
+
+@pytest.mark.parametrize("case", ["chain16", "overlap10"])
+def test_cholesky_vs_reference(golden, case):
+    g, sysm, P, p = golden(case)
+    D = po.compute_D(sysm.coord, p)
+    L = po.cholesky(D)
+    Lref = g["chol.L"]
+    assert np.max(np.abs(L - Lref)) / np.max(np.abs(Lref)) < 1e-13
+    assert np.max(np.abs(L @ L.T - D)) / np.max(np.abs(D)) < 1e-14
+    assert np.allclose(L, np.linalg.cholesky(D), rtol=0, atol=1e-14 * np.abs(L).max())
+
+
+@pytest.mark.parametrize("case", ["chain16", "chain40"])
+def test_forces_vs_reference(golden, case):
+    g, sysm, P, p = golden(case)
+    F, E, rep = po.forces(sysm, p)
+    Fref = g["forces.F"]
+    scale = np.max(np.abs(Fref))
+    assert np.max(np.abs(F - Fref)) <= 1e-13 * scale
+    # E[ENERGY_LJ] = index 6, E[ENERGY_COULOMB] = 7, E[ENERGY_BOND] = 1 (data.h:278-289)
+    Eref = g["forces.E"]
+    assert abs(E[0] - Eref[6]) <= 1e-12 * max(1, abs(Eref[6]))
+    assert abs(E[1] - Eref[7]) <= 1e-12 * max(1, abs(Eref[7]))
+    assert abs(E[2] - Eref[1]) <= 1e-12 * max(1, abs(Eref[1]))
+    assert rep == int(g["forces.repeat"])
+    # Newton's third law
+    assert np.max(np.abs(F.reshape(-1, 3).sum(0))) < 1e-12 * scale
+
+
+def test_force_is_minus_energy_gradient():
+    sysm = wl.chain(12, seed=4)
+    sysm.bond_parms[:, 1] = 20.0     # FENE with a finite r_max keeps the bond energy finite
+    P = wl.parse_params(wl.chain_params(bond_lj_scale=0.5, bond_c_scale=0.5))
+    p = po.params_from(P)
+    F, E, _ = po.forces(sysm, p)
+    h = 1e-6
+    c0 = sysm.coord.copy()
+    for (i, k) in [(0, 0), (3, 1), (7, 2), (11, 0)]:
+        cp, cm = c0.copy(), c0.copy()
+        cp[i, k] += h
+        cm[i, k] -= h
+        _, Ep, _ = po.forces(sysm, p, cp)
+        _, Em, _ = po.forces(sysm, p, cm)
+        dE = (Ep.sum() - Em.sum()) / (2 * h)
+        assert abs(F[3 * i + k] + dE) < 1e-5 * max(1.0, abs(dE)), (i, k, F[3 * i + k], -dE)
+
+
+def test_gauss_stream_vs_reference_and_glibc(golden):
+    for seed in (1, 12345):
+        g = golden(f"gauss_seed{seed}")[0]
+        ours, _ = po.gauss(seed, 15, 4)
+        assert np.array_equal(ours, g["gauss.g"])
+    g = golden("chain16")[0]
+    ours, _ = po.gauss(4242, 9, 3)
+    assert np.array_equal(ours, g["gauss.g"])
+    # the LCG against glibc's own drand48_r
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    buf = ctypes.create_string_buffer(64)
+    libc.srand48_r(ctypes.c_long(987654), buf)
+    st = ctypes.c_uint64(po.lib().orc_srand48(987654))
+    v = ctypes.c_double()
+    for _ in range(1000):
+        libc.drand48_r(buf, ctypes.byref(v))
+        assert po.lib().orc_drand48(ctypes.byref(st)) == v.value
+
+
+def _first_step(g, sysm, p, coord=None):
+    """orc pieces of the reference's first step (F0, Rf, Rs) given the reference's X."""
+    c = sysm.coord if coord is None else coord
+    F0, _, _ = po.forces(sysm, p, c)
+    return F0
+
+
+def test_first_step_cholesky_vs_reference(golden):
+    g, sysm, P, p = golden("chain16")
+    F0 = _first_step(g, sysm, p)
+    assert np.max(np.abs(F0 - g["step1.F0"])) <= 1e-13 * np.abs(F0).max()
+    sim = po.Sim(sysm, p, seed=P.rand_seed, refactor_every=1)
+    sim.run(1)
+    Rf, Rs = sim.last()
+    assert np.max(np.abs(Rf - g["step1.Rf"])) <= 1e-12 * np.abs(g["step1.Rf"]).max()
+    assert np.max(np.abs(Rs - g["step1.Rs"])) <= 1e-12 * np.abs(g["step1.Rs"]).max()
+    c = sim.coords()
+    assert np.max(np.abs(c - g["step1.coord"].reshape(-1, 4))) < 1e-12
+
+
+@pytest.mark.parametrize("case,steps", [("chain16", 6), ("chain40", 4)])
+def test_trajectory_vs_reference(golden, case, steps):
+    g, sysm, P, p = golden(case)
+    sim = po.Sim(sysm, p, seed=P.rand_seed, refactor_every=1)
+    traj = g["traj.coord"].reshape(steps, -1, 4)
+    for k in range(steps):
+        sim.run(1)
+        err = np.max(np.abs(sim.coords() - traj[k]))
+        assert err < 1e-11, (k, err)
+
+
+def test_geyer_first_step_and_traj_vs_reference(golden):
+    g, sysm, P, p = golden("geyer24")
+    sim = po.Sim(sysm, p, seed=P.rand_seed)
+    sim.run(1)
+    Rf, Rs = sim.last()
+    assert np.max(np.abs(Rf - g["step1.Rf"])) <= 1e-12 * np.abs(g["step1.Rf"]).max()
+    assert np.max(np.abs(Rs - g["step1.Rs"])) <= 1e-12 * np.abs(g["step1.Rs"]).max()
+    sim2 = po.Sim(sysm, p, seed=P.rand_seed)
+    traj = g["traj.coord"].reshape(3, -1, 4)
+    for k in range(3):
+        sim2.run(1)
+        assert np.max(np.abs(sim2.coords() - traj[k])) < 1e-11
+
+
+def test_geyer_reduces_to_diagonal_without_hi():
+    # all off-diagonal D_ij -> 0 (beads far apart): Rs_i = sqrt(2 D_ii dt) xi_i exactly
+    N = 6
+    coord = np.zeros((N, 4))
+    coord[:, 0] = np.arange(N) * 1e12
+    coord[:, 3] = np.linspace(1.0, 2.0, N)
+    p = po.params_from(wl.parse_params(wl.chain_params(hydro="geyer")))
+    dii, _ = po.prefactors(p)
+    xi = np.random.default_rng(0).normal(size=3 * N)
+    Rs, _ = po.geyer_rs(coord, p, p.dt, xi)
+    expect = np.sqrt(2 * (dii / np.repeat(coord[:, 3], 3)) * p.dt) * xi
+    assert np.allclose(Rs, expect, rtol=1e-9)
+
+
+def test_beenakker_step_vs_reference(golden):
+    for case in ("beenakker12", "beenakker12_chol"):
+        g, sysm, P, p = golden(case)
+        coord = golden("beenakker12")[0]["params.coord"].reshape(-1, 4)
+        sim = po.Sim(sysm, p, seed=P.rand_seed, coord=coord)
+        sim.run(1)
+        Rf, Rs = sim.last()
+        assert np.max(np.abs(Rs - g["step1.Rs"])) <= 1e-11 * np.abs(g["step1.Rs"]).max(), case
+        scale = max(np.abs(g["step1.Rf"]).max(), 1e-300)
+        assert np.max(np.abs(Rf - g["step1.Rf"])) <= 1e-11 * scale, case
+
+
+def test_refactor_interval_one_is_reference_semantics():
+    sysm = wl.chain(20, seed=9)
+    p = po.params_from(wl.parse_params(wl.chain_params()))
+    a, b = po.Sim(sysm, p, 5, refactor_every=1), po.Sim(sysm, p, 5, refactor_every=3)
+    a.run(1); b.run(1)
+    assert np.array_equal(a.coords(), b.coords())       # first step factors in both
+    a.run(1); b.run(1)
+    assert not np.array_equal(a.coords(), b.coords())   # second uses the stale factor
+    d = np.max(np.abs(a.coords() - b.coords()))
+    assert d < 1e-2                                      # but stays close
