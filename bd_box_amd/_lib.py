@@ -1,0 +1,95 @@
+"""ctypes binding of libbdbox_hip.so (the C ABI declared in include/bdbox_hip.h).
+
+The library is built in-tree (`make -C bd_box_amd/csrc`).  Loading fails loudly when it is
+missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbdbox_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_u64p = C.POINTER(C.c_uint64)
+
+# every symbol declared in include/bdbox_hip.h
+EXPORTS = [
+    "bdh_last_error", "bdh_init", "bdh_create", "bdh_destroy", "bdh_step", "bdh_compute_D",
+    "bdh_compute_forces", "bdh_factor", "bdh_noise_product", "bdh_compute_Rf",
+    "bdh_get_coords", "bdh_set_coords", "bdh_get_forces", "bdh_get_last_displacement",
+    "bdh_get_rng_state", "bdh_set_rng_state", "bdh_dpotrf", "bdh_dtrmv", "bdh_gauss",
+    "bdh_seed_state", "bdh_probe_mfma", "bdh_time_factor", "bdh_time_steps", "bdh_sync",
+]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("T", C.c_double), ("visc", C.c_double), ("vfactor", C.c_double),
+        ("alpha_lj", C.c_double), ("cutoff_lj", C.c_double), ("lj_6_term", C.c_int),
+        ("elec", C.c_int), ("epsilon_c", C.c_double), ("kappa_c", C.c_double),
+        ("gamma_c", C.c_double), ("cutoff_c", C.c_double), ("bond_lj_scale", C.c_double),
+        ("bond_c_scale", C.c_double), ("check_overlap", C.c_int), ("bc", C.c_int),
+        ("box", C.c_double), ("hydro", C.c_int), ("ewald_method", C.c_int),
+        ("ewald_real", C.c_int), ("precision", C.c_int), ("refactor_every", C.c_int),
+        ("rand_seed", C.c_uint32),
+    ]
+
+
+class BackendError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BackendError(f"{LIB_PATH} is not built; run `make -C bd_box_amd/csrc`")
+    L = C.CDLL(LIB_PATH)
+    L.bdh_last_error.restype = C.c_char_p
+    L.bdh_init.argtypes = [C.c_int]
+    L.bdh_create.argtypes = [C.POINTER(Config), C.c_int, _dp, _dp, _dp, C.c_int, _ip, _dp,
+                             C.POINTER(C.c_void_p)]
+    L.bdh_destroy.argtypes = [C.c_void_p]
+    L.bdh_destroy.restype = None
+    L.bdh_step.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.bdh_compute_D.argtypes = [C.c_void_p, _dp, C.c_int]
+    L.bdh_compute_forces.argtypes = [C.c_void_p, _dp, _dp, _ip]
+    L.bdh_factor.argtypes = [C.c_void_p, _dp, C.c_int]
+    L.bdh_noise_product.argtypes = [C.c_void_p, _dp, _dp]
+    L.bdh_compute_Rf.argtypes = [C.c_void_p, _dp]
+    L.bdh_get_coords.argtypes = [C.c_void_p, _dp]
+    L.bdh_set_coords.argtypes = [C.c_void_p, _dp]
+    L.bdh_get_forces.argtypes = [C.c_void_p, _dp]
+    L.bdh_get_last_displacement.argtypes = [C.c_void_p, _dp, _dp]
+    L.bdh_get_rng_state.argtypes = [C.c_void_p, _u64p]
+    L.bdh_set_rng_state.argtypes = [C.c_void_p, C.c_uint64]
+    L.bdh_dpotrf.argtypes = [C.c_char, _dp, C.c_int, C.c_int, C.c_int]
+    L.bdh_dtrmv.argtypes = [C.c_char, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
+    L.bdh_gauss.argtypes = [_u64p, _dp, C.c_int]
+    L.bdh_seed_state.argtypes = [C.c_uint32]
+    L.bdh_seed_state.restype = C.c_uint64
+    L.bdh_probe_mfma.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.bdh_time_factor.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.bdh_time_steps.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.bdh_sync.argtypes = [C.c_void_p]
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> int:
+    if rc < 0:
+        msg = load().bdh_last_error().decode(errors="replace")
+        raise BackendError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)

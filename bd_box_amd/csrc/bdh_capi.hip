@@ -1,0 +1,781 @@
+// C ABI of the backend (include/bdbox_hip.h): device state of one simulated system and
+// the orchestration of one Ermak-McCammon step on a single HIP stream.
+//
+// Step (reference: comp_step_ermak / compute_hydro_ermak, src/diff_alg/diff_comp.c:152-179,
+// diff_equation.c:216-264, diff_rand.c:47-85):
+//   1. xi        Gaussians: device drand48 + polar Box-Muller stream, or caller-supplied
+//   2. refactor  every `refactor_every` steps: assemble D (lower, T) and factor it in place
+//   3. Rf        (dt/kT) D F with RPY blocks recomputed on the fly (D is not re-read)
+//   4. Rs        sqrt(2 dt) L xi (triangular product over the stored factor)
+//   5. move      r += Rf + Rs in FP64 (periodic wrap when bc is periodic)
+//   6. forces    at the new positions, for the next step
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bdbox_hip.h"
+#include "bdh_common.hpp"
+#include "bdh_kernels.hpp"
+
+namespace bdh {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+// Rf/Rs without hydrodynamic interactions (diff_equation.c:138-146, diff_rand.c:54-61):
+// Pf[r] = D_ii F_r ; noise[r] = xi_r sqrt(D_ii 2 dt)
+__global__ void diag_mobility_kernel(const double4* __restrict__ pos, const double* __restrict__ F,
+                                     const double* __restrict__ xi, int N, double preDii, double dt,
+                                     double* __restrict__ Pf, double* __restrict__ noise)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 3 * N) return;
+    const double dii = preDii / pos[r / 3].w;
+    Pf[r] = dii * F[r];
+    noise[r] = xi[r] * sqrt(dii * 2.0 * dt);
+}
+
+// Copy the lower triangle of a column-major T matrix to a full symmetric FP64 matrix.
+template <typename T>
+__global__ void lower_to_full_kernel(const T* __restrict__ A, long long lda, int n,
+                                     double* __restrict__ D, long long ldd, int symmetric)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= n) return;
+    double v;
+    if (r >= c) v = (double)A[(long long)c * lda + r];
+    else v = symmetric ? (double)A[(long long)r * lda + c] : 0.0;
+    D[(long long)c * ldd + r] = v;
+}
+
+template <typename T>
+__global__ void full_to_lower_kernel(const double* __restrict__ D, long long ldd, int n,
+                                     T* __restrict__ A, long long lda, int from_upper)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= n || r < c) return;
+    const double v = from_upper ? D[(long long)r * ldd + c] : D[(long long)c * ldd + r];
+    A[(long long)c * lda + r] = (T)v;
+}
+
+}  // namespace bdh
+
+using namespace bdh;
+
+struct bdh_system {
+    bdh_config cfg{};
+    int N = 0, n = 0, n_pad = 0;
+    long long ld = 0;
+    bool mixed = false;
+    int device = 0;
+    hipStream_t st = nullptr;
+    double preDii = 0, preDij = 0;
+    ForceParams fp{};
+    Topology topo{};
+    // device buffers
+    double4* pos = nullptr;
+    double* Q = nullptr;
+    double2* LJ = nullptr;
+    int *conn_off = nullptr, *conn = nullptr, *bond_off = nullptr, *bond_p = nullptr;
+    double* bond_parm = nullptr;
+    double *F = nullptr, *E = nullptr, *PF = nullptr, *PE = nullptr;
+    int fsplits = 0, fblocks = 0;
+    void* A = nullptr;
+    double* xi = nullptr;
+    size_t xi_cap_steps = 0;
+    double *Pf = nullptr, *Pr = nullptr, *Rf = nullptr, *Rs = nullptr, *noise = nullptr;
+    int msplits = 0, tblocks = 0;
+    uint64_t* rng = nullptr;
+    long long* rng_off = nullptr;
+    int* rng_cnt = nullptr;
+    int* flags = nullptr;   // [0] force flags, [1] Cholesky status, [2] rng status
+    long long steps_done = 0;
+    bool have_L = false;
+};
+
+namespace {
+
+template <typename P>
+int dmalloc(P** p, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    BDH_HIP(hipMalloc((void**)p, bytes));
+    return 0;
+}
+
+template <typename P>
+void dfree(P*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+int check_flags(bdh_system* s)
+{
+    int h[3];
+    BDH_HIP(hipMemcpyAsync(h, s->flags, sizeof(h), hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    if (h[1]) {
+        set_error("Cholesky decomposition error: non-positive pivot at row " + std::to_string(h[1]));
+        return -3;
+    }
+    if (h[0] & 2) {
+        set_error("Overstretched bond");
+        return -4;
+    }
+    if (h[2]) {
+        set_error("Gaussian stream did not complete");
+        return -5;
+    }
+    return 0;
+}
+
+int compute_forces_dev(bdh_system* s)
+{
+    if (s->mixed) {
+        if (launch_forces<float>(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, s->PF, s->PE,
+                                 s->flags, s->fsplits, s->st)) return -1;
+    } else {
+        if (launch_forces<double>(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, s->PF, s->PE,
+                                  s->flags, s->fsplits, s->st)) return -1;
+    }
+    return launch_forces_reduce(s->PF, s->PE, s->N, s->fsplits, s->fblocks, s->F, s->E, s->st);
+}
+
+int assemble_and_factor(bdh_system* s)
+{
+    if (s->mixed) {
+        float* A = (float*)s->A;
+        if (launch_rpy_assemble<float>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)) return -1;
+        if (chol_factor<float>(A, s->ld, s->n_pad, s->flags + 1, SYRK_F32, s->st)) return -1;
+    } else {
+        double* A = (double*)s->A;
+        if (launch_rpy_assemble<double>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)) return -1;
+        if (chol_factor<double>(A, s->ld, s->n_pad, s->flags + 1, SYRK_F64, s->st)) return -1;
+    }
+    s->have_L = true;
+    return 0;
+}
+
+int matvec_dev(bdh_system* s)
+{
+    if (s->mixed)
+        return launch_rpy_matvec<float>(s->pos, s->F, s->N, s->preDii, s->preDij, s->Pf, s->msplits, s->st);
+    return launch_rpy_matvec<double>(s->pos, s->F, s->N, s->preDii, s->preDij, s->Pf, s->msplits, s->st);
+}
+
+int trmv_dev(bdh_system* s, const double* x)
+{
+    if (s->mixed)
+        return launch_trmv_partials<float>((const float*)s->A, s->ld, s->n, x, s->Pr, s->st);
+    return launch_trmv_partials<double>((const double*)s->A, s->ld, s->n, x, s->Pr, s->st);
+}
+
+int one_step(bdh_system* s, const double* xi_dev)
+{
+    const bdh_config& c = s->cfg;
+    const double dt = c.dt;
+    const double rf_scale = dt / (kB * c.T);
+    if (c.hydro == BDH_HYDRO_CHOLESKY) {
+        if (!s->have_L || (s->steps_done % c.refactor_every) == 0)
+            if (assemble_and_factor(s)) return -1;
+        if (matvec_dev(s)) return -1;
+        if (trmv_dev(s, xi_dev)) return -1;
+        if (launch_em_update(s->pos, s->N, s->Pf, s->msplits, rf_scale, s->Pr, s->tblocks, 256,
+                             std::sqrt(2.0 * dt), nullptr, c.bc == 1, c.box, s->Rf, s->Rs, s->st))
+            return -1;
+    } else {
+        hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st,
+                           s->pos, s->F, xi_dev, s->N, s->preDii, dt, s->Pf, s->noise);
+        BDH_LAUNCH_CHECK();
+        if (launch_em_update(s->pos, s->N, s->Pf, 1, rf_scale, nullptr, 0, 256, 0.0, s->noise,
+                             c.bc == 1, c.box, s->Rf, s->Rs, s->st))
+            return -1;
+    }
+    if (compute_forces_dev(s)) return -1;
+    s->steps_done++;
+    return 0;
+}
+
+int draw_noise(bdh_system* s)
+{
+    return launch_gauss_streams(s->rng, s->xi, s->rng_off, s->rng_cnt, 1, s->flags + 2, s->st);
+}
+
+}  // namespace
+
+namespace {
+template <typename T>
+int dpotrf_impl(char uplo, double* A, int n, int ld, int* info)
+{
+    const int n_pad = ceil_div(n, PAD_TO) * PAD_TO;
+    const long long lda = n_pad;
+    hipStream_t st;
+    BDH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    T* d_A = nullptr;
+    double* d_full = nullptr;
+    int* d_status = nullptr;
+    int rc = 0;
+    if (dmalloc(&d_A, sizeof(T) * lda * n_pad) || dmalloc(&d_full, sizeof(double) * (size_t)n * n) ||
+        dmalloc(&d_status, sizeof(int))) {
+        rc = -1;
+    } else {
+        const bool up = (uplo == 'U' || uplo == 'u');
+        dim3 grid(ceil_div(n, 256), n);
+        (void)hipMemsetAsync(d_A, 0, sizeof(T) * lda * n_pad, st);
+        (void)hipMemsetAsync(d_status, 0, sizeof(int), st);
+        (void)hipMemcpy2DAsync(d_full, sizeof(double) * n, A, sizeof(double) * ld, sizeof(double) * n, n,
+                               hipMemcpyHostToDevice, st);
+        hipLaunchKernelGGL(full_to_lower_kernel<T>, grid, dim3(256), 0, st, d_full, (long long)n, n,
+                           d_A, lda, up ? 1 : 0);
+        rc = launch_set_identity_pad<T>(d_A, lda, n, n_pad, st);
+        if (!rc) rc = chol_factor<T>(d_A, lda, n_pad, d_status, sizeof(T) == 4 ? SYRK_F32 : SYRK_F64, st);
+        if (!rc) {
+            hipLaunchKernelGGL(lower_to_full_kernel<T>, grid, dim3(256), 0, st, (const T*)d_A, lda, n,
+                               d_full, (long long)n, 0);
+            std::vector<double> Lh((size_t)n * n);
+            (void)hipMemcpyAsync(Lh.data(), d_full, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, st);
+            (void)hipMemcpyAsync(info, d_status, sizeof(int), hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) { set_error("bdh_dpotrf: sync failed"); rc = -1; }
+            if (!rc) {
+                for (int c = 0; c < n; ++c)
+                    for (int r = c; r < n; ++r) {
+                        const double v = Lh[(size_t)c * n + r];
+                        if (up) A[(size_t)r * ld + c] = v;   // U = L^T in the upper triangle
+                        else A[(size_t)c * ld + r] = v;
+                    }
+            }
+        }
+    }
+    dfree(d_A); dfree(d_full); dfree(d_status);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bdh_last_error(void) { return bdh::last_error(); }
+
+int bdh_init(int device)
+{
+    int n = 0;
+    BDH_HIP(hipGetDeviceCount(&n));
+    BDH_CHECK(device >= 0 && device < n, "bdh_init: no such device");
+    BDH_HIP(hipSetDevice(device));
+    return 0;
+}
+
+void bdh_destroy(bdh_system* s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    dfree(s->pos); dfree(s->Q); dfree(s->LJ);
+    dfree(s->conn_off); dfree(s->conn); dfree(s->bond_off); dfree(s->bond_p); dfree(s->bond_parm);
+    dfree(s->F); dfree(s->E); dfree(s->PF); dfree(s->PE);
+    dfree(s->A); dfree(s->xi); dfree(s->Pf); dfree(s->Pr); dfree(s->Rf); dfree(s->Rs);
+    dfree(s->noise); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+}
+
+static int create_impl(const bdh_config* cfg, int N, const double* coord, const double* Q,
+                       const double* LJ, int nbonds, const int* bonds, const double* bond_parms,
+                       bdh_system* s)
+{
+    BDH_CHECK(cfg && coord && Q && LJ, "bdh_create: null argument");
+    BDH_CHECK(N > 0, "bdh_create: no beads");
+    BDH_CHECK(cfg->hydro == BDH_HYDRO_NONE || cfg->hydro == BDH_HYDRO_CHOLESKY,
+              "bdh_create: hydro must be none or cholesky in this build");
+    BDH_CHECK(!(cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE),
+              "bdh_create: Ewald-summed tensors are not available in this build");
+    BDH_CHECK(cfg->precision == BDH_FP64 || cfg->precision == BDH_MIXED, "bdh_create: precision");
+    BDH_CHECK(cfg->dt > 0 && cfg->T > 0 && cfg->visc != 0 && cfg->vfactor != 0, "bdh_create: dt/T/visc");
+    BDH_CHECK(nbonds == 0 || (bonds && bond_parms), "bdh_create: bonds");
+    s->cfg = *cfg;
+    if (s->cfg.refactor_every < 1) s->cfg.refactor_every = 1;
+    BDH_HIP(hipGetDevice(&s->device));
+    BDH_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    s->N = N;
+    s->n = 3 * N;
+    s->n_pad = ceil_div(s->n, PAD_TO) * PAD_TO;
+    s->ld = s->n_pad;
+    s->mixed = cfg->precision == BDH_MIXED;
+    s->preDii = kB_1_6PI * cfg->T / (cfg->visc * cfg->vfactor);   // diff_decl.c:93
+    s->preDij = kB_1_8PI * cfg->T / (cfg->visc * cfg->vfactor);   // diff_decl.c:100
+    for (int i = 0; i < N; ++i)
+        BDH_CHECK(coord[4 * i + 3] > 0.0, "bdh_create: bead radius must be positive");
+    for (int b = 0; b < nbonds; ++b)
+        BDH_CHECK(bonds[2 * b] >= 0 && bonds[2 * b] < N && bonds[2 * b + 1] >= 0 &&
+                  bonds[2 * b + 1] < N && bonds[2 * b] != bonds[2 * b + 1],
+                  "bdh_create: bond index out of range");
+
+    // force parameters
+    ForceParams& fp = s->fp;
+    fp.alpha_lj = cfg->alpha_lj;
+    fp.cutoff_lj = cfg->cutoff_lj;
+    fp.lj_6_term = cfg->lj_6_term ? 1.0 : 0.0;
+    fp.elec = cfg->elec;
+    fp.gam = cfg->gamma_c / cfg->epsilon_c;
+    fp.kappa_c = cfg->kappa_c;
+    fp.cutoff_c = cfg->cutoff_c;
+    fp.bond_lj_scale = cfg->bond_lj_scale;
+    fp.bond_c_scale = cfg->bond_c_scale;
+    fp.use_conn = nbonds && ((cfg->elec && cfg->bond_c_scale != 1) ||
+                             (cfg->alpha_lj != 0.0 && cfg->bond_lj_scale != 1));
+    fp.check_overlap = cfg->check_overlap;
+    fp.pbc = cfg->bc == 1;
+    fp.box = cfg->box;
+
+    // per-bead adjacency (CSR), host-built
+    std::vector<int> deg(N + 1, 0);
+    for (int b = 0; b < nbonds; ++b) { deg[bonds[2 * b]]++; deg[bonds[2 * b + 1]]++; }
+    std::vector<int> off(N + 1, 0);
+    for (int i = 0; i < N; ++i) off[i + 1] = off[i] + deg[i];
+    std::vector<int> fill(off.begin(), off.end() - 1), partner(std::max(1, off[N]));
+    std::vector<double> parm(3 * std::max(1, off[N]));
+    for (int b = 0; b < nbonds; ++b) {
+        const int i = bonds[2 * b], j = bonds[2 * b + 1];
+        for (int e = 0; e < 2; ++e) {
+            const int me = e ? j : i, other = e ? i : j;
+            const int k = fill[me]++;
+            partner[k] = other;
+            parm[3 * k] = bond_parms[3 * b];
+            parm[3 * k + 1] = bond_parms[3 * b + 1];
+            parm[3 * k + 2] = bond_parms[3 * b + 2];
+        }
+    }
+    if (dmalloc(&s->conn_off, sizeof(int) * (N + 1))) return -1;
+    if (dmalloc(&s->bond_off, sizeof(int) * (N + 1))) return -1;
+    if (dmalloc(&s->conn, sizeof(int) * partner.size())) return -1;
+    if (dmalloc(&s->bond_p, sizeof(int) * partner.size())) return -1;
+    if (dmalloc(&s->bond_parm, sizeof(double) * parm.size())) return -1;
+    BDH_HIP(hipMemcpy(s->conn_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(s->bond_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(s->conn, partner.data(), sizeof(int) * partner.size(), hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(s->bond_p, partner.data(), sizeof(int) * partner.size(), hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(s->bond_parm, parm.data(), sizeof(double) * parm.size(), hipMemcpyHostToDevice));
+    s->topo = Topology{s->conn_off, s->conn, s->bond_off, s->bond_p, s->bond_parm};
+
+    // beads
+    if (dmalloc(&s->pos, sizeof(double4) * N)) return -1;
+    if (dmalloc(&s->Q, sizeof(double) * N)) return -1;
+    if (dmalloc(&s->LJ, sizeof(double2) * N)) return -1;
+    BDH_HIP(hipMemcpy(s->pos, coord, sizeof(double) * 4 * N, hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(s->Q, Q, sizeof(double) * N, hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(s->LJ, LJ, sizeof(double) * 2 * N, hipMemcpyHostToDevice));
+
+    // work buffers
+    s->fsplits = force_splits(N);
+    s->fblocks = ceil_div(N, 64) * s->fsplits;
+    s->msplits = matvec_splits(N);
+    s->tblocks = trmv_col_blocks(s->n);
+    if (dmalloc(&s->F, sizeof(double) * s->n)) return -1;
+    if (dmalloc(&s->E, sizeof(double) * 4)) return -1;
+    if (dmalloc(&s->PF, sizeof(double) * s->n * s->fsplits)) return -1;
+    if (dmalloc(&s->PE, sizeof(double) * 3 * s->fblocks)) return -1;
+    if (dmalloc(&s->Pf, sizeof(double) * s->n * std::max(1, s->msplits))) return -1;
+    if (dmalloc(&s->Pr, sizeof(double) * s->n * s->tblocks)) return -1;
+    if (dmalloc(&s->Rf, sizeof(double) * s->n)) return -1;
+    if (dmalloc(&s->Rs, sizeof(double) * s->n)) return -1;
+    if (dmalloc(&s->noise, sizeof(double) * s->n)) return -1;
+    s->xi_cap_steps = 1;
+    if (dmalloc(&s->xi, sizeof(double) * s->n)) return -1;
+    if (dmalloc(&s->flags, sizeof(int) * 4)) return -1;
+    BDH_HIP(hipMemset(s->flags, 0, sizeof(int) * 4));
+    BDH_HIP(hipMemset(s->Rf, 0, sizeof(double) * s->n));
+    BDH_HIP(hipMemset(s->Rs, 0, sizeof(double) * s->n));
+    if (dmalloc(&s->rng, sizeof(uint64_t))) return -1;
+    if (dmalloc(&s->rng_off, sizeof(long long))) return -1;
+    if (dmalloc(&s->rng_cnt, sizeof(int))) return -1;
+    {
+        uint64_t st0 = lcg_seed(cfg->rand_seed);
+        long long o = 0;
+        int cnt = s->n;
+        BDH_HIP(hipMemcpy(s->rng, &st0, sizeof(st0), hipMemcpyHostToDevice));
+        BDH_HIP(hipMemcpy(s->rng_off, &o, sizeof(o), hipMemcpyHostToDevice));
+        BDH_HIP(hipMemcpy(s->rng_cnt, &cnt, sizeof(cnt), hipMemcpyHostToDevice));
+    }
+    if (cfg->hydro == BDH_HYDRO_CHOLESKY) {
+        const size_t es = s->mixed ? sizeof(float) : sizeof(double);
+        const size_t bytes = es * (size_t)s->ld * (size_t)s->n_pad;
+        if (dmalloc(&s->A, bytes)) return -1;
+        BDH_HIP(hipMemset(s->A, 0, bytes));
+        if (s->mixed) {
+            if (launch_set_identity_pad<float>((float*)s->A, s->ld, s->n, s->n_pad, s->st)) return -1;
+        } else {
+            if (launch_set_identity_pad<double>((double*)s->A, s->ld, s->n, s->n_pad, s->st)) return -1;
+        }
+    }
+    // initial forces (main_loop.c:91)
+    if (compute_forces_dev(s)) return -1;
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return check_flags(s);
+}
+
+int bdh_create(const bdh_config* cfg, int N, const double* coord, const double* Q,
+               const double* LJ, int nbonds, const int* bonds, const double* bond_parms,
+               bdh_system** out)
+{
+    if (!out) { set_error("bdh_create: null out"); return -2; }
+    *out = nullptr;
+    bdh_system* s = new bdh_system();
+    int rc = create_impl(cfg, N, coord, Q, LJ, nbonds, bonds, bond_parms, s);
+    if (rc) {
+        std::string keep = g_err;
+        bdh_destroy(s);
+        g_err = keep;
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+int bdh_step(bdh_system* s, int steps, const double* xi)
+{
+    BDH_CHECK(s && steps >= 0, "bdh_step: bad arguments");
+    BDH_HIP(hipSetDevice(s->device));
+    if (xi) {
+        if ((size_t)steps > s->xi_cap_steps) {
+            dfree(s->xi);
+            if (dmalloc(&s->xi, sizeof(double) * s->n * (size_t)steps)) return -1;
+            s->xi_cap_steps = steps;
+        }
+        BDH_HIP(hipMemcpyAsync(s->xi, xi, sizeof(double) * s->n * (size_t)steps,
+                               hipMemcpyHostToDevice, s->st));
+    }
+    for (int k = 0; k < steps; ++k) {
+        const double* x = s->xi;
+        if (xi) x = s->xi + (size_t)k * s->n;
+        else if (draw_noise(s)) return -1;
+        if (one_step(s, x)) return -1;
+    }
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return check_flags(s);
+}
+
+int bdh_compute_D(bdh_system* s, double* D, int ld)
+{
+    BDH_CHECK(s && D && ld >= s->n, "bdh_compute_D: bad arguments");
+    BDH_HIP(hipSetDevice(s->device));
+    const size_t es = s->mixed ? sizeof(float) : sizeof(double);
+    void* tmp = nullptr;
+    double* full = nullptr;
+    BDH_HIP(hipMallocAsync(&tmp, es * (size_t)s->ld * s->n_pad, s->st));
+    BDH_HIP(hipMallocAsync((void**)&full, sizeof(double) * (size_t)s->n * s->n, s->st));
+    BDH_HIP(hipMemsetAsync(tmp, 0, es * (size_t)s->ld * s->n_pad, s->st));
+    dim3 grid(ceil_div(s->n, 256), s->n);
+    int rc = 0;
+    if (s->mixed) {
+        rc = launch_rpy_assemble<float>(s->pos, s->N, s->preDii, s->preDij, (float*)tmp, s->ld, s->st);
+        if (!rc) hipLaunchKernelGGL(lower_to_full_kernel<float>, grid, dim3(256), 0, s->st,
+                                    (const float*)tmp, s->ld, s->n, full, (long long)s->n, 1);
+    } else {
+        rc = launch_rpy_assemble<double>(s->pos, s->N, s->preDii, s->preDij, (double*)tmp, s->ld, s->st);
+        if (!rc) hipLaunchKernelGGL(lower_to_full_kernel<double>, grid, dim3(256), 0, s->st,
+                                    (const double*)tmp, s->ld, s->n, full, (long long)s->n, 1);
+    }
+    if (!rc) {
+        BDH_HIP(hipMemcpy2DAsync(D, sizeof(double) * ld, full, sizeof(double) * s->n,
+                                 sizeof(double) * s->n, s->n, hipMemcpyDeviceToHost, s->st));
+    }
+    BDH_HIP(hipFreeAsync(tmp, s->st));
+    BDH_HIP(hipFreeAsync(full, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return rc;
+}
+
+int bdh_compute_forces(bdh_system* s, double* F, double* E, int* repeat)
+{
+    BDH_CHECK(s, "bdh_compute_forces: null system");
+    BDH_HIP(hipSetDevice(s->device));
+    BDH_HIP(hipMemsetAsync(s->flags, 0, sizeof(int), s->st));
+    if (compute_forces_dev(s)) return -1;
+    int fl = 0;
+    if (F) BDH_HIP(hipMemcpyAsync(F, s->F, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
+    if (E) BDH_HIP(hipMemcpyAsync(E, s->E, sizeof(double) * 3, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipMemcpyAsync(&fl, s->flags, sizeof(int), hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    if (repeat) *repeat = (fl & 1) ? 1 : 0;
+    BDH_CHECK(!(fl & 2), "Overstretched bond");
+    return 0;
+}
+
+int bdh_factor(bdh_system* s, double* L, int ld)
+{
+    BDH_CHECK(s && s->A, "bdh_factor: system has no Cholesky state");
+    BDH_HIP(hipSetDevice(s->device));
+    if (assemble_and_factor(s)) return -1;
+    if (L) {
+        BDH_CHECK(ld >= s->n, "bdh_factor: ld < 3N");
+        double* full = nullptr;
+        BDH_HIP(hipMallocAsync((void**)&full, sizeof(double) * (size_t)s->n * s->n, s->st));
+        dim3 grid(ceil_div(s->n, 256), s->n);
+        if (s->mixed)
+            hipLaunchKernelGGL(lower_to_full_kernel<float>, grid, dim3(256), 0, s->st,
+                               (const float*)s->A, s->ld, s->n, full, (long long)s->n, 0);
+        else
+            hipLaunchKernelGGL(lower_to_full_kernel<double>, grid, dim3(256), 0, s->st,
+                               (const double*)s->A, s->ld, s->n, full, (long long)s->n, 0);
+        BDH_LAUNCH_CHECK();
+        BDH_HIP(hipMemcpy2DAsync(L, sizeof(double) * ld, full, sizeof(double) * s->n,
+                                 sizeof(double) * s->n, s->n, hipMemcpyDeviceToHost, s->st));
+        BDH_HIP(hipFreeAsync(full, s->st));
+    }
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return check_flags(s);
+}
+
+int bdh_noise_product(bdh_system* s, const double* x, double* y)
+{
+    BDH_CHECK(s && s->A && s->have_L && x && y, "bdh_noise_product: no factor");
+    BDH_HIP(hipSetDevice(s->device));
+    BDH_HIP(hipMemcpyAsync(s->noise, x, sizeof(double) * s->n, hipMemcpyHostToDevice, s->st));
+    if (trmv_dev(s, s->noise)) return -1;
+    if (launch_sum_partials(s->Pr, s->n, s->tblocks, 256, 1.0, s->Rs, s->st)) return -1;
+    BDH_HIP(hipMemcpyAsync(y, s->Rs, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int bdh_compute_Rf(bdh_system* s, double* Rf)
+{
+    BDH_CHECK(s && Rf, "bdh_compute_Rf: bad arguments");
+    BDH_HIP(hipSetDevice(s->device));
+    const double rf_scale = s->cfg.dt / (kB * s->cfg.T);
+    if (s->cfg.hydro == BDH_HYDRO_CHOLESKY) {
+        if (matvec_dev(s)) return -1;
+        if (launch_sum_partials(s->Pf, s->n, s->msplits, 0, rf_scale, s->Rf, s->st)) return -1;
+    } else {
+        hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st,
+                           s->pos, s->F, s->F, s->N, s->preDii, s->cfg.dt, s->Pf, s->noise);
+        BDH_LAUNCH_CHECK();
+        if (launch_sum_partials(s->Pf, s->n, 1, 0, rf_scale, s->Rf, s->st)) return -1;
+    }
+    BDH_HIP(hipMemcpyAsync(Rf, s->Rf, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int bdh_get_coords(bdh_system* s, double* coord)
+{
+    BDH_CHECK(s && coord, "bdh_get_coords: bad arguments");
+    BDH_HIP(hipMemcpyAsync(coord, s->pos, sizeof(double) * 4 * s->N, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int bdh_set_coords(bdh_system* s, const double* coord)
+{
+    BDH_CHECK(s && coord, "bdh_set_coords: bad arguments");
+    BDH_HIP(hipMemcpyAsync(s->pos, coord, sizeof(double) * 4 * s->N, hipMemcpyHostToDevice, s->st));
+    if (compute_forces_dev(s)) return -1;
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int bdh_get_forces(bdh_system* s, double* F)
+{
+    BDH_CHECK(s && F, "bdh_get_forces: bad arguments");
+    BDH_HIP(hipMemcpyAsync(F, s->F, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int bdh_get_last_displacement(bdh_system* s, double* Rf, double* Rs)
+{
+    BDH_CHECK(s, "bdh_get_last_displacement: null system");
+    if (Rf) BDH_HIP(hipMemcpyAsync(Rf, s->Rf, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
+    if (Rs) BDH_HIP(hipMemcpyAsync(Rs, s->Rs, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int bdh_get_rng_state(bdh_system* s, uint64_t* state)
+{
+    BDH_CHECK(s && state, "bdh_get_rng_state: bad arguments");
+    BDH_HIP(hipMemcpyAsync(state, s->rng, sizeof(uint64_t), hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int bdh_set_rng_state(bdh_system* s, uint64_t state)
+{
+    BDH_CHECK(s, "bdh_set_rng_state: null system");
+    state &= (1ULL << 48) - 1;
+    BDH_HIP(hipMemcpyAsync(s->rng, &state, sizeof(uint64_t), hipMemcpyHostToDevice, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+uint64_t bdh_seed_state(uint32_t seed) { return lcg_seed(seed); }
+
+int bdh_gauss(uint64_t* state, double* out, int count)
+{
+    BDH_CHECK(state && (out || count == 0) && count >= 0, "bdh_gauss: bad arguments");
+    if (count == 0) return 0;
+    hipStream_t st;
+    BDH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    uint64_t* d_state = nullptr;
+    double* d_out = nullptr;
+    long long* d_off = nullptr;
+    int *d_cnt = nullptr, *d_status = nullptr;
+    int rc = 0;
+    long long off = 0;
+    int h_status = 0;
+    if (dmalloc(&d_state, 8) || dmalloc(&d_out, sizeof(double) * count) || dmalloc(&d_off, 8) ||
+        dmalloc(&d_cnt, 4) || dmalloc(&d_status, 4)) {
+        rc = -1;
+    } else {
+        (void)hipMemcpyAsync(d_state, state, 8, hipMemcpyHostToDevice, st);
+        (void)hipMemcpyAsync(d_off, &off, 8, hipMemcpyHostToDevice, st);
+        (void)hipMemcpyAsync(d_cnt, &count, 4, hipMemcpyHostToDevice, st);
+        (void)hipMemsetAsync(d_status, 0, 4, st);
+        rc = launch_gauss_streams(d_state, d_out, d_off, d_cnt, 1, d_status, st);
+        if (!rc) {
+            (void)hipMemcpyAsync(out, d_out, sizeof(double) * count, hipMemcpyDeviceToHost, st);
+            (void)hipMemcpyAsync(state, d_state, 8, hipMemcpyDeviceToHost, st);
+            (void)hipMemcpyAsync(&h_status, d_status, 4, hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) { set_error("bdh_gauss: sync failed"); rc = -1; }
+            if (!rc && h_status) { set_error("bdh_gauss: stream did not complete"); rc = -5; }
+        }
+    }
+    dfree(d_state); dfree(d_out); dfree(d_off); dfree(d_cnt); dfree(d_status);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int bdh_dpotrf(char uplo, double* A, int n, int ld, int precision)
+{
+    BDH_CHECK(A && n > 0 && ld >= n, "bdh_dpotrf: bad arguments");
+    BDH_CHECK(uplo == 'U' || uplo == 'u' || uplo == 'L' || uplo == 'l', "bdh_dpotrf: uplo");
+    int info = 0;
+    int rc = precision == BDH_MIXED ? dpotrf_impl<float>(uplo, A, n, ld, &info)
+                                    : dpotrf_impl<double>(uplo, A, n, ld, &info);
+    if (rc) return rc;
+    return info;
+}
+
+int bdh_dtrmv(char uplo, const double* A, const double* v, double* o, int n, int ld, int precision)
+{
+    BDH_CHECK(A && v && o && n > 0 && ld >= n, "bdh_dtrmv: bad arguments");
+    const bool up = (uplo == 'U' || uplo == 'u');
+    const int n_pad = ceil_div(n, PAD_TO) * PAD_TO;
+    hipStream_t st;
+    BDH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    double *d_full = nullptr, *d_x = nullptr, *d_P = nullptr, *d_y = nullptr;
+    void* d_A = nullptr;
+    const size_t es = precision == BDH_MIXED ? 4 : 8;
+    const int nblk = trmv_col_blocks(n);
+    int rc = 0;
+    if (dmalloc(&d_full, sizeof(double) * (size_t)n * n) || dmalloc(&d_A, es * (size_t)n_pad * n_pad) ||
+        dmalloc(&d_x, sizeof(double) * n) || dmalloc(&d_P, sizeof(double) * (size_t)n * nblk) ||
+        dmalloc(&d_y, sizeof(double) * n)) {
+        rc = -1;
+    } else {
+        dim3 grid(ceil_div(n, 256), n);
+        (void)hipMemsetAsync(d_A, 0, es * (size_t)n_pad * n_pad, st);
+        (void)hipMemcpy2DAsync(d_full, sizeof(double) * n, A, sizeof(double) * ld, sizeof(double) * n, n,
+                               hipMemcpyHostToDevice, st);
+        (void)hipMemcpyAsync(d_x, v, sizeof(double) * n, hipMemcpyHostToDevice, st);
+        if (es == 4) {
+            hipLaunchKernelGGL(full_to_lower_kernel<float>, grid, dim3(256), 0, st, d_full, (long long)n, n,
+                               (float*)d_A, (long long)n_pad, up ? 1 : 0);
+            rc = launch_trmv_partials<float>((const float*)d_A, n_pad, n, d_x, d_P, st);
+        } else {
+            hipLaunchKernelGGL(full_to_lower_kernel<double>, grid, dim3(256), 0, st, d_full, (long long)n, n,
+                               (double*)d_A, (long long)n_pad, up ? 1 : 0);
+            rc = launch_trmv_partials<double>((const double*)d_A, n_pad, n, d_x, d_P, st);
+        }
+        if (!rc) rc = launch_sum_partials(d_P, n, nblk, 256, 1.0, d_y, st);
+        if (!rc) {
+            (void)hipMemcpyAsync(o, d_y, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) { set_error("bdh_dtrmv: sync failed"); rc = -1; }
+        }
+    }
+    dfree(d_full); dfree(d_A); dfree(d_x); dfree(d_P); dfree(d_y);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int bdh_probe_mfma(int kind, const void* A, const void* B, void* C)
+{
+    static const int dims[5][3] = {{32, 2, 32}, {16, 4, 16}, {16, 4, 16}, {32, 16, 32}, {16, 32, 16}};
+    BDH_CHECK(kind >= 0 && kind < 5 && A && B && C, "bdh_probe_mfma: bad arguments");
+    const int M = dims[kind][0], K = dims[kind][1], N = dims[kind][2];
+    const size_t es = kind == 2 ? 8 : 4;
+    void *dA = nullptr, *dB = nullptr, *dC = nullptr;
+    int rc = 0;
+    if (dmalloc(&dA, es * M * K) || dmalloc(&dB, es * K * N) || dmalloc(&dC, es * M * N)) {
+        rc = -1;
+    } else {
+        (void)hipMemcpy(dA, A, es * M * K, hipMemcpyHostToDevice);
+        (void)hipMemcpy(dB, B, es * K * N, hipMemcpyHostToDevice);
+        (void)hipMemset(dC, 0, es * M * N);
+        rc = probe_mfma(kind, dA, dB, dC, nullptr);
+        if (!rc && hipMemcpy(C, dC, es * M * N, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("bdh_probe_mfma: copy back failed");
+            rc = -1;
+        }
+    }
+    dfree(dA); dfree(dB); dfree(dC);
+    return rc;
+}
+
+int bdh_time_factor(bdh_system* s, int reps, double* ms)
+{
+    BDH_CHECK(s && s->A && reps > 0 && ms, "bdh_time_factor: bad arguments");
+    hipEvent_t e0, e1;
+    BDH_HIP(hipEventCreate(&e0));
+    BDH_HIP(hipEventCreate(&e1));
+    BDH_HIP(hipEventRecord(e0, s->st));
+    for (int r = 0; r < reps; ++r)
+        if (assemble_and_factor(s)) return -1;
+    BDH_HIP(hipEventRecord(e1, s->st));
+    BDH_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BDH_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return check_flags(s);
+}
+
+int bdh_time_steps(bdh_system* s, int steps, double* ms)
+{
+    BDH_CHECK(s && steps > 0 && ms, "bdh_time_steps: bad arguments");
+    hipEvent_t e0, e1;
+    BDH_HIP(hipEventCreate(&e0));
+    BDH_HIP(hipEventCreate(&e1));
+    BDH_HIP(hipEventRecord(e0, s->st));
+    for (int k = 0; k < steps; ++k) {
+        if (draw_noise(s)) return -1;
+        if (one_step(s, s->xi)) return -1;
+    }
+    BDH_HIP(hipEventRecord(e1, s->st));
+    BDH_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BDH_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return check_flags(s);
+}
+
+int bdh_sync(bdh_system* s)
+{
+    BDH_CHECK(s, "bdh_sync: null system");
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return check_flags(s);
+}
+
+}  // extern "C"

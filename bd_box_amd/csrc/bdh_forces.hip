@@ -1,0 +1,267 @@
+// Bonded and nonbonded forces, and the Ermak-McCammon position update, on gfx950.
+//
+// Forces restate the reference's compute_forces with brute-force pair lists
+// (src/potentials/calc_func.c:106-228, force_single :370-409):
+//   Debye-Hueckel  F = (gamma/eps_r) Q_i Q_j (1 + kappa r) e^{-kappa r} / r^2   r <= cutoff_c
+//                  (electro.c:36-78)
+//   Lennard-Jones  F = alpha eps_ij 6 (s6 (2 s6 - lj6)) / r, s6 = (sigma_ij/r)^6,
+//                  sigma_ij = R_i + R_j, eps_ij = sqrt(eps_i eps_j),
+//                  r <= cutoff_lj (or 1.122462 sigma_ij when cutoff_lj <= 0)   (LJ.c:37-99)
+//   bonds          F = H (r0 - r) / (1 - r^2/rmax^2)                           (bond.c:43-85)
+//   bonded partners use bond_lj_scale / bond_c_scale (0 removes the term) whenever a
+//   scale differs from 1, exactly when the reference allocates its partner table
+//   (bond.c:113-130); otherwise bonded pairs interact like any other pair.
+// Each workgroup owns 64 row beads and one split of the column beads and accumulates the
+// full row sums F_i = sum_j f_ij (every pair is evaluated from both sides; no atomics, so
+// the sums are deterministic).  Pair arithmetic runs in T (float in the mixed-precision
+// step), separations and accumulation in FP64.
+#include "bdh_common.hpp"
+#include "bdh_kernels.hpp"
+
+namespace bdh {
+
+constexpr int FR_ROWS = 64;
+constexpr int FR_CHUNK = 256;
+
+__device__ __forceinline__ void min_image(double& x, double& y, double& z, int pbc, double box)
+{
+    if (pbc) {
+        // trans_dist_vec (trans.c:111-121)
+        const double inv = 1.0 / box;
+        x -= box * floor(x * inv + 0.5);
+        y -= box * floor(y * inv + 0.5);
+        z -= box * floor(z * inv + 0.5);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
+              const double2* __restrict__ LJ, int N, Topology topo, ForceParams fp,
+              int cols_per_split, double* __restrict__ PF, double* __restrict__ PE,
+              int* __restrict__ flags)
+{
+    __shared__ double4 s_pos[FR_CHUNK];
+    __shared__ double s_q[FR_CHUNK];
+    __shared__ double2 s_lj[FR_CHUNK];
+    __shared__ double s_red[6][4][FR_ROWS];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FR_ROWS + tx;
+    const int i = blockIdx.x * FR_ROWS + tx;
+    const int jb = blockIdx.y * cols_per_split;
+    const int je = min(N, jb + cols_per_split);
+    const double4 pi = (i < N) ? pos[i] : make_double4(0, 0, 0, 1);
+    const double qi = (i < N) ? Q[i] : 0.0;
+    const double2 lji = (i < N) ? LJ[i] : make_double2(0, 0);
+    const int nb0 = (i < N && fp.use_conn) ? topo.conn_off[i] : 0;
+    const int nb1 = (i < N && fp.use_conn) ? topo.conn_off[i + 1] : 0;
+    double fx = 0, fy = 0, fz = 0, e_lj = 0, e_c = 0;
+    int rep = 0;
+    const T gam = (T)fp.gam, kap = (T)fp.kappa_c, alj = (T)fp.alpha_lj, lj6 = (T)fp.lj_6_term;
+    for (int c0 = jb; c0 < je; c0 += FR_CHUNK) {
+        const int cn = min(FR_CHUNK, je - c0);
+        __syncthreads();
+        for (int q = tid; q < cn; q += 256) {
+            s_pos[q] = pos[c0 + q];
+            s_q[q] = Q[c0 + q];
+            s_lj[q] = LJ[c0 + q];
+        }
+        __syncthreads();
+        if (i >= N) continue;
+        for (int q = ty; q < cn; q += 4) {
+            const int j = c0 + q;
+            if (j == i) continue;
+            double dx = s_pos[q].x - pi.x, dy = s_pos[q].y - pi.y, dz = s_pos[q].z - pi.z;
+            min_image(dx, dy, dz, fp.pbc, fp.box);
+            const double r2d = dx * dx + dy * dy + dz * dz;
+            const T r2 = (T)r2d;
+            const T r = sqrt(r2);
+            bool conn = false;
+            for (int b = nb0; b < nb1; ++b) conn |= (topo.conn[b] == j);
+            const T sig = (T)(lji.x + s_lj[q].x);
+            T F = 0;
+            int ov = 0;
+            if (fp.elec && r <= (T)fp.cutoff_c && (!conn || fp.bond_c_scale != 0.0)) {
+                T ex = exp(-kap * r);
+                T f = gam * (T)qi * (T)s_q[q] * ((T)1 + kap * r) * ex / r2;
+                T en = gam * (T)qi * (T)s_q[q] * ex / r;
+                if (conn) { f *= (T)fp.bond_c_scale; en *= (T)fp.bond_c_scale; }
+                F += f;
+                e_c += (double)en;
+            }
+            const T lj_cut = fp.cutoff_lj > 0.0 ? (T)fp.cutoff_lj : (T)1.122462f * sig;
+            if (fp.alpha_lj != 0.0 && r <= lj_cut) {
+                if (!conn || fp.bond_lj_scale != 0.0) {
+                    const T eps = (T)sqrt(lji.y * s_lj[q].y);
+                    T s6 = sig / r;
+                    s6 *= s6;
+                    s6 *= s6 * s6;
+                    T f = alj * eps * (T)6 * (s6 * ((T)2 * s6 - lj6)) / r;
+                    T en = alj * eps * (s6 * (s6 - lj6));
+                    if (conn) { f *= (T)fp.bond_lj_scale; en *= (T)fp.bond_lj_scale; }
+                    F += f;
+                    e_lj += (double)en;
+                    ov = (sig > r) || !isfinite(f);
+                } else {
+                    ov = sig * sig > r2;
+                }
+            } else {
+                ov = sig * sig > r2;
+            }
+            rep |= ov;
+            // force on i: -e_ij F with e_ij = (r_j - r_i)/r
+            const double s = (double)F / (double)r;
+            fx -= dx * s; fy -= dy * s; fz -= dz * s;
+        }
+    }
+    // bonds (only in the first column split, so each bond end is added once)
+    double e_b = 0.0;
+    if (i < N && blockIdx.y == 0 && ty == 0) {
+        for (int b = topo.bond_off[i]; b < topo.bond_off[i + 1]; ++b) {
+            const int p = topo.bond_p[b];
+            const double r0 = topo.bond_parm[3 * b], rmax = topo.bond_parm[3 * b + 1];
+            const double H = topo.bond_parm[3 * b + 2];
+            double dx = pos[p].x - pi.x, dy = pos[p].y - pi.y, dz = pos[p].z - pi.z;
+            min_image(dx, dy, dz, fp.pbc, fp.box);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double r = sqrt(r2);
+            if (!(r < rmax)) rep |= 2;  // overstretched bond: fatal in the reference
+            const double Ff = H * (r0 - r) / (1.0 - r2 / (rmax * rmax));
+            fx -= Ff * dx / r; fy -= Ff * dy / r; fz -= Ff * dz / r;
+            e_b += -0.5 * H * (rmax * rmax * log((rmax * rmax - r2) / (rmax * rmax - r0 * r0)) +
+                               r0 * rmax * log((rmax + r) * (rmax - r0) / ((rmax - r) * (rmax + r0))));
+        }
+    }
+    s_red[0][ty][tx] = fx;
+    s_red[1][ty][tx] = fy;
+    s_red[2][ty][tx] = fz;
+    s_red[3][ty][tx] = e_lj;
+    s_red[4][ty][tx] = e_c;
+    s_red[5][ty][tx] = e_b;
+    __syncthreads();
+    if (ty == 0 && i < N) {
+        double* out = PF + (long long)blockIdx.y * 3 * N + 3 * i;
+        out[0] = s_red[0][0][tx] + s_red[0][1][tx] + s_red[0][2][tx] + s_red[0][3][tx];
+        out[1] = s_red[1][0][tx] + s_red[1][1][tx] + s_red[1][2][tx] + s_red[1][3][tx];
+        out[2] = s_red[2][0][tx] + s_red[2][1][tx] + s_red[2][2][tx] + s_red[2][3][tx];
+    }
+    // energies: pair terms are seen from both ends (x 1/2), bonds likewise
+    __syncthreads();
+    if (tid < 3) {
+        double a = 0;
+        for (int y = 0; y < 4; ++y)
+            for (int x = 0; x < FR_ROWS; ++x) a += s_red[3 + tid][y][x];
+        PE[3 * (blockIdx.y * gridDim.x + blockIdx.x) + tid] = 0.5 * a;
+    }
+    if (rep && fp.check_overlap) atomicOr(flags, 1);
+    if (rep & 2) atomicOr(flags, 2);
+}
+
+int force_splits(int N)
+{
+    int rows = ceil_div(N, FR_ROWS);
+    int s = ceil_div(1024, rows);
+    int max_s = ceil_div(N, 64);
+    if (s > max_s) s = max_s;
+    return s < 1 ? 1 : s;
+}
+
+template <typename T>
+int launch_forces(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
+                  const Topology& topo, const ForceParams& fp, double* d_PF, double* d_PE,
+                  int* d_flags, int splits, hipStream_t st)
+{
+    dim3 grid(ceil_div(N, FR_ROWS), splits);
+    hipLaunchKernelGGL(forces_kernel<T>, grid, dim3(FR_ROWS, 4), 0, st, d_pos, d_Q, d_LJ, N,
+                       topo, fp, ceil_div(N, splits), d_PF, d_PE, d_flags);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+// F = sum over splits; E = (E_LJ, E_coulomb, E_bond) summed over workgroups
+__global__ void forces_reduce_kernel(const double* __restrict__ PF, const double* __restrict__ PE,
+                                     int N, int splits, int pe_blocks, double* __restrict__ F,
+                                     double* __restrict__ E)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < 3 * N) {
+        double s = 0.0;
+        for (int k = 0; k < splits; ++k) s += PF[(long long)k * 3 * N + r];
+        F[r] = s;
+    }
+    if (E && blockIdx.x == 0 && threadIdx.x < 3) {
+        double s = 0.0;
+        for (int k = 0; k < pe_blocks; ++k) s += PE[3 * k + threadIdx.x];
+        E[threadIdx.x] = s;
+    }
+}
+
+int launch_forces_reduce(const double* d_PF, const double* d_PE, int N, int splits,
+                         int pe_blocks, double* d_F, double* d_E, hipStream_t st)
+{
+    hipLaunchKernelGGL(forces_reduce_kernel, dim3(ceil_div(3 * N, 256)), dim3(256), 0, st, d_PF,
+                       d_PE, N, splits, pe_blocks, d_F, d_E);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+// ------------------------------------------------------------------ E-M position update
+// r_i <- r_i + Rf_i + Rs_i, then the periodic wrap of trans_to_box (diff_equation.c:254-264,
+// trans.c:101-107).  Rf = rf_scale * sum of the D.F partials; Rs = rs_scale * sum of the
+// L.xi partials, or, without hydrodynamic interactions (d_diag_noise != null),
+// Rs = sqrt(2 dt D_ii) xi with the prefactor already folded into d_diag_noise.
+__global__ void em_update_kernel(double4* __restrict__ pos, int N, const double* __restrict__ Pf,
+                                 int nf, double rf_scale, const double* __restrict__ Pr, int nr,
+                                 int r_blk_cols, double rs_scale,
+                                 const double* __restrict__ diag_noise, int pbc, double box,
+                                 double* __restrict__ Rf, double* __restrict__ Rs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int n = 3 * N;
+    double4 p = pos[i];
+    double d[3];
+    for (int k = 0; k < 3; ++k) {
+        const int r = 3 * i + k;
+        double f = 0.0;
+        for (int s = 0; s < nf; ++s) f += Pf[(long long)s * n + r];
+        f *= rf_scale;
+        double g = 0.0;
+        if (diag_noise) {
+            g = diag_noise[r];
+        } else {
+            const int last = min(nr - 1, r / r_blk_cols);
+            for (int s = 0; s <= last; ++s) g += Pr[(long long)s * n + r];
+            g *= rs_scale;
+        }
+        Rf[r] = f;
+        Rs[r] = g;
+        d[k] = f + g;
+    }
+    p.x = d[0] + p.x;
+    p.y = d[1] + p.y;
+    p.z = d[2] + p.z;
+    if (pbc) {
+        const double inv = 1.0 / box;
+        p.x -= box * floor(p.x * inv + 0.5);
+        p.y -= box * floor(p.y * inv + 0.5);
+        p.z -= box * floor(p.z * inv + 0.5);
+    }
+    pos[i] = p;
+}
+
+int launch_em_update(double4* d_pos, int N, const double* d_Pf, int nf, double rf_scale,
+                     const double* d_Pr, int nr, int r_blk_cols, double rs_scale,
+                     const double* d_diag_noise, int pbc, double box, double* d_Rf, double* d_Rs,
+                     hipStream_t st)
+{
+    hipLaunchKernelGGL(em_update_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, st, d_pos, N, d_Pf,
+                       nf, rf_scale, d_Pr, nr, r_blk_cols, rs_scale, d_diag_noise, pbc, box,
+                       d_Rf, d_Rs);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+template int launch_forces<float>(const double4*, const double*, const double2*, int, const Topology&, const ForceParams&, double*, double*, int*, int, hipStream_t);
+template int launch_forces<double>(const double4*, const double*, const double2*, int, const Topology&, const ForceParams&, double*, double*, int*, int, hipStream_t);
+
+}  // namespace bdh

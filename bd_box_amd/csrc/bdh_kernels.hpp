@@ -1,0 +1,78 @@
+// Kernel launchers of the HIP backend (all asynchronous on the given stream).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bdh {
+
+// ---- random stream (bdh_rng.hip)
+uint64_t lcg_seed(uint32_t seed);
+int launch_gauss_streams(uint64_t* d_states, double* d_out, const long long* d_offsets,
+                         const int* d_counts, int nstreams, int* d_status, hipStream_t st);
+
+// ---- RPY tensor (bdh_rpy.hip)
+template <typename T>
+int launch_rpy_assemble(const double4* d_pos, int N, double preDii, double preDij, T* d_A,
+                        long long ld, hipStream_t st);
+template <typename T>
+int launch_set_identity_pad(T* d_A, long long ld, int n, int n_pad, hipStream_t st);
+int matvec_splits(int N);
+template <typename T>
+int launch_rpy_matvec(const double4* d_pos, const double* d_F, int N, double preDii,
+                      double preDij, double* d_Pf, int splits, hipStream_t st);
+
+// ---- Cholesky and triangular products (bdh_chol.hip)
+constexpr int CHOL_NB = 64;   // panel width
+constexpr int PAD_TO = 128;   // n is padded to a multiple of this with an identity block
+enum SyrkMode { SYRK_F32 = 0, SYRK_F64 = 1, SYRK_BF16X3 = 2 };
+template <typename T>
+int chol_factor(T* d_A, long long ld, int n_pad, int* d_status, int syrk_mode, hipStream_t st);
+int trmv_col_blocks(int n);
+template <typename T>
+int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_x, double* d_P,
+                         hipStream_t st);
+int launch_sum_partials(const double* d_P, int n, int nblk, int blk_cols, double scale,
+                        double* d_y, hipStream_t st);
+
+// ---- forces (bdh_forces.hip)
+struct ForceParams {
+    double alpha_lj, cutoff_lj;   // cutoff_lj <= 0 -> 1.122462 (R_i + R_j)
+    double lj_6_term;
+    int elec;
+    double gam;                   // gamma_c / epsilon_c
+    double kappa_c, cutoff_c;
+    double bond_lj_scale, bond_c_scale;
+    int use_conn;                 // bonded-partner table active (scales != 1)
+    int check_overlap;
+    int pbc;
+    double box;
+};
+// Per-bead adjacency: conn_* lists bonded partners (pair exclusion / scaling), bond_* lists
+// each bead's bonds (partner, r0, rmax, H) so both ends are accumulated without atomics.
+struct Topology {
+    const int* conn_off;
+    const int* conn;
+    const int* bond_off;
+    const int* bond_p;
+    const double* bond_parm;
+};
+int force_splits(int N);
+template <typename T>
+int launch_forces(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
+                  const Topology& topo, const ForceParams& fp, double* d_PF, double* d_PE,
+                  int* d_flags, int splits, hipStream_t st);
+int launch_forces_reduce(const double* d_PF, const double* d_PE, int N, int splits,
+                         int pe_blocks, double* d_F, double* d_E, hipStream_t st);
+
+// ---- integrator (bdh_forces.hip)
+int launch_em_update(double4* d_pos, int N, const double* d_Pf, int nf, double rf_scale,
+                     const double* d_Pr, int nr, int r_blk_cols, double rs_scale,
+                     const double* d_diag_noise, int pbc, double box, double* d_Rf, double* d_Rs,
+                     hipStream_t st);
+
+// ---- probes (bdh_probe.hip)
+int probe_mfma(int kind, const void* A, const void* B, void* C, hipStream_t st);
+
+}  // namespace bdh

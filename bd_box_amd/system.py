@@ -1,0 +1,189 @@
+"""Host-side mirror of the reference's hot-path interface over the HIP C ABI.
+
+`BDSystem` holds one simulated system on the GPU and exposes the reference's entry
+points with the same meaning:
+
+    compute_D()          compute_D / compute_single_D   (src/diff_alg/diff_tensor.c:472-561)
+    compute_forces()     compute_forces                  (src/potentials/calc_func.c:106-186)
+    compute_Rf()         compute_Rf                      (src/diff_alg/diff_equation.c:119-214)
+    factor()             my_dpotrf on D                  (src/myblas.c:45-92)
+    noise_product(x)     my_dtrmv                        (src/myblas.c:94-118)
+    compute_hydro(k)     k x compute_hydro, ermak_const  (src/diff_alg/diff_comp.c:152-232)
+
+Errors raise BackendError (the reference calls UNERR and exits, src/err.c:48-90).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .workloads import Params, System
+
+FP64 = 0
+MIXED = 1
+
+
+def make_config(P: Params, precision: int = MIXED, refactor_every: int = 1) -> _lib.Config:
+    return _lib.Config(
+        dt=P.dt, T=P.T, visc=P.visc, vfactor=P.vfactor, alpha_lj=P.alpha_lj,
+        cutoff_lj=P.cutoff_lj, lj_6_term=P.lj_6_term, elec=P.elec, epsilon_c=P.epsilon_c,
+        kappa_c=P.kappa_c, gamma_c=P.gamma_c, cutoff_c=P.cutoff_c,
+        bond_lj_scale=P.bond_lj_scale, bond_c_scale=P.bond_c_scale,
+        check_overlap=P.check_overlap, bc=P.bc, box=P.box, hydro=P.hydro,
+        ewald_method=P.ewald_method, ewald_real=P.ewald_real, precision=precision,
+        refactor_every=refactor_every, rand_seed=P.rand_seed & 0xFFFFFFFF)
+
+
+class BDSystem:
+    def __init__(self, sysm: System, P: Params, precision: int = MIXED,
+                 refactor_every: int = 1, coord: Optional[np.ndarray] = None, device: int = 0):
+        L = _lib.load()
+        _lib.check(L.bdh_init(device), "bdh_init")
+        self.N = sysm.N
+        self.n = 3 * self.N
+        self.cfg = make_config(P, precision, refactor_every)
+        c = np.ascontiguousarray(sysm.coord if coord is None else coord, float).reshape(-1)
+        Q = np.ascontiguousarray(sysm.Q, float)
+        LJ = np.ascontiguousarray(sysm.LJ, float).reshape(-1)
+        bonds = np.ascontiguousarray(sysm.bonds, np.int32).reshape(-1)
+        parms = np.ascontiguousarray(sysm.bond_parms, float).reshape(-1)
+        h = C.c_void_p()
+        _lib.check(L.bdh_create(C.byref(self.cfg), self.N, _lib.dptr(c), _lib.dptr(Q),
+                                _lib.dptr(LJ), len(sysm.bonds),
+                                bonds.ctypes.data_as(C.POINTER(C.c_int)), _lib.dptr(parms),
+                                C.byref(h)), "bdh_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().bdh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ reference calls
+    def compute_D(self) -> np.ndarray:
+        D = np.zeros((self.n, self.n))
+        _lib.check(_lib.load().bdh_compute_D(self._h, _lib.dptr(D), self.n), "compute_D")
+        return D.T.copy()  # column-major buffer -> row-major array
+
+    def compute_forces(self):
+        F = np.zeros(self.n)
+        E = np.zeros(3)
+        rep = C.c_int()
+        _lib.check(_lib.load().bdh_compute_forces(self._h, _lib.dptr(F), _lib.dptr(E),
+                                                  C.byref(rep)), "compute_forces")
+        return F, E, rep.value
+
+    def compute_Rf(self) -> np.ndarray:
+        Rf = np.zeros(self.n)
+        _lib.check(_lib.load().bdh_compute_Rf(self._h, _lib.dptr(Rf)), "compute_Rf")
+        return Rf
+
+    def factor(self) -> np.ndarray:
+        Lm = np.zeros((self.n, self.n))
+        rc = _lib.load().bdh_factor(self._h, _lib.dptr(Lm), self.n)
+        _lib.check(rc, "factor")
+        return Lm.T.copy()
+
+    def noise_product(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, float)
+        y = np.zeros(self.n)
+        _lib.check(_lib.load().bdh_noise_product(self._h, _lib.dptr(x), _lib.dptr(y)),
+                   "noise_product")
+        return y
+
+    def compute_hydro(self, steps: int = 1, xi: Optional[np.ndarray] = None):
+        x = None if xi is None else np.ascontiguousarray(xi, float).reshape(-1)
+        if x is not None and x.size != steps * self.n:
+            raise ValueError("xi must hold steps * 3N values")
+        _lib.check(_lib.load().bdh_step(self._h, steps, None if x is None else _lib.dptr(x)),
+                   "compute_hydro")
+
+    # ------------------------------------------------------------------ state access
+    def coords(self) -> np.ndarray:
+        c = np.zeros(4 * self.N)
+        _lib.check(_lib.load().bdh_get_coords(self._h, _lib.dptr(c)), "get_coords")
+        return c.reshape(-1, 4)
+
+    def set_coords(self, coord):
+        c = np.ascontiguousarray(coord, float).reshape(-1)
+        _lib.check(_lib.load().bdh_set_coords(self._h, _lib.dptr(c)), "set_coords")
+
+    def forces(self) -> np.ndarray:
+        F = np.zeros(self.n)
+        _lib.check(_lib.load().bdh_get_forces(self._h, _lib.dptr(F)), "get_forces")
+        return F
+
+    def last_displacement(self):
+        Rf, Rs = np.zeros(self.n), np.zeros(self.n)
+        _lib.check(_lib.load().bdh_get_last_displacement(self._h, _lib.dptr(Rf), _lib.dptr(Rs)),
+                   "get_last_displacement")
+        return Rf, Rs
+
+    def rng_state(self) -> int:
+        s = C.c_uint64()
+        _lib.check(_lib.load().bdh_get_rng_state(self._h, C.byref(s)), "get_rng_state")
+        return s.value
+
+    def set_rng_state(self, state: int):
+        _lib.check(_lib.load().bdh_set_rng_state(self._h, state), "set_rng_state")
+
+    def time_factor(self, reps: int = 1) -> float:
+        ms = C.c_double()
+        _lib.check(_lib.load().bdh_time_factor(self._h, reps, C.byref(ms)), "time_factor")
+        return ms.value
+
+    def time_steps(self, steps: int) -> float:
+        ms = C.c_double()
+        _lib.check(_lib.load().bdh_time_steps(self._h, steps, C.byref(ms)), "time_steps")
+        return ms.value
+
+
+def gauss(state: int, count: int):
+    """`count` Gaussians of the drand48 polar Box-Muller stream from 48-bit `state`."""
+    st = C.c_uint64(state)
+    out = np.zeros(count)
+    _lib.check(_lib.load().bdh_gauss(C.byref(st), _lib.dptr(out), count), "gauss")
+    return out, st.value
+
+
+def seed_state(seed: int) -> int:
+    return _lib.load().bdh_seed_state(seed & 0xFFFFFFFF)
+
+
+def dpotrf(A: np.ndarray, uplo: str = "L", precision: int = FP64):
+    """In-place Cholesky of a host matrix (my_dpotrf); returns (factor, info)."""
+    n = A.shape[0]
+    M = np.asfortranarray(A, dtype=float).copy(order="F")
+    rc = _lib.load().bdh_dpotrf(uplo.encode(), M.ctypes.data_as(C.POINTER(C.c_double)), n, n,
+                                precision)
+    _lib.check(rc, "dpotrf")
+    return np.asarray(M), rc
+
+
+def dtrmv(A: np.ndarray, v: np.ndarray, uplo: str = "L", precision: int = FP64):
+    n = A.shape[0]
+    M = np.asfortranarray(A, dtype=float)
+    v = np.ascontiguousarray(v, float)
+    o = np.zeros(n)
+    _lib.check(_lib.load().bdh_dtrmv(uplo.encode(), M.ctypes.data_as(C.POINTER(C.c_double)),
+                                     _lib.dptr(v), _lib.dptr(o), n, n, precision), "dtrmv")
+    return o
+
+
+def probe_mfma(kind: int, A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    dt = np.float64 if kind == 2 else np.float32
+    A = np.ascontiguousarray(A, dt)
+    B = np.ascontiguousarray(B, dt)
+    Cm = np.zeros((A.shape[0], B.shape[1]), dt)
+    _lib.check(_lib.load().bdh_probe_mfma(kind, A.ctypes.data, B.ctypes.data, Cm.ctypes.data),
+               "probe_mfma")
+    return Cm

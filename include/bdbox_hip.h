@@ -1,0 +1,139 @@
+/*
+ * bdbox_hip.h -- C ABI of the MI355X (gfx950) backend for BD_BOX's Ermak-McCammon step
+ * with hydrodynamic interactions.
+ *
+ * Plain C types only (pointers, sizes, doubles).  Host arrays use the reference's layouts
+ * (src/data.h:252-253, src/data.c:85-109):
+ *   coord  double[4N]  (x, y, z, a) per bead
+ *   Q      double[N]
+ *   LJ     double[2N]  (R, eps) per bead, R = half the LJ diameter of the .str file
+ *   bonds  int[2B] bead indices, bond_parms double[3B] = (r0, r_max, H)
+ *   D, L   column-major n x n (n = 3N), leading dimension ld >= n
+ * All functions return 0 on success and a negative value on error; bdh_last_error() then
+ * describes it.  Functions are synchronous: results are in host memory when they return.
+ * One bdh_system may be driven by one host thread at a time (the reference's own
+ * threading contract: one thread calls compute_hydro per step).
+ *
+ * The entry points replace the reference's backend interface (the TU set substituted at
+ * link time in src/Makefile.am:40-70); each cites the function it stands in for.  The
+ * adapter that maps the reference's global-state calls onto these entry points is
+ * bd_box_amd/csrc/bdbox_dropin.c.
+ */
+#ifndef BDBOX_HIP_H
+#define BDBOX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bdh_system bdh_system;
+
+enum bdh_precision {
+    BDH_FP64 = 0,   /* everything in FP64 (the comparator)                              */
+    BDH_MIXED = 1   /* FP32 tensor/forces/trailing updates, FP64 panels and positions   */
+};
+
+enum bdh_hydro { BDH_HYDRO_NONE = 0, BDH_HYDRO_CHOLESKY = 1, BDH_HYDRO_GEYER = 2 };
+
+/* Control parameters after the reference's .prm parsing (src/input.c:353-503): DOUBLE
+ * keywords already rounded through float (src/input.c:933). */
+typedef struct bdh_config {
+    double dt, T, visc, vfactor;             /* dt [ps], T [K], visc [P], vfactor       */
+    double alpha_lj, cutoff_lj;              /* cutoff_lj <= 0: 1.122462 (R_i + R_j)    */
+    int lj_6_term;
+    int elec;
+    double epsilon_c, kappa_c, gamma_c, cutoff_c;
+    double bond_lj_scale, bond_c_scale;
+    int check_overlap;
+    int bc;                                  /* 0 none, 1 periodic cube                  */
+    double box;                              /* cube edge when bc == 1                   */
+    int hydro;                               /* enum bdh_hydro                           */
+    int ewald_method;                        /* 1 Beenakker (bc periodic)                */
+    int ewald_real;                          /* lattice sphere radius 0..5               */
+    int precision;                           /* enum bdh_precision                       */
+    int refactor_every;                      /* refactor D every k steps (reference: 1)  */
+    uint32_t rand_seed;                      /* drand48 seed of the Gaussian stream      */
+} bdh_config;
+
+/* Last error of the calling thread ("" if none). */
+const char* bdh_last_error(void);
+
+/* Select the HIP device for systems created afterwards by this thread.
+ * Stands in for init_cuda (src/cuda.c:88-148). */
+int bdh_init(int device);
+
+/* Allocate device state and upload the bead table; computes the initial forces
+ * (main_loop.c:91).  Stands in for init_diff_tensor (src/diff_alg/diff_decl.c:83-175)
+ * and init_forces (src/potentials/calc_func.c:411-415). */
+int bdh_create(const bdh_config* cfg, int N, const double* coord, const double* Q,
+               const double* LJ, int nbonds, const int* bonds, const double* bond_parms,
+               bdh_system** out);
+/* Stands in for free_diff_tensor / free_forces (diff_decl.c:177-217, calc_func.c:417-421). */
+void bdh_destroy(bdh_system* sys);
+
+/* Advance `steps` Ermak-McCammon steps: Rf = (dt/kT) D F, Rs = sqrt(2 dt) L xi, r += Rf+Rs,
+ * then forces at the new positions.  xi == NULL draws the Gaussians from the device
+ * reproduction of the reference's drand48 stream (src/rand_move.c:132-168); otherwise
+ * xi[steps * 3N] supplies them.  Stands in for compute_hydro with algorithm ermak_const
+ * (src/diff_alg/diff_comp.c:152-179, 210-232; diff_equation.c:216-286; diff_rand.c:47-85). */
+int bdh_step(bdh_system* sys, int steps, const double* xi);
+
+/* Assemble D at the current positions and copy it out (full symmetric matrix).
+ * Stands in for compute_D / compute_single_D (src/diff_alg/diff_tensor.c:472-544, 561). */
+int bdh_compute_D(bdh_system* sys, double* D, int ld);
+
+/* Forces F[3N] and energies E[3] = (E_LJ, E_coulomb, E_bond) at the current positions;
+ * *repeat receives the reference's "repeat step" flag (overlap/NaN with check_overlap).
+ * Stands in for compute_forces (src/potentials/calc_func.c:106-186). */
+int bdh_compute_forces(bdh_system* sys, double* F, double* E, int* repeat);
+
+/* Assemble and factor D = L L^T at the current positions (the refactor step), copy L out
+ * (lower triangle; zeros above).  Stands in for my_dpotrf on D (src/myblas.c:45-92,
+ * called from diff_rand.c:76-82). */
+int bdh_factor(bdh_system* sys, double* L, int ld);
+
+/* y = L x with the current factor.  Stands in for my_dtrmv (src/myblas.c:94-118). */
+int bdh_noise_product(bdh_system* sys, const double* x, double* y);
+
+/* D.F from blocks evaluated on the fly (never stored); Rf = (dt/kT) D F.
+ * Stands in for compute_Rf (src/diff_alg/diff_equation.c:119-214). */
+int bdh_compute_Rf(bdh_system* sys, double* Rf);
+
+int bdh_get_coords(bdh_system* sys, double* coord);          /* 4N, for save_output       */
+int bdh_set_coords(bdh_system* sys, const double* coord);    /* load_coord (diff_decl.c:229) */
+int bdh_get_forces(bdh_system* sys, double* F);              /* F[0] of the last step     */
+int bdh_get_last_displacement(bdh_system* sys, double* Rf, double* Rs);
+int bdh_get_rng_state(bdh_system* sys, uint64_t* state);     /* get_state (rand_move.c:170) */
+int bdh_set_rng_state(bdh_system* sys, uint64_t state);      /* set_state (rand_move.c:182) */
+
+/* Host-matrix LA on the device, replacing myblas (src/myblas.c:45-118):
+ * my_dpotrf(A, n, LD) factors in place; uplo 'U' returns U with A = U^T U in the upper
+ * triangle (the LAPACK convention of the reference), 'L' returns L in the lower triangle.
+ * Returns the LAPACK INFO (0, or the 1-based index of the failed pivot).
+ * my_dtrmv: o = L v where A holds the factor as written by bdh_dpotrf with the same uplo. */
+int bdh_dpotrf(char uplo, double* A, int n, int ld, int precision);
+int bdh_dtrmv(char uplo, const double* A, const double* v, double* o, int n, int ld,
+              int precision);
+
+/* `count` Gaussians of the reference's stream continuing from *state (updated).
+ * Stands in for gauss_rand (src/rand_move.c:132-168). */
+int bdh_gauss(uint64_t* state, double* out, int count);
+/* 48-bit drand48 state for srand48_r(seed) (src/rand_move.c:53-80). */
+uint64_t bdh_seed_state(uint32_t seed);
+
+/* Test hook: one MFMA on a single wavefront with the operand layouts this backend assumes
+ * (kind 0 f32 32x32x2, 1 f32 16x16x4, 2 f64 16x16x4, 3 bf16 32x32x16, 4 bf16 16x16x32).
+ * A (MxK) and B (KxN) row-major host arrays (float, or double for kind 2); C = A B. */
+int bdh_probe_mfma(int kind, const void* A, const void* B, void* C);
+
+/* Timing helpers for measurement: device-side events bracketing the hot kernels. */
+int bdh_time_factor(bdh_system* sys, int reps, double* ms_per_factor);
+int bdh_time_steps(bdh_system* sys, int steps, double* ms_total);
+int bdh_sync(bdh_system* sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
