@@ -180,7 +180,10 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int c0, int t0, int n_pad)
 }
 
 // ------------------------------------------------------------------ FP64 SYRK (MFMA)
-// Same tiling; each wave 64x64 = 4x4 blocks of 16x16 (v_mfma_f64_16x16x4_f64).
+// Same tiling; each wave 64x64 = 4x4 blocks of 16x16 (v_mfma_f64_16x16x4_f64).  The f64
+// MFMA returns row 4*r + l/16 in accumulator r of lane l; feeding the A operand rows in the
+// order perm(m) = 4*(m%4) + m/4 makes accumulator r hold row 4*(l/16) + r instead, so each
+// lane owns 4 consecutive rows of one column (two 16-byte stores).
 constexpr int SY_KC64 = 16;
 
 __global__ void __launch_bounds__(256)
@@ -196,6 +199,7 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int c0, int t0, int n_pad)
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const int l16 = lane & 15, lq = lane >> 4;
+    const int prow = 4 * (l16 & 3) + (l16 >> 2);
 
     doublex4 acc[4][4];
 #pragma unroll
@@ -222,7 +226,7 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int c0, int t0, int n_pad)
         for (int k = 0; k < SY_KC64; k += 4) {
             double av[4], bv[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = sA[k + lq][wr * 64 + a * 16 + l16];
+            for (int a = 0; a < 4; ++a) av[a] = sA[k + lq][wr * 64 + a * 16 + prow];
 #pragma unroll
             for (int b = 0; b < 4; ++b) bv[b] = sB[k + lq][wc * 64 + b * 16 + l16];
 #pragma unroll

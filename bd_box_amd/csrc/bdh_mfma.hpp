@@ -8,7 +8,8 @@
 //     D: acc r of lane l is D[8*(r/4) + 4*(l/32) + r%4][l%32]
 //  f32_16x16x4 : A 16x4, B 4x16, D 16x16 (4 floats/lane)
 //     A: A[l%16][l/16]   B: B[l/16][l%16]   D: acc r -> D[4*(l/16) + r][l%16]
-//  f64_16x16x4 : as f32_16x16x4 with doubles
+//  f64_16x16x4 : A, B as f32_16x16x4 (doubles); D: acc r -> D[4*r + l/16][l%16]
+//     (rows interleaved, unlike the f32 form; measured with the probe)
 //  bf16 32x32x16 (gfx950): A 32x16, B 16x32, 8 bf16 per lane
 //     A: lane l holds A[l%32][8*(l/32) + 0..7]   B: B[8*(l/32) + 0..7][l%32]
 //     D: as f32_32x32x2

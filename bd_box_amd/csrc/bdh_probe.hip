@@ -36,7 +36,7 @@ __global__ void probe_f64_16x16x4(const double* A, const double* B, double* C)
     double b = B[(l / 16) * 16 + l % 16];
     doublex4 acc = {0.0, 0.0, 0.0, 0.0};
     acc = mfma_f64_16x16x4(a, b, acc);
-    for (int r = 0; r < 4; ++r) C[(4 * (l / 16) + r) * 16 + l % 16] = acc[r];
+    for (int r = 0; r < 4; ++r) C[(4 * r + l / 16) * 16 + l % 16] = acc[r];
 }
 
 __global__ void probe_bf16_32x32x16(const float* A, const float* B, float* C)
