@@ -22,6 +22,7 @@ EXPORTS = [
     "bdh_get_coords", "bdh_set_coords", "bdh_get_forces", "bdh_get_last_displacement",
     "bdh_get_rng_state", "bdh_set_rng_state", "bdh_dpotrf", "bdh_dtrmv", "bdh_gauss",
     "bdh_seed_state", "bdh_probe_mfma", "bdh_time_factor", "bdh_time_steps", "bdh_sync",
+    "bdh_phase_timing", "bdh_phase_times",
 ]
 
 
@@ -80,6 +81,8 @@ def load():
     L.bdh_time_factor.argtypes = [C.c_void_p, C.c_int, _dp]
     L.bdh_time_steps.argtypes = [C.c_void_p, C.c_int, _dp]
     L.bdh_sync.argtypes = [C.c_void_p]
+    L.bdh_phase_timing.argtypes = [C.c_void_p, C.c_int]
+    L.bdh_phase_times.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_longlong)]
     _lib = L
     return L
 

@@ -28,16 +28,16 @@ void set_error(const std::string& msg) { g_err = msg; }
 const char* last_error() { return g_err.c_str(); }
 
 // Rf/Rs without hydrodynamic interactions (diff_equation.c:138-146, diff_rand.c:54-61):
-// Pf[r] = D_ii F_r ; noise[r] = xi_r sqrt(D_ii 2 dt)
+// Rf[r] = (dt/kT) D_ii F_r ; Rs[r] = xi_r sqrt(D_ii 2 dt)
 __global__ void diag_mobility_kernel(const double4* __restrict__ pos, const double* __restrict__ F,
                                      const double* __restrict__ xi, int N, double preDii, double dt,
-                                     double* __restrict__ Pf, double* __restrict__ noise)
+                                     double rf_scale, double* __restrict__ Rf, double* __restrict__ Rs)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= 3 * N) return;
     const double dii = preDii / pos[r / 3].w;
-    Pf[r] = dii * F[r];
-    noise[r] = xi[r] * sqrt(dii * 2.0 * dt);
+    Rf[r] = rf_scale * dii * F[r];
+    if (Rs) Rs[r] = xi[r] * sqrt(dii * 2.0 * dt);
 }
 
 // Copy the lower triangle of a column-major T matrix to a full symmetric FP64 matrix.
@@ -64,6 +64,34 @@ __global__ void full_to_lower_kernel(const double* __restrict__ D, long long ldd
     const double v = from_upper ? D[(long long)r * ldd + c] : D[(long long)c * ldd + r];
     A[(long long)c * lda + r] = (T)v;
 }
+
+// HIP-event recorder for the measurement hook (one event pair per bracketed launch).
+struct EventRecorder : PhaseRecorder {
+    struct Rec { int phase; hipEvent_t e0, e1; };
+    std::vector<Rec> recs;
+    hipEvent_t open_ev[PH_COUNT] = {};
+    void begin(int phase, hipStream_t st) override
+    {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, st);
+        open_ev[phase] = e;
+    }
+    void end(int phase, hipStream_t st) override
+    {
+        hipEvent_t e;
+        if (!open_ev[phase] || hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, st);
+        recs.push_back({phase, open_ev[phase], e});
+        open_ev[phase] = nullptr;
+    }
+    void clear()
+    {
+        for (auto& r : recs) { (void)hipEventDestroy(r.e0); (void)hipEventDestroy(r.e1); }
+        recs.clear();
+    }
+    ~EventRecorder() override { clear(); }
+};
 
 }  // namespace bdh
 
@@ -98,6 +126,7 @@ struct bdh_system {
     int* flags = nullptr;   // [0] force flags, [1] Cholesky status, [2] rng status
     long long steps_done = 0;
     bool have_L = false;
+    EventRecorder* rec = nullptr;   // non-null while phase timing is enabled
 };
 
 namespace {
@@ -137,8 +166,13 @@ int check_flags(bdh_system* s)
     return 0;
 }
 
-int compute_forces_dev(bdh_system* s)
+#define REC_BEGIN(ph) do { if (s->rec) s->rec->begin(ph, s->st); } while (0)
+#define REC_END(ph) do { if (s->rec) s->rec->end(ph, s->st); } while (0)
+
+int compute_forces_dev(bdh_system* s, bool energies = false)
 {
+    REC_BEGIN(PH_FORCES);
+    int rc = 0;
     if (s->mixed) {
         if (launch_forces<float>(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, s->PF, s->PE,
                                  s->flags, s->fsplits, s->st)) return -1;
@@ -146,19 +180,28 @@ int compute_forces_dev(bdh_system* s)
         if (launch_forces<double>(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, s->PF, s->PE,
                                   s->flags, s->fsplits, s->st)) return -1;
     }
-    return launch_forces_reduce(s->PF, s->PE, s->N, s->fsplits, s->fblocks, s->F, s->E, s->st);
+    ReduceJobs jobs{};
+    jobs.job[0] = ReduceJob{s->PF, s->F, s->n, s->fsplits, 0, 1.0};
+    rc = launch_reduce_partials(jobs, 1, s->n, s->st);
+    if (!rc && energies) rc = launch_energy_reduce(s->PE, s->fblocks, s->E, s->st);
+    REC_END(PH_FORCES);
+    return rc;
 }
 
 int assemble_and_factor(bdh_system* s)
 {
     if (s->mixed) {
         float* A = (float*)s->A;
+        REC_BEGIN(PH_ASSEMBLE);
         if (launch_rpy_assemble<float>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)) return -1;
-        if (chol_factor<float>(A, s->ld, s->n_pad, s->flags + 1, SYRK_F32, s->st)) return -1;
+        REC_END(PH_ASSEMBLE);
+        if (chol_factor<float>(A, s->ld, s->n_pad, s->flags + 1, SYRK_F32, s->st, s->rec)) return -1;
     } else {
         double* A = (double*)s->A;
+        REC_BEGIN(PH_ASSEMBLE);
         if (launch_rpy_assemble<double>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)) return -1;
-        if (chol_factor<double>(A, s->ld, s->n_pad, s->flags + 1, SYRK_F64, s->st)) return -1;
+        REC_END(PH_ASSEMBLE);
+        if (chol_factor<double>(A, s->ld, s->n_pad, s->flags + 1, SYRK_F64, s->st, s->rec)) return -1;
     }
     s->have_L = true;
     return 0;
@@ -186,18 +229,26 @@ int one_step(bdh_system* s, const double* xi_dev)
     if (c.hydro == BDH_HYDRO_CHOLESKY) {
         if (!s->have_L || (s->steps_done % c.refactor_every) == 0)
             if (assemble_and_factor(s)) return -1;
+        REC_BEGIN(PH_MATVEC);
         if (matvec_dev(s)) return -1;
+        REC_END(PH_MATVEC);
+        REC_BEGIN(PH_TRMV);
         if (trmv_dev(s, xi_dev)) return -1;
-        if (launch_em_update(s->pos, s->N, s->Pf, s->msplits, rf_scale, s->Pr, s->tblocks, 256,
-                             std::sqrt(2.0 * dt), nullptr, c.bc == 1, c.box, s->Rf, s->Rs, s->st))
-            return -1;
+        REC_END(PH_TRMV);
+        REC_BEGIN(PH_MOVE);
+        ReduceJobs jobs{};
+        jobs.job[0] = ReduceJob{s->Pf, s->Rf, s->n, s->msplits, 0, rf_scale};
+        jobs.job[1] = ReduceJob{s->Pr, s->Rs, s->n, s->tblocks, trmv_blk_cols(), std::sqrt(2.0 * dt)};
+        if (launch_reduce_partials(jobs, 2, s->n, s->st)) return -1;
+        if (launch_em_update(s->pos, s->N, s->Rf, s->Rs, c.bc == 1, c.box, s->st)) return -1;
+        REC_END(PH_MOVE);
     } else {
+        REC_BEGIN(PH_MOVE);
         hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st,
-                           s->pos, s->F, xi_dev, s->N, s->preDii, dt, s->Pf, s->noise);
+                           s->pos, s->F, xi_dev, s->N, s->preDii, dt, rf_scale, s->Rf, s->Rs);
         BDH_LAUNCH_CHECK();
-        if (launch_em_update(s->pos, s->N, s->Pf, 1, rf_scale, nullptr, 0, 256, 0.0, s->noise,
-                             c.bc == 1, c.box, s->Rf, s->Rs, s->st))
-            return -1;
+        if (launch_em_update(s->pos, s->N, s->Rf, s->Rs, c.bc == 1, c.box, s->st)) return -1;
+        REC_END(PH_MOVE);
     }
     if (compute_forces_dev(s)) return -1;
     s->steps_done++;
@@ -206,7 +257,10 @@ int one_step(bdh_system* s, const double* xi_dev)
 
 int draw_noise(bdh_system* s)
 {
-    return launch_gauss_streams(s->rng, s->xi, s->rng_off, s->rng_cnt, 1, s->flags + 2, s->st);
+    REC_BEGIN(PH_NOISE);
+    int rc = launch_gauss_streams(s->rng, s->xi, s->rng_off, s->rng_cnt, 1, s->flags + 2, s->st);
+    REC_END(PH_NOISE);
+    return rc;
 }
 
 }  // namespace
@@ -279,6 +333,8 @@ void bdh_destroy(bdh_system* s)
     if (!s) return;
     (void)hipSetDevice(s->device);
     if (s->st) (void)hipStreamSynchronize(s->st);
+    delete s->rec;
+    s->rec = nullptr;
     dfree(s->pos); dfree(s->Q); dfree(s->LJ);
     dfree(s->conn_off); dfree(s->conn); dfree(s->bond_off); dfree(s->bond_p); dfree(s->bond_parm);
     dfree(s->F); dfree(s->E); dfree(s->PF); dfree(s->PE);
@@ -391,9 +447,11 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     s->xi_cap_steps = 1;
     if (dmalloc(&s->xi, sizeof(double) * s->n)) return -1;
     if (dmalloc(&s->flags, sizeof(int) * 4)) return -1;
-    BDH_HIP(hipMemset(s->flags, 0, sizeof(int) * 4));
-    BDH_HIP(hipMemset(s->Rf, 0, sizeof(double) * s->n));
-    BDH_HIP(hipMemset(s->Rs, 0, sizeof(double) * s->n));
+    // every clear goes on the system's own (non-blocking) stream, ordered before the
+    // kernels that follow; a null-stream hipMemset would not be
+    BDH_HIP(hipMemsetAsync(s->flags, 0, sizeof(int) * 4, s->st));
+    BDH_HIP(hipMemsetAsync(s->Rf, 0, sizeof(double) * s->n, s->st));
+    BDH_HIP(hipMemsetAsync(s->Rs, 0, sizeof(double) * s->n, s->st));
     if (dmalloc(&s->rng, sizeof(uint64_t))) return -1;
     if (dmalloc(&s->rng_off, sizeof(long long))) return -1;
     if (dmalloc(&s->rng_cnt, sizeof(int))) return -1;
@@ -409,7 +467,7 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
         const size_t es = s->mixed ? sizeof(float) : sizeof(double);
         const size_t bytes = es * (size_t)s->ld * (size_t)s->n_pad;
         if (dmalloc(&s->A, bytes)) return -1;
-        BDH_HIP(hipMemset(s->A, 0, bytes));
+        BDH_HIP(hipMemsetAsync(s->A, 0, bytes, s->st));
         if (s->mixed) {
             if (launch_set_identity_pad<float>((float*)s->A, s->ld, s->n, s->n_pad, s->st)) return -1;
         } else {
@@ -499,7 +557,7 @@ int bdh_compute_forces(bdh_system* s, double* F, double* E, int* repeat)
     BDH_CHECK(s, "bdh_compute_forces: null system");
     BDH_HIP(hipSetDevice(s->device));
     BDH_HIP(hipMemsetAsync(s->flags, 0, sizeof(int), s->st));
-    if (compute_forces_dev(s)) return -1;
+    if (compute_forces_dev(s, E != nullptr)) return -1;
     int fl = 0;
     if (F) BDH_HIP(hipMemcpyAsync(F, s->F, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
     if (E) BDH_HIP(hipMemcpyAsync(E, s->E, sizeof(double) * 3, hipMemcpyDeviceToHost, s->st));
@@ -541,7 +599,9 @@ int bdh_noise_product(bdh_system* s, const double* x, double* y)
     BDH_HIP(hipSetDevice(s->device));
     BDH_HIP(hipMemcpyAsync(s->noise, x, sizeof(double) * s->n, hipMemcpyHostToDevice, s->st));
     if (trmv_dev(s, s->noise)) return -1;
-    if (launch_sum_partials(s->Pr, s->n, s->tblocks, 256, 1.0, s->Rs, s->st)) return -1;
+    ReduceJobs jobs{};
+    jobs.job[0] = ReduceJob{s->Pr, s->Rs, s->n, s->tblocks, trmv_blk_cols(), 1.0};
+    if (launch_reduce_partials(jobs, 1, s->n, s->st)) return -1;
     BDH_HIP(hipMemcpyAsync(y, s->Rs, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
     BDH_HIP(hipStreamSynchronize(s->st));
     return 0;
@@ -554,12 +614,14 @@ int bdh_compute_Rf(bdh_system* s, double* Rf)
     const double rf_scale = s->cfg.dt / (kB * s->cfg.T);
     if (s->cfg.hydro == BDH_HYDRO_CHOLESKY) {
         if (matvec_dev(s)) return -1;
-        if (launch_sum_partials(s->Pf, s->n, s->msplits, 0, rf_scale, s->Rf, s->st)) return -1;
+        ReduceJobs jobs{};
+        jobs.job[0] = ReduceJob{s->Pf, s->Rf, s->n, s->msplits, 0, rf_scale};
+        if (launch_reduce_partials(jobs, 1, s->n, s->st)) return -1;
     } else {
         hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st,
-                           s->pos, s->F, s->F, s->N, s->preDii, s->cfg.dt, s->Pf, s->noise);
+                           s->pos, s->F, s->F, s->N, s->preDii, s->cfg.dt, rf_scale, s->Rf,
+                           (double*)nullptr);
         BDH_LAUNCH_CHECK();
-        if (launch_sum_partials(s->Pf, s->n, 1, 0, rf_scale, s->Rf, s->st)) return -1;
     }
     BDH_HIP(hipMemcpyAsync(Rf, s->Rf, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
     BDH_HIP(hipStreamSynchronize(s->st));
@@ -696,7 +758,11 @@ int bdh_dtrmv(char uplo, const double* A, const double* v, double* o, int n, int
                                (double*)d_A, (long long)n_pad, up ? 1 : 0);
             rc = launch_trmv_partials<double>((const double*)d_A, n_pad, n, d_x, d_P, st);
         }
-        if (!rc) rc = launch_sum_partials(d_P, n, nblk, 256, 1.0, d_y, st);
+        if (!rc) {
+            ReduceJobs jobs{};
+            jobs.job[0] = ReduceJob{d_P, d_y, n, nblk, trmv_blk_cols(), 1.0};
+            rc = launch_reduce_partials(jobs, 1, n, st);
+        }
         if (!rc) {
             (void)hipMemcpyAsync(o, d_y, sizeof(double) * n, hipMemcpyDeviceToHost, st);
             if (hipStreamSynchronize(st) != hipSuccess) { set_error("bdh_dtrmv: sync failed"); rc = -1; }
@@ -769,6 +835,30 @@ int bdh_time_steps(bdh_system* s, int steps, double* ms)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return check_flags(s);
+}
+
+int bdh_phase_timing(bdh_system* s, int enable)
+{
+    BDH_CHECK(s, "bdh_phase_timing: null system");
+    BDH_HIP(hipStreamSynchronize(s->st));
+    delete s->rec;
+    s->rec = enable ? new EventRecorder() : nullptr;
+    return 0;
+}
+
+int bdh_phase_times(bdh_system* s, double* ms, long long* counts)
+{
+    BDH_CHECK(s && s->rec && ms && counts, "bdh_phase_times: timing not enabled");
+    BDH_HIP(hipStreamSynchronize(s->st));
+    for (int p = 0; p < PH_COUNT; ++p) { ms[p] = 0.0; counts[p] = 0; }
+    for (auto& r : s->rec->recs) {
+        float t = 0.f;
+        BDH_HIP(hipEventElapsedTime(&t, r.e0, r.e1));
+        ms[r.phase] += t;
+        counts[r.phase] += 1;
+    }
+    s->rec->clear();
+    return 0;
 }
 
 int bdh_sync(bdh_system* s)

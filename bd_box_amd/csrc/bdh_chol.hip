@@ -6,10 +6,11 @@
 // multiple of PAD_TO with an identity block so every tile is whole.
 //
 // Per panel of CHOL_NB = 64 columns:
-//   panel kernel   one workgroup per 32 rows at and below the diagonal block: each loads
-//                  the 64x64 diagonal block, factors it in FP64 in LDS (redundantly in every
-//                  workgroup, 64^3/3 flops), then solves its own 32 rows X L_kk^T = B in
-//                  FP64 and stores them in T.  No separate POTRF launch, no inverse.
+//   panel kernel   one workgroup per 256 rows below the diagonal block (plus one that stores
+//                  L_kk): each factors the 64x64 diagonal block in FP64 in registers of one
+//                  wavefront (redundantly, 64^3/3 flops), then every thread solves one row
+//                  x L_kk^T = b in FP64 and stores it in T.  No separate POTRF launch, no
+//                  inverse.
 //   trailing SYRK  A22 -= L21 L21^T over 128x128 lower tiles on MFMA:
 //                  T = float : v_mfma_f32_32x32x2_f32, FP32 accumulation
 //                  T = double: v_mfma_f64_16x16x4_f64
@@ -21,76 +22,87 @@
 
 namespace bdh {
 
-constexpr int PANEL_ROWS = 32;
+constexpr int PANEL_ROWS = 256;  // rows solved per workgroup, one per thread
 
+// Value of v held by lane `src` (uniform), through two v_readlane_b32.
+__device__ __forceinline__ double readlane_f64(double v, int src)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Panel kernel.  Workgroup 0 stores L_kk; workgroup b >= 1 solves rows
+// c0 + 64 + 256 (b - 1) + tid.  In every workgroup, wave 0 first factors the 64x64
+// diagonal block with each lane holding one row in registers (FP64): for column j the pivot
+// and the entries of the scaled column come from the lanes holding them (v_readlane: scalar
+// operands of the FMAs), and every lane applies the rank-1 update to its own row -- no LDS
+// traffic and no barriers inside the 64 steps.  After one
+// barrier each thread loads and solves its row
+// x L_kk^T = b column by column (right-looking: x_c is final, then x_k -= x_c L[k][c] for
+// k > c) against L_kk stored transposed in LDS, so every read is a broadcast.
 template <typename T>
 __global__ void __launch_bounds__(256)
-chol_panel_kernel(T* __restrict__ A, long long ld, int c0, int* __restrict__ status)
+chol_panel_kernel(T* __restrict__ A, long long ld, int c0, int n_pad, int* __restrict__ status)
 {
-    __shared__ double sL[CHOL_NB][CHOL_NB + 1];
-    __shared__ double sX[PANEL_ROWS][CHOL_NB + 1];
-    const int tid = threadIdx.x;
-    const int rb = blockIdx.x;
-    const int r0 = c0 + rb * PANEL_ROWS;
-    const T* Ad = A + (long long)c0 * ld + c0;
-
-    for (int idx = tid; idx < CHOL_NB * CHOL_NB; idx += 256) {
-        int r = idx % CHOL_NB, c = idx / CHOL_NB;
-        sL[r][c] = (r >= c) ? (double)Ad[(long long)c * ld + r] : 0.0;
-    }
-    const bool solve = rb >= CHOL_NB / PANEL_ROWS;
-    if (solve) {
-        const T* Ax = A + (long long)c0 * ld + r0;
-        for (int idx = tid; idx < PANEL_ROWS * CHOL_NB; idx += 256) {
-            int r = idx % PANEL_ROWS, c = idx / PANEL_ROWS;
-            sX[r][c] = (double)Ax[(long long)c * ld + r];
+    __shared__ __align__(16) double sLT[CHOL_NB][CHOL_NB + 2];  // sLT[c][k] = L_kk[k][c]
+    __shared__ double s_inv[CHOL_NB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.x;
+    const int r = c0 + CHOL_NB + (b - 1) * PANEL_ROWS + tid;  // row solved by this thread
+    const bool solves = b > 0 && r < n_pad;
+    if (wave == 0) {
+        const T* Ad = A + (long long)c0 * ld + c0 + lane;
+        double a[CHOL_NB];
+#pragma unroll
+        for (int c = 0; c < CHOL_NB; ++c) a[c] = (c <= lane) ? (double)Ad[(long long)c * ld] : 0.0;
+#pragma unroll
+        for (int j = 0; j < CHOL_NB; ++j) {
+            double d = readlane_f64(a[j], j);
+            if (!(d > 0.0)) {
+                if (lane == 0) atomicCAS(status, 0, c0 + j + 1);
+                d = __builtin_nan("");
+            }
+            const double sd = sqrt(d);
+            const double inv = 1.0 / sd;
+            a[j] = (lane == j) ? sd : a[j] * inv;
+#pragma unroll
+            for (int k = j + 1; k < CHOL_NB; ++k) a[k] -= a[j] * readlane_f64(a[j], k);
         }
+#pragma unroll
+        for (int c = 0; c < CHOL_NB; ++c) sLT[c][lane] = (c <= lane) ? a[c] : 0.0;
+        __builtin_amdgcn_wave_barrier();
+        // (a[lane] would index the register array at run time and push it to scratch)
+        s_inv[lane] = 1.0 / sLT[lane][lane];
     }
     __syncthreads();
-    // right-looking factorization of the diagonal block
-    for (int j = 0; j < CHOL_NB; ++j) {
-        double d = sL[j][j];
-        if (!(d > 0.0)) {
-            if (tid == 0) atomicCAS(status, 0, c0 + j + 1);
-            d = __builtin_nan("");
+    if (b == 0) {
+        // store the lower part of L_kk
+        T* Aw = A + (long long)c0 * ld + c0;
+        for (int idx = tid; idx < CHOL_NB * CHOL_NB; idx += 256) {
+            const int rr = idx % CHOL_NB, cc = idx / CHOL_NB;
+            if (rr >= cc) Aw[(long long)cc * ld + rr] = (T)sLT[cc][rr];
         }
-        const double s = sqrt(d);
-        const double inv = 1.0 / s;
-        __syncthreads();
-        if (tid < CHOL_NB) {
-            if (tid > j) sL[tid][j] *= inv;
-            else if (tid == j) sL[j][j] = s;
-        }
-        __syncthreads();
-        const int r = tid & (CHOL_NB - 1);
-        const double lrj = sL[r][j];
-        for (int c = j + 1 + (tid >> 6); c <= r; c += 4) sL[r][c] -= lrj * sL[c][j];
-        __syncthreads();
+        return;
     }
-    if (solve) {
-        // X L^T = B, column by column
-        for (int c = 0; c < CHOL_NB; ++c) {
-            if (tid < PANEL_ROWS) sX[tid][c] /= sL[c][c];
-            __syncthreads();
-            const int r = tid & (PANEL_ROWS - 1);
-            const double xc = sX[r][c];
-            for (int c2 = c + 1 + (tid >> 5); c2 < CHOL_NB; c2 += 8) sX[r][c2] -= xc * sL[c2][c];
-            __syncthreads();
-        }
-        T* Ax = A + (long long)c0 * ld + r0;
-        for (int idx = tid; idx < PANEL_ROWS * CHOL_NB; idx += 256) {
-            int r = idx % PANEL_ROWS, c = idx / PANEL_ROWS;
-            Ax[(long long)c * ld + r] = (T)sX[r][c];
-        }
-    } else {
-        // rows of the diagonal block itself: store the lower part of L_kk
-        T* Aw = A + (long long)c0 * ld + r0;
-        const int rbase = rb * PANEL_ROWS;
-        for (int idx = tid; idx < PANEL_ROWS * CHOL_NB; idx += 256) {
-            int r = idx % PANEL_ROWS, c = idx / PANEL_ROWS;
-            if (rbase + r >= c) Aw[(long long)c * ld + r] = (T)sL[rbase + r][c];
-        }
+    if (!solves) return;
+    // the row is loaded only now: keeping it live across the factorization would double
+    // the register footprint of wave 0
+    double x[CHOL_NB];
+    const T* Axl = A + (long long)c0 * ld + r;
+#pragma unroll
+    for (int c = 0; c < CHOL_NB; ++c) x[c] = (double)Axl[(long long)c * ld];
+#pragma unroll
+    for (int c = 0; c < CHOL_NB; ++c) {
+        x[c] *= s_inv[c];
+        const double xc = x[c];
+#pragma unroll
+        for (int k = c + 1; k < CHOL_NB; ++k) x[k] -= xc * sLT[c][k];
     }
+    T* Ax = A + (long long)c0 * ld + r;
+#pragma unroll
+    for (int c = 0; c < CHOL_NB; ++c) Ax[(long long)c * ld] = (T)x[c];
 }
 
 // Lower-triangular tile index (ti >= tj) from a linear index x.
@@ -254,19 +266,23 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int c0, int t0, int n_pad)
 }
 
 template <typename T>
-int chol_factor(T* d_A, long long ld, int n_pad, int* d_status, int syrk_mode, hipStream_t st)
+int chol_factor(T* d_A, long long ld, int n_pad, int* d_status, int syrk_mode, hipStream_t st,
+                PhaseRecorder* rec)
 {
     BDH_CHECK(n_pad % PAD_TO == 0, "chol_factor: n_pad must be a multiple of 128");
     BDH_CHECK(ld >= n_pad && ld % 4 == 0, "chol_factor: bad leading dimension");
     for (int c0 = 0; c0 < n_pad; c0 += CHOL_NB) {
-        const int rows = n_pad - c0;
-        hipLaunchKernelGGL(chol_panel_kernel<T>, dim3(rows / PANEL_ROWS), dim3(256), 0, st,
-                           d_A, ld, c0, d_status);
+        const int below = n_pad - c0 - CHOL_NB;
+        if (rec) rec->begin(PH_PANEL, st);
+        hipLaunchKernelGGL(chol_panel_kernel<T>, dim3(1 + ceil_div(below, PANEL_ROWS)), dim3(256),
+                           0, st, d_A, ld, c0, n_pad, d_status);
         BDH_LAUNCH_CHECK();
+        if (rec) rec->end(PH_PANEL, st);
         const int t0 = c0 + CHOL_NB;
         if (t0 >= n_pad) break;
         const int nt = ceil_div(n_pad - t0, SY_TILE);
         const int ntiles = nt * (nt + 1) / 2;
+        if (rec) rec->begin(PH_SYRK, st);
         if constexpr (sizeof(T) == 4) {
             (void)syrk_mode;
             hipLaunchKernelGGL(syrk_f32_kernel, dim3(ntiles), dim3(256), 0, st,
@@ -276,15 +292,36 @@ int chol_factor(T* d_A, long long ld, int n_pad, int* d_status, int syrk_mode, h
                                reinterpret_cast<double*>(d_A), ld, c0, t0, n_pad);
         }
         BDH_LAUNCH_CHECK();
+        if (rec) rec->end(PH_SYRK, st);
     }
     return 0;
 }
 
 // ------------------------------------------------------------------ y = L x (partials)
 // Tile: (64 lanes x V rows) x TRMV_COLS columns, V = 16 / sizeof(T) so each lane issues one
-// 16-byte load per column; the 4 waves split the tile's columns and meet in LDS.
-// P[cb][r] receives row r's partial over column block cb (FP64 accumulation).
+// 16-byte load per column; the 4 waves split the tile's columns (unrolled by 8 for loads in
+// flight) and meet in LDS.  Entries above the diagonal are masked with a select after the
+// load (the storage there is finite but meaningless), so diagonal tiles take the same
+// vector path.  P[cb][r] receives row r's partial over column block cb (FP64 accumulation).
 constexpr int TRMV_COLS = 256;
+
+template <typename T>
+__device__ __forceinline__ void trmv_accum(const T* __restrict__ col, int rbase, int c, double xc,
+                                           bool mask, double* acc)
+{
+    constexpr int V = 16 / sizeof(T);
+    if constexpr (V == 4) {
+        const float4 v = *reinterpret_cast<const float4*>(col + rbase);
+        acc[0] += (double)((!mask || rbase + 0 >= c) ? v.x : 0.f) * xc;
+        acc[1] += (double)((!mask || rbase + 1 >= c) ? v.y : 0.f) * xc;
+        acc[2] += (double)((!mask || rbase + 2 >= c) ? v.z : 0.f) * xc;
+        acc[3] += (double)((!mask || rbase + 3 >= c) ? v.w : 0.f) * xc;
+    } else {
+        const double2 v = *reinterpret_cast<const double2*>(col + rbase);
+        acc[0] += ((!mask || rbase + 0 >= c) ? v.x : 0.0) * xc;
+        acc[1] += ((!mask || rbase + 1 >= c) ? v.y : 0.0) * xc;
+    }
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -305,37 +342,24 @@ trmv_partial_kernel(const T* __restrict__ L, long long ld, int n, const double* 
     const int cend = min(n, cbeg + TRMV_COLS);
     const int wcols = TRMV_COLS / 4;
     const int c_lo = cbeg + wave * wcols, c_hi = min(cend, c_lo + wcols);
+    // the whole tile is below the diagonal when its last column precedes its first row
+    const bool mask = (cbeg + TRMV_COLS - 1) > r0;
     if (rbase < n) {
-        const bool full = (c_hi - 1) <= rbase;  // every column of this wave is left of the lane's rows
-        if (full) {
-#pragma unroll 4
-            for (int c = c_lo; c < c_hi; ++c) {
-                const double xc = x[c];
-                if constexpr (V == 4) {
-                    float4 v = *reinterpret_cast<const float4*>(L + (long long)c * ld + rbase);
-                    acc[0] += (double)v.x * xc; acc[1] += (double)v.y * xc;
-                    acc[2] += (double)v.z * xc; acc[3] += (double)v.w * xc;
-                } else {
-                    double2 v = *reinterpret_cast<const double2*>(L + (long long)c * ld + rbase);
-                    acc[0] += v.x * xc; acc[1] += v.y * xc;
-                }
-            }
-        } else {
-            for (int c = c_lo; c < c_hi && c <= rbase + V - 1; ++c) {
-                const double xc = x[c];
+        int c = c_lo;
+        for (; c + 8 <= c_hi; c += 8) {
 #pragma unroll
-                for (int q = 0; q < V; ++q)
-                    if (rbase + q >= c) acc[q] += (double)L[(long long)c * ld + rbase + q] * xc;
-            }
+            for (int u = 0; u < 8; ++u)
+                trmv_accum<T>(L + (long long)(c + u) * ld, rbase, c + u, x[c + u], mask, acc);
         }
+        for (; c < c_hi; ++c) trmv_accum<T>(L + (long long)c * ld, rbase, c, x[c], mask, acc);
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) s_red[wave][lane * V + q] = acc[q];
     __syncthreads();
-    for (int r = tid; r < ROWS; r += 256) {
-        const int row = r0 + r;
+    for (int rr = tid; rr < ROWS; rr += 256) {
+        const int row = r0 + rr;
         if (row < n)
-            P[(long long)cb * n + row] = s_red[0][r] + s_red[1][r] + s_red[2][r] + s_red[3][r];
+            P[(long long)cb * n + row] = (s_red[0][rr] + s_red[1][rr]) + (s_red[2][rr] + s_red[3][rr]);
     }
 }
 
@@ -353,29 +377,10 @@ int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_x, d
     return 0;
 }
 
-// y[r] = scale * sum_{cb <= r / blk_cols} P[cb][r]
-__global__ void sum_partials_kernel(const double* __restrict__ P, int n, int nblk, int blk_cols,
-                                    double scale, double* __restrict__ y)
-{
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    int last = blk_cols > 0 ? min(nblk - 1, r / blk_cols) : nblk - 1;
-    double s = 0.0;
-    for (int cb = 0; cb <= last; ++cb) s += P[(long long)cb * n + r];
-    y[r] = scale * s;
-}
+int trmv_blk_cols() { return TRMV_COLS; }
 
-int launch_sum_partials(const double* d_P, int n, int nblk, int blk_cols, double scale,
-                        double* d_y, hipStream_t st)
-{
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, d_P, n,
-                       nblk, blk_cols, scale, d_y);
-    BDH_LAUNCH_CHECK();
-    return 0;
-}
-
-template int chol_factor<float>(float*, long long, int, int*, int, hipStream_t);
-template int chol_factor<double>(double*, long long, int, int*, int, hipStream_t);
+template int chol_factor<float>(float*, long long, int, int*, int, hipStream_t, PhaseRecorder*);
+template int chol_factor<double>(double*, long long, int, int*, int, hipStream_t, PhaseRecorder*);
 template int launch_trmv_partials<float>(const float*, long long, int, const double*, double*, hipStream_t);
 template int launch_trmv_partials<double>(const double*, long long, int, const double*, double*, hipStream_t);
 

@@ -177,69 +177,85 @@ int launch_forces(const double4* d_pos, const double* d_Q, const double2* d_LJ, 
     return 0;
 }
 
-// F = sum over splits; E = (E_LJ, E_coulomb, E_bond) summed over workgroups
-__global__ void forces_reduce_kernel(const double* __restrict__ PF, const double* __restrict__ PE,
-                                     int N, int splits, int pe_blocks, double* __restrict__ F,
-                                     double* __restrict__ E)
+// E = (E_LJ, E_coulomb, E_bond): sum of the per-workgroup partials (one block, tree sum)
+__global__ void __launch_bounds__(256)
+energy_reduce_kernel(const double* __restrict__ PE, int pe_blocks, double* __restrict__ E)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < 3 * N) {
-        double s = 0.0;
-        for (int k = 0; k < splits; ++k) s += PF[(long long)k * 3 * N + r];
-        F[r] = s;
+    __shared__ double s[3][256];
+    const int t = threadIdx.x;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int k = t; k < pe_blocks; k += 256) {
+        a0 += PE[3 * k];
+        a1 += PE[3 * k + 1];
+        a2 += PE[3 * k + 2];
     }
-    if (E && blockIdx.x == 0 && threadIdx.x < 3) {
-        double s = 0.0;
-        for (int k = 0; k < pe_blocks; ++k) s += PE[3 * k + threadIdx.x];
-        E[threadIdx.x] = s;
+    s[0][t] = a0; s[1][t] = a1; s[2][t] = a2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) { s[0][t] += s[0][t + w]; s[1][t] += s[1][t + w]; s[2][t] += s[2][t + w]; }
+        __syncthreads();
     }
+    if (t < 3) E[t] = s[t][0];
 }
 
-int launch_forces_reduce(const double* d_PF, const double* d_PE, int N, int splits,
-                         int pe_blocks, double* d_F, double* d_E, hipStream_t st)
+int launch_energy_reduce(const double* d_PE, int pe_blocks, double* d_E, hipStream_t st)
 {
-    hipLaunchKernelGGL(forces_reduce_kernel, dim3(ceil_div(3 * N, 256)), dim3(256), 0, st, d_PF,
-                       d_PE, N, splits, pe_blocks, d_F, d_E);
+    hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(256), 0, st, d_PE, pe_blocks, d_E);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+// out_j[r] = scale_j * sum_{s <= last_j(r)} P_j[s][r] for up to three jobs (blockIdx.y),
+// last_j(r) = S_j - 1, or min(S_j - 1, r / blk_cols_j) for the triangular partials of L.x.
+// 64 rows per block; the 4 waves take every 4th partial and meet in LDS.
+__global__ void __launch_bounds__(256)
+reduce_partials_kernel(ReduceJobs jobs)
+{
+    __shared__ double s_red[4][64];
+    const ReduceJob& J = jobs.job[blockIdx.y];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = blockIdx.x * 64 + lane;
+    double acc = 0.0;
+    if (r < J.n) {
+        const int last = J.blk_cols > 0 ? min(J.S - 1, r / J.blk_cols) : J.S - 1;
+        const double* p = J.P + r;
+        int sidx = wave;
+        for (; sidx + 12 <= last; sidx += 16) {
+            const double v0 = p[(long long)sidx * J.n];
+            const double v1 = p[(long long)(sidx + 4) * J.n];
+            const double v2 = p[(long long)(sidx + 8) * J.n];
+            const double v3 = p[(long long)(sidx + 12) * J.n];
+            acc += (v0 + v1) + (v2 + v3);
+        }
+        for (; sidx <= last; sidx += 4) acc += p[(long long)sidx * J.n];
+    }
+    s_red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && r < J.n)
+        J.out[r] = J.scale * ((s_red[0][lane] + s_red[1][lane]) + (s_red[2][lane] + s_red[3][lane]));
+}
+
+int launch_reduce_partials(const ReduceJobs& jobs, int njobs, int max_n, hipStream_t st)
+{
+    if (njobs <= 0) return 0;
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(ceil_div(max_n, 64), njobs), dim3(256), 0, st,
+                       jobs);
     BDH_LAUNCH_CHECK();
     return 0;
 }
 
 // ------------------------------------------------------------------ E-M position update
 // r_i <- r_i + Rf_i + Rs_i, then the periodic wrap of trans_to_box (diff_equation.c:254-264,
-// trans.c:101-107).  Rf = rf_scale * sum of the D.F partials; Rs = rs_scale * sum of the
-// L.xi partials, or, without hydrodynamic interactions (d_diag_noise != null),
-// Rs = sqrt(2 dt D_ii) xi with the prefactor already folded into d_diag_noise.
-__global__ void em_update_kernel(double4* __restrict__ pos, int N, const double* __restrict__ Pf,
-                                 int nf, double rf_scale, const double* __restrict__ Pr, int nr,
-                                 int r_blk_cols, double rs_scale,
-                                 const double* __restrict__ diag_noise, int pbc, double box,
-                                 double* __restrict__ Rf, double* __restrict__ Rs)
+// trans.c:101-107).
+__global__ void em_update_kernel(double4* __restrict__ pos, int N, const double* __restrict__ Rf,
+                                 const double* __restrict__ Rs, int pbc, double box)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const int n = 3 * N;
     double4 p = pos[i];
-    double d[3];
-    for (int k = 0; k < 3; ++k) {
-        const int r = 3 * i + k;
-        double f = 0.0;
-        for (int s = 0; s < nf; ++s) f += Pf[(long long)s * n + r];
-        f *= rf_scale;
-        double g = 0.0;
-        if (diag_noise) {
-            g = diag_noise[r];
-        } else {
-            const int last = min(nr - 1, r / r_blk_cols);
-            for (int s = 0; s <= last; ++s) g += Pr[(long long)s * n + r];
-            g *= rs_scale;
-        }
-        Rf[r] = f;
-        Rs[r] = g;
-        d[k] = f + g;
-    }
-    p.x = d[0] + p.x;
-    p.y = d[1] + p.y;
-    p.z = d[2] + p.z;
+    p.x = (Rf[3 * i] + Rs[3 * i]) + p.x;
+    p.y = (Rf[3 * i + 1] + Rs[3 * i + 1]) + p.y;
+    p.z = (Rf[3 * i + 2] + Rs[3 * i + 2]) + p.z;
     if (pbc) {
         const double inv = 1.0 / box;
         p.x -= box * floor(p.x * inv + 0.5);
@@ -249,14 +265,11 @@ __global__ void em_update_kernel(double4* __restrict__ pos, int N, const double*
     pos[i] = p;
 }
 
-int launch_em_update(double4* d_pos, int N, const double* d_Pf, int nf, double rf_scale,
-                     const double* d_Pr, int nr, int r_blk_cols, double rs_scale,
-                     const double* d_diag_noise, int pbc, double box, double* d_Rf, double* d_Rs,
-                     hipStream_t st)
+int launch_em_update(double4* d_pos, int N, const double* d_Rf, const double* d_Rs, int pbc,
+                     double box, hipStream_t st)
 {
-    hipLaunchKernelGGL(em_update_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, st, d_pos, N, d_Pf,
-                       nf, rf_scale, d_Pr, nr, r_blk_cols, rs_scale, d_diag_noise, pbc, box,
-                       d_Rf, d_Rs);
+    hipLaunchKernelGGL(em_update_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, st, d_pos, N, d_Rf,
+                       d_Rs, pbc, box);
     BDH_LAUNCH_CHECK();
     return 0;
 }

@@ -7,6 +7,16 @@
 
 namespace bdh {
 
+// Measurement hook: when a recorder is passed, launches are bracketed by begin/end on the
+// launch stream (HIP events in the C ABI's implementation).
+enum Phase { PH_NOISE = 0, PH_ASSEMBLE, PH_PANEL, PH_SYRK, PH_MATVEC, PH_TRMV, PH_MOVE,
+             PH_FORCES, PH_COUNT };
+struct PhaseRecorder {
+    virtual ~PhaseRecorder() = default;
+    virtual void begin(int phase, hipStream_t st) = 0;
+    virtual void end(int phase, hipStream_t st) = 0;
+};
+
 // ---- random stream (bdh_rng.hip)
 uint64_t lcg_seed(uint32_t seed);
 int launch_gauss_streams(uint64_t* d_states, double* d_out, const long long* d_offsets,
@@ -28,13 +38,13 @@ constexpr int CHOL_NB = 64;   // panel width
 constexpr int PAD_TO = 128;   // n is padded to a multiple of this with an identity block
 enum SyrkMode { SYRK_F32 = 0, SYRK_F64 = 1, SYRK_BF16X3 = 2 };
 template <typename T>
-int chol_factor(T* d_A, long long ld, int n_pad, int* d_status, int syrk_mode, hipStream_t st);
+int chol_factor(T* d_A, long long ld, int n_pad, int* d_status, int syrk_mode, hipStream_t st,
+                PhaseRecorder* rec = nullptr);
 int trmv_col_blocks(int n);
 template <typename T>
 int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_x, double* d_P,
                          hipStream_t st);
-int launch_sum_partials(const double* d_P, int n, int nblk, int blk_cols, double scale,
-                        double* d_y, hipStream_t st);
+int trmv_blk_cols();   // columns per partial of the L.x kernel
 
 // ---- forces (bdh_forces.hip)
 struct ForceParams {
@@ -63,14 +73,23 @@ template <typename T>
 int launch_forces(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
                   const Topology& topo, const ForceParams& fp, double* d_PF, double* d_PE,
                   int* d_flags, int splits, hipStream_t st);
-int launch_forces_reduce(const double* d_PF, const double* d_PE, int N, int splits,
-                         int pe_blocks, double* d_F, double* d_E, hipStream_t st);
+int launch_energy_reduce(const double* d_PE, int pe_blocks, double* d_E, hipStream_t st);
+
+// Sums of partial rows written by the split kernels (D.F, L.xi, forces).
+struct ReduceJob {
+    const double* P;   // S partial rows of length n
+    double* out;
+    int n, S, blk_cols;  // blk_cols > 0: row r sums partials 0..r/blk_cols only
+    double scale;
+};
+struct ReduceJobs {
+    ReduceJob job[3];
+};
+int launch_reduce_partials(const ReduceJobs& jobs, int njobs, int max_n, hipStream_t st);
 
 // ---- integrator (bdh_forces.hip)
-int launch_em_update(double4* d_pos, int N, const double* d_Pf, int nf, double rf_scale,
-                     const double* d_Pr, int nr, int r_blk_cols, double rs_scale,
-                     const double* d_diag_noise, int pbc, double box, double* d_Rf, double* d_Rs,
-                     hipStream_t st);
+int launch_em_update(double4* d_pos, int N, const double* d_Rf, const double* d_Rs, int pbc,
+                     double box, hipStream_t st);
 
 // ---- probes (bdh_probe.hip)
 int probe_mfma(int kind, const void* A, const void* B, void* C, hipStream_t st);

@@ -13,8 +13,8 @@
 // its pair index p.  The stream state after the call is the state after the attempt that
 // produced the last needed pair -- so the integer stream is bit-identical to the host's,
 // and repeated calls continue it exactly as repeated gauss_rand calls do.
-// v1*v1 + v2*v2 and -2 log(rsq) / rsq are evaluated with explicit round-to-nearest
-// operations (no FMA contraction) as the x86-64 reference does; the device log differs
+// v1*v1 + v2*v2 and -2 log(rsq) / rsq are evaluated without FMA contraction, as the
+// x86-64 reference does; the device log differs
 // from glibc's by an ulp now and then, so Gaussian values agree to ~1e-15 while the
 // integer stream (and so which attempts are accepted) is identical.
 #include "bdh_common.hpp"
@@ -45,11 +45,29 @@ __device__ __forceinline__ uint64_t lcg_jump(uint64_t x, uint64_t k)
 
 __device__ __forceinline__ double u48(uint64_t x) { return (double)x * 0x1p-48; }
 
+// log(r) for r in (0.5, 1) with small relative error as log(r) -> 0:
+// log(1 + d) = 2 atanh(s), s = d / (2 + d), d = r - 1 exact (Sterbenz); |s| < 1/3 so the
+// odd series 2 s (1 + s^2/3 + s^4/5 + ...) is summed to 20 terms (s^40/41 < 1e-20).
+__device__ __forceinline__ double log_near_one(double r)
+{
+#pragma clang fp contract(off)
+    const double d = r - 1.0;
+    const double s = d / (2.0 + d);
+    const double s2 = s * s;
+    double p = 1.0 / 41.0;
+#pragma unroll
+    for (int k = 19; k >= 0; --k) p = 1.0 / (2 * k + 1) + s2 * p;
+    return (2.0 * s) * p;
+}
+
 __global__ void __launch_bounds__(RNG_THREADS)
 gauss_streams_kernel(uint64_t* __restrict__ states, double* __restrict__ out,
                      const long long* __restrict__ offsets, const int* __restrict__ counts,
                      int* __restrict__ status)
 {
+    // no FMA contraction in this function (the pragma covers only expressions written
+    // here, not inlined helpers): v1*v1 + v2*v2 must round exactly as on the host
+#pragma clang fp contract(off)
     __shared__ int s_scan[RNG_THREADS];
     __shared__ long long s_last;  // index (within the round) of the attempt giving the last pair
     const int s = blockIdx.x;
@@ -73,10 +91,10 @@ gauss_streams_kernel(uint64_t* __restrict__ states, double* __restrict__ out,
 #pragma unroll
         for (int q = 0; q < RNG_K; ++q) {
             x = lcg_step(x);
-            v1[q] = __dadd_rn(__dmul_rn(2.0, u48(x)), -1.0);
+            v1[q] = 2.0 * u48(x) - 1.0;
             x = lcg_step(x);
-            v2[q] = __dadd_rn(__dmul_rn(2.0, u48(x)), -1.0);
-            rsq[q] = __dadd_rn(__dmul_rn(v1[q], v1[q]), __dmul_rn(v2[q], v2[q]));
+            v2[q] = 2.0 * u48(x) - 1.0;
+            rsq[q] = v1[q] * v1[q] + v2[q] * v2[q];
             acc += (rsq[q] < 1.0 && rsq[q] != 0.0);
         }
         // inclusive scan of accept counts over the workgroup
@@ -96,12 +114,12 @@ gauss_streams_kernel(uint64_t* __restrict__ states, double* __restrict__ out,
         for (int q = 0; q < RNG_K; ++q) {
             if (rsq[q] < 1.0 && rsq[q] != 0.0) {
                 if (p < npairs) {
-                    // log near 1 through log1p of the exact difference rsq - 1 (Sterbenz), so
-                    // the relative error stays at the ulp level where log(rsq) -> 0
-                    const double lg = rsq[q] > 0.5 ? log1p(__dadd_rn(rsq[q], -1.0)) : log(rsq[q]);
-                    double fac = sqrt(__ddiv_rn(__dmul_rn(-2.0, lg), rsq[q]));
-                    o[2 * p] = __dmul_rn(v2[q], fac);
-                    if (2 * p + 1 < count) o[2 * p + 1] = __dmul_rn(v1[q], fac);
+                    // near 1 the library log is accurate only in absolute terms; the series
+                    // keeps the relative error at the ulp level where log(rsq) -> 0
+                    const double lg = rsq[q] > 0.5 ? log_near_one(rsq[q]) : log(rsq[q]);
+                    double fac = sqrt(-2.0 * lg / rsq[q]);
+                    o[2 * p] = v2[q] * fac;
+                    if (2 * p + 1 < count) o[2 * p + 1] = v1[q] * fac;
                     if (p == npairs - 1) s_last = (long long)t * RNG_K + q;
                 }
                 ++p;

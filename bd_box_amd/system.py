@@ -141,6 +141,18 @@ class BDSystem:
         _lib.check(_lib.load().bdh_time_factor(self._h, reps, C.byref(ms)), "time_factor")
         return ms.value
 
+    PHASES = ("noise", "assemble", "panel", "syrk", "matvec", "trmv", "move", "forces")
+
+    def phase_timing(self, enable: bool = True):
+        _lib.check(_lib.load().bdh_phase_timing(self._h, int(enable)), "phase_timing")
+
+    def phase_times(self):
+        """{phase: (total_ms, launches)} since the previous call."""
+        ms = np.zeros(8)
+        cnt = (C.c_longlong * 8)()
+        _lib.check(_lib.load().bdh_phase_times(self._h, _lib.dptr(ms), cnt), "phase_times")
+        return {p: (float(ms[i]), int(cnt[i])) for i, p in enumerate(self.PHASES)}
+
     def time_steps(self, steps: int) -> float:
         ms = C.c_double()
         _lib.check(_lib.load().bdh_time_steps(self._h, steps, C.byref(ms)), "time_steps")

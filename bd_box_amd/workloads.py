@@ -78,26 +78,62 @@ def _unit_vectors(rng: np.random.Generator, k: int) -> np.ndarray:
 
 def saw_chain(N: int, seed: int, bond: float = 3.8, dmin: float = 4.0) -> np.ndarray:
     """Self-avoiding random walk: each new bead sits `bond` from the previous one and at
-    least `dmin` from every earlier bead (trial directions uniform on the sphere)."""
+    least `dmin` from every earlier bead (trial directions uniform on the sphere).  A bead
+    with no admissible direction among 64 x 8 trials makes the walk back up 8 beads.
+    Earlier beads are found through a hash grid of cell edge bond + dmin, so only the 27
+    cells around the previous bead are searched."""
     rng = np.random.default_rng(seed)
     X = np.zeros((N, 3))
     d2 = dmin * dmin
+    h = bond + dmin
+    grid: Dict[Tuple[int, int, int], List[int]] = {}
+
+    def key(p):
+        return (int(math.floor(p[0] / h)), int(math.floor(p[1] / h)), int(math.floor(p[2] / h)))
+
+    def near(p, upto):
+        cx, cy, cz = key(p)
+        out: List[int] = []
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    out.extend(j for j in grid.get((cx + dx, cy + dy, cz + dz), ()) if j < upto)
+        return out
+
+    grid.setdefault(key(X[0]), []).append(0)
     i = 1
+    backtracks = 0
     while i < N:
-        trials = X[i - 1] + bond * _unit_vectors(rng, 64)
-        if i >= 2:
-            ok = np.ones(len(trials), bool)
-            prev = X[: i - 1]
-            # squared distances of every trial to every earlier non-bonded bead
+        placed = False
+        for _ in range(8 if i >= 2 else 1):
+            trials = X[i - 1] + bond * _unit_vectors(rng, 64)
+            if i < 2:
+                X[i] = trials[0]
+                placed = True
+                break
+            cand = near(X[i - 1], i - 1)   # earlier non-bonded beads near the previous one
+            if not cand:
+                X[i] = trials[0]
+                placed = True
+                break
+            prev = X[cand]
             dd = ((trials[:, None, :] - prev[None, :, :]) ** 2).sum(-1).min(axis=1)
-            ok &= dd >= d2
-            idx = np.flatnonzero(ok)
-            if idx.size == 0:
-                raise RuntimeError(f"self-avoiding walk stuck at bead {i}")
-            X[i] = trials[idx[0]]
+            idx = np.flatnonzero(dd >= d2)
+            if idx.size:
+                X[i] = trials[idx[0]]
+                placed = True
+                break
+        if placed:
+            grid.setdefault(key(X[i]), []).append(i)
+            i += 1
         else:
-            X[i] = trials[0]
-        i += 1
+            backtracks += 1
+            if backtracks > 10 * N:
+                raise RuntimeError(f"self-avoiding walk stuck at bead {i}")
+            back = max(1, i - 8)
+            for j in range(back, i):
+                grid[key(X[j])].remove(j)
+            i = back
     return X - X.mean(axis=0)
 
 

@@ -1,0 +1,259 @@
+#!/usr/bin/env python3
+"""BD steps/s of the N=3000 free-space RPY + Cholesky Ermak-McCammon step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: one bead-spring chain of N=3000 beads (self-avoiding walk, 3.8 A bonds, >= 4 A
+non-bonded separation, a = 1.9 A, harmonic bonds k = 10, LJ sigma 4 A eps 0.6 cutoff 12 A,
+20 % of beads charged +-1 with Debye-Hueckel kappa^-1 10 A eps_r 78.5 cutoff 12 A,
+T = 293 K, eta = 0.01 P, dt = 10 fs), free-space RPY, Cholesky noise refactored every
+10 steps, Gaussians from the device drand48 stream.  One "step" = noise + (every 10th
+step) assembly and factorization of the 9000 x 9000 tensor + D.F + L.xi + move + forces,
+with all inputs resident in HBM.
+
+Printed JSON line: `value` = mixed-precision steps/s; the same steps on the all-FP64
+path, the per-step relative displacement error mixed vs FP64, ||L L^T - D|| / ||D||,
+the roofline of the dominant kernel (HIP events on the launch stream during a timed run)
+and the reference's own CPU path (compiled from its sources, OpenMP + OpenBLAS) timed on
+this host's cores beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BD steps/s, N=3000 RPY+Cholesky: mixed vs FP64 GPU at rel err≤1e-5"
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"f32_mfma": 157.3, "f64_mfma": 78.6}
+KB = 0.001986248532617
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(N, seed):
+    from bd_box_amd import workloads as wl
+    sysm = wl.chain(N, seed=seed)
+    prm = wl.chain_params(rand_seed=20240 + seed)
+    return sysm, prm, wl.parse_params(prm)
+
+
+# ----------------------------------------------------------------------------- roofline
+def syrk_flops_per_factor(n, n_pad, nb=64):
+    """Algorithmic flops of the trailing updates of one factorization (lower triangle
+    only, unpadded size): sum over panels of nb * m (m + 1), m = rows below the panel."""
+    f = 0
+    for c0 in range(0, n_pad, nb):
+        m = max(0, n - (c0 + nb))
+        f += nb * m * (m + 1)
+    return f
+
+
+def roofline(phases, n, precision, refactors):
+    n_pad = -(-n // 128) * 128
+    s = 4 if precision == "mixed" else 8
+    dom = max(phases, key=lambda k: phases[k][0])
+    ms, cnt = phases[dom]
+    out = {"kernel": dom, "launches": cnt, "avg_ms": ms / max(cnt, 1)}
+    if dom == "syrk":
+        fl = syrk_flops_per_factor(n, n_pad) * refactors
+        achieved = fl / (ms * 1e-3) / 1e12
+        peak = PEAK_TFLOPS["f32_mfma" if precision == "mixed" else "f64_mfma"]
+        out.update(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s",
+                   frac=achieved / peak, traffic=None,
+                   basis="n_alg=%d: sum over 64-wide panels of 64 m (m+1) flops" % n)
+    elif dom in ("trmv", "assemble"):
+        per = s * n * (n + 1) / 2
+        achieved = per * cnt / (ms * 1e-3) / 1e9
+        out.update(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                   frac=achieved / HBM_PEAK_GBS, traffic=None,
+                   basis="%d B per lower-triangle element, n(n+1)/2 elements" % s)
+    else:
+        out.update(bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
+                   traffic=None, basis="not bandwidth or MFMA shaped")
+    return out
+
+
+# ----------------------------------------------------------------------------- CPU baseline
+def cpu_baseline(N, seed, steps, threads):
+    """Time the reference binary (compiled from its sources) on this host."""
+    from bd_box_amd import workloads as wl
+    exe = os.path.join(ROOT, "oracle", "_ref", "bd_box_lapack")
+    kind = "reference"
+    if not os.path.exists(exe):
+        exe = os.path.join(ROOT, "oracle", "_ref", "bd_box")
+        if not os.path.exists(exe):
+            return None
+    sysm, prm, _ = workload(N, seed)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads))
+    times = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        sp = os.path.join(tmp, "chain.str")
+        wl.write_str(sp, sysm)
+        for b in (1, steps + 1):
+            pp = os.path.join(tmp, f"chain{b}.prm")
+            wl.write_prm(pp, dict(prm, bdsteps=b), sp, tmp)
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, pp], cwd=tmp, env=env, capture_output=True, text=True,
+                               timeout=600)
+            times[b] = time.perf_counter() - t0
+            if r.returncode != 0:
+                log(f"reference run failed: {r.stdout[-500:]} {r.stderr[-500:]}")
+                return None
+    dt = times[steps + 1] - times[1]
+    return {"value": steps / dt, "unit": "steps/s", "cores": threads, "kind": kind,
+            "sample": f"{steps} steps of the N={N} chain, reference semantics (refactor every "
+                      f"step), {os.path.basename(exe)}: wall(bdsteps={steps + 1}) - "
+                      f"wall(bdsteps=1) = {dt:.2f} s"}
+
+
+# ----------------------------------------------------------------------------- accuracy
+def displacement_error(mixed, fp64, steps):
+    """Per-step relative displacement error of the mixed path against the FP64 path from
+    the same positions and the same Gaussians (both systems share step count and stream)."""
+    errs = []
+    for _ in range(steps):
+        c = fp64.coords()
+        mixed.set_coords(c)
+        mixed.set_rng_state(fp64.rng_state())
+        mixed.compute_hydro(1)
+        fp64.compute_hydro(1)
+        rf_m, rs_m = mixed.last_displacement()
+        rf_d, rs_d = fp64.last_displacement()
+        dm, dd = rf_m + rs_m, rf_d + rs_d
+        errs.append(float(np.linalg.norm(dm - dd) / np.linalg.norm(dd)))
+    return errs
+
+
+def factor_residual(mixed, fp64, probes=6):
+    """||L L^T - D||_F / ||D||_F of the mixed factor, estimated with Gaussian probes
+    (E ||A v||^2 = ||A||_F^2) against the FP64-assembled D."""
+    L = mixed.factor()
+    D = fp64.compute_D()
+    rng = np.random.default_rng(0)
+    num = den = 0.0
+    for _ in range(probes):
+        v = rng.normal(size=D.shape[0])
+        r = L @ (L.T @ v) - D @ v
+        num += r @ r
+        den += (D @ v) @ (D @ v)
+    return float(np.sqrt(num / den))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--N", type=int, default=3000)
+    ap.add_argument("--refactor", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the timed mixed-precision steps (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step of a "
+              "single chain runs on one MI355X; sharding is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from bd_box_amd.system import BDSystem, FP64, MIXED
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    torch.cuda.init()
+
+    sysm, prm, P = workload(args.N, args.seed)
+    n = 3 * sysm.N
+    log(f"workload: N={sysm.N} n={n} refactor_every={args.refactor}")
+    mixed = BDSystem(sysm, P, precision=MIXED, refactor_every=args.refactor)
+    mixed.compute_hydro(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mixed.compute_hydro(args.steps)
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    log(f"mixed: {args.steps} steps in {t_mixed:.3f} s")
+    if args.profile_only:
+        print(json.dumps({"profile_only": True, "ms_per_step": 1e3 * t_mixed / args.steps}))
+        return
+
+    # per-phase HIP events over a timed run of whole refactor cycles
+    cyc = max(args.refactor, 1) * 2
+    mixed.phase_timing(True)
+    mixed.compute_hydro(cyc)
+    ph_mixed = mixed.phase_times()
+    mixed.phase_timing(False)
+    log("mixed phases: " + json.dumps(ph_mixed))
+
+    fp64 = BDSystem(sysm, P, precision=FP64, refactor_every=args.refactor)
+    fp64.compute_hydro(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fp64.compute_hydro(args.steps)
+    torch.cuda.synchronize()
+    t_fp64 = time.perf_counter() - t0
+    log(f"fp64: {args.steps} steps in {t_fp64:.3f} s")
+    fp64.phase_timing(True)
+    fp64.compute_hydro(cyc)
+    ph_fp64 = fp64.phase_times()
+    fp64.phase_timing(False)
+    log("fp64 phases: " + json.dumps(ph_fp64))
+
+    acc = {}
+    if not args.no_accuracy:
+        # start the comparison on a refactor step of both systems
+        done = args.warmup + args.steps + cyc
+        pad = (-done) % max(args.refactor, 1)
+        if pad:
+            mixed.compute_hydro(pad)
+            fp64.compute_hydro(pad)
+        errs = displacement_error(mixed, fp64, max(args.refactor, 1))
+        acc["rel_disp_err_max"] = max(errs)
+        acc["rel_disp_err_mean"] = float(np.mean(errs))
+        acc["chol_rel_resid"] = factor_residual(mixed, fp64)
+        log(f"accuracy: {acc}")
+
+    refactors = cyc // max(args.refactor, 1)
+    roof = roofline(ph_mixed, n, "mixed", refactors)
+    value = args.steps / t_mixed
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: reference binary, {threads} threads, {args.cpu_steps} steps")
+        cpu = cpu_baseline(args.N, args.seed, args.cpu_steps, threads)
+    line = {
+        "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t_mixed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"N={args.N} bead-spring chain, free-space RPY + Cholesky "
+                               f"refactored every {args.refactor} steps, mixed precision",
+                   "N": args.N, "n": n, "refactor_every": args.refactor,
+                   "noise": "device drand48 polar Box-Muller stream", "parallelism": "1 GPU"},
+        "fp64_steps_per_s": args.steps / t_fp64,
+        "speedup_vs_fp64": t_fp64 / t_mixed,
+        **acc,
+        "phases_ms_per_step": {k: v[0] / cyc for k, v in ph_mixed.items()},
+        "fp64_phases_ms_per_step": {k: v[0] / cyc for k, v in ph_fp64.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

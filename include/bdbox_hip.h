@@ -128,9 +128,17 @@ uint64_t bdh_seed_state(uint32_t seed);
  * A (MxK) and B (KxN) row-major host arrays (float, or double for kind 2); C = A B. */
 int bdh_probe_mfma(int kind, const void* A, const void* B, void* C);
 
-/* Timing helpers for measurement: device-side events bracketing the hot kernels. */
+/* Measurement (no counterpart in the reference, which has no timers: diff_comp.c:58-74).
+ * bdh_time_factor: mean ms of assembly + factorization over `reps` refactors.
+ * bdh_time_steps : ms of `steps` device-stream steps, HIP events on the system's stream.
+ * bdh_phase_timing(sys, 1) brackets every later launch with HIP events on the system's
+ * stream; bdh_phase_times returns, per phase, summed ms and launch counts since the last
+ * call, phases 0 noise, 1 assembly, 2 panel factor, 3 trailing update, 4 D.F, 5 L.xi,
+ * 6 move, 7 forces (arrays of 8). */
 int bdh_time_factor(bdh_system* sys, int reps, double* ms_per_factor);
 int bdh_time_steps(bdh_system* sys, int steps, double* ms_total);
+int bdh_phase_timing(bdh_system* sys, int enable);
+int bdh_phase_times(bdh_system* sys, double* ms, long long* counts);
 int bdh_sync(bdh_system* sys);
 
 #ifdef __cplusplus

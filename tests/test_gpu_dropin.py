@@ -1,0 +1,61 @@
+"""The unmodified reference driver on the HIP backend.
+
+oracle/_ref/bd_box_hip is the reference's own driver (main.c, main_loop.c, input.c,
+output.c, ...) linked with bd_box_amd/csrc/bdbox_dropin.c + libbdbox_hip.so in place of
+its backend TUs; oracle/_ref/bd_box is the reference itself.  Both run the same .prm/.str
+and the xyz trajectories they write (%f, 6 decimals; src/output.c:64-75) must agree.
+Both binaries are built where the reference sources are mounted (make -C oracle ref dropin)
+and travel with the tree; the test skips when they are absent.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from bd_box_amd import workloads as wl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref", "bd_box")
+HIP = os.path.join(ROOT, "oracle", "_ref", "bd_box_hip")
+
+pytestmark = pytest.mark.gpu
+
+
+def read_xyz(path):
+    frames, cur = [], []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    i = 0
+    while i < len(lines):
+        n = int(lines[i].split()[0])
+        block = lines[i + 2:i + 2 + n]
+        frames.append([[float(v) for v in ln.split()[1:4]] for ln in block])
+        i += 2 + n
+    return np.array(frames)
+
+
+def run(exe, sysm, prm, tmp, env_extra=None):
+    sp = os.path.join(tmp, "c.str")
+    pp = os.path.join(tmp, "c.prm")
+    wl.write_str(sp, sysm)
+    wl.write_prm(pp, prm, sp, tmp)
+    env = dict(os.environ, OMP_NUM_THREADS="1", **(env_extra or {}))
+    r = subprocess.run([exe, pp], cwd=tmp, env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (exe, r.stdout[-2000:], r.stderr[-2000:])
+    return read_xyz(os.path.join(tmp, "out.xyz"))
+
+
+@pytest.mark.skipif(not (os.path.exists(REF) and os.path.exists(HIP)),
+                    reason="reference / drop-in binaries not built here")
+@pytest.mark.parametrize("N,hydro", [(48, "cholesky"), (30, "none")])
+def test_dropin_trajectory_matches_reference(N, hydro):
+    sysm = wl.chain(N, seed=N)
+    prm = wl.chain_params(hydro=hydro, rand_seed=321, bdsteps=9, save_xyz_freq=1)
+    with tempfile.TemporaryDirectory() as t1, tempfile.TemporaryDirectory() as t2:
+        a = run(REF, sysm, prm, t1)
+        b = run(HIP, sysm, prm, t2, {"BDBOX_HIP_PRECISION": "fp64"})
+    assert a.shape == b.shape == (9, N, 3)
+    # 6-decimal text output: allow one unit in the last printed place
+    assert np.max(np.abs(a - b)) <= 1.5e-6, np.max(np.abs(a - b))
