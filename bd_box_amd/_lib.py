@@ -22,7 +22,7 @@ EXPORTS = [
     "bdh_get_coords", "bdh_set_coords", "bdh_get_forces", "bdh_get_last_displacement",
     "bdh_get_rng_state", "bdh_set_rng_state", "bdh_dpotrf", "bdh_dtrmv", "bdh_gauss",
     "bdh_seed_state", "bdh_probe_mfma", "bdh_time_factor", "bdh_time_steps", "bdh_sync",
-    "bdh_phase_timing", "bdh_phase_times",
+    "bdh_phase_timing", "bdh_phase_times", "bdh_neighbour_pairs",
 ]
 
 
@@ -35,7 +35,7 @@ class Config(C.Structure):
         ("bond_c_scale", C.c_double), ("check_overlap", C.c_int), ("bc", C.c_int),
         ("box", C.c_double), ("hydro", C.c_int), ("ewald_method", C.c_int),
         ("ewald_real", C.c_int), ("precision", C.c_int), ("refactor_every", C.c_int),
-        ("rand_seed", C.c_uint32),
+        ("rand_seed", C.c_uint32), ("nb_list", C.c_int),
     ]
 
 
@@ -63,6 +63,8 @@ def load():
     L.bdh_step.argtypes = [C.c_void_p, C.c_int, _dp]
     L.bdh_compute_D.argtypes = [C.c_void_p, _dp, C.c_int]
     L.bdh_compute_forces.argtypes = [C.c_void_p, _dp, _dp, _ip]
+    L.bdh_neighbour_pairs.argtypes = [C.c_void_p, C.c_double, _ip, C.c_longlong,
+                                      C.POINTER(C.c_longlong)]
     L.bdh_factor.argtypes = [C.c_void_p, _dp, C.c_int]
     L.bdh_noise_product.argtypes = [C.c_void_p, _dp, _dp]
     L.bdh_compute_Rf.argtypes = [C.c_void_p, _dp]

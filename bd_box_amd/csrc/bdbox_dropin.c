@@ -13,8 +13,10 @@
  * Compiled against the reference's headers (it is the binding a BD_BOX maintainer adds);
  * see oracle/Makefile target `dropin` for the link line.
  *
- * Supported: algorithm ermak_const / ermak_var without rejections, hydro none / cholesky,
- * bc none, brute-force pair lists (the backend's own), FP64 or mixed precision
+ * Supported: algorithm ermak_const / ermak_var without rejections, hydro none / cholesky /
+ * geyer, bc none or periodic (cube; Beenakker-Ewald tensor with hydrodynamics), nb_list
+ * brute or either bucket method (both map to the backend's device cell list, which finds
+ * the same interacting pairs), FP64 or mixed precision
  * (environment BDBOX_HIP_PRECISION=fp64|mixed, default fp64), refactor interval
  * (BDBOX_HIP_REFACTOR, default 1 = the reference's semantics).  Anything else stops with
  * UNERR like the reference does for unimplemented methods (diff_comp.c:230).
@@ -39,6 +41,7 @@
 #include "potentials/bond.h"
 #include "potentials/electro.h"
 #include "potentials/LJ.h"
+#include "potentials/bucket.h"
 
 #include "../../include/bdbox_hip.h"
 
@@ -112,16 +115,21 @@ void init_diff_tensor(void)
     int i, rc;
     if (algorithm != DIFF_ALG_ERMAK_CONST && algorithm != DIFF_ALG_ERMAK_VAR)
         UNERR("HIP backend: only algorithm ermak_const / ermak_var");
-    if (hydro != DIFF_NONE && hydro != DIFF_CHOLESKY)
-        UNERR("HIP backend: only hydro none / cholesky");
+    if (hydro != DIFF_NONE && hydro != DIFF_CHOLESKY && hydro != DIFF_GEYER)
+        UNERR("HIP backend: hydro none / cholesky / geyer");
     if (vel_grad_tensor) UNERR("HIP backend: no flow field");
-    if (bc != BC_NONE) UNERR("HIP backend: only bc none");
+    if (bc != BC_NONE && bc != BC_BOX) UNERR("HIP backend: bc none or periodic");
+    if (bc == BC_BOX && (box[0] != box[1] || box[0] != box[2]))
+        UNERR("HIP backend: periodic box must be a cube");
+    if (bc == BC_BOX && hydro != DIFF_NONE && ewald_method != EWALD_METHOD_BEENAKKER)
+        UNERR("HIP backend: periodic hydrodynamics need ewald_method beenakker");
     memset(&cfg, 0, sizeof(cfg));
     cfg.dt = dt; cfg.T = T; cfg.visc = visc; cfg.vfactor = vfactor;
     cfg.alpha_lj = alpha_lj; cfg.cutoff_lj = cutoff_lj; cfg.lj_6_term = lj_6_term;
     cfg.elec = elec; cfg.epsilon_c = epsilon_c; cfg.kappa_c = kappa_c; cfg.gamma_c = gamma_c;
     cfg.cutoff_c = cutoff_c; cfg.bond_lj_scale = bond_lj_scale; cfg.bond_c_scale = bond_c_scale;
-    cfg.check_overlap = check_overlap; cfg.bc = 0; cfg.box = box[0];
+    cfg.check_overlap = check_overlap; cfg.bc = bc == BC_BOX; cfg.box = box[0];
+    cfg.nb_list = nb_list;
     cfg.hydro = hydro; cfg.ewald_method = ewald_method; cfg.ewald_real = ewald_real;
     cfg.precision = (prec && !strcmp(prec, "mixed")) ? BDH_MIXED : BDH_FP64;
     cfg.refactor_every = refac ? atoi(refac) : 1;

@@ -102,6 +102,7 @@ struct bdh_system {
     int N = 0, n = 0, n_pad = 0;
     long long ld = 0;
     bool mixed = false;
+    int syrk_mode = SYRK_F64;
     int device = 0;
     hipStream_t st = nullptr;
     double preDii = 0, preDij = 0;
@@ -115,11 +116,20 @@ struct bdh_system {
     double* bond_parm = nullptr;
     double *F = nullptr, *E = nullptr, *PF = nullptr, *PE = nullptr;
     int fsplits = 0, fblocks = 0;
+    bool cells = false;             // nonbonded pairs over a device cell list
+    double cell_cut = 0;            // cell edge lower bound (largest cutoff + 0.01)
+    CellList cl{};
     void* A = nullptr;
+    double* Lbuf = nullptr;         // FP64 diagonal blocks of the factorization
     double* xi = nullptr;
-    size_t xi_cap_steps = 0;
+    size_t xi_cap = 0;              // doubles in xi
     double *Pf = nullptr, *Pr = nullptr, *Rf = nullptr, *Rs = nullptr, *noise = nullptr;
+    double* RsB = nullptr;          // sqrt(2 dt) L xi for the steps of one refactor cycle
+    const double* last_Rs = nullptr;  // Rs of the last step taken
+    long long ldx = 0;              // stride of drawn noise vectors: n, or n + 1 if n is odd
     int msplits = 0, tblocks = 0;
+    bool ewald = false, tea = false;
+    double *Px = nullptr, *Pc = nullptr, *Pe = nullptr, *Feff = nullptr, *Cs = nullptr, *bt = nullptr;
     uint64_t* rng = nullptr;
     long long* rng_off = nullptr;
     int* rng_cnt = nullptr;
@@ -169,10 +179,33 @@ int check_flags(bdh_system* s)
 #define REC_BEGIN(ph) do { if (s->rec) s->rec->begin(ph, s->st); } while (0)
 #define REC_END(ph) do { if (s->rec) s->rec->end(ph, s->st); } while (0)
 
+int alloc_cells(bdh_system* s)
+{
+    if (s->cl.grid) return 0;
+    if (dmalloc(&s->cl.grid, cell_grid_bytes())) return -1;
+    if (dmalloc(&s->cl.key, sizeof(int) * s->N)) return -1;
+    if (dmalloc(&s->cl.count, sizeof(int) * CELL_MAX)) return -1;
+    if (dmalloc(&s->cl.fill, sizeof(int) * CELL_MAX)) return -1;
+    if (dmalloc(&s->cl.start, sizeof(int) * (CELL_MAX + 1))) return -1;
+    if (dmalloc(&s->cl.sorted, sizeof(int) * s->N)) return -1;
+    return 0;
+}
+
 int compute_forces_dev(bdh_system* s, bool energies = false)
 {
     REC_BEGIN(PH_FORCES);
     int rc = 0;
+    if (s->cells) {
+        if (launch_cell_build(s->pos, s->N, s->cell_cut, s->fp.pbc, s->fp.box, s->cl, s->st))
+            return -1;
+        rc = s->mixed ? launch_forces_cells<float>(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, s->cl,
+                                                   s->F, s->PE, s->flags, s->st)
+                      : launch_forces_cells<double>(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, s->cl,
+                                                    s->F, s->PE, s->flags, s->st);
+        if (!rc && energies) rc = launch_energy_reduce(s->PE, s->fblocks, s->E, s->st);
+        REC_END(PH_FORCES);
+        return rc;
+    }
     if (s->mixed) {
         if (launch_forces<float>(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, s->PF, s->PE,
                                  s->flags, s->fsplits, s->st)) return -1;
@@ -188,58 +221,110 @@ int compute_forces_dev(bdh_system* s, bool energies = false)
     return rc;
 }
 
+// The Ewald tables live in constant memory of the module: reload them when the system
+// asks for a different box or lattice radius than the last one loaded.
+double g_ew_box = -1.0;
+int g_ew_R = -1;
+int g_ew_dev = -1;
+
+int ensure_ewald(bdh_system* s)
+{
+    if (!s->ewald) return 0;
+    if (g_ew_box == s->cfg.box && g_ew_R == s->cfg.ewald_real && g_ew_dev == s->device) return 0;
+    if (ewald_upload(s->cfg.box, s->cfg.ewald_real) < 0) return -1;
+    g_ew_box = s->cfg.box;
+    g_ew_R = s->cfg.ewald_real;
+    g_ew_dev = s->device;
+    return 0;
+}
+
 int assemble_and_factor(bdh_system* s)
 {
+    if (ensure_ewald(s)) return -1;
     if (s->mixed) {
         float* A = (float*)s->A;
         REC_BEGIN(PH_ASSEMBLE);
-        if (launch_rpy_assemble<float>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)) return -1;
+        if (s->ewald ? launch_ewald_assemble<float>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)
+                     : launch_rpy_assemble<float>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st))
+            return -1;
         REC_END(PH_ASSEMBLE);
-        if (chol_factor<float>(A, s->ld, s->n_pad, s->flags + 1, SYRK_F32, s->st, s->rec)) return -1;
+        if (chol_factor<float>(A, s->ld, s->n_pad, s->Lbuf, s->flags + 1, s->syrk_mode, s->st, s->rec)) return -1;
     } else {
         double* A = (double*)s->A;
         REC_BEGIN(PH_ASSEMBLE);
-        if (launch_rpy_assemble<double>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)) return -1;
+        if (s->ewald ? launch_ewald_assemble<double>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)
+                     : launch_rpy_assemble<double>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st))
+            return -1;
         REC_END(PH_ASSEMBLE);
-        if (chol_factor<double>(A, s->ld, s->n_pad, s->flags + 1, SYRK_F64, s->st, s->rec)) return -1;
+        if (chol_factor<double>(A, s->ld, s->n_pad, s->Lbuf, s->flags + 1, SYRK_F64, s->st, s->rec)) return -1;
     }
     s->have_L = true;
     return 0;
 }
 
-int matvec_dev(bdh_system* s)
+// Pair sums over blocks evaluated on the fly: D.F, and for TEA also D.X, C and eps.
+int matvec_dev(bdh_system* s, const double* xi = nullptr, bool tea = false)
 {
+    if (ensure_ewald(s)) return -1;
     if (s->mixed)
-        return launch_rpy_matvec<float>(s->pos, s->F, s->N, s->preDii, s->preDij, s->Pf, s->msplits, s->st);
-    return launch_rpy_matvec<double>(s->pos, s->F, s->N, s->preDii, s->preDij, s->Pf, s->msplits, s->st);
+        return launch_pair_sums<float>(s->pos, s->F, xi, s->N, s->preDii, s->preDij, s->cfg.dt, s->ewald,
+                                       tea, s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st);
+    return launch_pair_sums<double>(s->pos, s->F, xi, s->N, s->preDii, s->preDij, s->cfg.dt, s->ewald,
+                                    tea, s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st);
 }
 
-int trmv_dev(bdh_system* s, const double* x)
+int trmv_dev(bdh_system* s, const double* X, long long ldx, int m)
 {
     if (s->mixed)
-        return launch_trmv_partials<float>((const float*)s->A, s->ld, s->n, x, s->Pr, s->st);
-    return launch_trmv_partials<double>((const double*)s->A, s->ld, s->n, x, s->Pr, s->st);
+        return launch_trmv_partials<float>((const float*)s->A, s->ld, s->n, X, ldx, m, s->Pr, s->st);
+    return launch_trmv_partials<double>((const double*)s->A, s->ld, s->n, X, ldx, m, s->Pr, s->st);
 }
 
-int one_step(bdh_system* s, const double* xi_dev)
+// Rs of m consecutive steps, sqrt(2 dt) L [xi_1 .. xi_m], in one pass over L (the factor
+// is fixed within a refactor cycle); stacked into RsB[j n + r].
+int cycle_noise(bdh_system* s, const double* X, long long ldx, int m)
+{
+    REC_BEGIN(PH_TRMV);
+    if (trmv_dev(s, X, ldx, m)) return -1;
+    ReduceJobs jobs{};
+    jobs.job[0] = ReduceJob{s->Pr, s->RsB, m * s->n, s->tblocks, trmv_blk_cols(),
+                            std::sqrt(2.0 * s->cfg.dt), s->n};
+    if (launch_reduce_partials(jobs, 1, m * s->n, s->st)) return -1;
+    REC_END(PH_TRMV);
+    return 0;
+}
+
+// One E-M step.  Cholesky: Rs_dev = this step's precomputed sqrt(2 dt) L xi.
+int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev = nullptr)
 {
     const bdh_config& c = s->cfg;
     const double dt = c.dt;
     const double rf_scale = dt / (kB * c.T);
     if (c.hydro == BDH_HYDRO_CHOLESKY) {
-        if (!s->have_L || (s->steps_done % c.refactor_every) == 0)
-            if (assemble_and_factor(s)) return -1;
         REC_BEGIN(PH_MATVEC);
         if (matvec_dev(s)) return -1;
         REC_END(PH_MATVEC);
-        REC_BEGIN(PH_TRMV);
-        if (trmv_dev(s, xi_dev)) return -1;
-        REC_END(PH_TRMV);
         REC_BEGIN(PH_MOVE);
         ReduceJobs jobs{};
         jobs.job[0] = ReduceJob{s->Pf, s->Rf, s->n, s->msplits, 0, rf_scale};
-        jobs.job[1] = ReduceJob{s->Pr, s->Rs, s->n, s->tblocks, trmv_blk_cols(), std::sqrt(2.0 * dt)};
-        if (launch_reduce_partials(jobs, 2, s->n, s->st)) return -1;
+        if (launch_reduce_partials(jobs, 1, s->n, s->st)) return -1;
+        if (launch_em_update(s->pos, s->N, s->Rf, Rs_dev, c.bc == 1, c.box, s->st)) return -1;
+        s->last_Rs = Rs_dev;
+        REC_END(PH_MOVE);
+    } else if (c.hydro == BDH_HYDRO_GEYER) {
+        // TEA-HI: one pass over the pairs gives D.F, D.X, C and eps (diff_rand.c:86-230)
+        REC_BEGIN(PH_MATVEC);
+        if (matvec_dev(s, xi_dev, true)) return -1;
+        REC_END(PH_MATVEC);
+        REC_BEGIN(PH_MOVE);
+        ReduceJobs jobs{};
+        jobs.job[0] = ReduceJob{s->Pf, s->Rf, s->n, s->msplits, 0, rf_scale};
+        jobs.job[1] = ReduceJob{s->Px, s->Feff, s->n, s->msplits, 0, 1.0};
+        jobs.job[2] = ReduceJob{s->Pc, s->Cs, s->n, s->msplits, 0, 1.0};
+        if (launch_reduce_partials(jobs, 3, s->n, s->st)) return -1;
+        if (launch_tea_close(s->pos, xi_dev, s->Feff, s->Cs, s->Pe, s->msplits, s->N, s->preDii, dt,
+                             s->bt, s->Rs, s->st))
+            return -1;
         if (launch_em_update(s->pos, s->N, s->Rf, s->Rs, c.bc == 1, c.box, s->st)) return -1;
         REC_END(PH_MOVE);
     } else {
@@ -255,19 +340,59 @@ int one_step(bdh_system* s, const double* xi_dev)
     return 0;
 }
 
-int draw_noise(bdh_system* s)
+// Gaussians of m consecutive steps in one launch: m calls of gauss_rand for n values equal
+// one call for m * ldx values with ldx = n + (n odd) -- every call starts a fresh pair and
+// drops the last value of its last pair when n is odd -- so step j reads xi + j * ldx.
+int draw_noise(bdh_system* s, int m)
 {
     REC_BEGIN(PH_NOISE);
-    int rc = launch_gauss_streams(s->rng, s->xi, s->rng_off, s->rng_cnt, 1, s->flags + 2, s->st);
+    int rc = launch_gauss_streams(s->rng, s->xi, s->rng_off, s->rng_cnt + m, 1, s->flags + 2, s->st);
     REC_END(PH_NOISE);
     return rc;
+}
+
+// steps E-M steps; have_xi: the caller's Gaussians are in s->xi ([step][n]).  Steps are
+// taken in chunks of up to TRMV_MAX_RHS that never cross a refactorization.
+int run_steps(bdh_system* s, int steps, bool have_xi)
+{
+    const int n = s->n;
+    const int R = s->cfg.refactor_every;
+    const bool chol = s->cfg.hydro == BDH_HYDRO_CHOLESKY;
+    for (int k = 0; k < steps;) {
+        int m = std::min(steps - k, TRMV_MAX_RHS);
+        bool refactor = false;
+        if (chol) {
+            const int ph = (int)(s->steps_done % R);
+            refactor = !s->have_L || ph == 0;
+            m = std::min(m, refactor ? R : R - ph);
+        }
+        const double* X;
+        long long ldx;
+        if (have_xi) {
+            X = s->xi + (size_t)k * n;
+            ldx = n;
+        } else {
+            if (draw_noise(s, m)) return -1;
+            X = s->xi;
+            ldx = s->ldx;
+        }
+        if (chol) {
+            // D and L at the positions of the cycle's first step (diff_comp.c:156)
+            if (refactor && assemble_and_factor(s)) return -1;
+            if (cycle_noise(s, X, ldx, m)) return -1;
+        }
+        for (int j = 0; j < m; ++j)
+            if (one_step(s, X + (size_t)j * ldx, chol ? s->RsB + (size_t)j * n : nullptr)) return -1;
+        k += m;
+    }
+    return 0;
 }
 
 }  // namespace
 
 namespace {
 template <typename T>
-int dpotrf_impl(char uplo, double* A, int n, int ld, int* info)
+int dpotrf_impl(char uplo, double* A, int n, int ld, int* info, int syrk_mode)
 {
     const int n_pad = ceil_div(n, PAD_TO) * PAD_TO;
     const long long lda = n_pad;
@@ -275,10 +400,11 @@ int dpotrf_impl(char uplo, double* A, int n, int ld, int* info)
     BDH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     T* d_A = nullptr;
     double* d_full = nullptr;
+    double* d_Lbuf = nullptr;
     int* d_status = nullptr;
     int rc = 0;
     if (dmalloc(&d_A, sizeof(T) * lda * n_pad) || dmalloc(&d_full, sizeof(double) * (size_t)n * n) ||
-        dmalloc(&d_status, sizeof(int))) {
+        dmalloc(&d_Lbuf, sizeof(double) * chol_lbuf_doubles(n_pad)) || dmalloc(&d_status, sizeof(int))) {
         rc = -1;
     } else {
         const bool up = (uplo == 'U' || uplo == 'u');
@@ -290,7 +416,7 @@ int dpotrf_impl(char uplo, double* A, int n, int ld, int* info)
         hipLaunchKernelGGL(full_to_lower_kernel<T>, grid, dim3(256), 0, st, d_full, (long long)n, n,
                            d_A, lda, up ? 1 : 0);
         rc = launch_set_identity_pad<T>(d_A, lda, n, n_pad, st);
-        if (!rc) rc = chol_factor<T>(d_A, lda, n_pad, d_status, sizeof(T) == 4 ? SYRK_F32 : SYRK_F64, st);
+        if (!rc) rc = chol_factor<T>(d_A, lda, n_pad, d_Lbuf, d_status, syrk_mode, st);
         if (!rc) {
             hipLaunchKernelGGL(lower_to_full_kernel<T>, grid, dim3(256), 0, st, (const T*)d_A, lda, n,
                                d_full, (long long)n, 0);
@@ -308,7 +434,7 @@ int dpotrf_impl(char uplo, double* A, int n, int ld, int* info)
             }
         }
     }
-    dfree(d_A); dfree(d_full); dfree(d_status);
+    dfree(d_A); dfree(d_full); dfree(d_Lbuf); dfree(d_status);
     (void)hipStreamDestroy(st);
     return rc;
 }
@@ -338,8 +464,11 @@ void bdh_destroy(bdh_system* s)
     dfree(s->pos); dfree(s->Q); dfree(s->LJ);
     dfree(s->conn_off); dfree(s->conn); dfree(s->bond_off); dfree(s->bond_p); dfree(s->bond_parm);
     dfree(s->F); dfree(s->E); dfree(s->PF); dfree(s->PE);
-    dfree(s->A); dfree(s->xi); dfree(s->Pf); dfree(s->Pr); dfree(s->Rf); dfree(s->Rs);
-    dfree(s->noise); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
+    dfree(s->A); dfree(s->Lbuf); dfree(s->xi); dfree(s->Pf); dfree(s->Pr); dfree(s->Rf); dfree(s->Rs);
+    dfree(s->noise); dfree(s->RsB); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
+    dfree(s->Px); dfree(s->Pc); dfree(s->Pe); dfree(s->Feff); dfree(s->Cs); dfree(s->bt);
+    dfree(s->cl.grid); dfree(s->cl.key); dfree(s->cl.count); dfree(s->cl.fill);
+    dfree(s->cl.start); dfree(s->cl.sorted);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
@@ -350,11 +479,17 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
 {
     BDH_CHECK(cfg && coord && Q && LJ, "bdh_create: null argument");
     BDH_CHECK(N > 0, "bdh_create: no beads");
-    BDH_CHECK(cfg->hydro == BDH_HYDRO_NONE || cfg->hydro == BDH_HYDRO_CHOLESKY,
-              "bdh_create: hydro must be none or cholesky in this build");
-    BDH_CHECK(!(cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE),
-              "bdh_create: Ewald-summed tensors are not available in this build");
-    BDH_CHECK(cfg->precision == BDH_FP64 || cfg->precision == BDH_MIXED, "bdh_create: precision");
+    BDH_CHECK(cfg->hydro == BDH_HYDRO_NONE || cfg->hydro == BDH_HYDRO_CHOLESKY ||
+              cfg->hydro == BDH_HYDRO_GEYER, "bdh_create: hydro must be none, cholesky or geyer");
+    BDH_CHECK(cfg->bc == 0 || cfg->bc == 1, "bdh_create: bc must be none or periodic");
+    // periodic boundaries with hydrodynamics mean an Ewald-summed tensor (input.c:390)
+    BDH_CHECK(!(cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE && cfg->ewald_method != 1),
+              "bdh_create: only Beenakker Ewald summation is available (ewald_method beenakker)");
+    BDH_CHECK(!(cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE) ||
+              (cfg->ewald_real >= 0 && cfg->ewald_real <= 5), "bdh_create: ewald_real 0..5");
+    BDH_CHECK(cfg->bc == 0 || cfg->box > 0, "bdh_create: box edge");
+    BDH_CHECK(cfg->precision == BDH_FP64 || cfg->precision == BDH_MIXED || cfg->precision == BDH_MIXED_SPLIT,
+              "bdh_create: precision");
     BDH_CHECK(cfg->dt > 0 && cfg->T > 0 && cfg->visc != 0 && cfg->vfactor != 0, "bdh_create: dt/T/visc");
     BDH_CHECK(nbonds == 0 || (bonds && bond_parms), "bdh_create: bonds");
     s->cfg = *cfg;
@@ -365,7 +500,8 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     s->n = 3 * N;
     s->n_pad = ceil_div(s->n, PAD_TO) * PAD_TO;
     s->ld = s->n_pad;
-    s->mixed = cfg->precision == BDH_MIXED;
+    s->mixed = cfg->precision != BDH_FP64;
+    s->syrk_mode = cfg->precision == BDH_MIXED_SPLIT ? SYRK_BF16X3 : (s->mixed ? SYRK_F32 : SYRK_F64);
     s->preDii = kB_1_6PI * cfg->T / (cfg->visc * cfg->vfactor);   // diff_decl.c:93
     s->preDij = kB_1_8PI * cfg->T / (cfg->visc * cfg->vfactor);   // diff_decl.c:100
     for (int i = 0; i < N; ++i)
@@ -431,21 +567,46 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     BDH_HIP(hipMemcpy(s->LJ, LJ, sizeof(double) * 2 * N, hipMemcpyHostToDevice));
 
     // work buffers
+    s->cells = cfg->nb_list != 0;
     s->fsplits = force_splits(N);
-    s->fblocks = ceil_div(N, 64) * s->fsplits;
-    s->msplits = matvec_splits(N);
+    s->fblocks = s->cells ? cell_force_blocks(N) : ceil_div(N, 64) * s->fsplits;
+    if (s->cells) {
+        // cell edge of init_cutoff (pre_func.c:197-231): every potential's cutoff and
+        // twice the largest radius (overlap test), plus 0.01
+        double rmax = 0;
+        for (int i = 0; i < N; ++i) rmax = std::max(rmax, LJ[2 * i]);
+        double cut = 2.0 * rmax;
+        if (cfg->elec) cut = std::max(cut, cfg->cutoff_c);
+        if (cfg->alpha_lj != 0.0)
+            cut = std::max(cut, cfg->cutoff_lj > 0 ? cfg->cutoff_lj : 1.1225 * 2.0 * rmax);
+        s->cell_cut = cut + 0.01;
+        if (alloc_cells(s)) return -1;
+    }
+    s->ewald = cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE;
+    s->tea = cfg->hydro == BDH_HYDRO_GEYER;
+    s->msplits = pair_splits(N, s->ewald);
     s->tblocks = trmv_col_blocks(s->n);
     if (dmalloc(&s->F, sizeof(double) * s->n)) return -1;
     if (dmalloc(&s->E, sizeof(double) * 4)) return -1;
     if (dmalloc(&s->PF, sizeof(double) * s->n * s->fsplits)) return -1;
     if (dmalloc(&s->PE, sizeof(double) * 3 * s->fblocks)) return -1;
     if (dmalloc(&s->Pf, sizeof(double) * s->n * std::max(1, s->msplits))) return -1;
-    if (dmalloc(&s->Pr, sizeof(double) * s->n * s->tblocks)) return -1;
+    if (s->tea) {
+        if (dmalloc(&s->Px, sizeof(double) * s->n * s->msplits)) return -1;
+        if (dmalloc(&s->Pc, sizeof(double) * s->n * s->msplits)) return -1;
+        if (dmalloc(&s->Pe, sizeof(double) * s->N * s->msplits)) return -1;
+        if (dmalloc(&s->Feff, sizeof(double) * s->n)) return -1;
+        if (dmalloc(&s->Cs, sizeof(double) * s->n)) return -1;
+        if (dmalloc(&s->bt, sizeof(double) * 2)) return -1;
+    }
+    if (dmalloc(&s->Pr, sizeof(double) * s->n * s->tblocks * TRMV_MAX_RHS)) return -1;
+    if (dmalloc(&s->RsB, sizeof(double) * s->n * TRMV_MAX_RHS)) return -1;
     if (dmalloc(&s->Rf, sizeof(double) * s->n)) return -1;
     if (dmalloc(&s->Rs, sizeof(double) * s->n)) return -1;
     if (dmalloc(&s->noise, sizeof(double) * s->n)) return -1;
-    s->xi_cap_steps = 1;
-    if (dmalloc(&s->xi, sizeof(double) * s->n)) return -1;
+    s->ldx = s->n + (s->n & 1);
+    s->xi_cap = (size_t)s->ldx * TRMV_MAX_RHS;
+    if (dmalloc(&s->xi, sizeof(double) * s->xi_cap)) return -1;
     if (dmalloc(&s->flags, sizeof(int) * 4)) return -1;
     // every clear goes on the system's own (non-blocking) stream, ordered before the
     // kernels that follow; a null-stream hipMemset would not be
@@ -454,19 +615,22 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     BDH_HIP(hipMemsetAsync(s->Rs, 0, sizeof(double) * s->n, s->st));
     if (dmalloc(&s->rng, sizeof(uint64_t))) return -1;
     if (dmalloc(&s->rng_off, sizeof(long long))) return -1;
-    if (dmalloc(&s->rng_cnt, sizeof(int))) return -1;
+    if (dmalloc(&s->rng_cnt, sizeof(int) * (TRMV_MAX_RHS + 1))) return -1;
     {
         uint64_t st0 = lcg_seed(cfg->rand_seed);
         long long o = 0;
-        int cnt = s->n;
+        int cnt[TRMV_MAX_RHS + 1];
+        for (int m = 0; m <= TRMV_MAX_RHS; ++m) cnt[m] = (int)(m * s->ldx);
+        cnt[1] = s->n;   // a single draw keeps the reference's own length
         BDH_HIP(hipMemcpy(s->rng, &st0, sizeof(st0), hipMemcpyHostToDevice));
         BDH_HIP(hipMemcpy(s->rng_off, &o, sizeof(o), hipMemcpyHostToDevice));
-        BDH_HIP(hipMemcpy(s->rng_cnt, &cnt, sizeof(cnt), hipMemcpyHostToDevice));
+        BDH_HIP(hipMemcpy(s->rng_cnt, cnt, sizeof(cnt), hipMemcpyHostToDevice));
     }
     if (cfg->hydro == BDH_HYDRO_CHOLESKY) {
         const size_t es = s->mixed ? sizeof(float) : sizeof(double);
         const size_t bytes = es * (size_t)s->ld * (size_t)s->n_pad;
         if (dmalloc(&s->A, bytes)) return -1;
+        if (dmalloc(&s->Lbuf, sizeof(double) * chol_lbuf_doubles(s->n_pad))) return -1;
         BDH_HIP(hipMemsetAsync(s->A, 0, bytes, s->st));
         if (s->mixed) {
             if (launch_set_identity_pad<float>((float*)s->A, s->ld, s->n, s->n_pad, s->st)) return -1;
@@ -503,20 +667,16 @@ int bdh_step(bdh_system* s, int steps, const double* xi)
     BDH_CHECK(s && steps >= 0, "bdh_step: bad arguments");
     BDH_HIP(hipSetDevice(s->device));
     if (xi) {
-        if ((size_t)steps > s->xi_cap_steps) {
+        if ((size_t)steps * s->n > s->xi_cap) {
+            BDH_HIP(hipStreamSynchronize(s->st));
             dfree(s->xi);
-            if (dmalloc(&s->xi, sizeof(double) * s->n * (size_t)steps)) return -1;
-            s->xi_cap_steps = steps;
+            s->xi_cap = (size_t)steps * s->n;
+            if (dmalloc(&s->xi, sizeof(double) * s->xi_cap)) return -1;
         }
         BDH_HIP(hipMemcpyAsync(s->xi, xi, sizeof(double) * s->n * (size_t)steps,
                                hipMemcpyHostToDevice, s->st));
     }
-    for (int k = 0; k < steps; ++k) {
-        const double* x = s->xi;
-        if (xi) x = s->xi + (size_t)k * s->n;
-        else if (draw_noise(s)) return -1;
-        if (one_step(s, x)) return -1;
-    }
+    if (run_steps(s, steps, xi != nullptr)) return -1;
     BDH_HIP(hipStreamSynchronize(s->st));
     return check_flags(s);
 }
@@ -532,13 +692,16 @@ int bdh_compute_D(bdh_system* s, double* D, int ld)
     BDH_HIP(hipMallocAsync((void**)&full, sizeof(double) * (size_t)s->n * s->n, s->st));
     BDH_HIP(hipMemsetAsync(tmp, 0, es * (size_t)s->ld * s->n_pad, s->st));
     dim3 grid(ceil_div(s->n, 256), s->n);
-    int rc = 0;
-    if (s->mixed) {
-        rc = launch_rpy_assemble<float>(s->pos, s->N, s->preDii, s->preDij, (float*)tmp, s->ld, s->st);
+    int rc = ensure_ewald(s);
+    if (rc) {
+    } else if (s->mixed) {
+        rc = s->ewald ? launch_ewald_assemble<float>(s->pos, s->N, s->preDii, s->preDij, (float*)tmp, s->ld, s->st)
+                      : launch_rpy_assemble<float>(s->pos, s->N, s->preDii, s->preDij, (float*)tmp, s->ld, s->st);
         if (!rc) hipLaunchKernelGGL(lower_to_full_kernel<float>, grid, dim3(256), 0, s->st,
                                     (const float*)tmp, s->ld, s->n, full, (long long)s->n, 1);
     } else {
-        rc = launch_rpy_assemble<double>(s->pos, s->N, s->preDii, s->preDij, (double*)tmp, s->ld, s->st);
+        rc = s->ewald ? launch_ewald_assemble<double>(s->pos, s->N, s->preDii, s->preDij, (double*)tmp, s->ld, s->st)
+                      : launch_rpy_assemble<double>(s->pos, s->N, s->preDii, s->preDij, (double*)tmp, s->ld, s->st);
         if (!rc) hipLaunchKernelGGL(lower_to_full_kernel<double>, grid, dim3(256), 0, s->st,
                                     (const double*)tmp, s->ld, s->n, full, (long long)s->n, 1);
     }
@@ -566,6 +729,41 @@ int bdh_compute_forces(bdh_system* s, double* F, double* E, int* repeat)
     if (repeat) *repeat = (fl & 1) ? 1 : 0;
     BDH_CHECK(!(fl & 2), "Overstretched bond");
     return 0;
+}
+
+int bdh_neighbour_pairs(bdh_system* s, double cutoff, int* pairs, long long cap, long long* count)
+{
+    BDH_CHECK(s && count && cutoff > 0.0 && cap >= 0 && (pairs || cap == 0),
+              "bdh_neighbour_pairs: bad arguments");
+    BDH_HIP(hipSetDevice(s->device));
+    if (alloc_cells(s)) return -1;
+    int* d_pairs = nullptr;
+    unsigned long long* d_count = nullptr;
+    const long long pcap = std::max(1LL, cap);
+    BDH_HIP(hipMallocAsync((void**)&d_pairs, sizeof(int) * 2 * pcap, s->st));
+    BDH_HIP(hipMallocAsync((void**)&d_count, sizeof(unsigned long long), s->st));
+    int rc = launch_cell_build(s->pos, s->N, cutoff, s->fp.pbc, s->fp.box, s->cl, s->st);
+    if (!rc) rc = launch_cell_pairs(s->pos, s->N, s->cl, cutoff, s->fp.pbc, s->fp.box, d_pairs, cap,
+                                    d_count, s->st);
+    unsigned long long got = 0;
+    if (!rc) {
+        BDH_HIP(hipMemcpyAsync(&got, d_count, sizeof(got), hipMemcpyDeviceToHost, s->st));
+        BDH_HIP(hipStreamSynchronize(s->st));
+        const long long kept = std::min<long long>((long long)got, cap);
+        if (kept > 0) {
+            BDH_HIP(hipMemcpyAsync(pairs, d_pairs, sizeof(int) * 2 * kept, hipMemcpyDeviceToHost, s->st));
+            BDH_HIP(hipStreamSynchronize(s->st));
+            std::vector<std::pair<int, int>> v(kept);
+            for (long long k = 0; k < kept; ++k) v[k] = {pairs[2 * k], pairs[2 * k + 1]};
+            std::sort(v.begin(), v.end());
+            for (long long k = 0; k < kept; ++k) { pairs[2 * k] = v[k].first; pairs[2 * k + 1] = v[k].second; }
+        }
+    }
+    (void)hipFreeAsync(d_pairs, s->st);
+    (void)hipFreeAsync(d_count, s->st);
+    BDH_HIP(hipStreamSynchronize(s->st));
+    *count = (long long)got;
+    return rc;
 }
 
 int bdh_factor(bdh_system* s, double* L, int ld)
@@ -598,7 +796,7 @@ int bdh_noise_product(bdh_system* s, const double* x, double* y)
     BDH_CHECK(s && s->A && s->have_L && x && y, "bdh_noise_product: no factor");
     BDH_HIP(hipSetDevice(s->device));
     BDH_HIP(hipMemcpyAsync(s->noise, x, sizeof(double) * s->n, hipMemcpyHostToDevice, s->st));
-    if (trmv_dev(s, s->noise)) return -1;
+    if (trmv_dev(s, s->noise, s->n, 1)) return -1;
     ReduceJobs jobs{};
     jobs.job[0] = ReduceJob{s->Pr, s->Rs, s->n, s->tblocks, trmv_blk_cols(), 1.0};
     if (launch_reduce_partials(jobs, 1, s->n, s->st)) return -1;
@@ -612,7 +810,7 @@ int bdh_compute_Rf(bdh_system* s, double* Rf)
     BDH_CHECK(s && Rf, "bdh_compute_Rf: bad arguments");
     BDH_HIP(hipSetDevice(s->device));
     const double rf_scale = s->cfg.dt / (kB * s->cfg.T);
-    if (s->cfg.hydro == BDH_HYDRO_CHOLESKY) {
+    if (s->cfg.hydro != BDH_HYDRO_NONE) {
         if (matvec_dev(s)) return -1;
         ReduceJobs jobs{};
         jobs.job[0] = ReduceJob{s->Pf, s->Rf, s->n, s->msplits, 0, rf_scale};
@@ -657,7 +855,8 @@ int bdh_get_last_displacement(bdh_system* s, double* Rf, double* Rs)
 {
     BDH_CHECK(s, "bdh_get_last_displacement: null system");
     if (Rf) BDH_HIP(hipMemcpyAsync(Rf, s->Rf, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
-    if (Rs) BDH_HIP(hipMemcpyAsync(Rs, s->Rs, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
+    const double* rs = s->last_Rs ? s->last_Rs : s->Rs;
+    if (Rs) BDH_HIP(hipMemcpyAsync(Rs, rs, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
     BDH_HIP(hipStreamSynchronize(s->st));
     return 0;
 }
@@ -721,8 +920,9 @@ int bdh_dpotrf(char uplo, double* A, int n, int ld, int precision)
     BDH_CHECK(A && n > 0 && ld >= n, "bdh_dpotrf: bad arguments");
     BDH_CHECK(uplo == 'U' || uplo == 'u' || uplo == 'L' || uplo == 'l', "bdh_dpotrf: uplo");
     int info = 0;
-    int rc = precision == BDH_MIXED ? dpotrf_impl<float>(uplo, A, n, ld, &info)
-                                    : dpotrf_impl<double>(uplo, A, n, ld, &info);
+    int rc = precision == BDH_FP64 ? dpotrf_impl<double>(uplo, A, n, ld, &info, SYRK_F64)
+                                   : dpotrf_impl<float>(uplo, A, n, ld, &info,
+                                                        precision == BDH_MIXED_SPLIT ? SYRK_BF16X3 : SYRK_F32);
     if (rc) return rc;
     return info;
 }
@@ -736,7 +936,7 @@ int bdh_dtrmv(char uplo, const double* A, const double* v, double* o, int n, int
     BDH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     double *d_full = nullptr, *d_x = nullptr, *d_P = nullptr, *d_y = nullptr;
     void* d_A = nullptr;
-    const size_t es = precision == BDH_MIXED ? 4 : 8;
+    const size_t es = precision != BDH_FP64 ? 4 : 8;
     const int nblk = trmv_col_blocks(n);
     int rc = 0;
     if (dmalloc(&d_full, sizeof(double) * (size_t)n * n) || dmalloc(&d_A, es * (size_t)n_pad * n_pad) ||
@@ -752,11 +952,11 @@ int bdh_dtrmv(char uplo, const double* A, const double* v, double* o, int n, int
         if (es == 4) {
             hipLaunchKernelGGL(full_to_lower_kernel<float>, grid, dim3(256), 0, st, d_full, (long long)n, n,
                                (float*)d_A, (long long)n_pad, up ? 1 : 0);
-            rc = launch_trmv_partials<float>((const float*)d_A, n_pad, n, d_x, d_P, st);
+            rc = launch_trmv_partials<float>((const float*)d_A, n_pad, n, d_x, n, 1, d_P, st);
         } else {
             hipLaunchKernelGGL(full_to_lower_kernel<double>, grid, dim3(256), 0, st, d_full, (long long)n, n,
                                (double*)d_A, (long long)n_pad, up ? 1 : 0);
-            rc = launch_trmv_partials<double>((const double*)d_A, n_pad, n, d_x, d_P, st);
+            rc = launch_trmv_partials<double>((const double*)d_A, n_pad, n, d_x, n, 1, d_P, st);
         }
         if (!rc) {
             ReduceJobs jobs{};
@@ -823,10 +1023,7 @@ int bdh_time_steps(bdh_system* s, int steps, double* ms)
     BDH_HIP(hipEventCreate(&e0));
     BDH_HIP(hipEventCreate(&e1));
     BDH_HIP(hipEventRecord(e0, s->st));
-    for (int k = 0; k < steps; ++k) {
-        if (draw_noise(s)) return -1;
-        if (one_step(s, s->xi)) return -1;
-    }
+    if (run_steps(s, steps, false)) return -1;
     BDH_HIP(hipEventRecord(e1, s->st));
     BDH_HIP(hipEventSynchronize(e1));
     float t = 0;

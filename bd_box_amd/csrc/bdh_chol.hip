@@ -5,24 +5,32 @@
 // build).  The factor is stored lower, column-major, in place of D; n is padded to a
 // multiple of PAD_TO with an identity block so every tile is whole.
 //
-// Per panel of CHOL_NB = 64 columns:
-//   panel kernel   one workgroup per 256 rows below the diagonal block (plus one that stores
-//                  L_kk): each factors the 64x64 diagonal block in FP64 in registers of one
-//                  wavefront (redundantly, 64^3/3 flops), then every thread solves one row
-//                  x L_kk^T = b in FP64 and stores it in T.  No separate POTRF launch, no
-//                  inverse.
-//   trailing SYRK  A22 -= L21 L21^T over 128x128 lower tiles on MFMA:
-//                  T = float : v_mfma_f32_32x32x2_f32, FP32 accumulation
-//                  T = double: v_mfma_f64_16x16x4_f64
-// FP64 is used for the diagonal-block factorization and the panel solve in both
-// precisions; in the FP32 variant the factor is rounded to FP32 when stored.
+// Per panel of CHOL_NB = 64 columns c0..c0+63:
+//   diagonal block  L_kk of the 64x64 block in FP64 (factor_diag64): 16-column sub-panels
+//                   factored by one wavefront in registers (pivot and column entries
+//                   broadcast with v_readlane), the rest of the block updated by all four
+//                   wavefronts from LDS.  Stored in T into A and in FP64 into Lbuf.
+//   TRSM            L_ik = A_ik L_kk^{-T} for every row below, FP64 forward substitution,
+//                   four lanes per row (a quad exchanges each finished entry through DPP).
+//   trailing SYRK   A22 -= L21 L21^T over 128x128 lower tiles on MFMA:
+//                   T = float : v_mfma_f32_32x32x2_f32, FP32 accumulation
+//                   T = double: v_mfma_f64_16x16x4_f64
+//                   Workgroup 0 owns the tile holding the next panel's diagonal block; it
+//                   keeps that block in LDS and factors it right after its update, so the
+//                   next diagonal factorization overlaps the rest of the SYRK (lookahead
+//                   without a second stream).  Only panel 0 needs its own launch.
+// FP64 is used for the diagonal blocks and the panel solve in both precisions; in the FP32
+// variant the factor is rounded to FP32 when stored.
+#include <algorithm>
+
 #include "bdh_common.hpp"
 #include "bdh_kernels.hpp"
 #include "bdh_mfma.hpp"
 
 namespace bdh {
 
-constexpr int PANEL_ROWS = 256;  // rows solved per workgroup, one per thread
+constexpr int DG_LD = CHOL_NB + 1;   // row pitch (doubles) of the diagonal block in LDS
+constexpr int TRSM_ROWS = 64;        // rows per TRSM workgroup (4 lanes per row)
 
 // Value of v held by lane `src` (uniform), through two v_readlane_b32.
 __device__ __forceinline__ double readlane_f64(double v, int src)
@@ -33,100 +41,209 @@ __device__ __forceinline__ double readlane_f64(double v, int src)
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// Panel kernel.  Workgroup 0 stores L_kk; workgroup b >= 1 solves rows
-// c0 + 64 + 256 (b - 1) + tid.  In every workgroup, wave 0 first factors the 64x64
-// diagonal block with each lane holding one row in registers (FP64): for column j the pivot
-// and the entries of the scaled column come from the lanes holding them (v_readlane: scalar
-// operands of the FMAs), and every lane applies the rank-1 update to its own row -- no LDS
-// traffic and no barriers inside the 64 steps.  After one
-// barrier each thread loads and solves its row
-// x L_kk^T = b column by column (right-looking: x_c is final, then x_k -= x_c L[k][c] for
-// k > c) against L_kk stored transposed in LDS, so every read is a broadcast.
-template <typename T>
-__global__ void __launch_bounds__(256)
-chol_panel_kernel(T* __restrict__ A, long long ld, int c0, int n_pad, int* __restrict__ status)
+// 1/sqrt(x): hardware estimate refined by two Newton steps y <- y (3 - x y^2) / 2.
+__device__ __forceinline__ double rsqrt_f64(double x)
 {
-    __shared__ __align__(16) double sLT[CHOL_NB][CHOL_NB + 2];  // sLT[c][k] = L_kk[k][c]
-    __shared__ double s_inv[CHOL_NB];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int b = blockIdx.x;
-    const int r = c0 + CHOL_NB + (b - 1) * PANEL_ROWS + tid;  // row solved by this thread
-    const bool solves = b > 0 && r < n_pad;
-    if (wave == 0) {
-        const T* Ad = A + (long long)c0 * ld + c0 + lane;
-        double a[CHOL_NB];
-#pragma unroll
-        for (int c = 0; c < CHOL_NB; ++c) a[c] = (c <= lane) ? (double)Ad[(long long)c * ld] : 0.0;
-#pragma unroll
-        for (int j = 0; j < CHOL_NB; ++j) {
-            double d = readlane_f64(a[j], j);
-            if (!(d > 0.0)) {
-                if (lane == 0) atomicCAS(status, 0, c0 + j + 1);
-                d = __builtin_nan("");
-            }
-            const double sd = sqrt(d);
-            const double inv = 1.0 / sd;
-            a[j] = (lane == j) ? sd : a[j] * inv;
-#pragma unroll
-            for (int k = j + 1; k < CHOL_NB; ++k) a[k] -= a[j] * readlane_f64(a[j], k);
-        }
-#pragma unroll
-        for (int c = 0; c < CHOL_NB; ++c) sLT[c][lane] = (c <= lane) ? a[c] : 0.0;
-        __builtin_amdgcn_wave_barrier();
-        // (a[lane] would index the register array at run time and push it to scratch)
-        s_inv[lane] = 1.0 / sLT[lane][lane];
-    }
-    __syncthreads();
-    if (b == 0) {
-        // store the lower part of L_kk
-        T* Aw = A + (long long)c0 * ld + c0;
-        for (int idx = tid; idx < CHOL_NB * CHOL_NB; idx += 256) {
-            const int rr = idx % CHOL_NB, cc = idx / CHOL_NB;
-            if (rr >= cc) Aw[(long long)cc * ld + rr] = (T)sLT[cc][rr];
-        }
-        return;
-    }
-    if (!solves) return;
-    // the row is loaded only now: keeping it live across the factorization would double
-    // the register footprint of wave 0
-    double x[CHOL_NB];
-    const T* Axl = A + (long long)c0 * ld + r;
-#pragma unroll
-    for (int c = 0; c < CHOL_NB; ++c) x[c] = (double)Axl[(long long)c * ld];
-#pragma unroll
-    for (int c = 0; c < CHOL_NB; ++c) {
-        x[c] *= s_inv[c];
-        const double xc = x[c];
-#pragma unroll
-        for (int k = c + 1; k < CHOL_NB; ++k) x[k] -= xc * sLT[c][k];
-    }
-    T* Ax = A + (long long)c0 * ld + r;
-#pragma unroll
-    for (int c = 0; c < CHOL_NB; ++c) Ax[(long long)c * ld] = (T)x[c];
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * (1.5 - 0.5 * x * y * y);
+    y = y * (1.5 - 0.5 * x * y * y);
+    return y;
 }
 
-// Lower-triangular tile index (ti >= tj) from a linear index x.
-__device__ __forceinline__ void tri_tile(int x, int& ti, int& tj)
+// In-place FP64 Cholesky of the 64x64 block sD[r * DG_LD + c] (lower part read; on return
+// the lower part holds L and the strict upper part zeros).  Called by all 256 threads.
+// A non-positive pivot records row c0 + j + 1 in *status (first one wins) and continues
+// with NaN, as the reference's dpotrf reports INFO.
+__device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, int* __restrict__ status)
 {
-    int t = (int)((sqrt(8.0 * (double)x + 1.0) - 1.0) * 0.5);
-    while ((t + 1) * (t + 2) / 2 <= x) ++t;
-    while (t * (t + 1) / 2 > x) --t;
-    ti = t;
-    tj = x - t * (t + 1) / 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll 1
+    for (int J = 0; J < 4; ++J) {
+        const int b0 = 16 * J;
+        if (wave == 0) {
+            double p[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) p[t] = (lane >= b0) ? sD[lane * DG_LD + b0 + t] : 0.0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                double piv = readlane_f64(p[t], b0 + t);
+                if (!(piv > 0.0)) {
+                    if (lane == 0) atomicCAS(status, 0, c0 + b0 + t + 1);
+                    piv = __builtin_nan("");
+                }
+                const double rs = rsqrt_f64(piv);
+                const double l = (lane == b0 + t) ? piv * rs : p[t] * rs;
+                p[t] = l;
+#pragma unroll
+                for (int u = t + 1; u < 16; ++u) p[u] -= l * readlane_f64(l, b0 + u);
+            }
+            if (lane >= b0) {
+#pragma unroll
+                for (int t = 0; t < 16; ++t) sD[lane * DG_LD + b0 + t] = (b0 + t <= lane) ? p[t] : 0.0;
+            }
+        }
+        __syncthreads();
+        const int R = 48 - b0;   // rows (and columns) after this sub-panel
+        if (R > 0) {
+            const int r0 = b0 + 16;
+            for (int e = tid; e < R * R; e += 256) {
+                const int r = r0 + e / R, c = r0 + e % R;
+                if (c > r) continue;
+                const double* Lr = sD + r * DG_LD + b0;
+                const double* Lc = sD + c * DG_LD + b0;
+                double acc = 0.0;
+#pragma unroll
+                for (int t = 0; t < 16; ++t) acc += Lr[t] * Lc[t];
+                sD[r * DG_LD + c] -= acc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// L (lower) of the factored block into A (as T) and, transposed to column-major with zeros
+// above the diagonal, into Lb (FP64, 64x64) for the TRSM.
+template <typename T>
+__device__ __forceinline__ void store_diag64(const double* __restrict__ sD, T* __restrict__ A,
+                                             long long ld, int c0, double* __restrict__ Lb)
+{
+    for (int e = threadIdx.x; e < CHOL_NB * CHOL_NB; e += 256) {
+        const int r = e % CHOL_NB, c = e / CHOL_NB;
+        const double v = (r >= c) ? sD[r * DG_LD + c] : 0.0;
+        if (r >= c) A[(long long)(c0 + c) * ld + c0 + r] = (T)v;
+        Lb[e] = v;
+    }
+}
+
+// Panel 0's diagonal block (later ones are factored inside the SYRK kernel).
+template <typename T>
+__global__ void __launch_bounds__(256)
+chol_diag_kernel(T* __restrict__ A, long long ld, int c0, double* __restrict__ Lb,
+                 int* __restrict__ status)
+{
+    __shared__ double sD[CHOL_NB * DG_LD];
+    for (int e = threadIdx.x; e < CHOL_NB * CHOL_NB; e += 256) {
+        const int r = e % CHOL_NB, c = e / CHOL_NB;
+        sD[r * DG_LD + c] = (r >= c) ? (double)A[(long long)(c0 + c) * ld + c0 + r] : 0.0;
+    }
+    __syncthreads();
+    factor_diag64(sD, c0, status);
+    store_diag64<T>(sD, A, ld, c0, Lb);
+}
+
+// Broadcast within each quad of lanes from quad lane Q (DPP quad_perm).
+template <int Q>
+__device__ __forceinline__ double quad_bcast_t(double v)
+{
+    constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), ctrl, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double quad_bcast(double v, int q)
+{
+    switch (q) {
+        case 0: return quad_bcast_t<0>(v);
+        case 1: return quad_bcast_t<1>(v);
+        case 2: return quad_bcast_t<2>(v);
+        default: return quad_bcast_t<3>(v);
+    }
+}
+
+// TRSM of the rows below panel c0: x L_kk^T = b.  Four lanes per row (a quad; lane q owns
+// columns 4m + q); column c is finished by its owner, sent to the quad through DPP, and
+// the quad subtracts its contribution from the later columns (right-looking), all in FP64.
+template <typename T>
+__global__ void __launch_bounds__(256)
+chol_trsm_kernel(T* __restrict__ A, long long ld, int c0, int n_pad, const double* __restrict__ Lb)
+{
+    __shared__ double sLT[CHOL_NB][CHOL_NB + 1];   // sLT[c][k] = L[k][c]
+    __shared__ double s_inv[CHOL_NB];
+    const int tid = threadIdx.x, q = tid & 3;
+    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256) sLT[e / CHOL_NB][e % CHOL_NB] = Lb[e];
+    __syncthreads();
+    if (tid < CHOL_NB) s_inv[tid] = 1.0 / sLT[tid][tid];
+    __syncthreads();
+    const int r = c0 + CHOL_NB + blockIdx.x * TRSM_ROWS + (tid >> 2);
+    if (r >= n_pad) return;
+    T* Ar = A + (long long)c0 * ld + r;
+    double x[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) x[m] = (double)Ar[(long long)(4 * m + q) * ld];
+#pragma unroll
+    for (int c = 0; c < CHOL_NB; ++c) {
+        const int m0 = c >> 2, q0 = c & 3;
+        const double xc = quad_bcast(x[m0] * s_inv[c], q0);
+        const double upd = x[m0] - xc * sLT[c][4 * m0 + q];
+        x[m0] = (q == q0) ? xc : ((q > q0) ? upd : x[m0]);
+#pragma unroll
+        for (int m = m0 + 1; m < 16; ++m) x[m] -= xc * sLT[c][4 * m + q];
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) Ar[(long long)(4 * m + q) * ld] = (T)x[m];
+}
+
+// Tile (ti, tj), ti >= tj, of a lower trapezoid with ni tile rows, enumerated column by
+// column (tile (0, 0) first).
+__device__ __forceinline__ void trap_tile(int x, int ni, int& ti, int& tj)
+{
+    int j = 0;
+    while (x >= ni - j) {
+        x -= ni - j;
+        ++j;
+    }
+    ti = j + x;
+    tj = j;
 }
 
 // ------------------------------------------------------------------ FP32 SYRK (MFMA)
 // Workgroup 256 = 4 waves in 2x2; tile 128x128; each wave 64x64 = 2x2 blocks of 32x32.
+//   SPLIT = false: v_mfma_f32_32x32x2_f32 on the FP32 operands.
+//   SPLIT = true : each FP32 operand x = hi + lo with hi = bf16(x), lo = bf16(x - hi)
+//                  (x - hi is exact in FP32), and a b = hi_a hi_b + hi_a lo_b + lo_a hi_b
+//                  on v_mfma_f32_32x32x16_bf16 with FP32 accumulation; the dropped lo_a lo_b
+//                  and the split residuals are below 2^-17 |a b|.
 constexpr int SY_TILE = 128;
+__device__ __forceinline__ int ceil_div_dev(int a, int b) { return (a + b - 1) / b; }
 constexpr int SY_KC32 = 32;
+constexpr int SY_KCH = 32;                 // k per stage of the split kernel
+constexpr int SY_HP = SY_KCH / 2 + 4;      // row pitch of a split stage, in 32-bit words
 
-__global__ void __launch_bounds__(256)
-syrk_f32_kernel(float* __restrict__ A, long long ld, int c0, int t0, int n_pad)
+// LDS of the SYRK kernels: the operand stages, reused for the next diagonal block.
+constexpr int SY_SMEM_F32 = 2 * SY_KC32 * (SY_TILE + 4) * 4;
+constexpr int SY_SMEM_SPLIT = 4 * SY_TILE * SY_HP * 4;
+constexpr int SY_SMEM = SY_SMEM_SPLIT > SY_SMEM_F32 ? SY_SMEM_SPLIT : SY_SMEM_F32;
+static_assert(SY_SMEM >= CHOL_NB * DG_LD * 8, "diagonal block does not fit the SYRK LDS");
+
+// bf16 bits of x, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned bf16_bits(float x)
 {
-    __shared__ float sA[SY_KC32][SY_TILE + 4];
-    __shared__ float sB[SY_KC32][SY_TILE + 4];
+    const unsigned u = __builtin_bit_cast(unsigned, x);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ void split_bf16(float x, unsigned& hi, unsigned& lo)
+{
+    hi = bf16_bits(x);
+    lo = bf16_bits(x - __builtin_bit_cast(float, hi << 16));
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(256)
+syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int t1, int n_pad,
+                double* __restrict__ Lb_next, int* __restrict__ status)
+{
+    __shared__ __align__(16) unsigned char smem[SY_SMEM];
+    float (*sA)[SY_TILE + 4] = reinterpret_cast<float (*)[SY_TILE + 4]>(smem);
+    float (*sB)[SY_TILE + 4] = reinterpret_cast<float (*)[SY_TILE + 4]>(smem + SY_KC32 * (SY_TILE + 4) * 4);
+    // split stages: [0] A hi, [1] A lo, [2] B hi, [3] B lo; row-major (row, k pairs)
+    unsigned (*sH)[SY_TILE][SY_HP] = reinterpret_cast<unsigned (*)[SY_TILE][SY_HP]>(smem);
+    double* sD = reinterpret_cast<double*>(smem);
+    // workgroup 0 holds tile (0, 0), whose top-left 64x64 block (wave 0) is the next
+    // panel's diagonal block
+    const bool diag = blockIdx.x == 0;
     int ti, tj;
-    tri_tile(blockIdx.x, ti, tj);
+    trap_tile(blockIdx.x, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
     const int ri0 = t0 + ti * SY_TILE;
     const int rj0 = t0 + tj * SY_TILE;
     const int tid = threadIdx.x;
@@ -142,40 +259,97 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int c0, int t0, int n_pad)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
 
-    for (int k0 = 0; k0 < CHOL_NB; k0 += SY_KC32) {
-        __syncthreads();
-        // 32 columns x 128 rows, float4 per thread-iteration
-        for (int idx = tid; idx < SY_KC32 * SY_TILE / 4; idx += 256) {
-            const int k = idx / (SY_TILE / 4);
-            const int r4 = (idx % (SY_TILE / 4)) * 4;
-            const float* col = A + (long long)(c0 + k0 + k) * ld;
-            float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;
-            if (ri0 + r4 < n_pad) va = *reinterpret_cast<const float4*>(col + ri0 + r4);
-            if (rj0 + r4 < n_pad) vb = *reinterpret_cast<const float4*>(col + rj0 + r4);
-            sA[k][r4 + 0] = va.x; sA[k][r4 + 1] = va.y; sA[k][r4 + 2] = va.z; sA[k][r4 + 3] = va.w;
-            sB[k][r4 + 0] = vb.x; sB[k][r4 + 1] = vb.y; sB[k][r4 + 2] = vb.z; sB[k][r4 + 3] = vb.w;
+    if constexpr (SPLIT) {
+        // staging: thread t loads row t of the A rows (t < 128) or of the B rows, SY_KCH
+        // consecutive k (one coalesced load per k across the wave), splits, and writes
+        // k-contiguous hi / lo words
+        const bool isB = tid >= SY_TILE;
+        const int srow = isB ? rj0 + tid - SY_TILE : ri0 + tid;
+        const int lrow = tid & (SY_TILE - 1);
+        const bool live = srow < n_pad;
+        for (int k0 = 0; k0 < K; k0 += SY_KCH) {
+            float v[SY_KCH];
+            const float* src = A + (long long)(kb + k0) * ld + srow;
+#pragma unroll
+            for (int k = 0; k < SY_KCH; ++k) v[k] = live ? src[(long long)k * ld] : 0.f;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < SY_KCH; k += 8) {
+                unsigned h[8], l[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) split_bf16(v[k + u], h[u], l[u]);
+                const uint4 hw = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16),
+                                            h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+                const uint4 lw = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16),
+                                            l[4] | (l[5] << 16), l[6] | (l[7] << 16));
+                *reinterpret_cast<uint4*>(&sH[isB ? 2 : 0][lrow][k / 2]) = hw;
+                *reinterpret_cast<uint4*>(&sH[isB ? 3 : 1][lrow][k / 2]) = lw;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < SY_KCH / 16; ++t) {
+                const int w = 8 * t + 4 * lh;   // lane holds k = 16 t + 8 lh + 0..7
+                bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const int r = wr * 64 + a * 32 + l32;
+                    ah[a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sH[0][r][w]));
+                    al[a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sH[1][r][w]));
+                }
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const int r = wc * 64 + b * 32 + l32;
+                    bh[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sH[2][r][w]));
+                    bl[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sH[3][r][w]));
+                }
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        acc[a][b] = mfma_bf16_32x32x16(al[a], bh[b], acc[a][b]);
+                        acc[a][b] = mfma_bf16_32x32x16(ah[a], bl[b], acc[a][b]);
+                        acc[a][b] = mfma_bf16_32x32x16(ah[a], bh[b], acc[a][b]);
+                    }
+            }
         }
-        __syncthreads();
+    } else {
+        for (int k0 = 0; k0 < K; k0 += SY_KC32) {
+            __syncthreads();
+            // 32 columns x 128 rows, float4 per thread-iteration
+            for (int idx = tid; idx < SY_KC32 * SY_TILE / 4; idx += 256) {
+                const int k = idx / (SY_TILE / 4);
+                const int r4 = (idx % (SY_TILE / 4)) * 4;
+                const float* col = A + (long long)(kb + k0 + k) * ld;
+                float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;
+                if (ri0 + r4 < n_pad) va = *reinterpret_cast<const float4*>(col + ri0 + r4);
+                if (rj0 + r4 < n_pad) vb = *reinterpret_cast<const float4*>(col + rj0 + r4);
+                sA[k][r4 + 0] = va.x; sA[k][r4 + 1] = va.y; sA[k][r4 + 2] = va.z; sA[k][r4 + 3] = va.w;
+                sB[k][r4 + 0] = vb.x; sB[k][r4 + 1] = vb.y; sB[k][r4 + 2] = vb.z; sB[k][r4 + 3] = vb.w;
+            }
+            __syncthreads();
 #pragma unroll 4
-        for (int k = 0; k < SY_KC32; k += 2) {
-            float av[2], bv[2];
+            for (int k = 0; k < SY_KC32; k += 2) {
+                float av[2], bv[2];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) av[a] = sA[k + lh][wr * 64 + a * 32 + l32];
+                for (int a = 0; a < 2; ++a) av[a] = sA[k + lh][wr * 64 + a * 32 + l32];
 #pragma unroll
-            for (int b = 0; b < 2; ++b) bv[b] = sB[k + lh][wc * 64 + b * 32 + l32];
+                for (int b = 0; b < 2; ++b) bv[b] = sB[k + lh][wc * 64 + b * 32 + l32];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+                for (int a = 0; a < 2; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f32_32x32x2(av[a], bv[b], acc[a][b]);
+                    for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f32_32x32x2(av[a], bv[b], acc[a][b]);
+            }
         }
     }
+    if (diag) __syncthreads();   // operand stages are reused for the diagonal block
     // C -= acc ; lane owns column l32 and rows 8*(r/4) + 4*lh + (0..3) of each 32x32 block
+    const bool to_lds = diag && wave == 0;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             const int col = rj0 + wc * 64 + b * 32 + l32;
-            if (col >= n_pad) continue;
+            if (col >= t1) continue;
             float* cp = A + (long long)col * ld;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -186,9 +360,22 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int c0, int t0, int n_pad)
                 c.y -= acc[a][b][4 * q + 1];
                 c.z -= acc[a][b][4 * q + 2];
                 c.w -= acc[a][b][4 * q + 3];
-                *reinterpret_cast<float4*>(cp + row) = c;
+                if (to_lds) {
+                    const int lr = row - ri0, lc = col - rj0;
+                    sD[(lr + 0) * DG_LD + lc] = c.x;
+                    sD[(lr + 1) * DG_LD + lc] = c.y;
+                    sD[(lr + 2) * DG_LD + lc] = c.z;
+                    sD[(lr + 3) * DG_LD + lc] = c.w;
+                } else {
+                    *reinterpret_cast<float4*>(cp + row) = c;
+                }
             }
         }
+    if (diag) {
+        __syncthreads();
+        factor_diag64(sD, t0, status);
+        store_diag64<float>(sD, A, ld, t0, Lb_next);
+    }
 }
 
 // ------------------------------------------------------------------ FP64 SYRK (MFMA)
@@ -199,12 +386,17 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int c0, int t0, int n_pad)
 constexpr int SY_KC64 = 16;
 
 __global__ void __launch_bounds__(256)
-syrk_f64_kernel(double* __restrict__ A, long long ld, int c0, int t0, int n_pad)
+syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int t1, int n_pad,
+                double* __restrict__ Lb_next, int* __restrict__ status)
 {
-    __shared__ double sA[SY_KC64][SY_TILE + 2];
-    __shared__ double sB[SY_KC64][SY_TILE + 2];
+    static_assert(2 * SY_KC64 * (SY_TILE + 2) * 8 <= SY_SMEM, "SYRK LDS");
+    __shared__ __align__(16) unsigned char smem[SY_SMEM];
+    double (*sA)[SY_TILE + 2] = reinterpret_cast<double (*)[SY_TILE + 2]>(smem);
+    double (*sB)[SY_TILE + 2] = reinterpret_cast<double (*)[SY_TILE + 2]>(smem + SY_KC64 * (SY_TILE + 2) * 8);
+    double* sD = reinterpret_cast<double*>(smem);
+    const bool diag = blockIdx.x == 0;
     int ti, tj;
-    tri_tile(blockIdx.x, ti, tj);
+    trap_tile(blockIdx.x, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
     const int ri0 = t0 + ti * SY_TILE;
     const int rj0 = t0 + tj * SY_TILE;
     const int tid = threadIdx.x;
@@ -221,12 +413,12 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int c0, int t0, int n_pad)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.0;
 
-    for (int k0 = 0; k0 < CHOL_NB; k0 += SY_KC64) {
+    for (int k0 = 0; k0 < K; k0 += SY_KC64) {
         __syncthreads();
         for (int idx = tid; idx < SY_KC64 * SY_TILE / 2; idx += 256) {
             const int k = idx / (SY_TILE / 2);
             const int r2 = (idx % (SY_TILE / 2)) * 2;
-            const double* col = A + (long long)(c0 + k0 + k) * ld;
+            const double* col = A + (long long)(kb + k0 + k) * ld;
             double2 va = make_double2(0.0, 0.0), vb = va;
             if (ri0 + r2 < n_pad) va = *reinterpret_cast<const double2*>(col + ri0 + r2);
             if (rj0 + r2 < n_pad) vb = *reinterpret_cast<const double2*>(col + rj0 + r2);
@@ -247,98 +439,161 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int c0, int t0, int n_pad)
                 for (int b = 0; b < 4; ++b) acc[a][b] = mfma_f64_16x16x4(av[a], bv[b], acc[a][b]);
         }
     }
+    if (diag) __syncthreads();   // operand stages are reused for the diagonal block
     // lane owns column l16 and rows 4*lq + (0..3) of each 16x16 block
+    const bool to_lds = diag && wave == 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int col = rj0 + wc * 64 + b * 16 + l16;
             const int row = ri0 + wr * 64 + a * 16 + 4 * lq;
-            if (col >= n_pad || row >= n_pad) continue;
+            if (col >= t1 || row >= n_pad) continue;
             double* cp = A + (long long)col * ld + row;
             double2 c0v = *reinterpret_cast<double2*>(cp);
             double2 c1v = *reinterpret_cast<double2*>(cp + 2);
             c0v.x -= acc[a][b][0]; c0v.y -= acc[a][b][1];
             c1v.x -= acc[a][b][2]; c1v.y -= acc[a][b][3];
-            *reinterpret_cast<double2*>(cp) = c0v;
-            *reinterpret_cast<double2*>(cp + 2) = c1v;
+            if (to_lds) {
+                const int lr = row - ri0, lc = col - rj0;
+                sD[(lr + 0) * DG_LD + lc] = c0v.x;
+                sD[(lr + 1) * DG_LD + lc] = c0v.y;
+                sD[(lr + 2) * DG_LD + lc] = c1v.x;
+                sD[(lr + 3) * DG_LD + lc] = c1v.y;
+            } else {
+                *reinterpret_cast<double2*>(cp) = c0v;
+                *reinterpret_cast<double2*>(cp + 2) = c1v;
+            }
         }
+    if (diag) {
+        __syncthreads();
+        factor_diag64(sD, t0, status);
+        store_diag64<double>(sD, A, ld, t0, Lb_next);
+    }
+}
+
+// Two-level blocking: outer panels of CHOL_OUTER columns are factored panel by panel
+// (64 columns) with updates restricted to the outer panel; the trailing matrix beyond it is
+// updated once per outer panel with K = CHOL_OUTER, which divides its HBM traffic (read +
+// write of the trailing lower triangle) by CHOL_OUTER / 64.
+constexpr int CHOL_OUTER = 256;
+
+template <typename T>
+int launch_update(T* d_A, long long ld, int kb, int K, int t0, int t1, int n_pad, double* d_Lb_next,
+                  int* d_status, int syrk_mode, hipStream_t st)
+{
+    const int ni = ceil_div(n_pad - t0, SY_TILE);
+    const int nj = ceil_div(t1 - t0, SY_TILE);
+    int ntiles = 0;
+    for (int j = 0; j < nj; ++j) ntiles += ni - j;
+    if constexpr (sizeof(T) == 4) {
+        if (syrk_mode == SYRK_BF16X3)
+            hipLaunchKernelGGL(syrk_f32_kernel<true>, dim3(ntiles), dim3(256), 0, st,
+                               reinterpret_cast<float*>(d_A), ld, kb, K, t0, t1, n_pad, d_Lb_next, d_status);
+        else
+            hipLaunchKernelGGL(syrk_f32_kernel<false>, dim3(ntiles), dim3(256), 0, st,
+                               reinterpret_cast<float*>(d_A), ld, kb, K, t0, t1, n_pad, d_Lb_next, d_status);
+    } else {
+        hipLaunchKernelGGL(syrk_f64_kernel, dim3(ntiles), dim3(256), 0, st,
+                           reinterpret_cast<double*>(d_A), ld, kb, K, t0, t1, n_pad, d_Lb_next, d_status);
+    }
+    BDH_LAUNCH_CHECK();
+    return 0;
 }
 
 template <typename T>
-int chol_factor(T* d_A, long long ld, int n_pad, int* d_status, int syrk_mode, hipStream_t st,
-                PhaseRecorder* rec)
+int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, int syrk_mode,
+                hipStream_t st, PhaseRecorder* rec)
 {
     BDH_CHECK(n_pad % PAD_TO == 0, "chol_factor: n_pad must be a multiple of 128");
     BDH_CHECK(ld >= n_pad && ld % 4 == 0, "chol_factor: bad leading dimension");
-    for (int c0 = 0; c0 < n_pad; c0 += CHOL_NB) {
-        const int below = n_pad - c0 - CHOL_NB;
-        if (rec) rec->begin(PH_PANEL, st);
-        hipLaunchKernelGGL(chol_panel_kernel<T>, dim3(1 + ceil_div(below, PANEL_ROWS)), dim3(256),
-                           0, st, d_A, ld, c0, n_pad, d_status);
-        BDH_LAUNCH_CHECK();
-        if (rec) rec->end(PH_PANEL, st);
-        const int t0 = c0 + CHOL_NB;
-        if (t0 >= n_pad) break;
-        const int nt = ceil_div(n_pad - t0, SY_TILE);
-        const int ntiles = nt * (nt + 1) / 2;
-        if (rec) rec->begin(PH_SYRK, st);
-        if constexpr (sizeof(T) == 4) {
-            (void)syrk_mode;
-            hipLaunchKernelGGL(syrk_f32_kernel, dim3(ntiles), dim3(256), 0, st,
-                               reinterpret_cast<float*>(d_A), ld, c0, t0, n_pad);
-        } else {
-            hipLaunchKernelGGL(syrk_f64_kernel, dim3(ntiles), dim3(256), 0, st,
-                               reinterpret_cast<double*>(d_A), ld, c0, t0, n_pad);
+    constexpr long long LB = (long long)CHOL_NB * CHOL_NB;
+    BDH_CHECK(syrk_mode != SYRK_BF16X3 || sizeof(T) == 4, "chol_factor: split SYRK needs FP32 storage");
+    if (rec) rec->begin(PH_PANEL, st);
+    hipLaunchKernelGGL(chol_diag_kernel<T>, dim3(1), dim3(256), 0, st, d_A, ld, 0, d_Lbuf, d_status);
+    BDH_LAUNCH_CHECK();
+    if (rec) rec->end(PH_PANEL, st);
+    for (int P0 = 0; P0 < n_pad; P0 += CHOL_OUTER) {
+        const int P1 = std::min(n_pad, P0 + CHOL_OUTER);
+        for (int c0 = P0; c0 < P1; c0 += CHOL_NB) {
+            const int below = n_pad - c0 - CHOL_NB;
+            if (below <= 0) return 0;
+            const long long k = c0 / CHOL_NB;
+            if (rec) rec->begin(PH_PANEL, st);
+            hipLaunchKernelGGL(chol_trsm_kernel<T>, dim3(ceil_div(below, TRSM_ROWS)), dim3(256), 0, st,
+                               d_A, ld, c0, n_pad, (const double*)(d_Lbuf + k * LB));
+            BDH_LAUNCH_CHECK();
+            if (rec) rec->end(PH_PANEL, st);
+            const int t0 = c0 + CHOL_NB;
+            if (rec) rec->begin(PH_SYRK, st);
+            // inside the outer panel: this panel's update of the panel's later columns;
+            // at its end: the whole outer panel's update of the trailing matrix
+            const int rc = (t0 < P1)
+                ? launch_update<T>(d_A, ld, c0, CHOL_NB, t0, P1, n_pad, d_Lbuf + (k + 1) * LB, d_status, syrk_mode, st)
+                : launch_update<T>(d_A, ld, P0, P1 - P0, t0, n_pad, n_pad, d_Lbuf + (k + 1) * LB, d_status, syrk_mode, st);
+            if (rc) return rc;
+            if (rec) rec->end(PH_SYRK, st);
         }
-        BDH_LAUNCH_CHECK();
-        if (rec) rec->end(PH_SYRK, st);
     }
     return 0;
 }
 
-// ------------------------------------------------------------------ y = L x (partials)
+size_t chol_lbuf_doubles(int n_pad) { return (size_t)(n_pad / CHOL_NB) * CHOL_NB * CHOL_NB; }
+
+// ------------------------------------------------------------------ Y = L X (partials)
+// X holds m right-hand sides (m <= M, X[k * ldx + c]): with L fixed between refactorizations,
+// the noise of a whole refactor cycle is one pass over L instead of one per step.
 // Tile: (64 lanes x V rows) x TRMV_COLS columns, V = 16 / sizeof(T) so each lane issues one
-// 16-byte load per column; the 4 waves split the tile's columns (unrolled by 8 for loads in
+// 16-byte load per column; the 4 waves split the tile's columns (unrolled for loads in
 // flight) and meet in LDS.  Entries above the diagonal are masked with a select after the
 // load (the storage there is finite but meaningless), so diagonal tiles take the same
-// vector path.  P[cb][r] receives row r's partial over column block cb (FP64 accumulation).
+// vector path.  P[cb][k][r] receives row r's partial of right-hand side k over column
+// block cb (FP64 products and accumulation).
 constexpr int TRMV_COLS = 256;
 
-template <typename T>
-__device__ __forceinline__ void trmv_accum(const T* __restrict__ col, int rbase, int c, double xc,
-                                           bool mask, double* acc)
+template <typename T, int M>
+__device__ __forceinline__ void trmv_accum(const T* __restrict__ col, int rbase, int c,
+                                           const double* __restrict__ xc, bool mask,
+                                           double (*acc)[M])
 {
     constexpr int V = 16 / sizeof(T);
+    double v[V];
     if constexpr (V == 4) {
-        const float4 v = *reinterpret_cast<const float4*>(col + rbase);
-        acc[0] += (double)((!mask || rbase + 0 >= c) ? v.x : 0.f) * xc;
-        acc[1] += (double)((!mask || rbase + 1 >= c) ? v.y : 0.f) * xc;
-        acc[2] += (double)((!mask || rbase + 2 >= c) ? v.z : 0.f) * xc;
-        acc[3] += (double)((!mask || rbase + 3 >= c) ? v.w : 0.f) * xc;
+        const float4 w = *reinterpret_cast<const float4*>(col + rbase);
+        v[0] = (!mask || rbase + 0 >= c) ? (double)w.x : 0.0;
+        v[1] = (!mask || rbase + 1 >= c) ? (double)w.y : 0.0;
+        v[2] = (!mask || rbase + 2 >= c) ? (double)w.z : 0.0;
+        v[3] = (!mask || rbase + 3 >= c) ? (double)w.w : 0.0;
     } else {
-        const double2 v = *reinterpret_cast<const double2*>(col + rbase);
-        acc[0] += ((!mask || rbase + 0 >= c) ? v.x : 0.0) * xc;
-        acc[1] += ((!mask || rbase + 1 >= c) ? v.y : 0.0) * xc;
+        const double2 w = *reinterpret_cast<const double2*>(col + rbase);
+        v[0] = (!mask || rbase + 0 >= c) ? w.x : 0.0;
+        v[1] = (!mask || rbase + 1 >= c) ? w.y : 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < V; ++q)
+#pragma unroll
+        for (int k = 0; k < M; ++k) acc[q][k] += v[q] * xc[k];
 }
 
-template <typename T>
+template <typename T, int M>
 __global__ void __launch_bounds__(256)
-trmv_partial_kernel(const T* __restrict__ L, long long ld, int n, const double* __restrict__ x,
-                    double* __restrict__ P)
+trmv_partial_kernel(const T* __restrict__ L, long long ld, int n, const double* __restrict__ X,
+                    long long ldx, int m, double* __restrict__ P)
 {
     constexpr int V = 16 / sizeof(T);
     constexpr int ROWS = 64 * V;
+    constexpr int UNR = M >= 8 ? 2 : (M >= 4 ? 4 : 8);
     const int rb = blockIdx.x, cb = blockIdx.y;
     const int r0 = rb * ROWS, cbeg = cb * TRMV_COLS;
     if (cbeg > r0 + ROWS - 1 || cbeg >= n) return;  // tile entirely above the diagonal
     __shared__ double s_red[4][ROWS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rbase = r0 + lane * V;
-    double acc[V];
+    double acc[V][M];
 #pragma unroll
-    for (int q = 0; q < V; ++q) acc[q] = 0.0;
+    for (int q = 0; q < V; ++q)
+#pragma unroll
+        for (int k = 0; k < M; ++k) acc[q][k] = 0.0;
     const int cend = min(n, cbeg + TRMV_COLS);
     const int wcols = TRMV_COLS / 4;
     const int c_lo = cbeg + wave * wcols, c_hi = min(cend, c_lo + wcols);
@@ -346,42 +601,71 @@ trmv_partial_kernel(const T* __restrict__ L, long long ld, int n, const double* 
     const bool mask = (cbeg + TRMV_COLS - 1) > r0;
     if (rbase < n) {
         int c = c_lo;
-        for (; c + 8 <= c_hi; c += 8) {
+        for (; c + UNR <= c_hi; c += UNR) {
+            double xc[UNR][M];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                trmv_accum<T>(L + (long long)(c + u) * ld, rbase, c + u, x[c + u], mask, acc);
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int k = 0; k < M; ++k) xc[u][k] = (k < m) ? X[k * ldx + c + u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+                trmv_accum<T, M>(L + (long long)(c + u) * ld, rbase, c + u, xc[u], mask, acc);
         }
-        for (; c < c_hi; ++c) trmv_accum<T>(L + (long long)c * ld, rbase, c, x[c], mask, acc);
-    }
+        for (; c < c_hi; ++c) {
+            double xc[M];
 #pragma unroll
-    for (int q = 0; q < V; ++q) s_red[wave][lane * V + q] = acc[q];
-    __syncthreads();
-    for (int rr = tid; rr < ROWS; rr += 256) {
-        const int row = r0 + rr;
-        if (row < n)
-            P[(long long)cb * n + row] = (s_red[0][rr] + s_red[1][rr]) + (s_red[2][rr] + s_red[3][rr]);
+            for (int k = 0; k < M; ++k) xc[k] = (k < m) ? X[k * ldx + c] : 0.0;
+            trmv_accum<T, M>(L + (long long)c * ld, rbase, c, xc, mask, acc);
+        }
+    }
+    for (int k = 0; k < m; ++k) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            double a = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < M; ++kk) a = (kk == k) ? acc[q][kk] : a;
+            s_red[wave][lane * V + q] = a;
+        }
+        __syncthreads();
+        for (int rr = tid; rr < ROWS; rr += 256) {
+            const int row = r0 + rr;
+            if (row < n)
+                P[((long long)cb * m + k) * n + row] =
+                    (s_red[0][rr] + s_red[1][rr]) + (s_red[2][rr] + s_red[3][rr]);
+        }
     }
 }
 
 int trmv_col_blocks(int n) { return ceil_div(n, TRMV_COLS); }
 
 template <typename T>
-int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_x, double* d_P,
-                         hipStream_t st)
+int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_X, long long ldx, int m,
+                         double* d_P, hipStream_t st)
 {
     constexpr int ROWS = 64 * (16 / sizeof(T));
     BDH_CHECK(ld % (16 / sizeof(T)) == 0, "trmv: leading dimension not 16-byte aligned");
+    BDH_CHECK(m >= 1 && m <= TRMV_MAX_RHS, "trmv: 1..16 right-hand sides");
     dim3 grid(ceil_div(n, ROWS), trmv_col_blocks(n));
-    hipLaunchKernelGGL(trmv_partial_kernel<T>, grid, dim3(256), 0, st, d_L, ld, n, d_x, d_P);
+    if (m == 1)
+        hipLaunchKernelGGL((trmv_partial_kernel<T, 1>), grid, dim3(256), 0, st, d_L, ld, n, d_X, ldx, m, d_P);
+    else if (m == 2)
+        hipLaunchKernelGGL((trmv_partial_kernel<T, 2>), grid, dim3(256), 0, st, d_L, ld, n, d_X, ldx, m, d_P);
+    else if (m <= 4)
+        hipLaunchKernelGGL((trmv_partial_kernel<T, 4>), grid, dim3(256), 0, st, d_L, ld, n, d_X, ldx, m, d_P);
+    else if (m <= 8)
+        hipLaunchKernelGGL((trmv_partial_kernel<T, 8>), grid, dim3(256), 0, st, d_L, ld, n, d_X, ldx, m, d_P);
+    else
+        hipLaunchKernelGGL((trmv_partial_kernel<T, 16>), grid, dim3(256), 0, st, d_L, ld, n, d_X, ldx, m, d_P);
     BDH_LAUNCH_CHECK();
     return 0;
 }
 
 int trmv_blk_cols() { return TRMV_COLS; }
 
-template int chol_factor<float>(float*, long long, int, int*, int, hipStream_t, PhaseRecorder*);
-template int chol_factor<double>(double*, long long, int, int*, int, hipStream_t, PhaseRecorder*);
-template int launch_trmv_partials<float>(const float*, long long, int, const double*, double*, hipStream_t);
-template int launch_trmv_partials<double>(const double*, long long, int, const double*, double*, hipStream_t);
+template int chol_factor<float>(float*, long long, int, double*, int*, int, hipStream_t, PhaseRecorder*);
+template int chol_factor<double>(double*, long long, int, double*, int*, int, hipStream_t, PhaseRecorder*);
+template int launch_trmv_partials<float>(const float*, long long, int, const double*, long long, int, double*, hipStream_t);
+template int launch_trmv_partials<double>(const double*, long long, int, const double*, long long, int, double*, hipStream_t);
 
 }  // namespace bdh

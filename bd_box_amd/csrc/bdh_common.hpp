@@ -48,6 +48,21 @@ constexpr double kB_1_6PI = 0.00010537375739171604;
 constexpr double SQRTPI = 1.77245385090551602729;
 constexpr double PI_F = (double)3.14159265358979323846f;  // the reference's float M_PIF
 
+// Pair separation shared by every pair kernel, evaluated without FMA contraction so the
+// brute-force and cell-list paths (and a plain FP64 host loop) see identical distances.
+// Minimum image of trans_dist_vec (trans.c:111-121) when pbc is set.
+__device__ __forceinline__ double pair_r2(double& dx, double& dy, double& dz, int pbc, double box)
+{
+#pragma clang fp contract(off)
+    if (pbc) {
+        const double inv = 1.0 / box;
+        dx = dx - box * floor(dx * inv + 0.5);
+        dy = dy - box * floor(dy * inv + 0.5);
+        dz = dz - box * floor(dz * inv + 0.5);
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
 inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 inline long long ceil_div_ll(long long a, long long b) { return (a + b - 1) / b; }
 

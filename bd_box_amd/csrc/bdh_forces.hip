@@ -1,38 +1,22 @@
 // Bonded and nonbonded forces, and the Ermak-McCammon position update, on gfx950.
 //
 // Forces restate the reference's compute_forces with brute-force pair lists
-// (src/potentials/calc_func.c:106-228, force_single :370-409):
-//   Debye-Hueckel  F = (gamma/eps_r) Q_i Q_j (1 + kappa r) e^{-kappa r} / r^2   r <= cutoff_c
-//                  (electro.c:36-78)
-//   Lennard-Jones  F = alpha eps_ij 6 (s6 (2 s6 - lj6)) / r, s6 = (sigma_ij/r)^6,
-//                  sigma_ij = R_i + R_j, eps_ij = sqrt(eps_i eps_j),
-//                  r <= cutoff_lj (or 1.122462 sigma_ij when cutoff_lj <= 0)   (LJ.c:37-99)
-//   bonds          F = H (r0 - r) / (1 - r^2/rmax^2)                           (bond.c:43-85)
-//   bonded partners use bond_lj_scale / bond_c_scale (0 removes the term) whenever a
-//   scale differs from 1, exactly when the reference allocates its partner table
-//   (bond.c:113-130); otherwise bonded pairs interact like any other pair.
+// (src/potentials/calc_func.c:106-228, force_single :370-409); the pair and bond terms are
+// in bdh_pairterm.hpp.  Bonded partners use bond_lj_scale / bond_c_scale (0 removes the
+// term) whenever a scale differs from 1, exactly when the reference allocates its partner
+// table (bond.c:113-130); otherwise bonded pairs interact like any other pair.
 // Each workgroup owns 64 row beads and one split of the column beads and accumulates the
 // full row sums F_i = sum_j f_ij (every pair is evaluated from both sides; no atomics, so
 // the sums are deterministic).  Pair arithmetic runs in T (float in the mixed-precision
 // step), separations and accumulation in FP64.
 #include "bdh_common.hpp"
 #include "bdh_kernels.hpp"
+#include "bdh_pairterm.hpp"
 
 namespace bdh {
 
 constexpr int FR_ROWS = 64;
 constexpr int FR_CHUNK = 256;
-
-__device__ __forceinline__ void min_image(double& x, double& y, double& z, int pbc, double box)
-{
-    if (pbc) {
-        // trans_dist_vec (trans.c:111-121)
-        const double inv = 1.0 / box;
-        x -= box * floor(x * inv + 0.5);
-        y -= box * floor(y * inv + 0.5);
-        z -= box * floor(z * inv + 0.5);
-    }
-}
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -54,9 +38,7 @@ forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
     const double2 lji = (i < N) ? LJ[i] : make_double2(0, 0);
     const int nb0 = (i < N && fp.use_conn) ? topo.conn_off[i] : 0;
     const int nb1 = (i < N && fp.use_conn) ? topo.conn_off[i + 1] : 0;
-    double fx = 0, fy = 0, fz = 0, e_lj = 0, e_c = 0;
-    int rep = 0;
-    const T gam = (T)fp.gam, kap = (T)fp.kappa_c, alj = (T)fp.alpha_lj, lj6 = (T)fp.lj_6_term;
+    PairAcc a{0, 0, 0, 0, 0, 0};
     for (int c0 = jb; c0 < je; c0 += FR_CHUNK) {
         const int cn = min(FR_CHUNK, je - c0);
         __syncthreads();
@@ -71,71 +53,18 @@ forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
             const int j = c0 + q;
             if (j == i) continue;
             double dx = s_pos[q].x - pi.x, dy = s_pos[q].y - pi.y, dz = s_pos[q].z - pi.z;
-            min_image(dx, dy, dz, fp.pbc, fp.box);
-            const double r2d = dx * dx + dy * dy + dz * dz;
-            const T r2 = (T)r2d;
-            const T r = sqrt(r2);
-            bool conn = false;
-            for (int b = nb0; b < nb1; ++b) conn |= (topo.conn[b] == j);
-            const T sig = (T)(lji.x + s_lj[q].x);
-            T F = 0;
-            int ov = 0;
-            if (fp.elec && r <= (T)fp.cutoff_c && (!conn || fp.bond_c_scale != 0.0)) {
-                T ex = exp(-kap * r);
-                T f = gam * (T)qi * (T)s_q[q] * ((T)1 + kap * r) * ex / r2;
-                T en = gam * (T)qi * (T)s_q[q] * ex / r;
-                if (conn) { f *= (T)fp.bond_c_scale; en *= (T)fp.bond_c_scale; }
-                F += f;
-                e_c += (double)en;
-            }
-            const T lj_cut = fp.cutoff_lj > 0.0 ? (T)fp.cutoff_lj : (T)1.122462f * sig;
-            if (fp.alpha_lj != 0.0 && r <= lj_cut) {
-                if (!conn || fp.bond_lj_scale != 0.0) {
-                    const T eps = (T)sqrt(lji.y * s_lj[q].y);
-                    T s6 = sig / r;
-                    s6 *= s6;
-                    s6 *= s6 * s6;
-                    T f = alj * eps * (T)6 * (s6 * ((T)2 * s6 - lj6)) / r;
-                    T en = alj * eps * (s6 * (s6 - lj6));
-                    if (conn) { f *= (T)fp.bond_lj_scale; en *= (T)fp.bond_lj_scale; }
-                    F += f;
-                    e_lj += (double)en;
-                    ov = (sig > r) || !isfinite(f);
-                } else {
-                    ov = sig * sig > r2;
-                }
-            } else {
-                ov = sig * sig > r2;
-            }
-            rep |= ov;
-            // force on i: -e_ij F with e_ij = (r_j - r_i)/r
-            const double s = (double)F / (double)r;
-            fx -= dx * s; fy -= dy * s; fz -= dz * s;
+            const double r2 = pair_r2(dx, dy, dz, fp.pbc, fp.box);
+            pair_term<T>(dx, dy, dz, r2, qi, s_q[q], lji, s_lj[q], bonded(topo, nb0, nb1, j), fp, a);
         }
     }
     // bonds (only in the first column split, so each bond end is added once)
     double e_b = 0.0;
-    if (i < N && blockIdx.y == 0 && ty == 0) {
-        for (int b = topo.bond_off[i]; b < topo.bond_off[i + 1]; ++b) {
-            const int p = topo.bond_p[b];
-            const double r0 = topo.bond_parm[3 * b], rmax = topo.bond_parm[3 * b + 1];
-            const double H = topo.bond_parm[3 * b + 2];
-            double dx = pos[p].x - pi.x, dy = pos[p].y - pi.y, dz = pos[p].z - pi.z;
-            min_image(dx, dy, dz, fp.pbc, fp.box);
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const double r = sqrt(r2);
-            if (!(r < rmax)) rep |= 2;  // overstretched bond: fatal in the reference
-            const double Ff = H * (r0 - r) / (1.0 - r2 / (rmax * rmax));
-            fx -= Ff * dx / r; fy -= Ff * dy / r; fz -= Ff * dz / r;
-            e_b += -0.5 * H * (rmax * rmax * log((rmax * rmax - r2) / (rmax * rmax - r0 * r0)) +
-                               r0 * rmax * log((rmax + r) * (rmax - r0) / ((rmax - r) * (rmax + r0))));
-        }
-    }
-    s_red[0][ty][tx] = fx;
-    s_red[1][ty][tx] = fy;
-    s_red[2][ty][tx] = fz;
-    s_red[3][ty][tx] = e_lj;
-    s_red[4][ty][tx] = e_c;
+    if (i < N && blockIdx.y == 0 && ty == 0) e_b = bond_terms(pos, i, pi, topo, fp, a);
+    s_red[0][ty][tx] = a.fx;
+    s_red[1][ty][tx] = a.fy;
+    s_red[2][ty][tx] = a.fz;
+    s_red[3][ty][tx] = a.e_lj;
+    s_red[4][ty][tx] = a.e_c;
     s_red[5][ty][tx] = e_b;
     __syncthreads();
     if (ty == 0 && i < N) {
@@ -147,13 +76,13 @@ forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
     // energies: pair terms are seen from both ends (x 1/2), bonds likewise
     __syncthreads();
     if (tid < 3) {
-        double a = 0;
+        double e = 0;
         for (int y = 0; y < 4; ++y)
-            for (int x = 0; x < FR_ROWS; ++x) a += s_red[3 + tid][y][x];
-        PE[3 * (blockIdx.y * gridDim.x + blockIdx.x) + tid] = 0.5 * a;
+            for (int x = 0; x < FR_ROWS; ++x) e += s_red[3 + tid][y][x];
+        PE[3 * (blockIdx.y * gridDim.x + blockIdx.x) + tid] = 0.5 * e;
     }
-    if (rep && fp.check_overlap) atomicOr(flags, 1);
-    if (rep & 2) atomicOr(flags, 2);
+    if (a.rep && fp.check_overlap) atomicOr(flags, 1);
+    if (a.rep & 2) atomicOr(flags, 2);
 }
 
 int force_splits(int N)
@@ -217,7 +146,8 @@ reduce_partials_kernel(ReduceJobs jobs)
     const int r = blockIdx.x * 64 + lane;
     double acc = 0.0;
     if (r < J.n) {
-        const int last = J.blk_cols > 0 ? min(J.S - 1, r / J.blk_cols) : J.S - 1;
+        const int rr = J.period > 0 ? r % J.period : r;
+        const int last = J.blk_cols > 0 ? min(J.S - 1, rr / J.blk_cols) : J.S - 1;
         const double* p = J.P + r;
         int sidx = wave;
         for (; sidx + 12 <= last; sidx += 16) {

@@ -28,22 +28,36 @@ int launch_rpy_assemble(const double4* d_pos, int N, double preDii, double preDi
                         long long ld, hipStream_t st);
 template <typename T>
 int launch_set_identity_pad(T* d_A, long long ld, int n, int n_pad, hipStream_t st);
-int matvec_splits(int N);
+
+// ---- pair sums, Ewald, TEA (bdh_pairs.hip)
+int ewald_upload(double box, int R);
+int pair_splits(int N, bool ewald);
 template <typename T>
-int launch_rpy_matvec(const double4* d_pos, const double* d_F, int N, double preDii,
-                      double preDij, double* d_Pf, int splits, hipStream_t st);
+int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi, int N,
+                     double preDii, double preDij, double dt, bool ewald, bool tea, int splits,
+                     double* d_Pf, double* d_Px, double* d_Pc, double* d_Pe, hipStream_t st);
+template <typename T>
+int launch_ewald_assemble(const double4* d_pos, int N, double preDii, double preDij, T* d_A,
+                          long long ld, hipStream_t st);
+int launch_tea_close(const double4* d_pos, const double* d_xi, const double* d_Feff,
+                     const double* d_C, const double* d_Pe, int S, int N, double preDii, double dt,
+                     double* d_bt, double* d_Rs, hipStream_t st);
 
 // ---- Cholesky and triangular products (bdh_chol.hip)
 constexpr int CHOL_NB = 64;   // panel width
 constexpr int PAD_TO = 128;   // n is padded to a multiple of this with an identity block
 enum SyrkMode { SYRK_F32 = 0, SYRK_F64 = 1, SYRK_BF16X3 = 2 };
+// d_Lbuf: chol_lbuf_doubles(n_pad) doubles of workspace (FP64 diagonal blocks).
+size_t chol_lbuf_doubles(int n_pad);
 template <typename T>
-int chol_factor(T* d_A, long long ld, int n_pad, int* d_status, int syrk_mode, hipStream_t st,
-                PhaseRecorder* rec = nullptr);
+int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, int syrk_mode,
+                hipStream_t st, PhaseRecorder* rec = nullptr);
 int trmv_col_blocks(int n);
+// Y = L X for m <= TRMV_MAX_RHS right-hand sides X[k * ldx + c]; partials P[cb][k][n].
+constexpr int TRMV_MAX_RHS = 16;
 template <typename T>
-int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_x, double* d_P,
-                         hipStream_t st);
+int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_X, long long ldx, int m,
+                         double* d_P, hipStream_t st);
 int trmv_blk_cols();   // columns per partial of the L.x kernel
 
 // ---- forces (bdh_forces.hip)
@@ -75,12 +89,36 @@ int launch_forces(const double4* d_pos, const double* d_Q, const double2* d_LJ, 
                   int* d_flags, int splits, hipStream_t st);
 int launch_energy_reduce(const double* d_PE, int pe_blocks, double* d_E, hipStream_t st);
 
+// ---- cell lists (bdh_cells.hip)
+constexpr int CELL_MAX_DIM = 48;                                   // cells per axis
+constexpr int CELL_MAX = CELL_MAX_DIM * CELL_MAX_DIM * CELL_MAX_DIM;
+struct CellList {
+    void* grid;     // device CellGrid (cell_grid_bytes())
+    int* key;       // N: cell of each bead
+    int* count;     // CELL_MAX
+    int* fill;      // CELL_MAX
+    int* start;     // CELL_MAX + 1
+    int* sorted;    // N: bead indices grouped by cell, ascending within a cell
+};
+size_t cell_grid_bytes();
+int launch_cell_build(const double4* d_pos, int N, double cutoff, int pbc, double box,
+                      const CellList& cl, hipStream_t st);
+int cell_force_blocks(int N);
+template <typename T>
+int launch_forces_cells(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
+                        const Topology& topo, const ForceParams& fp, const CellList& cl,
+                        double* d_F, double* d_PE, int* d_flags, hipStream_t st);
+int launch_cell_pairs(const double4* d_pos, int N, const CellList& cl, double cutoff, int pbc,
+                      double box, int* d_pairs, long long cap, unsigned long long* d_count,
+                      hipStream_t st);
+
 // Sums of partial rows written by the split kernels (D.F, L.xi, forces).
 struct ReduceJob {
     const double* P;   // S partial rows of length n
     double* out;
     int n, S, blk_cols;  // blk_cols > 0: row r sums partials 0..r/blk_cols only
     double scale;
+    int period;          // > 0: the triangular bound uses r % period (stacked vectors)
 };
 struct ReduceJobs {
     ReduceJob job[3];

@@ -1,5 +1,5 @@
 // Free-space Rotne-Prager-Yamakawa tensor on gfx950: tile-wise assembly of the lower
-// triangle of D, and the fused on-the-fly product D.F that never stores D.
+// triangle of D (the on-the-fly products over the same blocks are in bdh_pairs.hip).
 //
 // Block formula (src/diff_alg/diff_tensor.c:45-92), r = |r_i - r_j|, e = (r_i - r_j)/r:
 //   r >= a_i + a_j : (kT/8 pi eta r) [ (1 + (a_i^2+a_j^2)/(3 r^2)) I + (1 - (a_i^2+a_j^2)/r^2) e e^T ]
@@ -119,69 +119,6 @@ __global__ void set_identity_pad_kernel(T* __restrict__ A, long long ld, int n, 
     if (r < n_pad) A[(long long)r * ld + r] = (T)1;
 }
 
-// ------------------------------------------------------------------ on-the-fly D.F
-// Pf[s][3N] = sum over column beads j of split s of D_ij F_j (FP64 accumulation).
-// Grid (ceil(N/64), splits); workgroup (64, 4): tx -> row bead, ty -> every 4th column
-// bead of the LDS chunk; the 4 partial rows are combined through LDS.
-constexpr int MV_ROWS = 64;
-constexpr int MV_CHUNK = 256;
-
-template <typename T>
-__global__ void __launch_bounds__(256)
-rpy_matvec_kernel(const double4* __restrict__ pos, const double* __restrict__ F, int N,
-                  int cols_per_split, T preDii, T preDij, double* __restrict__ Pf)
-{
-    __shared__ double4 s_pos[MV_CHUNK];
-    __shared__ double s_f[MV_CHUNK * 3];
-    __shared__ double s_red[3][4][MV_ROWS];
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MV_ROWS + tx;
-    const int i = blockIdx.x * MV_ROWS + tx;
-    const int jb = blockIdx.y * cols_per_split;
-    const int je = min(N, jb + cols_per_split);
-    double4 pi = (i < N) ? pos[i] : make_double4(0, 0, 0, 1);
-    const T ai = (T)pi.w;
-    double y0 = 0, y1 = 0, y2 = 0;
-    for (int c0 = jb; c0 < je; c0 += MV_CHUNK) {
-        const int cn = min(MV_CHUNK, je - c0);
-        __syncthreads();
-        for (int q = tid; q < cn; q += 256) {
-            s_pos[q] = pos[c0 + q];
-            s_f[3 * q + 0] = F[3 * (c0 + q) + 0];
-            s_f[3 * q + 1] = F[3 * (c0 + q) + 1];
-            s_f[3 * q + 2] = F[3 * (c0 + q) + 2];
-        }
-        __syncthreads();
-        if (i < N) {
-            T a0 = 0, a1 = 0, a2 = 0;
-            for (int q = ty; q < cn; q += 4) {
-                const int j = c0 + q;
-                const double4 pj = s_pos[q];
-                const T fx = (T)s_f[3 * q], fy = (T)s_f[3 * q + 1], fz = (T)s_f[3 * q + 2];
-                if (j == i) {
-                    T dii = preDii / ai;
-                    a0 += dii * fx; a1 += dii * fy; a2 += dii * fz;
-                } else {
-                    Rpy6<T> b = rpy_pair<T>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, ai, (T)pj.w,
-                                            preDii, preDij);
-                    a0 += b.xx * fx + b.xy * fy + b.xz * fz;
-                    a1 += b.xy * fx + b.yy * fy + b.yz * fz;
-                    a2 += b.xz * fx + b.yz * fy + b.zz * fz;
-                }
-            }
-            y0 += (double)a0; y1 += (double)a1; y2 += (double)a2;
-        }
-    }
-    s_red[0][ty][tx] = y0;
-    s_red[1][ty][tx] = y1;
-    s_red[2][ty][tx] = y2;
-    __syncthreads();
-    if (ty == 0 && i < N) {
-        double* out = Pf + (long long)blockIdx.y * 3 * N + 3 * i;
-        for (int k = 0; k < 3; ++k)
-            out[k] = s_red[k][0][tx] + s_red[k][1][tx] + s_red[k][2][tx] + s_red[k][3][tx];
-    }
-}
-
 template <typename T>
 int launch_rpy_assemble(const double4* d_pos, int N, double preDii, double preDij, T* d_A,
                         long long ld, hipStream_t st)
@@ -203,34 +140,9 @@ int launch_set_identity_pad(T* d_A, long long ld, int n, int n_pad, hipStream_t 
     return 0;
 }
 
-int matvec_splits(int N)
-{
-    // enough workgroups to cover the chip: ~ 8 per CU, each split >= 64 columns
-    int rows = ceil_div(N, MV_ROWS);
-    int s = ceil_div(2048, rows);
-    int max_s = ceil_div(N, 64);
-    if (s > max_s) s = max_s;
-    if (s < 1) s = 1;
-    return s;
-}
-
-template <typename T>
-int launch_rpy_matvec(const double4* d_pos, const double* d_F, int N, double preDii,
-                      double preDij, double* d_Pf, int splits, hipStream_t st)
-{
-    int cps = ceil_div(N, splits);
-    dim3 grid(ceil_div(N, MV_ROWS), splits);
-    hipLaunchKernelGGL(rpy_matvec_kernel<T>, grid, dim3(MV_ROWS, 4), 0, st, d_pos, d_F, N, cps,
-                       (T)preDii, (T)preDij, d_Pf);
-    BDH_LAUNCH_CHECK();
-    return 0;
-}
-
 template int launch_rpy_assemble<float>(const double4*, int, double, double, float*, long long, hipStream_t);
 template int launch_rpy_assemble<double>(const double4*, int, double, double, double*, long long, hipStream_t);
 template int launch_set_identity_pad<float>(float*, long long, int, int, hipStream_t);
 template int launch_set_identity_pad<double>(double*, long long, int, int, hipStream_t);
-template int launch_rpy_matvec<float>(const double4*, const double*, int, double, double, double*, int, hipStream_t);
-template int launch_rpy_matvec<double>(const double4*, const double*, int, double, double, double*, int, hipStream_t);
 
 }  // namespace bdh

@@ -24,6 +24,7 @@ from .workloads import Params, System
 
 FP64 = 0
 MIXED = 1
+MIXED_SPLIT = 2
 
 
 def make_config(P: Params, precision: int = MIXED, refactor_every: int = 1) -> _lib.Config:
@@ -34,7 +35,7 @@ def make_config(P: Params, precision: int = MIXED, refactor_every: int = 1) -> _
         bond_lj_scale=P.bond_lj_scale, bond_c_scale=P.bond_c_scale,
         check_overlap=P.check_overlap, bc=P.bc, box=P.box, hydro=P.hydro,
         ewald_method=P.ewald_method, ewald_real=P.ewald_real, precision=precision,
-        refactor_every=refactor_every, rand_seed=P.rand_seed & 0xFFFFFFFF)
+        refactor_every=refactor_every, rand_seed=P.rand_seed & 0xFFFFFFFF, nb_list=P.nb_list)
 
 
 class BDSystem:
@@ -81,6 +82,20 @@ class BDSystem:
         _lib.check(_lib.load().bdh_compute_forces(self._h, _lib.dptr(F), _lib.dptr(E),
                                                   C.byref(rep)), "compute_forces")
         return F, E, rep.value
+
+    def neighbour_pairs(self, cutoff: float) -> np.ndarray:
+        """Pairs (i < j) within `cutoff` from the device cell list, sorted, shape (P, 2)."""
+        L = _lib.load()
+        cap = 1 << 16
+        while True:
+            out = np.zeros((cap, 2), dtype=np.int32)
+            cnt = C.c_longlong()
+            _lib.check(L.bdh_neighbour_pairs(self._h, float(cutoff),
+                                             out.ctypes.data_as(C.POINTER(C.c_int)), cap,
+                                             C.byref(cnt)), "neighbour_pairs")
+            if cnt.value <= cap:
+                return out[:cnt.value].copy()
+            cap = int(cnt.value)
 
     def compute_Rf(self) -> np.ndarray:
         Rf = np.zeros(self.n)

@@ -41,7 +41,7 @@ _ENUMS = {
                   "igt_var": 4, "igt_var_rev": 5},
     "bc": {"none": 0, "periodic": 1, "sphere": 2},
     "ewald_method": {"smith": 0, "beenakker": 1},
-    "nb_list": {"brute": 0, "spatial": 1, "verlet": 2},
+    "nb_list": {"brute": 0, "bucket_space": 1, "bucket_list": 2},
 }
 
 

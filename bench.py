@@ -34,7 +34,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BD steps/s, N=3000 RPY+Cholesky: mixed vs FP64 GPU at rel err≤1e-5"
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"f32_mfma": 157.3, "f64_mfma": 78.6}
+PEAK_TFLOPS = {"f32_mfma": 157.3, "f64_mfma": 78.6, "bf16_mfma": 2500.0}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_N3000.json")
 KB = 0.001986248532617
 
 
@@ -52,7 +53,8 @@ def workload(N, seed):
 # ----------------------------------------------------------------------------- roofline
 def syrk_flops_per_factor(n, n_pad, nb=64):
     """Algorithmic flops of the trailing updates of one factorization (lower triangle
-    only, unpadded size): sum over panels of nb * m (m + 1), m = rows below the panel."""
+    only, unpadded size): sum over 64-column panels of nb * m (m + 1), m = rows below the
+    panel.  The two-level schedule applies exactly these updates, grouped differently."""
     f = 0
     for c0 in range(0, n_pad, nb):
         m = max(0, n - (c0 + nb))
@@ -60,24 +62,40 @@ def syrk_flops_per_factor(n, n_pad, nb=64):
     return f
 
 
-def roofline(phases, n, precision, refactors):
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(phases, n, mode, refactors):
+    """Roofline of the phase with the largest share of the timed run."""
     n_pad = -(-n // 128) * 128
-    s = 4 if precision == "mixed" else 8
+    s = 8 if mode == "fp64" else 4
     dom = max(phases, key=lambda k: phases[k][0])
     ms, cnt = phases[dom]
     out = {"kernel": dom, "launches": cnt, "avg_ms": ms / max(cnt, 1)}
     if dom == "syrk":
         fl = syrk_flops_per_factor(n, n_pad) * refactors
         achieved = fl / (ms * 1e-3) / 1e12
-        peak = PEAK_TFLOPS["f32_mfma" if precision == "mixed" else "f64_mfma"]
+        peak = {"fp64": PEAK_TFLOPS["f64_mfma"], "f32": PEAK_TFLOPS["f32_mfma"],
+                "split": PEAK_TFLOPS["bf16_mfma"]}[mode]
+        basis = (f"n={n}: sum over 64-wide panels of 64 m (m+1) flops (lower-triangle "
+                 f"trailing updates), {refactors} factorizations / {cnt} launches")
+        if mode == "split":
+            basis += "; each FP32 product costs 3 bf16 MFMA products (ceiling peak/3)"
+        tr = measured_traffic("syrk")
         out.update(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s",
-                   frac=achieved / peak, traffic=None,
-                   basis="n_alg=%d: sum over 64-wide panels of 64 m (m+1) flops" % n)
+                   frac=achieved / peak, traffic=tr, basis=basis)
     elif dom in ("trmv", "assemble"):
         per = s * n * (n + 1) / 2
         achieved = per * cnt / (ms * 1e-3) / 1e9
         out.update(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                   frac=achieved / HBM_PEAK_GBS, traffic=None,
+                   frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(dom),
                    basis="%d B per lower-triangle element, n(n+1)/2 elements" % s)
     else:
         out.update(bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s", frac=None,
@@ -162,8 +180,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--precision", choices=("split", "f32"), default="split",
+                    help="mixed-precision variant timed as `value`: trailing updates as "
+                         "split-bf16 MFMA products (split) or FP32 MFMA (f32)")
     ap.add_argument("--profile-only", action="store_true",
-                    help="only the timed mixed-precision steps (for rocprofv3 runs)")
+                    help="only the timed mixed-precision steps (for rocprofv3 runs), then a "
+                         "256 MiB device copy that calibrates the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step of a "
@@ -172,14 +194,15 @@ def main():
         sys.exit(2)
 
     import torch
-    from bd_box_amd.system import BDSystem, FP64, MIXED
+    from bd_box_amd.system import BDSystem, FP64, MIXED, MIXED_SPLIT
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     torch.cuda.init()
 
     sysm, prm, P = workload(args.N, args.seed)
     n = 3 * sysm.N
     log(f"workload: N={sysm.N} n={n} refactor_every={args.refactor}")
-    mixed = BDSystem(sysm, P, precision=MIXED, refactor_every=args.refactor)
+    prec = MIXED_SPLIT if args.precision == "split" else MIXED
+    mixed = BDSystem(sysm, P, precision=prec, refactor_every=args.refactor)
     mixed.compute_hydro(args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -188,7 +211,13 @@ def main():
     t_mixed = time.perf_counter() - t0
     log(f"mixed: {args.steps} steps in {t_mixed:.3f} s")
     if args.profile_only:
-        print(json.dumps({"profile_only": True, "ms_per_step": 1e3 * t_mixed / args.steps}))
+        x = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_only": True, "ms_per_step": 1e3 * t_mixed / args.steps,
+                          "copy_bytes": 2 * x.numel() * 4}))
         return
 
     # per-phase HIP events over a timed run of whole refactor cycles
@@ -198,6 +227,19 @@ def main():
     ph_mixed = mixed.phase_times()
     mixed.phase_timing(False)
     log("mixed phases: " + json.dumps(ph_mixed))
+
+    other = {}
+    if args.precision == "split":
+        # the FP32-MFMA variant of the same step, for reference
+        m32 = BDSystem(sysm, P, precision=MIXED, refactor_every=args.refactor)
+        m32.compute_hydro(args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m32.compute_hydro(args.steps)
+        torch.cuda.synchronize()
+        other["f32_mfma_steps_per_s"] = args.steps / (time.perf_counter() - t0)
+        m32.close()
+        log(f"fp32-mfma variant: {other['f32_mfma_steps_per_s']:.1f} steps/s")
 
     fp64 = BDSystem(sysm, P, precision=FP64, refactor_every=args.refactor)
     fp64.compute_hydro(args.warmup)
@@ -228,7 +270,7 @@ def main():
         log(f"accuracy: {acc}")
 
     refactors = cyc // max(args.refactor, 1)
-    roof = roofline(ph_mixed, n, "mixed", refactors)
+    roof = roofline(ph_mixed, n, args.precision, refactors)
     value = args.steps / t_mixed
     cpu = None
     if not args.no_cpu_baseline:
@@ -241,12 +283,16 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"N={args.N} bead-spring chain, free-space RPY + Cholesky "
-                               f"refactored every {args.refactor} steps, mixed precision",
+                               f"refactored every {args.refactor} steps, mixed precision "
+                               f"({'split-bf16' if args.precision == 'split' else 'FP32'} "
+                               f"MFMA trailing updates, FP64 diagonal blocks)",
                    "N": args.N, "n": n, "refactor_every": args.refactor,
                    "noise": "device drand48 polar Box-Muller stream", "parallelism": "1 GPU"},
         "fp64_steps_per_s": args.steps / t_fp64,
         "speedup_vs_fp64": t_fp64 / t_mixed,
+        **other,
         **acc,
+        "accuracy_ok": (acc["rel_disp_err_max"] <= 1e-5) if acc else None,
         "phases_ms_per_step": {k: v[0] / cyc for k, v in ph_mixed.items()},
         "fp64_phases_ms_per_step": {k: v[0] / cyc for k, v in ph_fp64.items()},
         "roofline": roof,

@@ -32,7 +32,9 @@ typedef struct bdh_system bdh_system;
 
 enum bdh_precision {
     BDH_FP64 = 0,   /* everything in FP64 (the comparator)                              */
-    BDH_MIXED = 1   /* FP32 tensor/forces/trailing updates, FP64 panels and positions   */
+    BDH_MIXED = 1,  /* FP32 tensor/forces/trailing updates, FP64 panels and positions   */
+    BDH_MIXED_SPLIT = 2  /* as BDH_MIXED, trailing updates as split-bf16 products (three
+                            bf16 MFMAs per FP32 product, FP32 accumulation)                 */
 };
 
 enum bdh_hydro { BDH_HYDRO_NONE = 0, BDH_HYDRO_CHOLESKY = 1, BDH_HYDRO_GEYER = 2 };
@@ -55,6 +57,8 @@ typedef struct bdh_config {
     int precision;                           /* enum bdh_precision                       */
     int refactor_every;                      /* refactor D every k steps (reference: 1)  */
     uint32_t rand_seed;                      /* drand48 seed of the Gaussian stream      */
+    int nb_list;                             /* 0 brute force; 1 bucket_space, 2 bucket_list:
+                                                device cell list (src/potentials/bucket.h:40-46) */
 } bdh_config;
 
 /* Last error of the calling thread ("" if none). */
@@ -88,6 +92,13 @@ int bdh_compute_D(bdh_system* sys, double* D, int ld);
  * *repeat receives the reference's "repeat step" flag (overlap/NaN with check_overlap).
  * Stands in for compute_forces (src/potentials/calc_func.c:106-186). */
 int bdh_compute_forces(bdh_system* sys, double* F, double* E, int* repeat);
+
+/* Pairs i < j with |r_j - r_i| <= cutoff (minimum image when periodic) at the current
+ * positions, found through the device cell list; pairs[2k], pairs[2k+1] sorted ascending.
+ * *count receives the number of pairs; at most cap are stored.  Stands in for the
+ * neighbour queries of the reference's bucket lists (init_bucket_f / get_neighbours,
+ * src/potentials/bucket.h:112-117; init_keys / can_interact, :119-121). */
+int bdh_neighbour_pairs(bdh_system* sys, double cutoff, int* pairs, long long cap, long long* count);
 
 /* Assemble and factor D = L L^T at the current positions (the refactor step), copy L out
  * (lower triangle; zeros above).  Stands in for my_dpotrf on D (src/myblas.c:45-92,

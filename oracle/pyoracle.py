@@ -207,3 +207,26 @@ class Sim:
             lib().orc_sim_destroy(self.h)
         except Exception:
             pass
+
+
+def pairs_within(xyz, cutoff: float, pbc: bool = False, box: float = 0.0) -> np.ndarray:
+    """All pairs i < j with |r_j - r_i| <= cutoff, sorted, shape (P, 2), int32.
+
+    The brute-force pair loop of calc_func.c:188-228 with the separation of
+    trans_dist_vec (trans.c:111-121): d = r_j - r_i, then d -= L floor(d / L + 1/2) per
+    axis when periodic; r = sqrt(dx dx + dy dy + dz dz) in FP64, summed left to right.
+    """
+    xyz = np.ascontiguousarray(np.asarray(xyz, float)[:, :3])
+    N = len(xyz)
+    out = []
+    inv = 1.0 / box if pbc else 0.0
+    for i in range(N - 1):
+        d = xyz[i + 1:] - xyz[i]
+        if pbc:
+            d = d - box * np.floor(d * inv + 0.5)
+        r = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+        j = np.nonzero(r <= cutoff)[0] + i + 1
+        out.append(np.stack([np.full(len(j), i), j], axis=1))
+    if not out:
+        return np.zeros((0, 2), np.int32)
+    return np.concatenate(out).astype(np.int32)

@@ -123,7 +123,7 @@ def test_forces_unscaled_bonds_interact(api):
 
 
 # ----------------------------------------------------------------------------- Cholesky + L.x
-@pytest.mark.parametrize("N", [16, 100, 300])
+@pytest.mark.parametrize("N", [16, 100, 300, 700])
 def test_factor_vs_oracle(api, N):
     S, po = api
     sysm, P = _system(N, seed=100 + N)
@@ -135,6 +135,10 @@ def test_factor_vs_oracle(api, N):
     L32 = S.BDSystem(sysm, P, precision=S.MIXED).factor()
     res = np.linalg.norm(L32 @ L32.T - D) / np.linalg.norm(D)
     assert res < 1e-6, res
+    # split-bf16 trailing updates: products to ~2^-17, FP32 accumulation
+    Ls = S.BDSystem(sysm, P, precision=S.MIXED_SPLIT).factor()
+    res = np.linalg.norm(Ls @ Ls.T - D) / np.linalg.norm(D)
+    assert res < 5e-6, res
 
 
 def test_factor_vs_reference_golden(api, golden):
@@ -152,7 +156,7 @@ def test_noise_product(api, N):
     D = po.compute_D(sysm.coord, po.params_from(P))
     Lref = po.cholesky(D)
     x = np.random.default_rng(N).normal(size=3 * N)
-    for prec, tol in ((S.FP64, 1e-12), (S.MIXED, 1e-5)):
+    for prec, tol in ((S.FP64, 1e-12), (S.MIXED, 1e-5), (S.MIXED_SPLIT, 1e-5)):
         s = S.BDSystem(sysm, P, precision=prec)
         s.factor()
         y = s.noise_product(x)
@@ -178,6 +182,12 @@ def test_host_dpotrf_dtrmv(api):
     bad[5, 5] = -1.0
     _, info = S.dpotrf(bad, "L")
     assert info == 6
+    # a pivot in a later diagonal block (factored inside the trailing-update kernel)
+    bad = D.copy()
+    bad[150, 150] = -1.0
+    for prec in (S.FP64, S.MIXED, S.MIXED_SPLIT):
+        _, info = S.dpotrf(bad, "L", precision=prec)
+        assert info == 151, (prec, info)
 
 
 @pytest.mark.parametrize("N", [20, 150])
@@ -245,6 +255,26 @@ def test_device_stream_matches_oracle_stream(api):
     assert np.max(np.abs(s.coords() - ref.coords())) < 1e-10
 
 
+@pytest.mark.parametrize("N,refactor,steps", [(33, 5, 12), (40, 4, 9)])
+def test_device_stream_batched_cycles(api, N, refactor, steps):
+    """Noise drawn for a whole refactor cycle in one launch (odd and even 3N) and L xi of
+    the cycle in one pass over L: same stream state and trajectory as step-by-step."""
+    S, po = api
+    sysm, P = _system(N, seed=8, rand_seed=4242)
+    ref = po.Sim(sysm, po.params_from(P), seed=4242, refactor_every=refactor)
+    ref.run(steps)
+    s = S.BDSystem(sysm, P, precision=S.FP64, refactor_every=refactor)
+    s.compute_hydro(steps)
+    assert s.rng_state() == ref.rng_state()
+    assert np.max(np.abs(s.coords() - ref.coords())) < 1e-10
+    # the same steps one call at a time take the unbatched path
+    s1 = S.BDSystem(sysm, P, precision=S.FP64, refactor_every=refactor)
+    for _ in range(steps):
+        s1.compute_hydro(1)
+    assert s1.rng_state() == ref.rng_state()
+    assert np.max(np.abs(s1.coords() - s.coords())) < 1e-11
+
+
 def test_no_hydro_path(api):
     S, po = api
     sysm, P = _system(25, seed=3, hydro="none")
@@ -254,3 +284,59 @@ def test_no_hydro_path(api):
     s = S.BDSystem(sysm, P, precision=S.FP64)
     s.compute_hydro(2, xi)
     assert np.max(np.abs(s.coords() - ref.coords())) < 1e-11
+
+
+# ----------------------------------------------------------------------------- TEA, Ewald
+def _bk_coord(golden):
+    return golden("beenakker12")[0]["params.coord"].reshape(-1, 4)
+
+
+@pytest.mark.parametrize("case", ["beenakker12", "beenakker12_chol"])
+def test_beenakker_D_vs_reference_golden(api, golden, case):
+    S, _ = api
+    g, sysm, P, p = golden(case)
+    coord = _bk_coord(golden)
+    Dref = g["D.D"]
+    assert _rel(S.BDSystem(sysm, P, precision=S.FP64, coord=coord).compute_D(), Dref) < 1e-12
+    assert _rel(S.BDSystem(sysm, P, precision=S.MIXED, coord=coord).compute_D(), Dref) < 1e-5
+
+
+@pytest.mark.parametrize("case", ["beenakker12", "beenakker12_chol", "geyer24"])
+def test_first_step_tea_ewald_vs_reference_golden(api, golden, case):
+    """First step with the device stream: Rf and Rs against the reference's own
+    (TEA free space, TEA + Beenakker, Cholesky + Beenakker)."""
+    S, _ = api
+    g, sysm, P, p = golden(case)
+    coord = _bk_coord(golden) if case.startswith("beenakker") else None
+    for prec, tol in ((S.FP64, 1e-10), (S.MIXED, 2e-5)):
+        s = S.BDSystem(sysm, P, precision=prec, coord=coord)
+        s.compute_hydro(1)
+        Rf, Rs = s.last_displacement()
+        assert _rel(Rs, g["step1.Rs"]) < tol, (case, prec, _rel(Rs, g["step1.Rs"]))
+        if np.abs(g["step1.Rf"]).max() > 0:
+            assert _rel(Rf, g["step1.Rf"]) < max(tol, 1e-4 if prec == S.MIXED else 0), (case, prec)
+
+
+def test_geyer_trajectory_vs_reference_golden(api, golden):
+    S, _ = api
+    g, sysm, P, p = golden("geyer24")
+    traj = g["traj.coord"].reshape(3, -1, 4)
+    s = S.BDSystem(sysm, P, precision=S.FP64)
+    for k in range(3):
+        s.compute_hydro(1)
+        assert np.max(np.abs(s.coords() - traj[k])) < 1e-10, k
+
+
+@pytest.mark.parametrize("N", [50, 130])
+def test_tea_vs_oracle_given_noise(api, N):
+    S, po = api
+    sysm, P = _system(N, seed=500 + N, hydro="geyer")
+    xi = np.random.default_rng(N).normal(size=(3, 3 * N))
+    ref = po.Sim(sysm, po.params_from(P), seed=1)
+    ref.run(3, xi)
+    for prec, tol in ((S.FP64, 1e-10), (S.MIXED, 1e-5)):
+        s = S.BDSystem(sysm, P, precision=prec)
+        s.compute_hydro(3, xi)
+        d_gpu = s.coords()[:, :3] - sysm.coord[:, :3]
+        d_ref = ref.coords()[:, :3] - sysm.coord[:, :3]
+        assert np.linalg.norm(d_gpu - d_ref) / np.linalg.norm(d_ref) < tol, prec
