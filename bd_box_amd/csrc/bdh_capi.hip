@@ -161,6 +161,10 @@ int check_flags(bdh_system* s)
     int h[3];
     BDH_HIP(hipMemcpyAsync(h, s->flags, sizeof(h), hipMemcpyDeviceToHost, s->st));
     BDH_HIP(hipStreamSynchronize(s->st));
+    if (h[1] < 0) {
+        set_error("Cholesky decomposition error: a panel did not receive its diagonal block");
+        return -3;
+    }
     if (h[1]) {
         set_error("Cholesky decomposition error: non-positive pivot at row " + std::to_string(h[1]));
         return -3;
@@ -924,6 +928,7 @@ int bdh_dpotrf(char uplo, double* A, int n, int ld, int precision)
                                    : dpotrf_impl<float>(uplo, A, n, ld, &info,
                                                         precision == BDH_MIXED_SPLIT ? SYRK_BF16X3 : SYRK_F32);
     if (rc) return rc;
+    BDH_CHECK(info >= 0, "bdh_dpotrf: a panel did not receive its diagonal block");
     return info;
 }
 

@@ -30,6 +30,7 @@
 namespace bdh {
 
 constexpr int DG_LD = CHOL_NB + 1;   // row pitch (doubles) of the diagonal block in LDS
+constexpr int SY_TILE_C = 128;       // SYRK tile edge
 constexpr int TRSM_ROWS = 64;        // rows per TRSM workgroup (4 lanes per row)
 
 // Value of v held by lane `src` (uniform), through two v_readlane_b32.
@@ -50,55 +51,87 @@ __device__ __forceinline__ double rsqrt_f64(double x)
     return y;
 }
 
+// Sub-panel B0..B0+15 of the diagonal block, one wavefront, lane = row: pivot and column
+// entries broadcast with v_readlane (lane indices are compile-time constants), the rank-1
+// updates of the sub-panel's later columns in registers.  A non-positive pivot is noted in
+// `bad` (first one) and continues as NaN.
+template <int B0>
+__device__ __forceinline__ void diag_subpanel(double* __restrict__ sD, int lane, int& bad)
+{
+    double p[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) p[t] = (lane >= B0) ? sD[lane * DG_LD + B0 + t] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        double piv = readlane_f64(p[t], B0 + t);
+        const bool ok = piv > 0.0;
+        bad = (!ok && bad < 0) ? B0 + t : bad;
+        piv = ok ? piv : __builtin_nan("");
+        const double rs = rsqrt_f64(piv);
+        const double l = (lane == B0 + t) ? piv * rs : p[t] * rs;
+        p[t] = l;
+#pragma unroll
+        for (int u = t + 1; u < 16; ++u) p[u] -= l * readlane_f64(l, B0 + u);
+    }
+    if (lane >= B0) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) sD[lane * DG_LD + B0 + t] = (B0 + t <= lane) ? p[t] : 0.0;
+    }
+}
+
+// Update of the block's rows/columns >= B0 + 16 with sub-panel B0 (16 columns) on
+// v_mfma_f64_16x16x4_f64: the lower 16x16 tiles are shared out over the four wavefronts.
+template <int B0>
+__device__ __forceinline__ void diag_update(double* __restrict__ sD, int lane, int wave)
+{
+    constexpr int r0 = B0 + 16, NT = (CHOL_NB - r0) / 16;
+    constexpr int NTILES = NT * (NT + 1) / 2;
+    for (int tt = wave; tt < NTILES; tt += 4) {
+        int ti = 0, x = tt;
+        while (x > ti) {
+            x -= ti + 1;
+            ++ti;
+        }
+        const int rb = r0 + 16 * ti, cb = r0 + 16 * x;
+        doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 16; k += 4) {
+            const double a = sD[(rb + (lane & 15)) * DG_LD + B0 + k + (lane >> 4)];
+            const double b = sD[(cb + (lane & 15)) * DG_LD + B0 + k + (lane >> 4)];
+            acc = mfma_f64_16x16x4(a, b, acc);
+        }
+        // accumulator r holds row 4 r + lane / 16, column lane % 16 of the tile
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = rb + 4 * r + (lane >> 4), col = cb + (lane & 15);
+            if (row >= col) sD[row * DG_LD + col] -= acc[r];
+        }
+    }
+}
+
 // In-place FP64 Cholesky of the 64x64 block sD[r * DG_LD + c] (lower part read; on return
 // the lower part holds L and the strict upper part zeros).  Called by all 256 threads.
-// A non-positive pivot records row c0 + j + 1 in *status (first one wins) and continues
-// with NaN, as the reference's dpotrf reports INFO.
+// A non-positive pivot records row c0 + j + 1 in *status (first one wins) and the
+// factorization continues with NaN, as the reference's dpotrf reports INFO.
 __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, int* __restrict__ status)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-#pragma unroll 1
-    for (int J = 0; J < 4; ++J) {
-        const int b0 = 16 * J;
-        if (wave == 0) {
-            double p[16];
-#pragma unroll
-            for (int t = 0; t < 16; ++t) p[t] = (lane >= b0) ? sD[lane * DG_LD + b0 + t] : 0.0;
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                double piv = readlane_f64(p[t], b0 + t);
-                if (!(piv > 0.0)) {
-                    if (lane == 0) atomicCAS(status, 0, c0 + b0 + t + 1);
-                    piv = __builtin_nan("");
-                }
-                const double rs = rsqrt_f64(piv);
-                const double l = (lane == b0 + t) ? piv * rs : p[t] * rs;
-                p[t] = l;
-#pragma unroll
-                for (int u = t + 1; u < 16; ++u) p[u] -= l * readlane_f64(l, b0 + u);
-            }
-            if (lane >= b0) {
-#pragma unroll
-                for (int t = 0; t < 16; ++t) sD[lane * DG_LD + b0 + t] = (b0 + t <= lane) ? p[t] : 0.0;
-            }
-        }
-        __syncthreads();
-        const int R = 48 - b0;   // rows (and columns) after this sub-panel
-        if (R > 0) {
-            const int r0 = b0 + 16;
-            for (int e = tid; e < R * R; e += 256) {
-                const int r = r0 + e / R, c = r0 + e % R;
-                if (c > r) continue;
-                const double* Lr = sD + r * DG_LD + b0;
-                const double* Lc = sD + c * DG_LD + b0;
-                double acc = 0.0;
-#pragma unroll
-                for (int t = 0; t < 16; ++t) acc += Lr[t] * Lc[t];
-                sD[r * DG_LD + c] -= acc;
-            }
-        }
-        __syncthreads();
-    }
+    int bad = -1;
+    if (wave == 0) diag_subpanel<0>(sD, lane, bad);
+    __syncthreads();
+    diag_update<0>(sD, lane, wave);
+    __syncthreads();
+    if (wave == 0) diag_subpanel<16>(sD, lane, bad);
+    __syncthreads();
+    diag_update<16>(sD, lane, wave);
+    __syncthreads();
+    if (wave == 0) diag_subpanel<32>(sD, lane, bad);
+    __syncthreads();
+    diag_update<32>(sD, lane, wave);
+    __syncthreads();
+    if (wave == 0) diag_subpanel<48>(sD, lane, bad);
+    __syncthreads();
+    if (wave == 0 && lane == 0 && bad >= 0) atomicCAS(status, 0, c0 + bad + 1);
 }
 
 // L (lower) of the factored block into A (as T) and, transposed to column-major with zeros
@@ -119,9 +152,10 @@ __device__ __forceinline__ void store_diag64(const double* __restrict__ sD, T* _
 template <typename T>
 __global__ void __launch_bounds__(256)
 chol_diag_kernel(T* __restrict__ A, long long ld, int c0, double* __restrict__ Lb,
-                 int* __restrict__ status)
+                 int* __restrict__ flags, int nflags, int* __restrict__ status)
 {
     __shared__ double sD[CHOL_NB * DG_LD];
+    for (int k = threadIdx.x; k < nflags; k += 256) flags[k] = 0;
     for (int e = threadIdx.x; e < CHOL_NB * CHOL_NB; e += 256) {
         const int r = e % CHOL_NB, c = e / CHOL_NB;
         sD[r * DG_LD + c] = (r >= c) ? (double)A[(long long)(c0 + c) * ld + c0 + r] : 0.0;
@@ -151,37 +185,108 @@ __device__ __forceinline__ double quad_bcast(double v, int q)
     }
 }
 
-// TRSM of the rows below panel c0: x L_kk^T = b.  Four lanes per row (a quad; lane q owns
-// columns 4m + q); column c is finished by its owner, sent to the quad through DPP, and
-// the quad subtracts its contribution from the later columns (right-looking), all in FP64.
+// TRSM of rows rb..re-1 of panel c0: x L_kk^T = b.  Four lanes per row (a quad; lane q
+// owns columns 4m + q); column c is finished by its owner, sent to the quad through DPP,
+// and the quad subtracts its contribution from the later columns (right-looking), all in
+// FP64.  sLT[c * (CHOL_NB + 1) + k] = L_kk[k][c] and s_inv = 1 / diag(L_kk) in LDS.  All 256
+// threads call it (64 rows per round).
+template <typename T>
+__device__ __forceinline__ void trsm_rows(T* __restrict__ A, long long ld, int c0, int rb, int re,
+                                          const double* __restrict__ sLT,
+                                          const double* __restrict__ s_inv)
+{
+    const int tid = threadIdx.x, q = tid & 3;
+#pragma unroll 1
+    for (int base = rb; base < re; base += TRSM_ROWS) {
+        const int r = base + (tid >> 2);
+        if (r >= re) continue;
+        T* Ar = A + (long long)c0 * ld + r;
+        double x[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) x[m] = (double)Ar[(long long)(4 * m + q) * ld];
+#pragma unroll
+        for (int c = 0; c < CHOL_NB; ++c) {
+            const int m0 = c >> 2, q0 = c & 3;
+            const double* Lc = sLT + c * (CHOL_NB + 1);
+            const double xc = quad_bcast(x[m0] * s_inv[c], q0);
+            const double upd = x[m0] - xc * Lc[4 * m0 + q];
+            x[m0] = (q == q0) ? xc : ((q > q0) ? upd : x[m0]);
+#pragma unroll
+            for (int m = m0 + 1; m < 16; ++m) x[m] -= xc * Lc[4 * m + q];
+        }
+#pragma unroll
+        for (int m = 0; m < 16; ++m) Ar[(long long)(4 * m + q) * ld] = (T)x[m];
+    }
+}
+
+// L_kk (column-major FP64 in Lb) into LDS as sLT, with the inverse diagonal.
+__device__ __forceinline__ void load_lkk(const double* __restrict__ Lb, double* __restrict__ sLT,
+                                         double* __restrict__ s_inv)
+{
+    const int tid = threadIdx.x;
+    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256)
+        sLT[(e / CHOL_NB) * (CHOL_NB + 1) + e % CHOL_NB] = Lb[e];
+    __syncthreads();
+    if (tid < CHOL_NB) s_inv[tid] = 1.0 / sLT[tid * (CHOL_NB + 2)];
+    __syncthreads();
+}
+
+// Panel c0's TRSM as its own launch (panel 0 only; the others run inside the SYRK kernel).
 template <typename T>
 __global__ void __launch_bounds__(256)
 chol_trsm_kernel(T* __restrict__ A, long long ld, int c0, int n_pad, const double* __restrict__ Lb)
 {
-    __shared__ double sLT[CHOL_NB][CHOL_NB + 1];   // sLT[c][k] = L[k][c]
+    __shared__ double sLT[CHOL_NB * (CHOL_NB + 1)];
     __shared__ double s_inv[CHOL_NB];
-    const int tid = threadIdx.x, q = tid & 3;
-    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256) sLT[e / CHOL_NB][e % CHOL_NB] = Lb[e];
-    __syncthreads();
-    if (tid < CHOL_NB) s_inv[tid] = 1.0 / sLT[tid][tid];
-    __syncthreads();
-    const int r = c0 + CHOL_NB + blockIdx.x * TRSM_ROWS + (tid >> 2);
-    if (r >= n_pad) return;
-    T* Ar = A + (long long)c0 * ld + r;
-    double x[16];
-#pragma unroll
-    for (int m = 0; m < 16; ++m) x[m] = (double)Ar[(long long)(4 * m + q) * ld];
-#pragma unroll
-    for (int c = 0; c < CHOL_NB; ++c) {
-        const int m0 = c >> 2, q0 = c & 3;
-        const double xc = quad_bcast(x[m0] * s_inv[c], q0);
-        const double upd = x[m0] - xc * sLT[c][4 * m0 + q];
-        x[m0] = (q == q0) ? xc : ((q > q0) ? upd : x[m0]);
-#pragma unroll
-        for (int m = m0 + 1; m < 16; ++m) x[m] -= xc * sLT[c][4 * m + q];
+    load_lkk(Lb, sLT, s_inv);
+    const int rb = c0 + CHOL_NB + blockIdx.x * TRSM_ROWS;
+    trsm_rows<T>(A, ld, c0, rb, min(n_pad, rb + TRSM_ROWS), sLT, s_inv);
+}
+
+// Flags of the diagonal blocks, after the FP64 blocks in the workspace: flag k is set once
+// block k's factor is in Lb (cleared by the panel-0 diagonal kernel of each factorization).
+__device__ __forceinline__ int* chol_flags(double* Lbuf, int n_pad)
+{
+    return reinterpret_cast<int*>(Lbuf + (long long)(n_pad / CHOL_NB) * CHOL_NB * CHOL_NB);
+}
+
+// Bounded wait (about a second) for a flag set by another workgroup of the same grid.
+constexpr int CHOL_SPIN = 1 << 20;
+__device__ __forceinline__ bool wait_flag(int* flag)
+{
+    for (int it = 0; it < CHOL_SPIN; ++it) {
+        if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+        __builtin_amdgcn_s_sleep(2);
     }
-#pragma unroll
-    for (int m = 0; m < 16; ++m) Ar[(long long)(4 * m + q) * ld] = (T)x[m];
+    return false;
+}
+
+// Tail of the SYRK kernels (tile column 0): the next panel's diagonal block was factored by
+// workgroup 0 (diag); every tile of column 0 then solves its own rows of that panel.
+template <typename T>
+__device__ __forceinline__ void fused_trsm(T* __restrict__ A, long long ld, int t0, int ri0, int n_pad,
+                                           double* __restrict__ Lb_next, int* __restrict__ flag,
+                                           int* __restrict__ status, bool diag, unsigned char* smem)
+{
+    double* sLT = reinterpret_cast<double*>(smem);
+    double* s_inv = sLT + CHOL_NB * (CHOL_NB + 1);
+    __shared__ int s_ok;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (diag) {
+            __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            s_ok = 1;
+        } else {
+            s_ok = wait_flag(flag) ? 1 : 0;
+            if (!s_ok) atomicCAS(status, 0, -1);
+        }
+    }
+    __syncthreads();
+    if (!s_ok) return;
+    load_lkk(Lb_next, sLT, s_inv);
+    const int rb = max(ri0, t0 + CHOL_NB);
+    trsm_rows<T>(A, ld, t0, rb, min(n_pad, ri0 + SY_TILE_C), sLT, s_inv);
 }
 
 // Tile (ti, tj), ti >= tj, of a lower trapezoid with ni tile rows, enumerated column by
@@ -204,7 +309,7 @@ __device__ __forceinline__ void trap_tile(int x, int ni, int& ti, int& tj)
 //                  (x - hi is exact in FP32), and a b = hi_a hi_b + hi_a lo_b + lo_a hi_b
 //                  on v_mfma_f32_32x32x16_bf16 with FP32 accumulation; the dropped lo_a lo_b
 //                  and the split residuals are below 2^-17 |a b|.
-constexpr int SY_TILE = 128;
+constexpr int SY_TILE = SY_TILE_C;
 __device__ __forceinline__ int ceil_div_dev(int a, int b) { return (a + b - 1) / b; }
 constexpr int SY_KC32 = 32;
 constexpr int SY_KCH = 32;                 // k per stage of the split kernel
@@ -228,10 +333,11 @@ __device__ __forceinline__ void split_bf16(float x, unsigned& hi, unsigned& lo)
     lo = bf16_bits(x - __builtin_bit_cast(float, hi << 16));
 }
 
+// Three waves per SIMD: the accumulators live in VGPRs next to the fused TRSM's registers.
 template <bool SPLIT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
 syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int t1, int n_pad,
-                double* __restrict__ Lb_next, int* __restrict__ status)
+                double* __restrict__ Lb_next, int* __restrict__ flag_next, int* __restrict__ status)
 {
     __shared__ __align__(16) unsigned char smem[SY_SMEM];
     float (*sA)[SY_TILE + 4] = reinterpret_cast<float (*)[SY_TILE + 4]>(smem);
@@ -376,6 +482,7 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
         factor_diag64(sD, t0, status);
         store_diag64<float>(sD, A, ld, t0, Lb_next);
     }
+    if (tj == 0) fused_trsm<float>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem);
 }
 
 // ------------------------------------------------------------------ FP64 SYRK (MFMA)
@@ -387,7 +494,7 @@ constexpr int SY_KC64 = 16;
 
 __global__ void __launch_bounds__(256)
 syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int t1, int n_pad,
-                double* __restrict__ Lb_next, int* __restrict__ status)
+                double* __restrict__ Lb_next, int* __restrict__ flag_next, int* __restrict__ status)
 {
     static_assert(2 * SY_KC64 * (SY_TILE + 2) * 8 <= SY_SMEM, "SYRK LDS");
     __shared__ __align__(16) unsigned char smem[SY_SMEM];
@@ -470,6 +577,7 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int
         factor_diag64(sD, t0, status);
         store_diag64<double>(sD, A, ld, t0, Lb_next);
     }
+    if (tj == 0) fused_trsm<double>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem);
 }
 
 // Two-level blocking: outer panels of CHOL_OUTER columns are factored panel by panel
@@ -480,7 +588,7 @@ constexpr int CHOL_OUTER = 256;
 
 template <typename T>
 int launch_update(T* d_A, long long ld, int kb, int K, int t0, int t1, int n_pad, double* d_Lb_next,
-                  int* d_status, int syrk_mode, hipStream_t st)
+                  int* d_flag_next, int* d_status, int syrk_mode, hipStream_t st)
 {
     const int ni = ceil_div(n_pad - t0, SY_TILE);
     const int nj = ceil_div(t1 - t0, SY_TILE);
@@ -489,48 +597,57 @@ int launch_update(T* d_A, long long ld, int kb, int K, int t0, int t1, int n_pad
     if constexpr (sizeof(T) == 4) {
         if (syrk_mode == SYRK_BF16X3)
             hipLaunchKernelGGL(syrk_f32_kernel<true>, dim3(ntiles), dim3(256), 0, st,
-                               reinterpret_cast<float*>(d_A), ld, kb, K, t0, t1, n_pad, d_Lb_next, d_status);
+                               reinterpret_cast<float*>(d_A), ld, kb, K, t0, t1, n_pad, d_Lb_next,
+                               d_flag_next, d_status);
         else
             hipLaunchKernelGGL(syrk_f32_kernel<false>, dim3(ntiles), dim3(256), 0, st,
-                               reinterpret_cast<float*>(d_A), ld, kb, K, t0, t1, n_pad, d_Lb_next, d_status);
+                               reinterpret_cast<float*>(d_A), ld, kb, K, t0, t1, n_pad, d_Lb_next,
+                               d_flag_next, d_status);
     } else {
         hipLaunchKernelGGL(syrk_f64_kernel, dim3(ntiles), dim3(256), 0, st,
-                           reinterpret_cast<double*>(d_A), ld, kb, K, t0, t1, n_pad, d_Lb_next, d_status);
+                           reinterpret_cast<double*>(d_A), ld, kb, K, t0, t1, n_pad, d_Lb_next,
+                           d_flag_next, d_status);
     }
     BDH_LAUNCH_CHECK();
     return 0;
 }
 
+// One launch per panel: the update kernel of panel k also factors diagonal block k + 1
+// (workgroup 0) and solves panel k + 1 (its tile column 0, after workgroup 0's flag).
 template <typename T>
 int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, int syrk_mode,
                 hipStream_t st, PhaseRecorder* rec)
 {
     BDH_CHECK(n_pad % PAD_TO == 0, "chol_factor: n_pad must be a multiple of 128");
     BDH_CHECK(ld >= n_pad && ld % 4 == 0, "chol_factor: bad leading dimension");
-    constexpr long long LB = (long long)CHOL_NB * CHOL_NB;
     BDH_CHECK(syrk_mode != SYRK_BF16X3 || sizeof(T) == 4, "chol_factor: split SYRK needs FP32 storage");
+    constexpr long long LB = (long long)CHOL_NB * CHOL_NB;
+    const int nblk = n_pad / CHOL_NB;
+    int* d_flags = reinterpret_cast<int*>(d_Lbuf + nblk * LB);
     if (rec) rec->begin(PH_PANEL, st);
-    hipLaunchKernelGGL(chol_diag_kernel<T>, dim3(1), dim3(256), 0, st, d_A, ld, 0, d_Lbuf, d_status);
+    hipLaunchKernelGGL(chol_diag_kernel<T>, dim3(1), dim3(256), 0, st, d_A, ld, 0, d_Lbuf, d_flags,
+                       nblk, d_status);
     BDH_LAUNCH_CHECK();
+    if (n_pad > CHOL_NB) {
+        hipLaunchKernelGGL(chol_trsm_kernel<T>, dim3(ceil_div(n_pad - CHOL_NB, TRSM_ROWS)), dim3(256),
+                           0, st, d_A, ld, 0, n_pad, (const double*)d_Lbuf);
+        BDH_LAUNCH_CHECK();
+    }
     if (rec) rec->end(PH_PANEL, st);
     for (int P0 = 0; P0 < n_pad; P0 += CHOL_OUTER) {
         const int P1 = std::min(n_pad, P0 + CHOL_OUTER);
         for (int c0 = P0; c0 < P1; c0 += CHOL_NB) {
-            const int below = n_pad - c0 - CHOL_NB;
-            if (below <= 0) return 0;
-            const long long k = c0 / CHOL_NB;
-            if (rec) rec->begin(PH_PANEL, st);
-            hipLaunchKernelGGL(chol_trsm_kernel<T>, dim3(ceil_div(below, TRSM_ROWS)), dim3(256), 0, st,
-                               d_A, ld, c0, n_pad, (const double*)(d_Lbuf + k * LB));
-            BDH_LAUNCH_CHECK();
-            if (rec) rec->end(PH_PANEL, st);
             const int t0 = c0 + CHOL_NB;
+            if (t0 >= n_pad) return 0;
+            const long long k = c0 / CHOL_NB;
             if (rec) rec->begin(PH_SYRK, st);
             // inside the outer panel: this panel's update of the panel's later columns;
             // at its end: the whole outer panel's update of the trailing matrix
             const int rc = (t0 < P1)
-                ? launch_update<T>(d_A, ld, c0, CHOL_NB, t0, P1, n_pad, d_Lbuf + (k + 1) * LB, d_status, syrk_mode, st)
-                : launch_update<T>(d_A, ld, P0, P1 - P0, t0, n_pad, n_pad, d_Lbuf + (k + 1) * LB, d_status, syrk_mode, st);
+                ? launch_update<T>(d_A, ld, c0, CHOL_NB, t0, P1, n_pad, d_Lbuf + (k + 1) * LB,
+                                   d_flags + k + 1, d_status, syrk_mode, st)
+                : launch_update<T>(d_A, ld, P0, P1 - P0, t0, n_pad, n_pad, d_Lbuf + (k + 1) * LB,
+                                   d_flags + k + 1, d_status, syrk_mode, st);
             if (rc) return rc;
             if (rec) rec->end(PH_SYRK, st);
         }
@@ -538,7 +655,11 @@ int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, 
     return 0;
 }
 
-size_t chol_lbuf_doubles(int n_pad) { return (size_t)(n_pad / CHOL_NB) * CHOL_NB * CHOL_NB; }
+size_t chol_lbuf_doubles(int n_pad)
+{
+    const size_t nblk = n_pad / CHOL_NB;
+    return nblk * CHOL_NB * CHOL_NB + (nblk + 1) / 2;   // FP64 blocks, then int flags
+}
 
 // ------------------------------------------------------------------ Y = L X (partials)
 // X holds m right-hand sides (m <= M, X[k * ldx + c]): with L fixed between refactorizations,
