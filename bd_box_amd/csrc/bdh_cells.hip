@@ -48,13 +48,16 @@ __device__ CellGrid make_grid(double lx, double ly, double lz, double hx, double
         G.h = box / n;
         G.ox = G.oy = G.oz = -0.5 * box;
     } else {
-        const double ex = hx - lx, ey = hy - ly, ez = hz - lz;
+        // extents of the finite coordinates (none finite, e.g. after a failed step: one cell)
+        double ex = hx - lx, ey = hy - ly, ez = hz - lz;
+        const bool ok = ex >= 0.0 && ey >= 0.0 && ez >= 0.0 && ex < 1e30 && ey < 1e30 && ez < 1e30;
+        if (!ok) { ex = ey = ez = 0.0; lx = ly = lz = 0.0; }
         const double h = fmax(cutoff, fmax(fmax(ex, ey), ez) / (maxdim - 1));
         G.h = h;
         G.ox = lx; G.oy = ly; G.oz = lz;
-        G.nx = min(maxdim, (int)(ex / h) + 1);
-        G.ny = min(maxdim, (int)(ey / h) + 1);
-        G.nz = min(maxdim, (int)(ez / h) + 1);
+        G.nx = max(1, min(maxdim, (int)(ex / h) + 1));
+        G.ny = max(1, min(maxdim, (int)(ey / h) + 1));
+        G.nz = max(1, min(maxdim, (int)(ez / h) + 1));
     }
     G.inv_h = 1.0 / G.h;
     G.ncell = G.nx * G.ny * G.nz;
@@ -90,14 +93,13 @@ cell_setup_kernel(const double4* __restrict__ pos, int N, double cutoff, int pbc
                                cutoff, pbc, box, CELL_MAX_DIM);
 }
 
+// Cell index along one axis, always in [0, n): non-finite coordinates go to cell 0.
 __device__ __forceinline__ int cell_coord(double x, double o, double inv_h, int n, int pbc)
 {
-    int c = (int)floor((x - o) * inv_h);
-    if (pbc) {
-        c %= n;
-        return c < 0 ? c + n : c;
-    }
-    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+    double f = floor((x - o) * inv_h);
+    if (pbc) f -= (double)n * floor(f / (double)n);
+    if (!(f >= 0.0)) return 0;
+    return f >= (double)n ? n - 1 : (int)f;
 }
 
 __global__ void cell_zero_kernel(const CellGrid* __restrict__ g, int* __restrict__ count,

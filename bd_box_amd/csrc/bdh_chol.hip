@@ -134,17 +134,46 @@ __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, i
     if (wave == 0 && lane == 0 && bad >= 0) atomicCAS(status, 0, c0 + bad + 1);
 }
 
-// L (lower) of the factored block into A (as T) and, transposed to column-major with zeros
-// above the diagonal, into Lb (FP64, 64x64) for the TRSM.
+// Per diagonal block, the workspace Lb holds (FP64):
+//   Lb[0 .. 4095]      L_kk column-major, zeros above the diagonal
+//   Lb[4096 + 256 J]   inverse of its J-th 16x16 diagonal sub-block, column-major, J = 0..3
+constexpr int LB_STRIDE = CHOL_NB * CHOL_NB + 4 * 256;
+
+// FP64 copy of the factored block and the inverses of its four 16x16 diagonal sub-blocks
+// (wave J: lane c < 16 solves L_JJ y = e_c by forward substitution) into Lb.
+__device__ __forceinline__ void publish_diag64(const double* __restrict__ sD, double* __restrict__ Lb)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256) {
+        const int r = e % CHOL_NB, c = e / CHOL_NB;
+        Lb[e] = (r >= c) ? sD[r * DG_LD + c] : 0.0;
+    }
+    if (lane < 16) {
+        const int b = 16 * wave, c = lane;
+        double rd[16], y[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) rd[r] = 1.0 / sD[(b + r) * DG_LD + b + r];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            double acc = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+            for (int t = 0; t < r; ++t) acc -= sD[(b + r) * DG_LD + b + t] * y[t];
+            y[r] = acc * rd[r];
+        }
+        double* inv = Lb + CHOL_NB * CHOL_NB + 256 * wave;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) inv[c * 16 + r] = y[r];
+    }
+}
+
+// L (lower part) of the factored block into A, in T.
 template <typename T>
 __device__ __forceinline__ void store_diag64(const double* __restrict__ sD, T* __restrict__ A,
-                                             long long ld, int c0, double* __restrict__ Lb)
+                                             long long ld, int c0)
 {
     for (int e = threadIdx.x; e < CHOL_NB * CHOL_NB; e += 256) {
         const int r = e % CHOL_NB, c = e / CHOL_NB;
-        const double v = (r >= c) ? sD[r * DG_LD + c] : 0.0;
-        if (r >= c) A[(long long)(c0 + c) * ld + c0 + r] = (T)v;
-        Lb[e] = v;
+        if (r >= c) A[(long long)(c0 + c) * ld + c0 + r] = (T)sD[r * DG_LD + c];
     }
 }
 
@@ -162,7 +191,8 @@ chol_diag_kernel(T* __restrict__ A, long long ld, int c0, double* __restrict__ L
     }
     __syncthreads();
     factor_diag64(sD, c0, status);
-    store_diag64<T>(sD, A, ld, c0, Lb);
+    publish_diag64(sD, Lb);
+    store_diag64<T>(sD, A, ld, c0);
 }
 
 // Broadcast within each quad of lanes from quad lane Q (DPP quad_perm).
@@ -185,69 +215,119 @@ __device__ __forceinline__ double quad_bcast(double v, int q)
     }
 }
 
-// TRSM of rows rb..re-1 of panel c0: x L_kk^T = b.  Four lanes per row (a quad; lane q
-// owns columns 4m + q); column c is finished by its owner, sent to the quad through DPP,
-// and the quad subtracts its contribution from the later columns (right-looking), all in
-// FP64.  sLT[c * (CHOL_NB + 1) + k] = L_kk[k][c] and s_inv = 1 / diag(L_kk) in LDS.  All 256
-// threads call it (64 rows per round).
+// TRSM x L_kk^T = b for 16 rows (one wavefront), blocked by the four 16-column sub-blocks J
+// of the panel, on v_mfma_f64_16x16x4_f64:
+//   Y_J = B_J - sum_{K<J} X_K L_JK^T,   X_J = Y_J Linv_JJ^T
+// Y and X live in the MFMA result layout (accumulator r of lane l = row 4 r + l/16, column
+// l%16 of the 16x16 block); a finished block goes through LDS (pitch 17) to become an A
+// operand (lane l: row l%16, k = 4 s + l/16).  sLT[k * 64 + c] = L_kk[c][k];
+// sInv[256 J + c' * 16 + c] = Linv_JJ[c][c'].  All FP64.
+constexpr int SC_PITCH = 17;
+
+struct Trsm16 {
+    doublex4 y[4];   // B_J in, X_J out, result layout
+};
+
 template <typename T>
-__device__ __forceinline__ void trsm_rows(T* __restrict__ A, long long ld, int c0, int rb, int re,
-                                          const double* __restrict__ sLT,
-                                          const double* __restrict__ s_inv)
+__device__ __forceinline__ void trsm16_load(const T* __restrict__ A, long long ld, int t0, int g0,
+                                            int re, int lane, Trsm16& g)
 {
-    const int tid = threadIdx.x, q = tid & 3;
-#pragma unroll 1
-    for (int base = rb; base < re; base += TRSM_ROWS) {
-        const int r = base + (tid >> 2);
-        if (r >= re) continue;
-        T* Ar = A + (long long)c0 * ld + r;
-        double x[16];
+    const int lr = lane & 15, lq = lane >> 4;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) x[m] = (double)Ar[(long long)(4 * m + q) * ld];
+    for (int J = 0; J < 4; ++J) {
+        const T* col = A + (long long)(t0 + 16 * J + lr) * ld;
 #pragma unroll
-        for (int c = 0; c < CHOL_NB; ++c) {
-            const int m0 = c >> 2, q0 = c & 3;
-            const double* Lc = sLT + c * (CHOL_NB + 1);
-            const double xc = quad_bcast(x[m0] * s_inv[c], q0);
-            const double upd = x[m0] - xc * Lc[4 * m0 + q];
-            x[m0] = (q == q0) ? xc : ((q > q0) ? upd : x[m0]);
-#pragma unroll
-            for (int m = m0 + 1; m < 16; ++m) x[m] -= xc * Lc[4 * m + q];
+        for (int r = 0; r < 4; ++r) {
+            const int row = g0 + 4 * r + lq;
+            g.y[J][r] = (row < re) ? (double)col[row] : 0.0;
         }
-#pragma unroll
-        for (int m = 0; m < 16; ++m) Ar[(long long)(4 * m + q) * ld] = (T)x[m];
     }
 }
 
-// L_kk (column-major FP64 in Lb) into LDS as sLT, with the inverse diagonal.
-__device__ __forceinline__ void load_lkk(const double* __restrict__ Lb, double* __restrict__ sLT,
-                                         double* __restrict__ s_inv)
+template <typename T>
+__device__ __forceinline__ void trsm16_solve_store(T* __restrict__ A, long long ld, int t0, int g0,
+                                                   int re, int lane, Trsm16& g,
+                                                   const double* __restrict__ sLT,
+                                                   const double* __restrict__ sInv,
+                                                   double* __restrict__ sc)
 {
-    const int tid = threadIdx.x;
-    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256)
-        sLT[(e / CHOL_NB) * (CHOL_NB + 1) + e % CHOL_NB] = Lb[e];
-    __syncthreads();
-    if (tid < CHOL_NB) s_inv[tid] = 1.0 / sLT[tid * (CHOL_NB + 2)];
+    const int lr = lane & 15, lq = lane >> 4;
+    double xa[4][4];   // A-operand form of the finished X_K: xa[K][s] = X_K[lr][4 s + lq]
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+        doublex4 yj = g.y[J];
+#pragma unroll
+        for (int K = 0; K < J; ++K)
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const double b = sLT[(16 * K + 4 * s4 + lq) * CHOL_NB + 16 * J + lr];
+                yj = mfma_f64_16x16x4(-xa[K][s4], b, yj);
+            }
+        // Y_J into A-operand form
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sc[(4 * r + lq) * SC_PITCH + lr] = yj[r];
+        __builtin_amdgcn_wave_barrier();
+        double ya[4];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) ya[s4] = sc[lr * SC_PITCH + 4 * s4 + lq];
+        __builtin_amdgcn_wave_barrier();
+        doublex4 xj = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+            xj = mfma_f64_16x16x4(ya[s4], sInv[256 * J + (4 * s4 + lq) * 16 + lr], xj);
+        T* col = A + (long long)(t0 + 16 * J + lr) * ld;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = g0 + 4 * r + lq;
+            if (row < re) col[row] = (T)xj[r];
+        }
+        if (J < 3) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sc[(4 * r + lq) * SC_PITCH + lr] = xj[r];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) xa[J][s4] = sc[lr * SC_PITCH + 4 * s4 + lq];
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+// L_kk and its diagonal sub-block inverses (Lb, FP64) into LDS.
+__device__ __forceinline__ void load_lkk(const double* __restrict__ Lb, double* __restrict__ sLT,
+                                         double* __restrict__ sInv)
+{
+    // Lb is column-major (Lb[k * CHOL_NB + c] = L_kk[c][k]), which is the sLT layout
+    for (int e = threadIdx.x; e < CHOL_NB * CHOL_NB; e += 256) sLT[e] = Lb[e];
+    for (int e = threadIdx.x; e < 4 * 256; e += 256) sInv[e] = Lb[CHOL_NB * CHOL_NB + e];
     __syncthreads();
 }
 
-// Panel c0's TRSM as its own launch (panel 0 only; the others run inside the SYRK kernel).
+// LDS of a TRSM workgroup: sLT, sInv, and 16 x SC_PITCH doubles per wavefront.
+constexpr int TRSM_LDS_DOUBLES = CHOL_NB * CHOL_NB + 4 * 256 + 4 * 16 * SC_PITCH;
+
+// Panel c0's TRSM as its own launch (panel 0 only; the others run inside the SYRK kernel):
+// 64 rows per workgroup, 16 per wavefront.
 template <typename T>
 __global__ void __launch_bounds__(256)
 chol_trsm_kernel(T* __restrict__ A, long long ld, int c0, int n_pad, const double* __restrict__ Lb)
 {
-    __shared__ double sLT[CHOL_NB * (CHOL_NB + 1)];
-    __shared__ double s_inv[CHOL_NB];
-    load_lkk(Lb, sLT, s_inv);
-    const int rb = c0 + CHOL_NB + blockIdx.x * TRSM_ROWS;
-    trsm_rows<T>(A, ld, c0, rb, min(n_pad, rb + TRSM_ROWS), sLT, s_inv);
+    __shared__ double smem[TRSM_LDS_DOUBLES];
+    double* sLT = smem;
+    double* sInv = sLT + CHOL_NB * CHOL_NB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* sc = sInv + 4 * 256 + wave * 16 * SC_PITCH;
+    const int g0 = c0 + CHOL_NB + blockIdx.x * TRSM_ROWS + 16 * wave;
+    Trsm16 g;
+    trsm16_load<T>(A, ld, c0, g0, n_pad, lane, g);
+    load_lkk(Lb, sLT, sInv);
+    if (g0 < n_pad) trsm16_solve_store<T>(A, ld, c0, g0, n_pad, lane, g, sLT, sInv, sc);
 }
 
 // Flags of the diagonal blocks, after the FP64 blocks in the workspace: flag k is set once
 // block k's factor is in Lb (cleared by the panel-0 diagonal kernel of each factorization).
 __device__ __forceinline__ int* chol_flags(double* Lbuf, int n_pad)
 {
-    return reinterpret_cast<int*>(Lbuf + (long long)(n_pad / CHOL_NB) * CHOL_NB * CHOL_NB);
+    return reinterpret_cast<int*>(Lbuf + (long long)(n_pad / CHOL_NB) * LB_STRIDE);
 }
 
 // Bounded wait (about a second) for a flag set by another workgroup of the same grid.
@@ -262,20 +342,37 @@ __device__ __forceinline__ bool wait_flag(int* flag)
 }
 
 // Tail of the SYRK kernels (tile column 0): the next panel's diagonal block was factored by
-// workgroup 0 (diag); every tile of column 0 then solves its own rows of that panel.
+// workgroup 0 (diag); every tile of column 0 then solves its own rows of that panel, 16 rows
+// per wavefront at a time.  The rows are loaded before the wait (this workgroup wrote
+// them), so the wait hides the loads.
 template <typename T>
 __device__ __forceinline__ void fused_trsm(T* __restrict__ A, long long ld, int t0, int ri0, int n_pad,
                                            double* __restrict__ Lb_next, int* __restrict__ flag,
-                                           int* __restrict__ status, bool diag, unsigned char* smem)
+                                           int* __restrict__ status, bool diag, unsigned char* smem,
+                                           const double* __restrict__ sD)
 {
     double* sLT = reinterpret_cast<double*>(smem);
-    double* s_inv = sLT + CHOL_NB * (CHOL_NB + 1);
+    double* sInv = sLT + CHOL_NB * CHOL_NB;
     __shared__ int s_ok;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double* sc = sInv + 4 * 256 + wave * 16 * SC_PITCH;
+    const int rb = max(ri0, t0 + CHOL_NB), re = min(n_pad, ri0 + SY_TILE_C);
+    const int g0 = rb + 16 * wave, g1 = g0 + 64;   // this wavefront's two 16-row groups
+    if (diag) {
+        // publish the FP64 block first; the T copy in A is not read by this kernel
+        publish_diag64(sD, Lb_next);
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        store_diag64<T>(sD, A, ld, t0);
+    }
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) {
+    Trsm16 x0, x1;
+    if (g0 < re) trsm16_load<T>(A, ld, t0, g0, re, lane, x0);
+    if (g1 < re) trsm16_load<T>(A, ld, t0, g1, re, lane, x1);
+    if (tid == 0) {
         if (diag) {
-            __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             s_ok = 1;
         } else {
             s_ok = wait_flag(flag) ? 1 : 0;
@@ -284,9 +381,9 @@ __device__ __forceinline__ void fused_trsm(T* __restrict__ A, long long ld, int 
     }
     __syncthreads();
     if (!s_ok) return;
-    load_lkk(Lb_next, sLT, s_inv);
-    const int rb = max(ri0, t0 + CHOL_NB);
-    trsm_rows<T>(A, ld, t0, rb, min(n_pad, ri0 + SY_TILE_C), sLT, s_inv);
+    load_lkk(Lb_next, sLT, sInv);
+    if (g0 < re) trsm16_solve_store<T>(A, ld, t0, g0, re, lane, x0, sLT, sInv, sc);
+    if (g1 < re) trsm16_solve_store<T>(A, ld, t0, g1, re, lane, x1, sLT, sInv, sc);
 }
 
 // Tile (ti, tj), ti >= tj, of a lower trapezoid with ni tile rows, enumerated column by
@@ -318,8 +415,10 @@ constexpr int SY_HP = SY_KCH / 2 + 4;      // row pitch of a split stage, in 32-
 // LDS of the SYRK kernels: the operand stages, reused for the next diagonal block.
 constexpr int SY_SMEM_F32 = 2 * SY_KC32 * (SY_TILE + 4) * 4;
 constexpr int SY_SMEM_SPLIT = 4 * SY_TILE * SY_HP * 4;
-constexpr int SY_SMEM = SY_SMEM_SPLIT > SY_SMEM_F32 ? SY_SMEM_SPLIT : SY_SMEM_F32;
+constexpr int SY_SMEM_MAX2 = SY_SMEM_SPLIT > SY_SMEM_F32 ? SY_SMEM_SPLIT : SY_SMEM_F32;
+constexpr int SY_SMEM = SY_SMEM_MAX2 > TRSM_LDS_DOUBLES * 8 ? SY_SMEM_MAX2 : TRSM_LDS_DOUBLES * 8;
 static_assert(SY_SMEM >= CHOL_NB * DG_LD * 8, "diagonal block does not fit the SYRK LDS");
+static_assert(SY_SMEM >= TRSM_LDS_DOUBLES * 8, "TRSM operands do not fit the SYRK LDS");
 
 // bf16 bits of x, round to nearest even (finite inputs)
 __device__ __forceinline__ unsigned bf16_bits(float x)
@@ -448,8 +547,24 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
         }
     }
     if (diag) __syncthreads();   // operand stages are reused for the diagonal block
-    // C -= acc ; lane owns column l32 and rows 8*(r/4) + 4*lh + (0..3) of each 32x32 block
+    // C -= acc ; lane owns column l32 and rows 8*(r/4) + 4*lh + (0..3) of each 32x32 block.
+    // All 16 loads of C are issued before any store (the stores could alias them as far as
+    // the compiler knows, which would serialise every load behind the previous store).
     const bool to_lds = diag && wave == 0;
+    float4 cv[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int col = rj0 + wc * 64 + b * 32 + l32;
+            const float* cp = A + (long long)col * ld;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = ri0 + wr * 64 + a * 32 + 8 * q + 4 * lh;
+                cv[a][b][q] = (col < t1 && row < n_pad) ? *reinterpret_cast<const float4*>(cp + row)
+                                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -461,7 +576,7 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
             for (int q = 0; q < 4; ++q) {
                 const int row = ri0 + wr * 64 + a * 32 + 8 * q + 4 * lh;
                 if (row >= n_pad) continue;
-                float4 c = *reinterpret_cast<float4*>(cp + row);
+                float4 c = cv[a][b][q];
                 c.x -= acc[a][b][4 * q + 0];
                 c.y -= acc[a][b][4 * q + 1];
                 c.z -= acc[a][b][4 * q + 2];
@@ -480,9 +595,8 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
     if (diag) {
         __syncthreads();
         factor_diag64(sD, t0, status);
-        store_diag64<float>(sD, A, ld, t0, Lb_next);
     }
-    if (tj == 0) fused_trsm<float>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem);
+    if (tj == 0) fused_trsm<float>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem, sD);
 }
 
 // ------------------------------------------------------------------ FP64 SYRK (MFMA)
@@ -547,8 +661,21 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int
         }
     }
     if (diag) __syncthreads();   // operand stages are reused for the diagonal block
-    // lane owns column l16 and rows 4*lq + (0..3) of each 16x16 block
+    // lane owns column l16 and rows 4*lq + (0..3) of each 16x16 block; every load of C is
+    // issued before the stores
     const bool to_lds = diag && wave == 0;
+    double2 cv[4][4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int col = rj0 + wc * 64 + b * 16 + l16;
+            const int row = ri0 + wr * 64 + a * 16 + 4 * lq;
+            const double* cp = A + (long long)col * ld + row;
+            const bool in = col < t1 && row < n_pad;
+            cv[a][b][0] = in ? *reinterpret_cast<const double2*>(cp) : make_double2(0.0, 0.0);
+            cv[a][b][1] = in ? *reinterpret_cast<const double2*>(cp + 2) : make_double2(0.0, 0.0);
+        }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -557,8 +684,8 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int
             const int row = ri0 + wr * 64 + a * 16 + 4 * lq;
             if (col >= t1 || row >= n_pad) continue;
             double* cp = A + (long long)col * ld + row;
-            double2 c0v = *reinterpret_cast<double2*>(cp);
-            double2 c1v = *reinterpret_cast<double2*>(cp + 2);
+            double2 c0v = cv[a][b][0];
+            double2 c1v = cv[a][b][1];
             c0v.x -= acc[a][b][0]; c0v.y -= acc[a][b][1];
             c1v.x -= acc[a][b][2]; c1v.y -= acc[a][b][3];
             if (to_lds) {
@@ -575,9 +702,8 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int
     if (diag) {
         __syncthreads();
         factor_diag64(sD, t0, status);
-        store_diag64<double>(sD, A, ld, t0, Lb_next);
     }
-    if (tj == 0) fused_trsm<double>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem);
+    if (tj == 0) fused_trsm<double>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem, sD);
 }
 
 // Two-level blocking: outer panels of CHOL_OUTER columns are factored panel by panel
@@ -621,7 +747,7 @@ int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, 
     BDH_CHECK(n_pad % PAD_TO == 0, "chol_factor: n_pad must be a multiple of 128");
     BDH_CHECK(ld >= n_pad && ld % 4 == 0, "chol_factor: bad leading dimension");
     BDH_CHECK(syrk_mode != SYRK_BF16X3 || sizeof(T) == 4, "chol_factor: split SYRK needs FP32 storage");
-    constexpr long long LB = (long long)CHOL_NB * CHOL_NB;
+    constexpr long long LB = LB_STRIDE;
     const int nblk = n_pad / CHOL_NB;
     int* d_flags = reinterpret_cast<int*>(d_Lbuf + nblk * LB);
     if (rec) rec->begin(PH_PANEL, st);
@@ -658,7 +784,7 @@ int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, 
 size_t chol_lbuf_doubles(int n_pad)
 {
     const size_t nblk = n_pad / CHOL_NB;
-    return nblk * CHOL_NB * CHOL_NB + (nblk + 1) / 2;   // FP64 blocks, then int flags
+    return nblk * LB_STRIDE + (nblk + 1) / 2;   // FP64 blocks with inverses, then int flags
 }
 
 // ------------------------------------------------------------------ Y = L X (partials)
