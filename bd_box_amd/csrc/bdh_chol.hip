@@ -185,9 +185,16 @@ chol_diag_kernel(T* __restrict__ A, long long ld, int c0, double* __restrict__ L
 {
     __shared__ double sD[CHOL_NB * DG_LD];
     for (int k = threadIdx.x; k < nflags; k += 256) flags[k] = 0;
-    for (int e = threadIdx.x; e < CHOL_NB * CHOL_NB; e += 256) {
-        const int r = e % CHOL_NB, c = e / CHOL_NB;
-        sD[r * DG_LD + c] = (r >= c) ? (double)A[(long long)(c0 + c) * ld + c0 + r] : 0.0;
+    double v[CHOL_NB * CHOL_NB / 256];
+#pragma unroll
+    for (int i = 0; i < CHOL_NB * CHOL_NB / 256; ++i) {
+        const int e = threadIdx.x + 256 * i, r = e % CHOL_NB, c = e / CHOL_NB;
+        v[i] = (r >= c) ? (double)A[(long long)(c0 + c) * ld + c0 + r] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < CHOL_NB * CHOL_NB / 256; ++i) {
+        const int e = threadIdx.x + 256 * i, r = e % CHOL_NB, c = e / CHOL_NB;
+        sD[r * DG_LD + c] = v[i];
     }
     __syncthreads();
     factor_diag64(sD, c0, status);
@@ -296,9 +303,17 @@ __device__ __forceinline__ void trsm16_solve_store(T* __restrict__ A, long long 
 __device__ __forceinline__ void load_lkk(const double* __restrict__ Lb, double* __restrict__ sLT,
                                          double* __restrict__ sInv)
 {
-    // Lb is column-major (Lb[k * CHOL_NB + c] = L_kk[c][k]), which is the sLT layout
-    for (int e = threadIdx.x; e < CHOL_NB * CHOL_NB; e += 256) sLT[e] = Lb[e];
-    for (int e = threadIdx.x; e < 4 * 256; e += 256) sInv[e] = Lb[CHOL_NB * CHOL_NB + e];
+    // Lb is column-major (Lb[k * CHOL_NB + c] = L_kk[c][k]), which is the sLT layout.  All
+    // loads are issued before the LDS stores (which the compiler would otherwise order
+    // between them).
+    constexpr int NL = CHOL_NB * CHOL_NB / 256, NI = 4;
+    double v[NL + NI];
+#pragma unroll
+    for (int i = 0; i < NL + NI; ++i) v[i] = Lb[threadIdx.x + 256 * i];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) sLT[threadIdx.x + 256 * i] = v[i];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) sInv[threadIdx.x + 256 * i] = v[NL + i];
     __syncthreads();
 }
 
