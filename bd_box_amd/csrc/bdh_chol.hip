@@ -487,11 +487,15 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
         const int srow = isB ? rj0 + tid - SY_TILE : ri0 + tid;
         const int lrow = tid & (SY_TILE - 1);
         const bool live = srow < n_pad;
-        for (int k0 = 0; k0 < K; k0 += SY_KCH) {
-            float v[SY_KCH];
-            const float* src = A + (long long)(kb + k0) * ld + srow;
+        // the next chunk's loads are issued before this chunk's MFMAs (register double
+        // buffering: v is converted into LDS before it is refilled)
+        float v[SY_KCH];
+        {
+            const float* src = A + (long long)kb * ld + srow;
 #pragma unroll
             for (int k = 0; k < SY_KCH; ++k) v[k] = live ? src[(long long)k * ld] : 0.f;
+        }
+        for (int k0 = 0; k0 < K; k0 += SY_KCH) {
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < SY_KCH; k += 8) {
@@ -506,6 +510,11 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
                 *reinterpret_cast<uint4*>(&sH[isB ? 3 : 1][lrow][k / 2]) = lw;
             }
             __syncthreads();
+            if (k0 + SY_KCH < K) {
+                const float* src = A + (long long)(kb + k0 + SY_KCH) * ld + srow;
+#pragma unroll
+                for (int k = 0; k < SY_KCH; ++k) v[k] = live ? src[(long long)k * ld] : 0.f;
+            }
 #pragma unroll
             for (int t = 0; t < SY_KCH / 16; ++t) {
                 const int w = 8 * t + 4 * lh;   // lane holds k = 16 t + 8 lh + 0..7
