@@ -53,26 +53,36 @@ __device__ __forceinline__ double rsqrt_f64(double x)
 
 // Sub-panel B0..B0+15 of the diagonal block, one wavefront, lane = row: pivot and column
 // entries broadcast with v_readlane (lane indices are compile-time constants), the rank-1
-// updates of the sub-panel's later columns in registers.  A non-positive pivot is noted in
-// `bad` (first one) and continues as NaN.
+// updates of the sub-panel's later columns in registers.  The chain from one pivot to the
+// next is kept short: pivot t+1 is formed from its two pre-update entries,
+//   d_{t+1} = a_{t+1,t+1} - (a_{t+1,t} / sqrt(d_t))^2,
+// with the same FMA the owning lane applies to its own entry, so both agree bit for bit.
+// A non-positive (or NaN) pivot propagates NaN; the first such row is noted in `bad`.
 template <int B0>
 __device__ __forceinline__ void diag_subpanel(double* __restrict__ sD, int lane, int& bad)
 {
     double p[16];
 #pragma unroll
     for (int t = 0; t < 16; ++t) p[t] = (lane >= B0) ? sD[lane * DG_LD + B0 + t] : 0.0;
+    double piv = readlane_f64(p[0], B0);
+    double mydiag = 1.0;   // this lane's pivot, if it owns one in the sub-panel
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-        double piv = readlane_f64(p[t], B0 + t);
-        const bool ok = piv > 0.0;
-        bad = (!ok && bad < 0) ? B0 + t : bad;
-        piv = ok ? piv : __builtin_nan("");
+        mydiag = (lane == B0 + t) ? piv : mydiag;
         const double rs = rsqrt_f64(piv);
+        double nxt = 0.0;
+        if (t < 15) {
+            const double le = readlane_f64(p[t], B0 + t + 1) * rs;
+            nxt = __builtin_fma(-le, le, readlane_f64(p[t + 1], B0 + t + 1));
+        }
         const double l = (lane == B0 + t) ? piv * rs : p[t] * rs;
         p[t] = l;
 #pragma unroll
-        for (int u = t + 1; u < 16; ++u) p[u] -= l * readlane_f64(l, B0 + u);
+        for (int u = t + 1; u < 16; ++u) p[u] = __builtin_fma(-l, readlane_f64(l, B0 + u), p[u]);
+        piv = nxt;
     }
+    const unsigned long long fail = __ballot(lane >= B0 && lane < B0 + 16 && !(mydiag > 0.0));
+    if (bad < 0 && fail) bad = __builtin_ctzll(fail);
     if (lane >= B0) {
 #pragma unroll
         for (int t = 0; t < 16; ++t) sD[lane * DG_LD + B0 + t] = (B0 + t <= lane) ? p[t] : 0.0;
