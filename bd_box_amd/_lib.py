@@ -22,7 +22,7 @@ EXPORTS = [
     "bdh_get_coords", "bdh_set_coords", "bdh_get_forces", "bdh_get_last_displacement",
     "bdh_get_rng_state", "bdh_set_rng_state", "bdh_dpotrf", "bdh_dtrmv", "bdh_gauss",
     "bdh_seed_state", "bdh_probe_mfma", "bdh_time_factor", "bdh_time_steps", "bdh_sync",
-    "bdh_phase_timing", "bdh_phase_times", "bdh_neighbour_pairs",
+    "bdh_phase_timing", "bdh_phase_times", "bdh_neighbour_pairs", "bdh_last_dt",
 ]
 
 
@@ -35,7 +35,8 @@ class Config(C.Structure):
         ("bond_c_scale", C.c_double), ("check_overlap", C.c_int), ("bc", C.c_int),
         ("box", C.c_double), ("hydro", C.c_int), ("ewald_method", C.c_int),
         ("ewald_real", C.c_int), ("precision", C.c_int), ("refactor_every", C.c_int),
-        ("rand_seed", C.c_uint32), ("nb_list", C.c_int),
+        ("rand_seed", C.c_uint32), ("nb_list", C.c_int), ("algorithm", C.c_int),
+        ("move_attempts", C.c_int),
     ]
 
 
@@ -72,6 +73,7 @@ def load():
     L.bdh_set_coords.argtypes = [C.c_void_p, _dp]
     L.bdh_get_forces.argtypes = [C.c_void_p, _dp]
     L.bdh_get_last_displacement.argtypes = [C.c_void_p, _dp, _dp]
+    L.bdh_last_dt.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.bdh_get_rng_state.argtypes = [C.c_void_p, _u64p]
     L.bdh_set_rng_state.argtypes = [C.c_void_p, C.c_uint64]
     L.bdh_dpotrf.argtypes = [C.c_char, _dp, C.c_int, C.c_int, C.c_int]

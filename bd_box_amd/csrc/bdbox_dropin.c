@@ -13,7 +13,8 @@
  * Compiled against the reference's headers (it is the binding a BD_BOX maintainer adds);
  * see oracle/Makefile target `dropin` for the link line.
  *
- * Supported: algorithm ermak_const / ermak_var without rejections, hydro none / cholesky /
+ * Supported: algorithm ermak_const / ermak_var (with the reference's rejection of steps that
+ * overlap when check_overlap is set, diff_comp.c:152-179), hydro none / cholesky /
  * geyer, bc none or periodic (cube; Beenakker-Ewald tensor with hydrodynamics), nb_list
  * brute or either bucket method (both map to the backend's device cell list, which finds
  * the same interacting pairs), FP64 or mixed precision
@@ -130,6 +131,7 @@ void init_diff_tensor(void)
     cfg.cutoff_c = cutoff_c; cfg.bond_lj_scale = bond_lj_scale; cfg.bond_c_scale = bond_c_scale;
     cfg.check_overlap = check_overlap; cfg.bc = bc == BC_BOX; cfg.box = box[0];
     cfg.nb_list = nb_list;
+    cfg.algorithm = algorithm; cfg.move_attempts = move_attempts;
     cfg.hydro = hydro; cfg.ewald_method = ewald_method; cfg.ewald_real = ewald_real;
     cfg.precision = (prec && !strcmp(prec, "mixed")) ? BDH_MIXED : BDH_FP64;
     cfg.refactor_every = refac ? atoi(refac) : 1;
@@ -200,14 +202,17 @@ static int output_step(INT s)
            (save_rst_freq && s % save_rst_freq == 0) || (save_enr_freq && s % save_enr_freq == 0);
 }
 
-/* compute_hydro (diff_comp.c:210-232) for ermak_const: one step on the device; positions,
-   forces and the random-stream state come back to the host on output steps only */
+/* compute_hydro (diff_comp.c:210-232) for ermak_const / ermak_var: one step on the device,
+   rejected attempts redrawn there; positions, forces and the random-stream state come back
+   to the host on output steps only */
 void compute_hydro(DOUBLE* _E)
 {
     uint64_t st;
-    curr_dt = dt;
+    double step_dt;
     curr_iter++;
     if (bdh_step(g_sys, 1, NULL)) fail("step");
+    if (bdh_last_dt(g_sys, &step_dt)) fail("step dt");
+    curr_dt = step_dt;
     curr_time += curr_dt;
     if (output_step(step) || step + 1 >= bdsteps)
     {

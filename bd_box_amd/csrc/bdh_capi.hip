@@ -126,6 +126,9 @@ struct bdh_system {
     double *Pf = nullptr, *Pr = nullptr, *Rf = nullptr, *Rs = nullptr, *noise = nullptr;
     double* RsB = nullptr;          // sqrt(2 dt) L xi for the steps of one refactor cycle
     const double* last_Rs = nullptr;  // Rs of the last step taken
+    double last_dt = 0;             // dt of the last step taken (halved by ermak_var retries)
+    double4* pos_prev = nullptr;    // positions and forces before a step that may be rejected
+    double* F_prev = nullptr;
     long long ldx = 0;              // stride of drawn noise vectors: n, or n + 1 if n is odd
     int msplits = 0, tblocks = 0;
     bool ewald = false, tea = false;
@@ -267,13 +270,14 @@ int assemble_and_factor(bdh_system* s)
 }
 
 // Pair sums over blocks evaluated on the fly: D.F, and for TEA also D.X, C and eps.
-int matvec_dev(bdh_system* s, const double* xi = nullptr, bool tea = false)
+int matvec_dev(bdh_system* s, const double* xi = nullptr, bool tea = false, double dt = 0.0)
 {
     if (ensure_ewald(s)) return -1;
+    if (dt == 0.0) dt = s->cfg.dt;
     if (s->mixed)
-        return launch_pair_sums<float>(s->pos, s->F, xi, s->N, s->preDii, s->preDij, s->cfg.dt, s->ewald,
+        return launch_pair_sums<float>(s->pos, s->F, xi, s->N, s->preDii, s->preDij, dt, s->ewald,
                                        tea, s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st);
-    return launch_pair_sums<double>(s->pos, s->F, xi, s->N, s->preDii, s->preDij, s->cfg.dt, s->ewald,
+    return launch_pair_sums<double>(s->pos, s->F, xi, s->N, s->preDii, s->preDij, dt, s->ewald,
                                     tea, s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st);
 }
 
@@ -286,23 +290,22 @@ int trmv_dev(bdh_system* s, const double* X, long long ldx, int m)
 
 // Rs of m consecutive steps, sqrt(2 dt) L [xi_1 .. xi_m], in one pass over L (the factor
 // is fixed within a refactor cycle); stacked into RsB[j n + r].
-int cycle_noise(bdh_system* s, const double* X, long long ldx, int m)
+int cycle_noise(bdh_system* s, const double* X, long long ldx, int m, double dt)
 {
     REC_BEGIN(PH_TRMV);
     if (trmv_dev(s, X, ldx, m)) return -1;
     ReduceJobs jobs{};
     jobs.job[0] = ReduceJob{s->Pr, s->RsB, m * s->n, s->tblocks, trmv_blk_cols(),
-                            std::sqrt(2.0 * s->cfg.dt), s->n};
+                            std::sqrt(2.0 * dt), s->n};
     if (launch_reduce_partials(jobs, 1, m * s->n, s->st)) return -1;
     REC_END(PH_TRMV);
     return 0;
 }
 
 // One E-M step.  Cholesky: Rs_dev = this step's precomputed sqrt(2 dt) L xi.
-int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev = nullptr)
+int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev, double dt)
 {
     const bdh_config& c = s->cfg;
-    const double dt = c.dt;
     const double rf_scale = dt / (kB * c.T);
     if (c.hydro == BDH_HYDRO_CHOLESKY) {
         REC_BEGIN(PH_MATVEC);
@@ -318,7 +321,7 @@ int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev = nullptr
     } else if (c.hydro == BDH_HYDRO_GEYER) {
         // TEA-HI: one pass over the pairs gives D.F, D.X, C and eps (diff_rand.c:86-230)
         REC_BEGIN(PH_MATVEC);
-        if (matvec_dev(s, xi_dev, true)) return -1;
+        if (matvec_dev(s, xi_dev, true, dt)) return -1;
         REC_END(PH_MATVEC);
         REC_BEGIN(PH_MOVE);
         ReduceJobs jobs{};
@@ -340,7 +343,6 @@ int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev = nullptr
         REC_END(PH_MOVE);
     }
     if (compute_forces_dev(s)) return -1;
-    s->steps_done++;
     return 0;
 }
 
@@ -357,11 +359,57 @@ int draw_noise(bdh_system* s, int m)
 
 // steps E-M steps; have_xi: the caller's Gaussians are in s->xi ([step][n]).  Steps are
 // taken in chunks of up to TRMV_MAX_RHS that never cross a refactorization.
+// One step with the reference's rejection loop (comp_step_ermak, diff_comp.c:152-179): when
+// the forces at the new positions report an overlap, the positions and forces are restored
+// and the step is redrawn with new Gaussians (ermak_var also halves dt, at most 16 times);
+// more than move_attempts rejections are fatal.  The factor of the step is kept (the
+// reference refactors only when curr_iter changes).
+int step_with_retries(bdh_system* s, const double* xi_given, bool refactor)
+{
+    const bool chol = s->cfg.hydro == BDH_HYDRO_CHOLESKY;
+    if (chol && refactor && assemble_and_factor(s)) return -1;
+    BDH_HIP(hipMemcpyAsync(s->pos_prev, s->pos, sizeof(double4) * s->N, hipMemcpyDeviceToDevice, s->st));
+    BDH_HIP(hipMemcpyAsync(s->F_prev, s->F, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->st));
+    double dt = s->cfg.dt;
+    for (int count = 0;; ) {
+        // supplied Gaussians serve the first attempt; redraws come from the device stream
+        const double* X = count == 0 ? xi_given : nullptr;
+        if (!X) {
+            if (draw_noise(s, 1)) return -1;
+            X = s->xi;
+        }
+        if (chol && cycle_noise(s, X, s->n, 1, dt)) return -1;
+        BDH_HIP(hipMemsetAsync(s->flags, 0, sizeof(int), s->st));
+        if (one_step(s, X, chol ? s->RsB : nullptr, dt)) return -1;
+        int fl = 0;
+        BDH_HIP(hipMemcpyAsync(&fl, s->flags, sizeof(int), hipMemcpyDeviceToHost, s->st));
+        BDH_HIP(hipStreamSynchronize(s->st));
+        BDH_CHECK(!(fl & 2), "Overstretched bond");
+        if (!(fl & 1)) break;
+        if (s->cfg.algorithm == BDH_ERMAK_VAR && count < 16) dt /= 2;
+        BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, sizeof(double4) * s->N, hipMemcpyDeviceToDevice, s->st));
+        BDH_HIP(hipMemcpyAsync(s->F, s->F_prev, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->st));
+        ++count;
+        BDH_CHECK(count <= s->cfg.move_attempts, "Wrong conformation, can't move");
+    }
+    s->last_dt = dt;
+    s->steps_done++;
+    return 0;
+}
+
 int run_steps(bdh_system* s, int steps, bool have_xi)
 {
     const int n = s->n;
     const int R = s->cfg.refactor_every;
     const bool chol = s->cfg.hydro == BDH_HYDRO_CHOLESKY;
+    if (s->cfg.check_overlap) {
+        // rejections possible: step by step, each draw in the reference's order
+        for (int k = 0; k < steps; ++k) {
+            const bool refactor = chol && (!s->have_L || s->steps_done % R == 0);
+            if (step_with_retries(s, have_xi ? s->xi + (size_t)k * n : nullptr, refactor)) return -1;
+        }
+        return 0;
+    }
     for (int k = 0; k < steps;) {
         int m = std::min(steps - k, TRMV_MAX_RHS);
         bool refactor = false;
@@ -383,10 +431,14 @@ int run_steps(bdh_system* s, int steps, bool have_xi)
         if (chol) {
             // D and L at the positions of the cycle's first step (diff_comp.c:156)
             if (refactor && assemble_and_factor(s)) return -1;
-            if (cycle_noise(s, X, ldx, m)) return -1;
+            if (cycle_noise(s, X, ldx, m, s->cfg.dt)) return -1;
         }
-        for (int j = 0; j < m; ++j)
-            if (one_step(s, X + (size_t)j * ldx, chol ? s->RsB + (size_t)j * n : nullptr)) return -1;
+        for (int j = 0; j < m; ++j) {
+            if (one_step(s, X + (size_t)j * ldx, chol ? s->RsB + (size_t)j * n : nullptr, s->cfg.dt))
+                return -1;
+            s->steps_done++;
+        }
+        s->last_dt = s->cfg.dt;
         k += m;
     }
     return 0;
@@ -469,7 +521,7 @@ void bdh_destroy(bdh_system* s)
     dfree(s->conn_off); dfree(s->conn); dfree(s->bond_off); dfree(s->bond_p); dfree(s->bond_parm);
     dfree(s->F); dfree(s->E); dfree(s->PF); dfree(s->PE);
     dfree(s->A); dfree(s->Lbuf); dfree(s->xi); dfree(s->Pf); dfree(s->Pr); dfree(s->Rf); dfree(s->Rs);
-    dfree(s->noise); dfree(s->RsB); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
+    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
     dfree(s->Px); dfree(s->Pc); dfree(s->Pe); dfree(s->Feff); dfree(s->Cs); dfree(s->bt);
     dfree(s->cl.grid); dfree(s->cl.key); dfree(s->cl.count); dfree(s->cl.fill);
     dfree(s->cl.start); dfree(s->cl.sorted);
@@ -496,8 +548,12 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
               "bdh_create: precision");
     BDH_CHECK(cfg->dt > 0 && cfg->T > 0 && cfg->visc != 0 && cfg->vfactor != 0, "bdh_create: dt/T/visc");
     BDH_CHECK(nbonds == 0 || (bonds && bond_parms), "bdh_create: bonds");
+    BDH_CHECK(cfg->algorithm == BDH_ERMAK_CONST || cfg->algorithm == BDH_ERMAK_VAR,
+              "bdh_create: algorithm (ermak_const or ermak_var)");
+    BDH_CHECK(cfg->move_attempts >= 0, "bdh_create: move_attempts");
     s->cfg = *cfg;
     if (s->cfg.refactor_every < 1) s->cfg.refactor_every = 1;
+    s->last_dt = cfg->dt;
     BDH_HIP(hipGetDevice(&s->device));
     BDH_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     s->N = N;
@@ -605,6 +661,10 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     }
     if (dmalloc(&s->Pr, sizeof(double) * s->n * s->tblocks * TRMV_MAX_RHS)) return -1;
     if (dmalloc(&s->RsB, sizeof(double) * s->n * TRMV_MAX_RHS)) return -1;
+    if (cfg->check_overlap) {
+        if (dmalloc(&s->pos_prev, sizeof(double4) * N)) return -1;
+        if (dmalloc(&s->F_prev, sizeof(double) * s->n)) return -1;
+    }
     if (dmalloc(&s->Rf, sizeof(double) * s->n)) return -1;
     if (dmalloc(&s->Rs, sizeof(double) * s->n)) return -1;
     if (dmalloc(&s->noise, sizeof(double) * s->n)) return -1;
@@ -852,6 +912,13 @@ int bdh_get_forces(bdh_system* s, double* F)
     BDH_CHECK(s && F, "bdh_get_forces: bad arguments");
     BDH_HIP(hipMemcpyAsync(F, s->F, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
     BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int bdh_last_dt(bdh_system* s, double* dt)
+{
+    BDH_CHECK(s && dt, "bdh_last_dt: bad arguments");
+    *dt = s->last_dt;
     return 0;
 }
 

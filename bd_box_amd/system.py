@@ -27,6 +27,18 @@ MIXED = 1
 MIXED_SPLIT = 2
 
 
+def _algorithm(P: Params) -> int:
+    """ermak_const / ermak_var as given; ermak_newton takes the ermak_const path, which is
+    the same step whenever no step is rejected (comp_step_ermak_newton,
+    diff_comp.c:181-208), so it is accepted only without check_overlap."""
+    if P.algorithm in (0, 1):
+        return P.algorithm
+    if P.algorithm == 2 and not P.check_overlap:
+        return 0
+    raise NotImplementedError("HIP backend: algorithm ermak_const / ermak_var "
+                              "(ermak_newton only with check_overlap no)")
+
+
 def make_config(P: Params, precision: int = MIXED, refactor_every: int = 1) -> _lib.Config:
     return _lib.Config(
         dt=P.dt, T=P.T, visc=P.visc, vfactor=P.vfactor, alpha_lj=P.alpha_lj,
@@ -35,7 +47,9 @@ def make_config(P: Params, precision: int = MIXED, refactor_every: int = 1) -> _
         bond_lj_scale=P.bond_lj_scale, bond_c_scale=P.bond_c_scale,
         check_overlap=P.check_overlap, bc=P.bc, box=P.box, hydro=P.hydro,
         ewald_method=P.ewald_method, ewald_real=P.ewald_real, precision=precision,
-        refactor_every=refactor_every, rand_seed=P.rand_seed & 0xFFFFFFFF, nb_list=P.nb_list)
+        refactor_every=refactor_every, rand_seed=P.rand_seed & 0xFFFFFFFF, nb_list=P.nb_list,
+        algorithm=_algorithm(P),
+        move_attempts=P.move_attempts)
 
 
 class BDSystem:
@@ -136,6 +150,12 @@ class BDSystem:
         F = np.zeros(self.n)
         _lib.check(_lib.load().bdh_get_forces(self._h, _lib.dptr(F)), "get_forces")
         return F
+
+    def last_dt(self) -> float:
+        """Length of the last step taken (below dt after ermak_var rejections)."""
+        d = C.c_double()
+        _lib.check(_lib.load().bdh_last_dt(self._h, C.byref(d)), "last_dt")
+        return d.value
 
     def last_displacement(self):
         Rf, Rs = np.zeros(self.n), np.zeros(self.n)

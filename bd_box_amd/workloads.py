@@ -290,6 +290,7 @@ class Params:
     rand_seed: int
     nb_list: int
     geyer_on_the_fly: int
+    move_attempts: int = 10000
 
 
 _DEFAULTS = dict(  # code defaults of the reference registry (MAKE_STR_IN lines)
@@ -298,7 +299,7 @@ _DEFAULTS = dict(  # code defaults of the reference registry (MAKE_STR_IN lines)
     bond_lj_scale=1.0, bond_c_scale=1.0, check_overlap="yes", bc="periodic",
     hydro="none", algorithm="ermak_newton", ewald_method="smith", ewald_real=10,
     ewald_recip=10, ewald_alpha=1.77245385090551602729, xbox=0.0, rand_seed=12345,
-    nb_list="brute", geyer_on_the_fly="yes",
+    nb_list="brute", geyer_on_the_fly="yes", move_attempts=10000,
 )
 
 
@@ -331,7 +332,7 @@ def parse_params(params: Dict[str, object]) -> Params:
         hydro=hydro, algorithm=algorithm, ewald_method=enum("ewald_method"),
         ewald_real=int(raw["ewald_real"]), ewald_recip=int(raw["ewald_recip"]),
         ewald_alpha=dbl("ewald_alpha"), box=dbl("xbox"), rand_seed=int(raw["rand_seed"]),
-        nb_list=enum("nb_list"), geyer_on_the_fly=g,
+        nb_list=enum("nb_list"), geyer_on_the_fly=g, move_attempts=int(raw["move_attempts"]),
     )
 
 

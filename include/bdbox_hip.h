@@ -39,6 +39,9 @@ enum bdh_precision {
 
 enum bdh_hydro { BDH_HYDRO_NONE = 0, BDH_HYDRO_CHOLESKY = 1, BDH_HYDRO_GEYER = 2 };
 
+/* integration algorithm (src/diff_alg/diff_tensor.h:29-30) */
+enum bdh_algorithm { BDH_ERMAK_CONST = 0, BDH_ERMAK_VAR = 1 };
+
 /* Control parameters after the reference's .prm parsing (src/input.c:353-503): DOUBLE
  * keywords already rounded through float (src/input.c:933). */
 typedef struct bdh_config {
@@ -59,6 +62,9 @@ typedef struct bdh_config {
     uint32_t rand_seed;                      /* drand48 seed of the Gaussian stream      */
     int nb_list;                             /* 0 brute force; 1 bucket_space, 2 bucket_list:
                                                 device cell list (src/potentials/bucket.h:40-46) */
+    int algorithm;                           /* enum bdh_algorithm                       */
+    int move_attempts;                       /* rejections allowed per step when
+                                                check_overlap (diff_decl.c:71: 10000)    */
 } bdh_config;
 
 /* Last error of the calling thread ("" if none). */
@@ -83,6 +89,14 @@ void bdh_destroy(bdh_system* sys);
  * xi[steps * 3N] supplies them.  Stands in for compute_hydro with algorithm ermak_const
  * (src/diff_alg/diff_comp.c:152-179, 210-232; diff_equation.c:216-286; diff_rand.c:47-85). */
 int bdh_step(bdh_system* sys, int steps, const double* xi);
+/* With check_overlap, a step whose new positions overlap is undone (positions and forces
+ * restored) and redrawn with the next Gaussians of the device stream (also when xi supplied
+ * the first draw), ermak_var halving dt for each of the first 16 rejections; more than
+ * move_attempts rejections of one step fail with "Wrong conformation, can't move".  The
+ * factor of the step is reused across its attempts.  Follows comp_step_ermak
+ * (src/diff_alg/diff_comp.c:152-179).  *dt receives the length of the last step taken (the
+ * reference's curr_dt, which its caller adds to curr_time, diff_comp.c:212-216). */
+int bdh_last_dt(bdh_system* sys, double* dt);
 
 /* Assemble D at the current positions and copy it out (full symmetric matrix).
  * Stands in for compute_D / compute_single_D (src/diff_alg/diff_tensor.c:472-544, 561). */

@@ -603,8 +603,12 @@ struct orc_sim {
     orc_params p;
     double *coord, *Q, *LJ, *bond_parms, *F, *D, *L, *Rf, *Rs, *X;
     int* bonds;
+    double *coord_prev, *F_prev;
     uint64_t rng;
     int have_L;
+    int algorithm, move_attempts;   /* 0 ermak_const, 1 ermak_var; rejections allowed */
+    double last_dt;
+    long long rejections;
 };
 
 orc_sim* orc_sim_create(int N, const double* coord, const double* Q, const double* LJ,
@@ -629,6 +633,10 @@ orc_sim* orc_sim_create(int N, const double* coord, const double* Q, const doubl
     s->Rf = (double*)calloc(n, sizeof(double));
     s->Rs = (double*)calloc(n, sizeof(double));
     s->X = (double*)calloc(n, sizeof(double));
+    s->coord_prev = (double*)malloc(sizeof(double) * 4 * N);
+    s->F_prev = (double*)malloc(sizeof(double) * n);
+    s->move_attempts = 10000;   /* diff_decl.c:71 */
+    s->last_dt = p->dt;
     if (p->hydro == 1)
     {
         s->D = (double*)malloc(sizeof(double) * n * n);
@@ -640,65 +648,106 @@ orc_sim* orc_sim_create(int N, const double* coord, const double* Q, const doubl
     return s;
 }
 
+/* one attempt of a step of length dt from the current state: Rf, Rs, move, forces;
+   returns orc_forces' status (> 0 repeat, < 0 overstretched bond) */
+static int sim_attempt(orc_sim* s, double dt, const double* X_in, double preDii)
+{
+    const int N = s->N, n = 3 * N;
+    const double pre = dt / (KB * s->p.T);
+    int i;
+    /* deterministic part Rf = dt/kT D F (diff_equation.c:119-214) */
+    if (s->p.hydro == 1)
+    {
+        orc_symv_lower(n, s->D, n, s->F, s->Rf);
+        for (i = 0; i < n; ++i) s->Rf[i] *= pre;
+    }
+    else if (s->p.hydro == 2)
+        orc_rf_onfly(N, s->coord, &s->p, dt, s->F, s->Rf);
+    else
+        for (i = 0; i < n; ++i) s->Rf[i] = pre * (preDii / s->coord[4 * (i / 3) + 3]) * s->F[i];
+    /* random part (diff_rand.c:47-236) */
+    if (X_in) memcpy(s->X, X_in, sizeof(double) * n);
+    else orc_gauss(&s->rng, s->X, n);
+    if (s->p.hydro == 1)
+    {
+        double a = sqrt(2 * dt);
+        for (i = 0; i < n; ++i) s->X[i] *= a;
+        orc_trmv_lower(n, s->L, n, s->X, s->Rs);
+    }
+    else if (s->p.hydro == 2)
+        orc_geyer_rs(N, s->coord, &s->p, dt, s->X, s->Rs);
+    else
+        for (i = 0; i < n; ++i) s->Rs[i] = s->X[i] * sqrt((preDii / s->coord[4 * (i / 3) + 3]) * 2 * dt);
+    /* move (diff_equation.c:254-264) */
+    for (i = 0; i < N; ++i)
+    {
+        int k;
+        for (k = 0; k < 3; ++k)
+            s->coord[4 * i + k] = s->Rf[3 * i + k] + s->Rs[3 * i + k] + s->coord[4 * i + k];
+        if (s->p.bc == 1)
+        {
+            double inv = 1 / s->p.box;
+            for (k = 0; k < 3; ++k)
+                s->coord[4 * i + k] -= s->p.box * floor(s->coord[4 * i + k] * inv + 0.5f);
+        }
+    }
+    return orc_forces(N, s->coord, s->Q, s->LJ, s->nb, s->bonds, s->bond_parms, &s->p, s->F, NULL);
+}
+
+/* comp_step_ermak (diff_comp.c:152-179): D and its factor once per step at the step's
+   starting positions; an attempt whose forces report an overlap (check_overlap) is undone
+   (positions and forces restored, load_coord / load_forces) and redrawn with the next
+   Gaussians of the stream, ermak_var halving dt for the first 16 rejections; more than
+   move_attempts rejections fail. */
 int orc_sim_run(orc_sim* s, int steps, const double* xi)
 {
     const int N = s->N, n = 3 * N;
     double preDii, preDij;
-    const double dt = s->p.dt;
-    const double pre = dt / (KB * s->p.T);
-    int st, i;
+    int st;
     orc_prefactors(&s->p, &preDii, &preDij);
     for (st = 0; st < steps; ++st)
     {
-        /* deterministic part Rf = dt/kT D F (diff_equation.c:119-214) */
+        double dt = s->p.dt;
+        int count = 0;
         if (s->p.hydro == 1)
         {
             orc_compute_D(N, s->coord, &s->p, s->D, n);
-            orc_symv_lower(n, s->D, n, s->F, s->Rf);
-            for (i = 0; i < n; ++i) s->Rf[i] *= pre;
-        }
-        else if (s->p.hydro == 2)
-            orc_rf_onfly(N, s->coord, &s->p, dt, s->F, s->Rf);
-        else
-            for (i = 0; i < n; ++i) s->Rf[i] = pre * (preDii / s->coord[4 * (i / 3) + 3]) * s->F[i];
-        /* random part (diff_rand.c:47-236) */
-        if (xi) memcpy(s->X, xi + (size_t)st * n, sizeof(double) * n);
-        else orc_gauss(&s->rng, s->X, n);
-        if (s->p.hydro == 1)
-        {
-            double a = sqrt(2 * dt);
-            for (i = 0; i < n; ++i) s->X[i] *= a;
             if (!s->have_L || (s->steps_done % s->refactor_every) == 0)
             {
                 memcpy(s->L, s->D, sizeof(double) * (size_t)n * n);
                 if (orc_cholesky(n, s->L, n)) return 1;
                 s->have_L = 1;
             }
-            orc_trmv_lower(n, s->L, n, s->X, s->Rs);
         }
-        else if (s->p.hydro == 2)
-            orc_geyer_rs(N, s->coord, &s->p, dt, s->X, s->Rs);
-        else
-            for (i = 0; i < n; ++i) s->Rs[i] = s->X[i] * sqrt((preDii / s->coord[4 * (i / 3) + 3]) * 2 * dt);
-        /* move (diff_equation.c:254-264) */
-        for (i = 0; i < N; ++i)
+        memcpy(s->coord_prev, s->coord, sizeof(double) * 4 * N);
+        memcpy(s->F_prev, s->F, sizeof(double) * n);
+        for (;;)
         {
-            int k;
-            for (k = 0; k < 3; ++k)
-                s->coord[4 * i + k] = s->Rf[3 * i + k] + s->Rs[3 * i + k] + s->coord[4 * i + k];
-            if (s->p.bc == 1)
-            {
-                double inv = 1 / s->p.box;
-                for (k = 0; k < 3; ++k)
-                    s->coord[4 * i + k] -= s->p.box * floor(s->coord[4 * i + k] * inv + 0.5f);
-            }
+            /* supplied Gaussians serve the first attempt; redraws come from the stream */
+            const double* X = (xi && count == 0) ? xi + (size_t)st * n : NULL;
+            int rep = sim_attempt(s, dt, X, preDii);
+            if (rep < 0) return 2;
+            if (rep == 0 || !s->p.check_overlap) break;
+            if (s->algorithm == 1 && count < 16) dt /= 2;
+            memcpy(s->coord, s->coord_prev, sizeof(double) * 4 * N);
+            memcpy(s->F, s->F_prev, sizeof(double) * n);
+            if (++count > s->move_attempts) return 3;
         }
-        if (orc_forces(N, s->coord, s->Q, s->LJ, s->nb, s->bonds, s->bond_parms, &s->p, s->F, NULL) < 0)
-            return 2;
+        s->last_dt = dt;
+        s->rejections += count;
         s->steps_done++;
     }
     return 0;
 }
+
+void orc_sim_set_algorithm(orc_sim* s, int algorithm, int move_attempts)
+{
+    s->algorithm = algorithm;
+    s->move_attempts = move_attempts;
+}
+
+double orc_sim_last_dt(const orc_sim* s) { return s->last_dt; }
+long long orc_sim_rejections(const orc_sim* s) { return s->rejections; }
 
 void orc_sim_coords(const orc_sim* s, double* c) { memcpy(c, s->coord, sizeof(double) * 4 * s->N); }
 void orc_sim_forces(const orc_sim* s, double* F) { memcpy(F, s->F, sizeof(double) * 3 * s->N); }
@@ -714,5 +763,6 @@ void orc_sim_destroy(orc_sim* s)
     if (!s) return;
     free(s->coord); free(s->Q); free(s->LJ); free(s->bonds); free(s->bond_parms);
     free(s->F); free(s->D); free(s->L); free(s->Rf); free(s->Rs); free(s->X);
+    free(s->coord_prev); free(s->F_prev);
     free(s);
 }

@@ -93,9 +93,16 @@ typedef struct orc_sim orc_sim;
 orc_sim* orc_sim_create(int N, const double* coord, const double* Q, const double* LJ,
                         int nbonds, const int* bonds, const double* bond_parms,
                         const orc_params* p, uint32_t seed, int refactor_every);
-/* Advance `steps` steps; when xi != NULL the Gaussians are taken from xi[steps*3N]
-   instead of the internal drand48 stream.  Returns 0, or nonzero on failure. */
+/* Advance `steps` steps; when xi != NULL the Gaussians of each step's first attempt are
+   taken from xi[steps*3N] instead of the internal drand48 stream (redraws after a rejected
+   attempt always come from the stream).  Returns 0; 1 on a failed factorization, 2 on an
+   overstretched bond, 3 after more than move_attempts rejections of one step. */
 int  orc_sim_run(orc_sim* s, int steps, const double* xi);
+/* integration algorithm (0 ermak_const, 1 ermak_var) and the rejection limit (default
+   10000, diff_decl.c:71) */
+void orc_sim_set_algorithm(orc_sim* s, int algorithm, int move_attempts);
+double orc_sim_last_dt(const orc_sim* s);          /* dt of the last step taken */
+long long orc_sim_rejections(const orc_sim* s);    /* rejected attempts so far */
 void orc_sim_coords(const orc_sim* s, double* coord_out);
 void orc_sim_forces(const orc_sim* s, double* F_out);
 /* displacement parts of the last step: Rf (deterministic) and Rs (random) */

@@ -70,6 +70,11 @@ def lib():
         L.orc_sim_rng_state.argtypes = [C.c_void_p]
         L.orc_sim_rng_state.restype = C.c_uint64
         L.orc_sim_destroy.argtypes = [C.c_void_p]
+        L.orc_sim_set_algorithm.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_sim_last_dt.argtypes = [C.c_void_p]
+        L.orc_sim_last_dt.restype = C.c_double
+        L.orc_sim_rejections.argtypes = [C.c_void_p]
+        L.orc_sim_rejections.restype = C.c_longlong
         _lib = L
     return _lib
 
@@ -165,7 +170,8 @@ def rf_onfly(coord, p: OrcParams, dt, F):
 class Sim:
     """Stateful FP64 Ermak-McCammon integrator (orc_sim_*)."""
 
-    def __init__(self, sysm, p: OrcParams, seed: int, refactor_every: int = 1, coord=None):
+    def __init__(self, sysm, p: OrcParams, seed: int, refactor_every: int = 1, coord=None,
+                 algorithm: int = 0, move_attempts: int = 10000):
         self.N = sysm.N
         c = np.ascontiguousarray(sysm.coord if coord is None else coord, float).reshape(-1)
         self._keep = [c, np.ascontiguousarray(sysm.Q, float),
@@ -177,6 +183,7 @@ class Sim:
                                       _d(self._keep[2]), len(sysm.bonds),
                                       self._keep[3].ctypes.data_as(_ip), _d(self._keep[4]),
                                       C.byref(self.p), seed, refactor_every)
+        lib().orc_sim_set_algorithm(self.h, algorithm, move_attempts)
 
     def run(self, steps: int, xi=None):
         x = None if xi is None else np.ascontiguousarray(xi, float).reshape(-1)
@@ -201,6 +208,12 @@ class Sim:
 
     def rng_state(self):
         return lib().orc_sim_rng_state(self.h)
+
+    def last_dt(self):
+        return lib().orc_sim_last_dt(self.h)
+
+    def rejections(self):
+        return lib().orc_sim_rejections(self.h)
 
     def __del__(self):
         try:

@@ -15,7 +15,8 @@
  *   chol          lower Cholesky factor L, D = L L^T        (my_dpotrf/my_dtrmv, myblas.c:45-118)
  *   gauss M K     K successive gauss_rand(.., M) draws      (rand_move.c:132-168)
  *   step1         X (scaled noise), Rf and Rs of the first step   (diff_equation.c:216-264)
- *   traj K        positions after each of K steps           (main_loop.c:91-117)
+ *   traj K        positions and curr_dt after each of K steps (main_loop.c:91-117;
+ *                 curr_dt < dt after ermak_var rejections, diff_comp.c:152-179)
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -182,6 +183,7 @@ int main(int argc, char* argv[])
             save_curr_forces();
             compute_hydro(NULL);
             print_vec("coord", coord, DIMS1 * size);
+            printf("dt %.17g\n", (double)curr_dt);
         }
     }
     else

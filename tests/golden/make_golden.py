@@ -6,6 +6,7 @@ with 17 significant digits) on small seeded systems, and stores inputs and outpu
 compressed `.npz` files.  Only the reference's outputs are stored, never its code.
 
     python tests/golden/make_golden.py            # rewrites tests/golden/*.npz
+    python tests/golden/make_golden.py NAME ...   # rewrites only the named cases
 
 Every run uses OMP_NUM_THREADS=1 so the reference's per-thread force and displacement
 buffers are reduced in one fixed order.
@@ -65,7 +66,12 @@ def run(mode_args, sysm, params, tmp):
     return parse(r.stdout)
 
 
+ONLY = set(sys.argv[1:])
+
+
 def case(name, sysm, params, modes):
+    if ONLY and name not in ONLY:
+        return
     res = {}
     with tempfile.TemporaryDirectory() as tmp:
         for m in modes:
@@ -123,6 +129,18 @@ def main():
     for seed in (1, 12345):
         case(f"gauss_seed{seed}", wl.chain(4, seed=1), wl.chain_params(rand_seed=seed),
              [["gauss", "15", "4"]])
+    # 6. rejected steps (comp_step_ermak, diff_comp.c:152-179): a chain with beads closer
+    #    than their LJ diameter after large steps; check_overlap yes makes each such step
+    #    redraw its Gaussians, ermak_var also halving curr_dt
+    s24r = wl.chain(24, seed=5, lj_diam=3.3)
+    for name, alg, dt, hydro in (("retry_const", "ermak_const", 0.5, "cholesky"),
+                                 ("retry_var", "ermak_var", 2.0, "cholesky"),
+                                 ("retry_var_none", "ermak_var", 2.0, "none"),
+                                 ("retry_var_geyer", "ermak_var", 2.0, "geyer")):
+        case(name, s24r, wl.chain_params(hydro=hydro, algorithm=alg, dt=dt,
+                                          check_overlap="yes", move_attempts=200,
+                                          rand_seed=11),
+             [["traj", "8"]])
 
 
 if __name__ == "__main__":

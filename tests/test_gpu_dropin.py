@@ -59,3 +59,31 @@ def test_dropin_trajectory_matches_reference(N, hydro):
     assert a.shape == b.shape == (9, N, 3)
     # 6-decimal text output: allow one unit in the last printed place
     assert np.max(np.abs(a - b)) <= 1.5e-6, np.max(np.abs(a - b))
+
+
+DROPIN_CASES = {
+    # rejected steps redrawn on the device (diff_comp.c:152-179), dt halved by ermak_var
+    "retry_var": (dict(lj_diam=3.3), dict(hydro="cholesky", algorithm="ermak_var", dt=2.0,
+                                          check_overlap="yes", move_attempts=200)),
+    "retry_const": (dict(lj_diam=3.3), dict(hydro="cholesky", algorithm="ermak_const", dt=0.5,
+                                            check_overlap="yes", move_attempts=1000)),
+    "geyer": ({}, dict(hydro="geyer")),
+    "bucket_list": ({}, dict(hydro="cholesky", nb_list="bucket_list")),
+    "bucket_space": ({}, dict(hydro="none", nb_list="bucket_space")),
+}
+
+
+@pytest.mark.skipif(not (os.path.exists(REF) and os.path.exists(HIP)),
+                    reason="reference / drop-in binaries not built here")
+@pytest.mark.parametrize("name", sorted(DROPIN_CASES))
+def test_dropin_cases_match_reference(name):
+    sopt, popt = DROPIN_CASES[name]
+    N = 24
+    sysm = wl.chain(N, seed=5, **sopt)
+    prm = wl.chain_params(rand_seed=11, bdsteps=8, save_xyz_freq=1, **popt)
+    print(f"drop-in {name}", flush=True)
+    with tempfile.TemporaryDirectory() as t1, tempfile.TemporaryDirectory() as t2:
+        a = run(REF, sysm, prm, t1)
+        b = run(HIP, sysm, prm, t2, {"BDBOX_HIP_PRECISION": "fp64"})
+    assert a.shape == b.shape == (8, N, 3)
+    assert np.max(np.abs(a - b)) <= 1.5e-6, (name, np.max(np.abs(a - b)))

@@ -229,3 +229,36 @@ def test_refactor_interval_one_is_reference_semantics():
     assert not np.array_equal(a.coords(), b.coords())   # second uses the stale factor
     d = np.max(np.abs(a.coords() - b.coords()))
     assert d < 1e-2                                      # but stays close
+
+
+@pytest.mark.parametrize("case", ["retry_const", "retry_var", "retry_var_none", "retry_var_geyer"])
+def test_rejected_steps_vs_reference(golden, case):
+    """comp_step_ermak's rejection loop (diff_comp.c:152-179) against the reference's own
+    trajectory: positions and curr_dt after each step, with steps redrawn after overlaps."""
+    g, sysm, P, p = golden(case)
+    assert P.check_overlap == 1
+    sim = po.Sim(sysm, p, seed=P.rand_seed, algorithm=P.algorithm,
+                 move_attempts=P.move_attempts)
+    traj = g["traj.coord"].reshape(8, -1, 4)
+    for k in range(8):
+        sim.run(1)
+        assert sim.last_dt() == g["traj.dt"][k], k
+        err = np.max(np.abs(sim.coords() - traj[k]))
+        assert err < 1e-11, (k, err)
+    assert sim.rejections() > 8        # the case exercises the loop
+
+
+def test_rejection_loop_discriminates(golden):
+    """A loop without the dt halving, or with a lower attempt limit, departs from the
+    reference's trajectory / stops where the reference's would."""
+    g, sysm, P, p = golden("retry_var")
+    sim = po.Sim(sysm, p, seed=P.rand_seed, algorithm=0, move_attempts=P.move_attempts)
+    try:
+        sim.run(1)
+        ok = np.max(np.abs(sim.coords() - g["traj.coord"].reshape(8, -1, 4)[0])) < 1e-11
+    except RuntimeError:
+        ok = False
+    assert not ok
+    tight = po.Sim(sysm, p, seed=P.rand_seed, algorithm=P.algorithm, move_attempts=0)
+    with pytest.raises(RuntimeError, match=r"\(3\)"):
+        tight.run(8)
