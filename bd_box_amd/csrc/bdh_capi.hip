@@ -329,7 +329,7 @@ int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev, double d
         jobs.job[1] = ReduceJob{s->Px, s->Feff, s->n, s->msplits, 0, 1.0};
         jobs.job[2] = ReduceJob{s->Pc, s->Cs, s->n, s->msplits, 0, 1.0};
         if (launch_reduce_partials(jobs, 3, s->n, s->st)) return -1;
-        if (launch_tea_close(s->pos, xi_dev, s->Feff, s->Cs, s->Pe, s->msplits, s->N, s->preDii, dt,
+        if (launch_tea_close(s->pos, xi_dev, s->Feff, s->Cs, s->Pe, pair_eparts(s->N), s->N, s->preDii, dt,
                              s->bt, s->Rs, s->st))
             return -1;
         if (launch_em_update(s->pos, s->N, s->Rf, s->Rs, c.bc == 1, c.box, s->st)) return -1;
@@ -654,7 +654,7 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     if (s->tea) {
         if (dmalloc(&s->Px, sizeof(double) * s->n * s->msplits)) return -1;
         if (dmalloc(&s->Pc, sizeof(double) * s->n * s->msplits)) return -1;
-        if (dmalloc(&s->Pe, sizeof(double) * s->N * s->msplits)) return -1;
+        if (dmalloc(&s->Pe, sizeof(double) * pair_eparts(N))) return -1;
         if (dmalloc(&s->Feff, sizeof(double) * s->n)) return -1;
         if (dmalloc(&s->Cs, sizeof(double) * s->n)) return -1;
         if (dmalloc(&s->bt, sizeof(double) * 2)) return -1;

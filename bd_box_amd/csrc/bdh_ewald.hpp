@@ -21,12 +21,13 @@ constexpr int EW_MAXV = 700;  // lattice vectors with |hfg| <= 5
 
 struct EwaldTables {
     int nv;                  // vectors in the sphere (including 0)
+    int R;                   // lattice sphere radius
     double box, inv_L, alpha, alpha2, alpha3, inv_alpha2, preQ;
     double self_a, self_b;   // self term: -6 alpha/sqrt(pi) and 40 alpha^3/(3 sqrt(pi))
 };
 
 extern __constant__ EwaldTables c_ew;
-extern __constant__ short4 c_hfg[EW_MAXV];       // (h, f, g, 0) in the reference's order
+// per lattice vector, h then f then g ascending over the sphere (the reference's order):
 extern __constant__ double4 c_recip[EW_MAXV];    // (G_k, Rk, e_x, e_y); e_z in c_recip_z
 extern __constant__ double c_recip_z[EW_MAXV];
 
@@ -36,6 +37,28 @@ int ewald_upload(double box, int R);
 struct Blk6 {
     double xx, yy, zz, xy, xz, yz;
 };
+
+// (re, im) *= (c + i s) and (re, im) *= (c - i s)
+__device__ __forceinline__ void cmul_step(double& re, double& im, double c, double s)
+{
+    const double r = re * c - im * s;
+    im = re * s + im * c;
+    re = r;
+}
+__device__ __forceinline__ void cmul_conj_step(double& re, double& im, double c, double s)
+{
+    const double r = re * c + im * s;
+    im = im * c - re * s;
+    re = r;
+}
+
+// floor(sqrt(m)) for 0 <= m <= 25 (lattice radius <= 5)
+__device__ __forceinline__ int isqrt_small(int m)
+{
+    int r = 0;
+    while ((r + 1) * (r + 1) <= m) ++r;
+    return r;
+}
 
 // Beenakker block for the pair (i, j): r = r_j - r_i (no minimum image, dist_vec), radii
 // ai, aj; self = (i == j) adds the self term and clears the off-diagonal entries.
@@ -72,45 +95,66 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
     const double vx = rx * E.inv_L, vy = ry * E.inv_L, vz = rz * E.inv_L;
     const T al = (T)E.alpha, al2 = (T)E.alpha2, al3 = (T)E.alpha3, sg = (T)sigma_sq;
     const T inv_sqrtpi = (T)(1.0 / SQRTPI);
-    for (int q = 0; q < E.nv; ++q) {
-        const short4 v = c_hfg[q];
-        // real space image r + L (h, f, g)
-        {
-            const T x = (T)(rx + E.box * v.x), y = (T)(ry + E.box * v.y), z = (T)(rz + E.box * v.z);
-            const T rr2 = x * x + y * y + z * z;
-            if (rr2 != (T)0) {
-                const T rn = sqrt(rr2);
-                const T inv_r = (T)1 / rn, inv_r2 = (T)1 / rr2, inv_r3 = inv_r * inv_r2;
-                const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;
-                const T eg = exp(-al2 * rr2) * inv_sqrtpi;
-                const T ec = erfc(rn * al);
-                T sc = al * sg * inv_r2 + (T)14 * al3 * sg - (T)4.5 * al - (T)20 * al2 * al3 * sg * rr2 +
-                       (T)3 * al3 * rr2 + (T)4 * al3 * al2 * al2 * sg * rr2 * rr2;
-                sc = sc * eg + ec * ((T)0.75 * inv_r + (T)0.5 * sg * inv_r3);
-                T sc2 = (T)-3 * al * sg * inv_r2 - (T)2 * al3 * sg + (T)1.5 * al + (T)16 * al2 * al3 * sg * rr2 -
-                        (T)3 * al3 * rr2 - (T)4 * al3 * al2 * al2 * sg * rr2 * rr2;
-                sc2 = sc2 * eg + ec * ((T)0.75 * inv_r - (T)1.5 * sg * inv_r3);
-                O[0] += (double)(sc + sc2 * ex * ex);
-                O[1] += (double)(sc + sc2 * ey * ey);
-                O[2] += (double)(sc + sc2 * ez * ez);
-                O[3] += (double)(sc2 * ex * ey);
-                O[4] += (double)(sc2 * ex * ez);
-                O[5] += (double)(sc2 * ey * ez);
+    // Reciprocal phases cos 2 pi v.(h, f, g) = Re(e^{i h tx} e^{i f ty} e^{i g tz}): one
+    // sincos per axis, the exponentials stepped along the lattice loops, which visit the
+    // vectors of the sphere in the table's order (h, then f, then g ascending).
+    double sx, cx, sy, cy, sz, cz;
+    sincos(2.0 * PI_F * vx, &sx, &cx);
+    sincos(2.0 * PI_F * vy, &sy, &cy);
+    sincos(2.0 * PI_F * vz, &sz, &cz);
+    const int R = E.R, RR = R * R;
+    double hr = 1.0, hi = 0.0;
+    for (int k = 0; k < R; ++k) cmul_conj_step(hr, hi, cx, sx);
+    int q = 0;
+    for (int h = -R; h <= R; ++h) {
+        const int nf = isqrt_small(RR - h * h);
+        double fr = 1.0, fi = 0.0;
+        for (int k = 0; k < nf; ++k) cmul_conj_step(fr, fi, cy, sy);
+        for (int f = -nf; f <= nf; ++f) {
+            const int ng = isqrt_small(RR - h * h - f * f);
+            const double hfr = hr * fr - hi * fi, hfi = hr * fi + hi * fr;
+            double gr = 1.0, gi = 0.0;
+            for (int k = 0; k < ng; ++k) cmul_conj_step(gr, gi, cz, sz);
+            for (int g = -ng; g <= ng; ++g, ++q) {
+                // real space image r + L (h, f, g)
+                const T x = (T)(rx + E.box * h), y = (T)(ry + E.box * f), z = (T)(rz + E.box * g);
+                const T rr2 = x * x + y * y + z * z;
+                if (rr2 != (T)0) {
+                    const T rn = sqrt(rr2);
+                    const T inv_r = (T)1 / rn, inv_r2 = (T)1 / rr2, inv_r3 = inv_r * inv_r2;
+                    const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;
+                    const T eg = exp(-al2 * rr2) * inv_sqrtpi;
+                    const T ec = erfc(rn * al);
+                    T sc = al * sg * inv_r2 + (T)14 * al3 * sg - (T)4.5 * al - (T)20 * al2 * al3 * sg * rr2 +
+                           (T)3 * al3 * rr2 + (T)4 * al3 * al2 * al2 * sg * rr2 * rr2;
+                    sc = sc * eg + ec * ((T)0.75 * inv_r + (T)0.5 * sg * inv_r3);
+                    T sc2 = (T)-3 * al * sg * inv_r2 - (T)2 * al3 * sg + (T)1.5 * al + (T)16 * al2 * al3 * sg * rr2 -
+                            (T)3 * al3 * rr2 - (T)4 * al3 * al2 * al2 * sg * rr2 * rr2;
+                    sc2 = sc2 * eg + ec * ((T)0.75 * inv_r - (T)1.5 * sg * inv_r3);
+                    O[0] += (double)(sc + sc2 * ex * ex);
+                    O[1] += (double)(sc + sc2 * ey * ey);
+                    O[2] += (double)(sc + sc2 * ez * ez);
+                    O[3] += (double)(sc2 * ex * ey);
+                    O[4] += (double)(sc2 * ex * ez);
+                    O[5] += (double)(sc2 * ey * ez);
+                }
+                // reciprocal space (FP64)
+                if (h != 0 || f != 0 || g != 0) {
+                    const double4 gk = c_recip[q];
+                    const double ekz = c_recip_z[q];
+                    const double sc = (1.0 - sigma_sq * gk.y) * gk.x * (hfr * gr - hfi * gi);
+                    Qr[0] += sc - sc * gk.z * gk.z;
+                    Qr[1] += sc - sc * gk.w * gk.w;
+                    Qr[2] += sc - sc * ekz * ekz;
+                    Qr[3] -= sc * gk.z * gk.w;
+                    Qr[4] -= sc * gk.z * ekz;
+                    Qr[5] -= sc * gk.w * ekz;
+                }
+                cmul_step(gr, gi, cz, sz);
             }
+            cmul_step(fr, fi, cy, sy);
         }
-        // reciprocal space (FP64)
-        if (v.x != 0 || v.y != 0 || v.z != 0) {
-            const double4 g = c_recip[q];
-            const double ez = c_recip_z[q];
-            const double sc = (1.0 - sigma_sq * g.y) * g.x *
-                              cos(2.0 * PI_F * (vx * v.x + vy * v.y + vz * v.z));
-            Qr[0] += sc - sc * g.z * g.z;
-            Qr[1] += sc - sc * g.w * g.w;
-            Qr[2] += sc - sc * ez * ez;
-            Qr[3] -= sc * g.z * g.w;
-            Qr[4] -= sc * g.z * ez;
-            Qr[5] -= sc * g.w * ez;
-        }
+        cmul_step(hr, hi, cx, sx);
     }
     out.xx += preDii * (O[0] + E.preQ * Qr[0]);
     out.yy += preDii * (O[1] + E.preQ * Qr[1]);

@@ -31,7 +31,8 @@ int launch_set_identity_pad(T* d_A, long long ld, int n, int n_pad, hipStream_t 
 
 // ---- pair sums, Ewald, TEA (bdh_pairs.hip)
 int ewald_upload(double box, int R);
-int pair_splits(int N, bool ewald);
+int pair_splits(int N, bool ewald);   // row partials per bead: one per 64-bead tile
+int pair_eparts(int N);                // eps partials of the TEA sums: one per tile pair
 template <typename T>
 int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi, int N,
                      double preDii, double preDij, double dt, bool ewald, bool tea, int splits,
@@ -39,6 +40,7 @@ int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi
 template <typename T>
 int launch_ewald_assemble(const double4* d_pos, int N, double preDii, double preDij, T* d_A,
                           long long ld, hipStream_t st);
+// S = pair_eparts(N) eps partials in d_Pe
 int launch_tea_close(const double4* d_pos, const double* d_xi, const double* d_Feff,
                      const double* d_C, const double* d_Pe, int S, int N, double preDii, double dt,
                      double* d_bt, double* d_Rs, hipStream_t st);

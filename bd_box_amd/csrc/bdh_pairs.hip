@@ -1,6 +1,6 @@
 // Pair sums over hydrodynamic blocks evaluated on the fly, and the Geyer-Winter TEA noise.
 //
-// pair_sums_kernel<T, EW, TEA>: for every row bead i, over the column beads j of one split,
+// pair_sums_kernel<T, EW, TEA>: for every row bead i, over the column beads j of a tile,
 //   f_i += D_ij F_j                        (D.F, diff_equation.c:55-117 / 119-214)
 // and with TEA (diff_rand.c:86-135, the sums of compute_r's DIFF_GEYER branch)
 //   x_i += D_ij X_j,  X_j = xi_j sqrt(2 dt / D_jj)          (j != i)
@@ -8,8 +8,8 @@
 //   e_i += sum of the 9 entries of D_ij                      (j != i)
 // D_ij is the free-space RPY block (EW = false) or the Beenakker-Ewald block (EW = true);
 // D_jj in the TEA sums is the free-space diagonal kT/(6 pi eta a_j) (diag_D, as the
-// reference uses it).  Each pair is evaluated once per row it belongs to (full rows, no
-// atomics); accumulation is FP64.
+// reference uses it).  Each unordered pair is evaluated once and serves both of its rows
+// (no atomics); accumulation is FP64.
 //
 // TEA closing (diff_rand.c:185-224): eps = 2 (sum_{i<j} D_ij entries) / (n (n - 1)),
 // clamped to 1; a = (n - 1) eps^2 - (n - 2) eps; beta = (1 - sqrt(1 - a)) / a (0.5 when
@@ -24,7 +24,6 @@ namespace bdh {
 
 // ------------------------------------------------------------------ Ewald tables
 __constant__ EwaldTables c_ew;
-__constant__ short4 c_hfg[EW_MAXV];
 __constant__ double4 c_recip[EW_MAXV];
 __constant__ double c_recip_z[EW_MAXV];
 
@@ -32,7 +31,7 @@ int ewald_upload(double box, int R)
 {
     BDH_CHECK(R >= 0 && R <= 5, "Ewald lattice radius must be 0..5");
     EwaldTables t{};
-    std::vector<short4> hfg;
+    int nv = 0;
     std::vector<double4> rec;
     std::vector<double> recz;
     t.box = box;
@@ -50,7 +49,7 @@ int ewald_upload(double box, int R)
         for (int f = -nf; f <= nf; ++f) {
             const int ng = (int)std::sqrt((double)(rr - h * h - f * f));
             for (int g = -ng; g <= ng; ++g) {
-                hfg.push_back(make_short4((short)h, (short)f, (short)g, 0));
+                ++nv;
                 const double s = (double)(h * h + f * f + g * g);
                 if (s == 0.0) {
                     rec.push_back(make_double4(0, 0, 0, 0));
@@ -66,10 +65,10 @@ int ewald_upload(double box, int R)
             }
         }
     }
-    t.nv = (int)hfg.size();
+    t.nv = nv;
+    t.R = R;
     BDH_CHECK(t.nv <= EW_MAXV, "Ewald table overflow");
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ew), &t, sizeof(t)));
-    BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_hfg), hfg.data(), sizeof(short4) * hfg.size()));
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip), rec.data(), sizeof(double4) * rec.size()));
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip_z), recz.data(), sizeof(double) * recz.size()));
     return t.nv;
@@ -127,98 +126,157 @@ __device__ __forceinline__ Blk6 pair_block(const double4& pi, const double4& pj,
 }
 
 // ------------------------------------------------------------------ pair sums
-constexpr int PS_ROWS = 64;
-constexpr int PS_CHUNK = 128;
+// Tiles of 64 beads.  Workgroup b takes the tile pair (I, J), I <= J, of the upper triangle
+// (b = J (J + 1) / 2 + I).  Lane t owns row bead I*64 + t; wave w takes the shifts
+// s = w, w + 4, ... and evaluates the pair (t, (t + s) mod 64) of tile J, so the 64 column
+// beads of one shift are distinct within a wave and its column sums go to the wave's own
+// LDS rows without conflicts.
+//   I < J : every unordered pair once; the block serves both rows, D_ji = D_ij (a block is
+//           an even function of the separation, and symmetric).
+//   I == J: every ordered pair, rows only.
+// Row sums of tile I land in split J, column sums of tile J in split I: every split of
+// every row is written exactly once, so the partials need no clearing.  The eps sums of
+// the workgroup (each unordered pair i != j counted twice, as from both rows) are reduced
+// into Pe[b].
+constexpr int PS_TILE = 64;
+
+int pair_splits(int N, bool) { return ceil_div(N, PS_TILE); }
+int pair_eparts(int N) { const int t = ceil_div(N, PS_TILE); return t * (t + 1) / 2; }
 
 template <typename T, bool EW, bool TEA>
 __global__ void __launch_bounds__(256)
 pair_sums_kernel(const double4* __restrict__ pos, const double* __restrict__ F,
-                 const double* __restrict__ xi, int N, int cols_per_split, double preDii,
-                 double preDij, double dt, double* __restrict__ Pf, double* __restrict__ Px,
-                 double* __restrict__ Pc, double* __restrict__ Pe)
+                 const double* __restrict__ xi, int N, double preDii, double preDij, double dt,
+                 double* __restrict__ Pf, double* __restrict__ Px, double* __restrict__ Pc,
+                 double* __restrict__ Pe)
 {
-    __shared__ double4 s_pos[PS_CHUNK];
-    __shared__ double s_f[PS_CHUNK * 3];
-    __shared__ double s_x[TEA ? PS_CHUNK * 3 : 1];
-    __shared__ double s_red[TEA ? 10 : 3][4][PS_ROWS];
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * PS_ROWS + tx;
-    const int i = blockIdx.x * PS_ROWS + tx;
-    const int jb = blockIdx.y * cols_per_split;
-    const int je = min(N, jb + cols_per_split);
-    const double4 pi = (i < N) ? pos[i] : make_double4(0, 0, 0, 1);
-    double f0 = 0, f1 = 0, f2 = 0, x0 = 0, x1 = 0, x2 = 0, c0 = 0, c1 = 0, c2 = 0, e = 0;
-    for (int b0 = jb; b0 < je; b0 += PS_CHUNK) {
-        const int cn = min(PS_CHUNK, je - b0);
-        __syncthreads();
-        for (int q = tid; q < cn; q += 256) {
-            const double4 p = pos[b0 + q];
-            s_pos[q] = p;
-            s_f[3 * q] = F[3 * (b0 + q)];
-            s_f[3 * q + 1] = F[3 * (b0 + q) + 1];
-            s_f[3 * q + 2] = F[3 * (b0 + q) + 2];
-            if constexpr (TEA) {
-                const double sx = sqrt(2.0 * dt / (preDii / p.w));
-                s_x[3 * q] = xi[3 * (b0 + q)] * sx;
-                s_x[3 * q + 1] = xi[3 * (b0 + q) + 1] * sx;
-                s_x[3 * q + 2] = xi[3 * (b0 + q) + 2] * sx;
-            }
-        }
-        __syncthreads();
-        if (i >= N) continue;
-        for (int q = ty; q < cn; q += 4) {
-            const int j = b0 + q;
-            const bool self = (j == i);
-            const double4 pj = s_pos[q];
-            const Blk6 B = pair_block<T, EW>(pi, pj, self, preDii, preDij);
-            const double fx = s_f[3 * q], fy = s_f[3 * q + 1], fz = s_f[3 * q + 2];
-            f0 += B.xx * fx + B.xy * fy + B.xz * fz;
-            f1 += B.xy * fx + B.yy * fy + B.yz * fz;
-            f2 += B.xz * fx + B.yz * fy + B.zz * fz;
-            if constexpr (TEA) {
-                if (!self) {
-                    const double X0 = s_x[3 * q], X1 = s_x[3 * q + 1], X2 = s_x[3 * q + 2];
-                    x0 += B.xx * X0 + B.xy * X1 + B.xz * X2;
-                    x1 += B.xy * X0 + B.yy * X1 + B.yz * X2;
-                    x2 += B.xz * X0 + B.yz * X1 + B.zz * X2;
-                    const double inv_d = pj.w / preDii;  // 1 / D_jj, isotropic
-                    c0 += (B.xx * B.xx + B.xy * B.xy + B.xz * B.xz) * inv_d;
-                    c1 += (B.xy * B.xy + B.yy * B.yy + B.yz * B.yz) * inv_d;
-                    c2 += (B.xz * B.xz + B.yz * B.yz + B.zz * B.zz) * inv_d;
-                    e += B.xx + B.yy + B.zz + 2.0 * (B.xy + B.xz + B.yz);
-                }
-            }
-        }
-    }
-    s_red[0][ty][tx] = f0; s_red[1][ty][tx] = f1; s_red[2][ty][tx] = f2;
-    if constexpr (TEA) {
-        s_red[3][ty][tx] = x0; s_red[4][ty][tx] = x1; s_red[5][ty][tx] = x2;
-        s_red[6][ty][tx] = c0; s_red[7][ty][tx] = c1; s_red[8][ty][tx] = c2;
-        s_red[9][ty][tx] = e;
-    }
-    __syncthreads();
-    if (ty == 0 && i < N) {
-        const long long o = (long long)blockIdx.y * 3 * N + 3 * i;
+    constexpr int NV = TEA ? 10 : 3;
+    __shared__ double4 s_pos[PS_TILE];
+    __shared__ double s_f[3 * PS_TILE];
+    __shared__ double s_x[TEA ? 3 * PS_TILE : 1];
+    __shared__ double s_col[4][NV][PS_TILE];
+    __shared__ double s_row[4][NV][PS_TILE];
+    const int t = threadIdx.x, w = threadIdx.y, tid = w * PS_TILE + t;
+    const int b = blockIdx.x;
+    int J = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while (J * (J + 1) / 2 > b) --J;
+    while ((J + 1) * (J + 2) / 2 <= b) ++J;
+    const int I = b - J * (J + 1) / 2;
+    const bool diag = I == J;
+    const int i = I * PS_TILE + t, j0 = J * PS_TILE;
+    if (tid < PS_TILE) {
+        const int j = j0 + tid;
+        const bool ok = j < N;
+        const double4 p = ok ? pos[j] : make_double4(0, 0, 0, 1);
+        s_pos[tid] = p;
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            Pf[o + k] = (s_red[k][0][tx] + s_red[k][1][tx]) + (s_red[k][2][tx] + s_red[k][3][tx]);
+        for (int k = 0; k < 3; ++k) s_f[3 * tid + k] = ok ? F[3 * j + k] : 0.0;
+        if constexpr (TEA) {
+            const double sx = sqrt(2.0 * dt / (preDii / p.w));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s_x[3 * tid + k] = ok ? xi[3 * j + k] * sx : 0.0;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s_col[w][v][t] = 0.0;
+    const bool iv = i < N;
+    const double4 pi = iv ? pos[i] : make_double4(0, 0, 0, 1);
+    double Fi[3] = {0, 0, 0}, Xi[3] = {0, 0, 0};
+    if (iv && !diag) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Fi[k] = F[3 * i + k];
+        if constexpr (TEA) {
+            const double sx = sqrt(2.0 * dt / (preDii / pi.w));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) Xi[k] = xi[3 * i + k] * sx;
+        }
+    }
+    const double inv_di = pi.w / preDii;  // 1 / D_ii, isotropic (diag_D)
+    __syncthreads();
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    for (int sh = w; sh < PS_TILE; sh += 4) {
+        const int jl = (t + sh) & (PS_TILE - 1);
+        if (!iv || j0 + jl >= N) continue;
+        const bool self = diag && sh == 0;
+        const double4 pj = s_pos[jl];
+        const Blk6 B = pair_block<T, EW>(pi, pj, self, preDii, preDij);
+        const double fx = s_f[3 * jl], fy = s_f[3 * jl + 1], fz = s_f[3 * jl + 2];
+        acc[0] += B.xx * fx + B.xy * fy + B.xz * fz;
+        acc[1] += B.xy * fx + B.yy * fy + B.yz * fz;
+        acc[2] += B.xz * fx + B.yz * fy + B.zz * fz;
+        double q0 = 0, q1 = 0, q2 = 0, e = 0;
+        if constexpr (TEA) {
+            if (!self) {
+                const double X0 = s_x[3 * jl], X1 = s_x[3 * jl + 1], X2 = s_x[3 * jl + 2];
+                acc[3] += B.xx * X0 + B.xy * X1 + B.xz * X2;
+                acc[4] += B.xy * X0 + B.yy * X1 + B.yz * X2;
+                acc[5] += B.xz * X0 + B.yz * X1 + B.zz * X2;
+                q0 = B.xx * B.xx + B.xy * B.xy + B.xz * B.xz;
+                q1 = B.xy * B.xy + B.yy * B.yy + B.yz * B.yz;
+                q2 = B.xz * B.xz + B.yz * B.yz + B.zz * B.zz;
+                const double inv_dj = pj.w / preDii;
+                acc[6] += q0 * inv_dj;
+                acc[7] += q1 * inv_dj;
+                acc[8] += q2 * inv_dj;
+                e = B.xx + B.yy + B.zz + 2.0 * (B.xy + B.xz + B.yz);
+                acc[9] += e;
+            }
+        }
+        if (!diag) {
+            double* c = &s_col[w][0][jl];
+            c[0] += B.xx * Fi[0] + B.xy * Fi[1] + B.xz * Fi[2];
+            c[PS_TILE] += B.xy * Fi[0] + B.yy * Fi[1] + B.yz * Fi[2];
+            c[2 * PS_TILE] += B.xz * Fi[0] + B.yz * Fi[1] + B.zz * Fi[2];
+            if constexpr (TEA) {
+                c[3 * PS_TILE] += B.xx * Xi[0] + B.xy * Xi[1] + B.xz * Xi[2];
+                c[4 * PS_TILE] += B.xy * Xi[0] + B.yy * Xi[1] + B.yz * Xi[2];
+                c[5 * PS_TILE] += B.xz * Xi[0] + B.yz * Xi[1] + B.zz * Xi[2];
+                c[6 * PS_TILE] += q0 * inv_di;
+                c[7 * PS_TILE] += q1 * inv_di;
+                c[8 * PS_TILE] += q2 * inv_di;
+                c[9 * PS_TILE] += e;
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s_row[w][v][t] = acc[v];
+    __syncthreads();
+#define sum4(a, v) (((a)[0][v][t] + (a)[1][v][t]) + ((a)[2][v][t] + (a)[3][v][t]))
+    if (w == 0 && iv) {
+        const long long o = (long long)J * 3 * N + 3 * i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Pf[o + k] = sum4(s_row, k);
         if constexpr (TEA) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                Px[o + k] = (s_red[3 + k][0][tx] + s_red[3 + k][1][tx]) + (s_red[3 + k][2][tx] + s_red[3 + k][3][tx]);
-                Pc[o + k] = (s_red[6 + k][0][tx] + s_red[6 + k][1][tx]) + (s_red[6 + k][2][tx] + s_red[6 + k][3][tx]);
+                Px[o + k] = sum4(s_row, 3 + k);
+                Pc[o + k] = sum4(s_row, 6 + k);
             }
-            Pe[(long long)blockIdx.y * N + i] = (s_red[9][0][tx] + s_red[9][1][tx]) + (s_red[9][2][tx] + s_red[9][3][tx]);
         }
     }
-}
-
-int pair_splits(int N, bool ewald)
-{
-    const int rows = ceil_div(N, PS_ROWS);
-    int s = ceil_div(ewald ? 4096 : 2048, rows);
-    const int max_s = ceil_div(N, 32);
-    if (s > max_s) s = max_s;
-    return s < 1 ? 1 : s;
+    if (w == 1 && !diag && j0 + t < N) {
+        const long long o = (long long)I * 3 * N + 3 * (j0 + t);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Pf[o + k] = sum4(s_col, k);
+        if constexpr (TEA) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Px[o + k] = sum4(s_col, 3 + k);
+                Pc[o + k] = sum4(s_col, 6 + k);
+            }
+        }
+    }
+    if constexpr (TEA) {
+        if (w == 2) {
+            double e = sum4(s_row, 9);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
+            if (t == 0) Pe[b] = diag ? e : 2.0 * e;
+        }
+    }
+#undef sum4
 }
 
 template <typename T>
@@ -226,20 +284,19 @@ int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi
                      double preDii, double preDij, double dt, bool ewald, bool tea, int splits,
                      double* d_Pf, double* d_Px, double* d_Pc, double* d_Pe, hipStream_t st)
 {
-    dim3 grid(ceil_div(N, PS_ROWS), splits);
-    dim3 blk(PS_ROWS, 4);
-    const int cps = ceil_div(N, splits);
+    BDH_CHECK(splits == pair_splits(N, ewald), "launch_pair_sums: split count");
+    const dim3 grid(pair_eparts(N)), blk(PS_TILE, 4);
     if (ewald && tea)
-        hipLaunchKernelGGL((pair_sums_kernel<T, true, true>), grid, blk, 0, st, d_pos, d_F, d_xi, N, cps,
+        hipLaunchKernelGGL((pair_sums_kernel<T, true, true>), grid, blk, 0, st, d_pos, d_F, d_xi, N,
                            preDii, preDij, dt, d_Pf, d_Px, d_Pc, d_Pe);
     else if (ewald)
-        hipLaunchKernelGGL((pair_sums_kernel<T, true, false>), grid, blk, 0, st, d_pos, d_F, d_xi, N, cps,
+        hipLaunchKernelGGL((pair_sums_kernel<T, true, false>), grid, blk, 0, st, d_pos, d_F, d_xi, N,
                            preDii, preDij, dt, d_Pf, d_Px, d_Pc, d_Pe);
     else if (tea)
-        hipLaunchKernelGGL((pair_sums_kernel<T, false, true>), grid, blk, 0, st, d_pos, d_F, d_xi, N, cps,
+        hipLaunchKernelGGL((pair_sums_kernel<T, false, true>), grid, blk, 0, st, d_pos, d_F, d_xi, N,
                            preDii, preDij, dt, d_Pf, d_Px, d_Pc, d_Pe);
     else
-        hipLaunchKernelGGL((pair_sums_kernel<T, false, false>), grid, blk, 0, st, d_pos, d_F, d_xi, N, cps,
+        hipLaunchKernelGGL((pair_sums_kernel<T, false, false>), grid, blk, 0, st, d_pos, d_F, d_xi, N,
                            preDii, preDij, dt, d_Pf, d_Px, d_Pc, d_Pe);
     BDH_LAUNCH_CHECK();
     return 0;
@@ -282,14 +339,14 @@ int launch_ewald_assemble(const double4* d_pos, int N, double preDii, double pre
 }
 
 // ------------------------------------------------------------------ TEA closing
-// One block: eps from the row partials (each unordered pair counted from both rows), then
-// beta; out[0] = beta, out[1] = eps.
+// One block: eps from the per-workgroup partials of the pair sums (each unordered pair
+// counted twice, as from both rows), then beta; out[0] = beta, out[1] = eps.
 __global__ void __launch_bounds__(256)
 tea_beta_kernel(const double* __restrict__ Pe, int S, int N, double* __restrict__ out)
 {
     __shared__ double s[256];
     double a = 0.0;
-    for (long long k = threadIdx.x; k < (long long)S * N; k += 256) a += Pe[k];
+    for (int k = threadIdx.x; k < S; k += 256) a += Pe[k];
     s[threadIdx.x] = a;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {

@@ -50,6 +50,20 @@ def workload(N, seed):
     return sysm, prm, wl.parse_params(prm)
 
 
+def periodic_workload(N, seed):
+    """N hard spheres (a = 25 A) at volume fraction 0.2, random sequential addition in a
+    periodic cube; WCA-like wall (sigma 50 A, eps 1, cutoff 2^(1/6) sigma); Beenakker-Ewald
+    RPY with the reference's alpha = sqrt(pi)/L and lattice sphere R = 2 (first omitted
+    real / reciprocal shells ~1e-8 / ~1e-7 relative); TEA noise; dt = 1 ps."""
+    from bd_box_amd import workloads as wl
+    sysm, L = wl.spheres(N, seed=seed, radius=25.0, phi=0.2)
+    prm = wl.chain_params(bc="periodic", xbox=L, ybox=L, zbox=L, hydro="geyer",
+                          ewald_method="beenakker", ewald_real=2, ewald_recip=2, elec="no",
+                          alpha_lj=4.0, cutoff_lj=-1, lj_6_term="yes", dt=1.0,
+                          nb_list="bucket_list", rand_seed=20240 + seed)
+    return sysm, prm, wl.parse_params(prm), L
+
+
 # ----------------------------------------------------------------------------- roofline
 def syrk_flops_per_factor(n, n_pad, nb=64):
     """Algorithmic flops of the trailing updates of one factorization (lower triangle
@@ -186,6 +200,9 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed mixed-precision steps (for rocprofv3 runs), then a "
                          "256 MiB device copy that calibrates the HBM byte counters")
+    ap.add_argument("--workload", choices=("chain", "periodic"), default="chain",
+                    help="chain: the N=3000 RPY + Cholesky step (default); periodic: N=8192 "
+                         "spheres, Beenakker-Ewald RPY + TEA (its own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step of a "
@@ -197,6 +214,9 @@ def main():
     from bd_box_amd.system import BDSystem, FP64, MIXED, MIXED_SPLIT
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     torch.cuda.init()
+
+    if args.workload == "periodic":
+        return bench_periodic(args)
 
     sysm, prm, P = workload(args.N, args.seed)
     n = 3 * sysm.N
@@ -297,6 +317,49 @@ def main():
         "fp64_phases_ms_per_step": {k: v[0] / cyc for k, v in ph_fp64.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def bench_periodic(args):
+    """Steps/s of the periodic suspension with Beenakker-Ewald TEA noise, mixed (FP32 pair
+    blocks, FP64 Ewald reciprocal/self sums and accumulation) and FP64."""
+    import torch
+    from bd_box_amd.system import BDSystem, FP64, MIXED
+    N = 8192 if args.N == 3000 else args.N
+    steps = min(args.steps, 50)
+    warm = min(args.warmup, 2)
+    sysm, prm, P, L = periodic_workload(N, args.seed)
+    log(f"periodic workload: N={N} box={L:.1f} A, {steps} timed steps")
+    out = {}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
+        s = BDSystem(sysm, P, precision=prec)
+        s.compute_hydro(warm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.compute_hydro(steps)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        s.phase_timing(True)
+        s.compute_hydro(2)
+        ph = s.phase_times()
+        s.phase_timing(False)
+        out[name] = (steps / t, {k: v[0] / 2 for k, v in ph.items()})
+        log(f"{name}: {steps / t:.2f} steps/s phases {out[name][1]}")
+        s.close()
+    line = {
+        "metric": "BD steps/s, N=8192 periodic Beenakker-Ewald RPY + TEA",
+        "value": out["mixed"][0], "unit": "steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "ms_per_step": 1e3 / out["mixed"][0], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"N={N} spheres a=25 A, phi=0.2, periodic cube L={L:.1f} A, "
+                               "WCA wall sigma 50 A eps 1, Beenakker-Ewald R=2, TEA, dt 1 ps",
+                   "N": N, "box": L, "parallelism": "1 GPU"},
+        "fp64_steps_per_s": out["fp64"][0],
+        "phases_ms_per_step": out["mixed"][1],
+        "fp64_phases_ms_per_step": out["fp64"][1],
+        "roofline": None,
+        "cpu_baseline": None,
     }
     print(json.dumps(line))
 
