@@ -18,7 +18,7 @@
  * geyer, bc none or periodic (cube; Beenakker-Ewald tensor with hydrodynamics), nb_list
  * brute or either bucket method (both map to the backend's device cell list, which finds
  * the same interacting pairs), FP64 or mixed precision
- * (environment BDBOX_HIP_PRECISION=fp64|mixed, default fp64), refactor interval
+ * (environment BDBOX_HIP_PRECISION=fp64|mixed|split, default fp64), refactor interval
  * (BDBOX_HIP_REFACTOR, default 1 = the reference's semantics).  Anything else stops with
  * UNERR like the reference does for unimplemented methods (diff_comp.c:230).
  */
@@ -133,7 +133,11 @@ void init_diff_tensor(void)
     cfg.nb_list = nb_list;
     cfg.algorithm = algorithm; cfg.move_attempts = move_attempts;
     cfg.hydro = hydro; cfg.ewald_method = ewald_method; cfg.ewald_real = ewald_real;
-    cfg.precision = (prec && !strcmp(prec, "mixed")) ? BDH_MIXED : BDH_FP64;
+    cfg.precision = !prec || !strcmp(prec, "fp64") ? BDH_FP64
+                  : !strcmp(prec, "mixed")            ? BDH_MIXED
+                  : !strcmp(prec, "split")            ? BDH_MIXED_SPLIT
+                                                      : -1;
+    if (cfg.precision < 0) UNERR("BDBOX_HIP_PRECISION: fp64, mixed or split");
     cfg.refactor_every = refac ? atoi(refac) : 1;
     cfg.rand_seed = (uint32_t)rand_seed;
     if (bdh_init(0)) fail("init");
