@@ -119,11 +119,54 @@ __device__ __forceinline__ void diag_update(double* __restrict__ sD, int lane, i
     }
 }
 
+// Per diagonal block, the workspace Lb holds (FP64):
+//   Lb[0 .. 4095]      L_kk column-major, zeros above the diagonal
+//   Lb[4096 + 256 J]   inverse of its J-th 16x16 diagonal sub-block, column-major, J = 0..3
+constexpr int LB_STRIDE = CHOL_NB * CHOL_NB + 4 * 256;
+
+// Columns 16 J .. 16 J + 15 of the factored block into Lb, by the 128 threads q of two
+// wavefronts (8 entries each).
+__device__ __forceinline__ void publish_cols(const double* __restrict__ sD, double* __restrict__ Lb,
+                                             int J, int q)
+{
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int e = q + 128 * i, r = e % CHOL_NB, c = 16 * J + e / CHOL_NB;
+        Lb[c * CHOL_NB + r] = (r >= c) ? sD[r * DG_LD + c] : 0.0;
+    }
+}
+
+// Inverse of the J-th 16x16 diagonal sub-block into Lb (lane c < 16 of one wavefront solves
+// L_JJ y = e_c by forward substitution; column-major result).
+__device__ __forceinline__ void publish_inv(const double* __restrict__ sD, double* __restrict__ Lb,
+                                            int J, int lane)
+{
+    if (lane >= 16) return;
+    const int b = 16 * J, c = lane;
+    double rd[16], y[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) rd[r] = 1.0 / sD[(b + r) * DG_LD + b + r];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        double acc = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int t = 0; t < r; ++t) acc -= sD[(b + r) * DG_LD + b + t] * y[t];
+        y[r] = acc * rd[r];
+    }
+    double* inv = Lb + CHOL_NB * CHOL_NB + 256 * J;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) inv[c * 16 + r] = y[r];
+}
+
 // In-place FP64 Cholesky of the 64x64 block sD[r * DG_LD + c] (lower part read; on return
 // the lower part holds L and the strict upper part zeros).  Called by all 256 threads.
 // A non-positive pivot records row c0 + j + 1 in *status (first one wins) and the
 // factorization continues with NaN, as the reference's dpotrf reports INFO.
-__device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, int* __restrict__ status)
+// The factor and its 16x16 diagonal-block inverses are also written to Lb: while wavefront
+// 0 factors sub-panel J + 1, wavefront 1 inverts block J and wavefronts 2-3 copy the columns
+// of sub-panel J (final once it is factored), so only sub-panel 3 is left at the end.
+__device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, int* __restrict__ status,
+                                              double* __restrict__ Lb)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int bad = -1;
@@ -132,48 +175,24 @@ __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, i
     diag_update<0>(sD, lane, wave);
     __syncthreads();
     if (wave == 0) diag_subpanel<16>(sD, lane, bad);
+    else if (wave == 1) publish_inv(sD, Lb, 0, lane);
+    else publish_cols(sD, Lb, 0, tid - 128);
     __syncthreads();
     diag_update<16>(sD, lane, wave);
     __syncthreads();
     if (wave == 0) diag_subpanel<32>(sD, lane, bad);
+    else if (wave == 1) publish_inv(sD, Lb, 1, lane);
+    else publish_cols(sD, Lb, 1, tid - 128);
     __syncthreads();
     diag_update<32>(sD, lane, wave);
     __syncthreads();
     if (wave == 0) diag_subpanel<48>(sD, lane, bad);
+    else if (wave == 1) publish_inv(sD, Lb, 2, lane);
+    else publish_cols(sD, Lb, 2, tid - 128);
     __syncthreads();
+    if (wave == 0) publish_inv(sD, Lb, 3, lane);
+    else if (wave >= 2) publish_cols(sD, Lb, 3, tid - 128);
     if (wave == 0 && lane == 0 && bad >= 0) atomicCAS(status, 0, c0 + bad + 1);
-}
-
-// Per diagonal block, the workspace Lb holds (FP64):
-//   Lb[0 .. 4095]      L_kk column-major, zeros above the diagonal
-//   Lb[4096 + 256 J]   inverse of its J-th 16x16 diagonal sub-block, column-major, J = 0..3
-constexpr int LB_STRIDE = CHOL_NB * CHOL_NB + 4 * 256;
-
-// FP64 copy of the factored block and the inverses of its four 16x16 diagonal sub-blocks
-// (wave J: lane c < 16 solves L_JJ y = e_c by forward substitution) into Lb.
-__device__ __forceinline__ void publish_diag64(const double* __restrict__ sD, double* __restrict__ Lb)
-{
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < CHOL_NB * CHOL_NB; e += 256) {
-        const int r = e % CHOL_NB, c = e / CHOL_NB;
-        Lb[e] = (r >= c) ? sD[r * DG_LD + c] : 0.0;
-    }
-    if (lane < 16) {
-        const int b = 16 * wave, c = lane;
-        double rd[16], y[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) rd[r] = 1.0 / sD[(b + r) * DG_LD + b + r];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            double acc = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-            for (int t = 0; t < r; ++t) acc -= sD[(b + r) * DG_LD + b + t] * y[t];
-            y[r] = acc * rd[r];
-        }
-        double* inv = Lb + CHOL_NB * CHOL_NB + 256 * wave;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) inv[c * 16 + r] = y[r];
-    }
 }
 
 // L (lower part) of the factored block into A, in T.
@@ -207,8 +226,7 @@ chol_diag_kernel(T* __restrict__ A, long long ld, int c0, double* __restrict__ L
         sD[r * DG_LD + c] = v[i];
     }
     __syncthreads();
-    factor_diag64(sD, c0, status);
-    publish_diag64(sD, Lb);
+    factor_diag64(sD, c0, status, Lb);
     store_diag64<T>(sD, A, ld, c0);
 }
 
@@ -309,6 +327,73 @@ __device__ __forceinline__ void trsm16_solve_store(T* __restrict__ A, long long 
     }
 }
 
+// Two 16-row groups a and b of one wavefront solved together: the same L operands, two
+// independent MFMA chains interleaved (each hides the other's latency).  Rows of b at or
+// beyond re are neither loaded nor stored.
+template <typename T>
+__device__ __forceinline__ void trsm16x2_solve_store(T* __restrict__ A, long long ld, int t0, int ga0,
+                                                     int gb0, int re, int lane, Trsm16& ga, Trsm16& gb,
+                                                     const double* __restrict__ sLT,
+                                                     const double* __restrict__ sInv,
+                                                     double* __restrict__ sc)
+{
+    const int lr = lane & 15, lq = lane >> 4;
+    double xa[4][4], xb[4][4];
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+        doublex4 ya = ga.y[J], yb = gb.y[J];
+#pragma unroll
+        for (int K = 0; K < J; ++K)
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const double b = sLT[(16 * K + 4 * s4 + lq) * CHOL_NB + 16 * J + lr];
+                ya = mfma_f64_16x16x4(-xa[K][s4], b, ya);
+                yb = mfma_f64_16x16x4(-xb[K][s4], b, yb);
+            }
+        double yaa[4], yba[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sc[(4 * r + lq) * SC_PITCH + lr] = ya[r];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) yaa[s4] = sc[lr * SC_PITCH + 4 * s4 + lq];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sc[(4 * r + lq) * SC_PITCH + lr] = yb[r];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) yba[s4] = sc[lr * SC_PITCH + 4 * s4 + lq];
+        __builtin_amdgcn_wave_barrier();
+        doublex4 xja = {0.0, 0.0, 0.0, 0.0}, xjb = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const double iv = sInv[256 * J + (4 * s4 + lq) * 16 + lr];
+            xja = mfma_f64_16x16x4(yaa[s4], iv, xja);
+            xjb = mfma_f64_16x16x4(yba[s4], iv, xjb);
+        }
+        T* col = A + (long long)(t0 + 16 * J + lr) * ld;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int rowa = ga0 + 4 * r + lq, rowb = gb0 + 4 * r + lq;
+            if (rowa < re) col[rowa] = (T)xja[r];
+            if (rowb < re) col[rowb] = (T)xjb[r];
+        }
+        if (J < 3) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sc[(4 * r + lq) * SC_PITCH + lr] = xja[r];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) xa[J][s4] = sc[lr * SC_PITCH + 4 * s4 + lq];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sc[(4 * r + lq) * SC_PITCH + lr] = xjb[r];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) xb[J][s4] = sc[lr * SC_PITCH + 4 * s4 + lq];
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
 // L_kk and its diagonal sub-block inverses (Lb, FP64) into LDS.
 __device__ __forceinline__ void load_lkk(const double* __restrict__ Lb, double* __restrict__ sLT,
                                          double* __restrict__ sInv)
@@ -384,8 +469,7 @@ __device__ __forceinline__ void fused_trsm(T* __restrict__ A, long long ld, int 
     const int rb = max(ri0, t0 + CHOL_NB), re = min(n_pad, ri0 + SY_TILE_C);
     const int g0 = rb + 16 * wave, g1 = g0 + 64;   // this wavefront's two 16-row groups
     if (diag) {
-        // publish the FP64 block first; the T copy in A is not read by this kernel
-        publish_diag64(sD, Lb_next);
+        // the FP64 block is in Lb (factor_diag64); the T copy in A is not read by this kernel
         __threadfence();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -396,6 +480,7 @@ __device__ __forceinline__ void fused_trsm(T* __restrict__ A, long long ld, int 
     Trsm16 x0, x1;
     if (g0 < re) trsm16_load<T>(A, ld, t0, g0, re, lane, x0);
     if (g1 < re) trsm16_load<T>(A, ld, t0, g1, re, lane, x1);
+    else x1 = Trsm16{};
     if (tid == 0) {
         if (diag) {
             s_ok = 1;
@@ -407,8 +492,8 @@ __device__ __forceinline__ void fused_trsm(T* __restrict__ A, long long ld, int 
     __syncthreads();
     if (!s_ok) return;
     load_lkk(Lb_next, sLT, sInv);
-    if (g0 < re) trsm16_solve_store<T>(A, ld, t0, g0, re, lane, x0, sLT, sInv, sc);
-    if (g1 < re) trsm16_solve_store<T>(A, ld, t0, g1, re, lane, x1, sLT, sInv, sc);
+    if (g1 < re) trsm16x2_solve_store<T>(A, ld, t0, g0, g1, re, lane, x0, x1, sLT, sInv, sc);
+    else if (g0 < re) trsm16_solve_store<T>(A, ld, t0, g0, re, lane, x0, sLT, sInv, sc);
 }
 
 // Tile (ti, tj), ti >= tj, of a lower trapezoid with ni tile rows, enumerated column by
@@ -628,7 +713,7 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
         }
     if (diag) {
         __syncthreads();
-        factor_diag64(sD, t0, status);
+        factor_diag64(sD, t0, status, Lb_next);
     }
     if (tj == 0) fused_trsm<float>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem, sD);
 }
@@ -735,7 +820,7 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int
         }
     if (diag) {
         __syncthreads();
-        factor_diag64(sD, t0, status);
+        factor_diag64(sD, t0, status, Lb_next);
     }
     if (tj == 0) fused_trsm<double>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem, sD);
 }
