@@ -14,7 +14,8 @@
  * see oracle/Makefile target `dropin` for the link line.
  *
  * Supported: algorithm ermak_const / ermak_var (with the reference's rejection of steps that
- * overlap when check_overlap is set, diff_comp.c:152-179), hydro none / cholesky /
+ * overlap when check_overlap is set, diff_comp.c:152-179) and the IG-T predictor-corrector
+ * igt_const / igt_var (diff_comp.c:76-144; hydro none or cholesky), hydro none / cholesky /
  * geyer, bc none or periodic (cube; Beenakker-Ewald tensor with hydrodynamics), nb_list
  * brute or either bucket method (both map to the backend's device cell list, which finds
  * the same interacting pairs), FP64 or mixed precision
@@ -114,8 +115,11 @@ void init_diff_tensor(void)
     const char* prec = getenv("BDBOX_HIP_PRECISION");
     const char* refac = getenv("BDBOX_HIP_REFACTOR");
     int i, rc;
-    if (algorithm != DIFF_ALG_ERMAK_CONST && algorithm != DIFF_ALG_ERMAK_VAR)
-        UNERR("HIP backend: only algorithm ermak_const / ermak_var");
+    if (algorithm != DIFF_ALG_ERMAK_CONST && algorithm != DIFF_ALG_ERMAK_VAR &&
+        algorithm != DIFF_ALG_IGT_CONST && algorithm != DIFF_ALG_IGT_VAR)
+        UNERR("HIP backend: algorithm ermak_const / ermak_var / igt_const / igt_var");
+    if ((algorithm == DIFF_ALG_IGT_CONST || algorithm == DIFF_ALG_IGT_VAR) && hydro == DIFF_GEYER)
+        UNERR("HIP backend: IG-T with hydro geyer is not supported");
     if (hydro != DIFF_NONE && hydro != DIFF_CHOLESKY && hydro != DIFF_GEYER)
         UNERR("HIP backend: hydro none / cholesky / geyer");
     if (vel_grad_tensor) UNERR("HIP backend: no flow field");
@@ -206,7 +210,7 @@ static int output_step(INT s)
            (save_rst_freq && s % save_rst_freq == 0) || (save_enr_freq && s % save_enr_freq == 0);
 }
 
-/* compute_hydro (diff_comp.c:210-232) for ermak_const / ermak_var: one step on the device,
+/* compute_hydro (diff_comp.c:210-232) for the E-M and IG-T steps: one step on the device,
    rejected attempts redrawn there; positions, forces and the random-stream state come back
    to the host on output steps only */
 void compute_hydro(DOUBLE* _E)

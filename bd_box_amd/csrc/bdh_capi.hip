@@ -36,8 +36,16 @@ __global__ void diag_mobility_kernel(const double4* __restrict__ pos, const doub
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= 3 * N) return;
     const double dii = preDii / pos[r / 3].w;
-    Rf[r] = rf_scale * dii * F[r];
+    if (Rf) Rf[r] = rf_scale * dii * F[r];
     if (Rs) Rs[r] = xi[r] * sqrt(dii * 2.0 * dt);
+}
+
+// out = (a + b) / 2, the IG-T deterministic move Rf0 (diff_equation.c:322-325)
+__global__ void mean2_kernel(const double* __restrict__ a, const double* __restrict__ b, int n,
+                             double* __restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = 0.5 * (a[r] + b[r]);
 }
 
 // Copy the lower triangle of a column-major T matrix to a full symmetric FP64 matrix.
@@ -129,6 +137,8 @@ struct bdh_system {
     double last_dt = 0;             // dt of the last step taken (halved by ermak_var retries)
     double4* pos_prev = nullptr;    // positions and forces before a step that may be rejected
     double* F_prev = nullptr;
+    void* A2 = nullptr;             // IG-T: the averaged tensor and its factor
+    double *Rf0 = nullptr, *Rfs = nullptr;   // IG-T: the mean deterministic move, Rf at r*
     long long ldx = 0;              // stride of drawn noise vectors: n, or n + 1 if n is odd
     int msplits = 0, tblocks = 0;
     bool ewald = false, tea = false;
@@ -245,27 +255,30 @@ int ensure_ewald(bdh_system* s)
     return 0;
 }
 
-int assemble_and_factor(bdh_system* s)
+// D at the current positions (with pos2: the mean of the tensors at both sets of positions)
+// into Abuf (default: the system's matrix), factored in place.
+int assemble_and_factor(bdh_system* s, void* Abuf = nullptr, const double4* pos2 = nullptr)
 {
     if (ensure_ewald(s)) return -1;
+    void* M = Abuf ? Abuf : s->A;
     if (s->mixed) {
-        float* A = (float*)s->A;
+        float* A = (float*)M;
         REC_BEGIN(PH_ASSEMBLE);
-        if (s->ewald ? launch_ewald_assemble<float>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)
-                     : launch_rpy_assemble<float>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st))
+        if (s->ewald ? launch_ewald_assemble<float>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st, pos2)
+                     : launch_rpy_assemble<float>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st, pos2))
             return -1;
         REC_END(PH_ASSEMBLE);
         if (chol_factor<float>(A, s->ld, s->n_pad, s->Lbuf, s->flags + 1, s->syrk_mode, s->st, s->rec)) return -1;
     } else {
-        double* A = (double*)s->A;
+        double* A = (double*)M;
         REC_BEGIN(PH_ASSEMBLE);
-        if (s->ewald ? launch_ewald_assemble<double>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st)
-                     : launch_rpy_assemble<double>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st))
+        if (s->ewald ? launch_ewald_assemble<double>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st, pos2)
+                     : launch_rpy_assemble<double>(s->pos, s->N, s->preDii, s->preDij, A, s->ld, s->st, pos2))
             return -1;
         REC_END(PH_ASSEMBLE);
         if (chol_factor<double>(A, s->ld, s->n_pad, s->Lbuf, s->flags + 1, SYRK_F64, s->st, s->rec)) return -1;
     }
-    s->have_L = true;
+    if (M == s->A) s->have_L = true;
     return 0;
 }
 
@@ -281,19 +294,20 @@ int matvec_dev(bdh_system* s, const double* xi = nullptr, bool tea = false, doub
                                     tea, s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st);
 }
 
-int trmv_dev(bdh_system* s, const double* X, long long ldx, int m)
+int trmv_dev(bdh_system* s, const double* X, long long ldx, int m, const void* L = nullptr)
 {
+    const void* M = L ? L : s->A;
     if (s->mixed)
-        return launch_trmv_partials<float>((const float*)s->A, s->ld, s->n, X, ldx, m, s->Pr, s->st);
-    return launch_trmv_partials<double>((const double*)s->A, s->ld, s->n, X, ldx, m, s->Pr, s->st);
+        return launch_trmv_partials<float>((const float*)M, s->ld, s->n, X, ldx, m, s->Pr, s->st);
+    return launch_trmv_partials<double>((const double*)M, s->ld, s->n, X, ldx, m, s->Pr, s->st);
 }
 
 // Rs of m consecutive steps, sqrt(2 dt) L [xi_1 .. xi_m], in one pass over L (the factor
 // is fixed within a refactor cycle); stacked into RsB[j n + r].
-int cycle_noise(bdh_system* s, const double* X, long long ldx, int m, double dt)
+int cycle_noise(bdh_system* s, const double* X, long long ldx, int m, double dt, const void* L = nullptr)
 {
     REC_BEGIN(PH_TRMV);
-    if (trmv_dev(s, X, ldx, m)) return -1;
+    if (trmv_dev(s, X, ldx, m, L)) return -1;
     ReduceJobs jobs{};
     jobs.job[0] = ReduceJob{s->Pr, s->RsB, m * s->n, s->tblocks, trmv_blk_cols(),
                             std::sqrt(2.0 * dt), s->n};
@@ -357,8 +371,6 @@ int draw_noise(bdh_system* s, int m)
     return rc;
 }
 
-// steps E-M steps; have_xi: the caller's Gaussians are in s->xi ([step][n]).  Steps are
-// taken in chunks of up to TRMV_MAX_RHS that never cross a refactorization.
 // One step with the reference's rejection loop (comp_step_ermak, diff_comp.c:152-179): when
 // the forces at the new positions report an overlap, the positions and forces are restored
 // and the step is redrawn with new Gaussians (ermak_var also halves dt, at most 16 times);
@@ -397,8 +409,121 @@ int step_with_retries(bdh_system* s, const double* xi_given, bool refactor)
     return 0;
 }
 
+// Rf = (dt / kT) D F at the current positions and forces into out (D.F on the fly, or
+// diag_D without hydrodynamics).
+int rf_dev(bdh_system* s, double* out, double dt)
+{
+    const double scale = dt / (kB * s->cfg.T);
+    if (s->cfg.hydro == BDH_HYDRO_CHOLESKY) {
+        if (matvec_dev(s)) return -1;
+        ReduceJobs jobs{};
+        jobs.job[0] = ReduceJob{s->Pf, out, s->n, s->msplits, 0, scale};
+        return launch_reduce_partials(jobs, 1, s->n, s->st);
+    }
+    hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st, s->pos,
+                       s->F, (const double*)nullptr, s->N, s->preDii, dt, scale, out, (double*)nullptr);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+// Rs from the next Gaussians of the device stream: sqrt(2 dt) L xi (L the factor in Lm), or
+// xi sqrt(2 dt diag_D) without hydrodynamics.
+const double* rs_dev(bdh_system* s, double dt, const void* Lm)
+{
+    if (draw_noise(s, 1)) return nullptr;
+    if (s->cfg.hydro == BDH_HYDRO_CHOLESKY)
+        return cycle_noise(s, s->xi, s->n, 1, dt, Lm) ? nullptr : s->RsB;
+    hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st, s->pos,
+                       s->F, (const double*)s->xi, s->N, s->preDii, dt, 0.0, (double*)nullptr, s->Rs);
+    return hipGetLastError() == hipSuccess ? s->Rs : nullptr;
+}
+
+// Forces at the current positions; returns the force flags (1 overlap, 2 overstretched bond).
+int forces_flags(bdh_system* s, int& fl)
+{
+    BDH_HIP(hipMemsetAsync(s->flags, 0, sizeof(int), s->st));
+    if (compute_forces_dev(s)) return -1;
+    BDH_HIP(hipMemcpyAsync(&fl, s->flags, sizeof(int), hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    BDH_CHECK(!(fl & 2), "Overstretched bond");
+    return 0;
+}
+
+// One IG-T predictor-corrector step, comp_step_IGT_const / _var (diff_comp.c:76-144) over
+// compute_hydro_IGT and its second phase (diff_equation.c:288-374):
+//   predictor  r* = r0 + Rf(r0) + sqrt(2 dt) L(r0) xi1, forces at r*;
+//   corrector  r  = r0 + (Rf(r0) + Rf(r*)) / 2 + sqrt(2 dt) L_avg xi2, forces at r,
+//              L_avg the factor of (D(r0) + D(r*)) / 2.
+// A rejected predictor is redrawn with L(r0) kept (igt_var also halves dt).  A rejected
+// corrector is redrawn alone with L_avg kept (igt_const; its first rejection is not
+// counted), or undoes the whole step (igt_var: dt halved, L(r0) kept).
+int step_igt(bdh_system* s)
+{
+    const bool chol = s->cfg.hydro == BDH_HYDRO_CHOLESKY;
+    const bool var = s->cfg.algorithm == BDH_IGT_VAR;
+    const size_t pb = sizeof(double4) * s->N, fb = sizeof(double) * s->n;
+    double dt = s->cfg.dt;
+    int count = 0, fl = 0;
+    BDH_HIP(hipMemcpyAsync(s->pos_prev, s->pos, pb, hipMemcpyDeviceToDevice, s->st));
+    BDH_HIP(hipMemcpyAsync(s->F_prev, s->F, fb, hipMemcpyDeviceToDevice, s->st));
+    if (chol && assemble_and_factor(s)) return -1;
+    const double* Rs = nullptr;
+    for (;;) {
+        // predictor
+        if (rf_dev(s, s->Rf, dt)) return -1;
+        if (!(Rs = rs_dev(s, dt, s->A))) return -1;
+        if (launch_em_update(s->pos, s->N, s->Rf, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+        if (forces_flags(s, fl)) return -1;
+        if (fl & 1) {
+            BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, pb, hipMemcpyDeviceToDevice, s->st));
+            BDH_HIP(hipMemcpyAsync(s->F, s->F_prev, fb, hipMemcpyDeviceToDevice, s->st));
+            if (var && count < 16) dt /= 2;
+            BDH_CHECK(++count <= s->cfg.move_attempts, "Wrong conformation, can't move");
+            continue;
+        }
+        // mean deterministic move and mean tensor of (r0, r*)
+        if (rf_dev(s, s->Rfs, dt)) return -1;
+        hipLaunchKernelGGL(mean2_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st, s->Rf, s->Rfs,
+                           s->n, s->Rf0);
+        BDH_LAUNCH_CHECK();
+        if (chol && assemble_and_factor(s, s->A2, s->pos_prev)) return -1;
+        BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, pb, hipMemcpyDeviceToDevice, s->st));
+        // corrector
+        bool ok = false;
+        for (int fails = 0;; ++fails) {
+            if (!(Rs = rs_dev(s, dt, s->A2))) return -1;
+            if (launch_em_update(s->pos, s->N, s->Rf0, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+            if (forces_flags(s, fl)) return -1;
+            if (!(fl & 1)) {
+                ok = true;
+                break;
+            }
+            BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, pb, hipMemcpyDeviceToDevice, s->st));
+            if (var) break;
+            if (fails > 0) BDH_CHECK(++count <= s->cfg.move_attempts, "Wrong conformation, can't move");
+        }
+        if (ok) break;
+        BDH_HIP(hipMemcpyAsync(s->F, s->F_prev, fb, hipMemcpyDeviceToDevice, s->st));
+        if (count < 16) dt /= 2;
+        BDH_CHECK(++count <= s->cfg.move_attempts, "Wrong conformation, can't move");
+    }
+    BDH_HIP(hipMemcpyAsync(s->Rf, s->Rf0, fb, hipMemcpyDeviceToDevice, s->st));
+    s->last_Rs = Rs;
+    s->last_dt = dt;
+    s->steps_done++;
+    return 0;
+}
+
+// steps steps; have_xi: the caller's Gaussians are in s->xi ([step][n]).  E-M steps without
+// rejections are taken in chunks of up to TRMV_MAX_RHS that never cross a refactorization.
 int run_steps(bdh_system* s, int steps, bool have_xi)
 {
+    if (s->cfg.algorithm == BDH_IGT_CONST || s->cfg.algorithm == BDH_IGT_VAR) {
+        BDH_CHECK(!have_xi, "IG-T steps draw their Gaussians from the device stream");
+        for (int k = 0; k < steps; ++k)
+            if (step_igt(s)) return -1;
+        return 0;
+    }
     const int n = s->n;
     const int R = s->cfg.refactor_every;
     const bool chol = s->cfg.hydro == BDH_HYDRO_CHOLESKY;
@@ -521,7 +646,7 @@ void bdh_destroy(bdh_system* s)
     dfree(s->conn_off); dfree(s->conn); dfree(s->bond_off); dfree(s->bond_p); dfree(s->bond_parm);
     dfree(s->F); dfree(s->E); dfree(s->PF); dfree(s->PE);
     dfree(s->A); dfree(s->Lbuf); dfree(s->xi); dfree(s->Pf); dfree(s->Pr); dfree(s->Rf); dfree(s->Rs);
-    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
+    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->A2); dfree(s->Rf0); dfree(s->Rfs); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
     dfree(s->Px); dfree(s->Pc); dfree(s->Pe); dfree(s->Feff); dfree(s->Cs); dfree(s->bt);
     dfree(s->cl.grid); dfree(s->cl.key); dfree(s->cl.count); dfree(s->cl.fill);
     dfree(s->cl.start); dfree(s->cl.sorted);
@@ -548,8 +673,11 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
               "bdh_create: precision");
     BDH_CHECK(cfg->dt > 0 && cfg->T > 0 && cfg->visc != 0 && cfg->vfactor != 0, "bdh_create: dt/T/visc");
     BDH_CHECK(nbonds == 0 || (bonds && bond_parms), "bdh_create: bonds");
-    BDH_CHECK(cfg->algorithm == BDH_ERMAK_CONST || cfg->algorithm == BDH_ERMAK_VAR,
-              "bdh_create: algorithm (ermak_const or ermak_var)");
+    const bool igt = cfg->algorithm == BDH_IGT_CONST || cfg->algorithm == BDH_IGT_VAR;
+    BDH_CHECK(cfg->algorithm == BDH_ERMAK_CONST || cfg->algorithm == BDH_ERMAK_VAR || igt,
+              "bdh_create: algorithm (ermak_const, ermak_var, igt_const or igt_var)");
+    BDH_CHECK(!igt || cfg->hydro != BDH_HYDRO_GEYER, "bdh_create: IG-T with TEA noise is not supported");
+    BDH_CHECK(!igt || cfg->refactor_every <= 1, "bdh_create: IG-T factors D at every step (refactor_every 1)");
     BDH_CHECK(cfg->move_attempts >= 0, "bdh_create: move_attempts");
     s->cfg = *cfg;
     if (s->cfg.refactor_every < 1) s->cfg.refactor_every = 1;
@@ -661,9 +789,13 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     }
     if (dmalloc(&s->Pr, sizeof(double) * s->n * s->tblocks * TRMV_MAX_RHS)) return -1;
     if (dmalloc(&s->RsB, sizeof(double) * s->n * TRMV_MAX_RHS)) return -1;
-    if (cfg->check_overlap) {
+    if (cfg->check_overlap || igt) {
         if (dmalloc(&s->pos_prev, sizeof(double4) * N)) return -1;
         if (dmalloc(&s->F_prev, sizeof(double) * s->n)) return -1;
+    }
+    if (igt) {
+        if (dmalloc(&s->Rf0, sizeof(double) * s->n)) return -1;
+        if (dmalloc(&s->Rfs, sizeof(double) * s->n)) return -1;
     }
     if (dmalloc(&s->Rf, sizeof(double) * s->n)) return -1;
     if (dmalloc(&s->Rs, sizeof(double) * s->n)) return -1;
@@ -693,13 +825,16 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     if (cfg->hydro == BDH_HYDRO_CHOLESKY) {
         const size_t es = s->mixed ? sizeof(float) : sizeof(double);
         const size_t bytes = es * (size_t)s->ld * (size_t)s->n_pad;
-        if (dmalloc(&s->A, bytes)) return -1;
         if (dmalloc(&s->Lbuf, sizeof(double) * chol_lbuf_doubles(s->n_pad))) return -1;
-        BDH_HIP(hipMemsetAsync(s->A, 0, bytes, s->st));
-        if (s->mixed) {
-            if (launch_set_identity_pad<float>((float*)s->A, s->ld, s->n, s->n_pad, s->st)) return -1;
-        } else {
-            if (launch_set_identity_pad<double>((double*)s->A, s->ld, s->n, s->n_pad, s->st)) return -1;
+        for (void** M : {&s->A, &s->A2}) {
+            if (M == &s->A2 && !igt) break;
+            if (dmalloc(M, bytes)) return -1;
+            BDH_HIP(hipMemsetAsync(*M, 0, bytes, s->st));
+            if (s->mixed) {
+                if (launch_set_identity_pad<float>((float*)*M, s->ld, s->n, s->n_pad, s->st)) return -1;
+            } else {
+                if (launch_set_identity_pad<double>((double*)*M, s->ld, s->n, s->n_pad, s->st)) return -1;
+            }
         }
     }
     // initial forces (main_loop.c:91)

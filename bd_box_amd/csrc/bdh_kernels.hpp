@@ -25,7 +25,7 @@ int launch_gauss_streams(uint64_t* d_states, double* d_out, const long long* d_o
 // ---- RPY tensor (bdh_rpy.hip)
 template <typename T>
 int launch_rpy_assemble(const double4* d_pos, int N, double preDii, double preDij, T* d_A,
-                        long long ld, hipStream_t st);
+                        long long ld, hipStream_t st, const double4* d_pos2 = nullptr);
 template <typename T>
 int launch_set_identity_pad(T* d_A, long long ld, int n, int n_pad, hipStream_t st);
 
@@ -39,7 +39,7 @@ int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi
                      double* d_Pf, double* d_Px, double* d_Pc, double* d_Pe, hipStream_t st);
 template <typename T>
 int launch_ewald_assemble(const double4* d_pos, int N, double preDii, double preDij, T* d_A,
-                          long long ld, hipStream_t st);
+                          long long ld, hipStream_t st, const double4* d_pos2 = nullptr);
 // S = pair_eparts(N) eps partials in d_Pe
 int launch_tea_close(const double4* d_pos, const double* d_xi, const double* d_Feff,
                      const double* d_C, const double* d_Pe, int S, int N, double preDii, double dt,

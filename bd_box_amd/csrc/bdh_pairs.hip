@@ -305,10 +305,11 @@ int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi
 // ------------------------------------------------------------------ Ewald assembly
 // Lower triangle of the Beenakker-Ewald D (rows of i, columns of j, i >= j), the same
 // tiling as the free-space assembly: (64 row beads) x (16 column beads) per workgroup.
+// With pos2, the mean of the blocks at pos and pos2 (the IG-T tensor).
 template <typename T>
 __global__ void __launch_bounds__(256)
-ewald_assemble_kernel(const double4* __restrict__ pos, int N, double preDii, double preDij,
-                      T* __restrict__ A, long long ld)
+ewald_assemble_kernel(const double4* __restrict__ pos, const double4* __restrict__ pos2, int N,
+                      double preDii, double preDij, T* __restrict__ A, long long ld)
 {
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 16;
     if (j0 > i0 + 63) return;
@@ -318,8 +319,15 @@ ewald_assemble_kernel(const double4* __restrict__ pos, int N, double preDii, dou
     for (int q = 0; q < 4; ++q) {
         const int j = j0 + threadIdx.y * 4 + q;
         if (j >= N || j > i) continue;
-        const Blk6 B = beenakker_block<T>(pos[j].x - pi.x, pos[j].y - pi.y, pos[j].z - pi.z, pi.w,
-                                          pos[j].w, i == j, preDii, preDij);
+        Blk6 B = beenakker_block<T>(pos[j].x - pi.x, pos[j].y - pi.y, pos[j].z - pi.z, pi.w,
+                                    pos[j].w, i == j, preDii, preDij);
+        if (pos2 && i != j) {
+            const double4 qi = pos2[i], qj = pos2[j];
+            const Blk6 B2 = beenakker_block<T>(qj.x - qi.x, qj.y - qi.y, qj.z - qi.z, qi.w, qj.w, false,
+                                               preDii, preDij);
+            B.xx = (B.xx + B2.xx) / 2; B.yy = (B.yy + B2.yy) / 2; B.zz = (B.zz + B2.zz) / 2;
+            B.xy = (B.xy + B2.xy) / 2; B.xz = (B.xz + B2.xz) / 2; B.yz = (B.yz + B2.yz) / 2;
+        }
         T* c0 = A + (long long)(3 * j) * ld + 3 * i;
         c0[0] = (T)B.xx; c0[1] = (T)B.xy; c0[2] = (T)B.xz;
         c0[ld] = (T)B.xy; c0[ld + 1] = (T)B.yy; c0[ld + 2] = (T)B.yz;
@@ -329,11 +337,11 @@ ewald_assemble_kernel(const double4* __restrict__ pos, int N, double preDii, dou
 
 template <typename T>
 int launch_ewald_assemble(const double4* d_pos, int N, double preDii, double preDij, T* d_A,
-                          long long ld, hipStream_t st)
+                          long long ld, hipStream_t st, const double4* d_pos2)
 {
     dim3 grid(ceil_div(N, 64), ceil_div(N, 16));
-    hipLaunchKernelGGL(ewald_assemble_kernel<T>, grid, dim3(64, 4), 0, st, d_pos, N, preDii, preDij,
-                       d_A, ld);
+    hipLaunchKernelGGL(ewald_assemble_kernel<T>, grid, dim3(64, 4), 0, st, d_pos, d_pos2, N, preDii,
+                       preDij, d_A, ld);
     BDH_LAUNCH_CHECK();
     return 0;
 }
@@ -396,7 +404,9 @@ int launch_tea_close(const double4* d_pos, const double* d_xi, const double* d_F
 
 template int launch_pair_sums<float>(const double4*, const double*, const double*, int, double, double, double, bool, bool, int, double*, double*, double*, double*, hipStream_t);
 template int launch_pair_sums<double>(const double4*, const double*, const double*, int, double, double, double, bool, bool, int, double*, double*, double*, double*, hipStream_t);
-template int launch_ewald_assemble<float>(const double4*, int, double, double, float*, long long, hipStream_t);
-template int launch_ewald_assemble<double>(const double4*, int, double, double, double*, long long, hipStream_t);
+template int launch_ewald_assemble<float>(const double4*, int, double, double, float*, long long, hipStream_t,
+                                          const double4*);
+template int launch_ewald_assemble<double>(const double4*, int, double, double, double*, long long, hipStream_t,
+                                           const double4*);
 
 }  // namespace bdh

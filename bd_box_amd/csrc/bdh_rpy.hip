@@ -69,25 +69,30 @@ __device__ __forceinline__ Rpy6<T> rpy_pair(double dx, double dy, double dz, T a
 constexpr int ASM_ROWS = 64;
 constexpr int ASM_COLS = 16;
 
+// With pos2, the blocks are the mean of those at pos and pos2 (the IG-T tensor
+// (D(r0) + D(r*)) / 2 of compute_hydro_IGT, diff_equation.c:327-352; the diagonal blocks do
+// not depend on the positions).
 template <typename T>
 __global__ void __launch_bounds__(256)
-rpy_assemble_kernel(const double4* __restrict__ pos, int N, T preDii, T preDij,
-                    T* __restrict__ A, long long ld)
+rpy_assemble_kernel(const double4* __restrict__ pos, const double4* __restrict__ pos2, int N,
+                    T preDii, T preDij, T* __restrict__ A, long long ld)
 {
     const int i0 = blockIdx.x * ASM_ROWS;
     const int j0 = blockIdx.y * ASM_COLS;
     if (j0 > i0 + ASM_ROWS - 1) return;  // tile above the diagonal
-    __shared__ double4 s_col[ASM_COLS];
+    __shared__ double4 s_col[ASM_COLS], s_col2[ASM_COLS];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int tid = ty * ASM_ROWS + tx;
     if (tid < ASM_COLS) {
         int j = j0 + tid;
         s_col[tid] = (j < N) ? pos[j] : make_double4(0, 0, 0, 1);
+        if (pos2) s_col2[tid] = (j < N) ? pos2[j] : make_double4(0, 0, 0, 1);
     }
     __syncthreads();
     const int i = i0 + tx;
     if (i >= N) return;
     const double4 pi = pos[i];
+    const double4 pi2 = pos2 ? pos2[i] : pi;
     const T ai = (T)pi.w;
 #pragma unroll
     for (int q = 0; q < ASM_COLS / 4; ++q) {
@@ -104,6 +109,13 @@ rpy_assemble_kernel(const double4* __restrict__ pos, int N, T preDii, T preDij,
         }
         const double4 pj = s_col[jl];
         Rpy6<T> b = rpy_pair<T>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, ai, (T)pj.w, preDii, preDij);
+        if (pos2) {
+            const double4 pj2 = s_col2[jl];
+            const Rpy6<T> b2 = rpy_pair<T>(pi2.x - pj2.x, pi2.y - pj2.y, pi2.z - pj2.z, ai, (T)pj2.w,
+                                           preDii, preDij);
+            b.xx = (b.xx + b2.xx) / (T)2; b.yy = (b.yy + b2.yy) / (T)2; b.zz = (b.zz + b2.zz) / (T)2;
+            b.xy = (b.xy + b2.xy) / (T)2; b.xz = (b.xz + b2.xz) / (T)2; b.yz = (b.yz + b2.yz) / (T)2;
+        }
         c0[0] = b.xx; c0[1] = b.xy; c0[2] = b.xz;
         c0[ld + 0] = b.xy; c0[ld + 1] = b.yy; c0[ld + 2] = b.yz;
         c0[2 * ld + 0] = b.xz; c0[2 * ld + 1] = b.yz; c0[2 * ld + 2] = b.zz;
@@ -121,10 +133,10 @@ __global__ void set_identity_pad_kernel(T* __restrict__ A, long long ld, int n, 
 
 template <typename T>
 int launch_rpy_assemble(const double4* d_pos, int N, double preDii, double preDij, T* d_A,
-                        long long ld, hipStream_t st)
+                        long long ld, hipStream_t st, const double4* d_pos2)
 {
     dim3 grid(ceil_div(N, ASM_ROWS), ceil_div(N, ASM_COLS));
-    hipLaunchKernelGGL(rpy_assemble_kernel<T>, grid, dim3(ASM_ROWS, 4), 0, st, d_pos, N,
+    hipLaunchKernelGGL(rpy_assemble_kernel<T>, grid, dim3(ASM_ROWS, 4), 0, st, d_pos, d_pos2, N,
                        (T)preDii, (T)preDij, d_A, ld);
     BDH_LAUNCH_CHECK();
     return 0;
@@ -140,8 +152,10 @@ int launch_set_identity_pad(T* d_A, long long ld, int n, int n_pad, hipStream_t 
     return 0;
 }
 
-template int launch_rpy_assemble<float>(const double4*, int, double, double, float*, long long, hipStream_t);
-template int launch_rpy_assemble<double>(const double4*, int, double, double, double*, long long, hipStream_t);
+template int launch_rpy_assemble<float>(const double4*, int, double, double, float*, long long, hipStream_t,
+                                        const double4*);
+template int launch_rpy_assemble<double>(const double4*, int, double, double, double*, long long, hipStream_t,
+                                         const double4*);
 template int launch_set_identity_pad<float>(float*, long long, int, int, hipStream_t);
 template int launch_set_identity_pad<double>(double*, long long, int, int, hipStream_t);
 

@@ -28,15 +28,16 @@ MIXED_SPLIT = 2
 
 
 def _algorithm(P: Params) -> int:
-    """ermak_const / ermak_var as given; ermak_newton takes the ermak_const path, which is
-    the same step whenever no step is rejected (comp_step_ermak_newton,
-    diff_comp.c:181-208), so it is accepted only without check_overlap."""
-    if P.algorithm in (0, 1):
+    """ermak_const / ermak_var / igt_const / igt_var as given; ermak_newton takes the
+    ermak_const path, which is the same step whenever no step is rejected
+    (comp_step_ermak_newton, diff_comp.c:181-208), so it is accepted only without
+    check_overlap."""
+    if P.algorithm in (0, 1, 3, 4):
         return P.algorithm
     if P.algorithm == 2 and not P.check_overlap:
         return 0
-    raise NotImplementedError("HIP backend: algorithm ermak_const / ermak_var "
-                              "(ermak_newton only with check_overlap no)")
+    raise NotImplementedError("HIP backend: algorithm ermak_const / ermak_var / igt_const / "
+                              "igt_var (ermak_newton only with check_overlap no)")
 
 
 def make_config(P: Params, precision: int = MIXED, refactor_every: int = 1) -> _lib.Config:

@@ -39,8 +39,9 @@ enum bdh_precision {
 
 enum bdh_hydro { BDH_HYDRO_NONE = 0, BDH_HYDRO_CHOLESKY = 1, BDH_HYDRO_GEYER = 2 };
 
-/* integration algorithm (src/diff_alg/diff_tensor.h:29-30) */
-enum bdh_algorithm { BDH_ERMAK_CONST = 0, BDH_ERMAK_VAR = 1 };
+/* integration algorithm (src/diff_alg/diff_tensor.h:29-33); IG-T with hydro none or
+ * cholesky */
+enum bdh_algorithm { BDH_ERMAK_CONST = 0, BDH_ERMAK_VAR = 1, BDH_IGT_CONST = 3, BDH_IGT_VAR = 4 };
 
 /* Control parameters after the reference's .prm parsing (src/input.c:353-503): DOUBLE
  * keywords already rounded through float (src/input.c:933). */

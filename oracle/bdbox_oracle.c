@@ -604,6 +604,7 @@ struct orc_sim {
     double *coord, *Q, *LJ, *bond_parms, *F, *D, *L, *Rf, *Rs, *X;
     int* bonds;
     double *coord_prev, *F_prev;
+    double *D1, *L2, *Rf0;          /* IG-T: D at the predictor, the averaged factor, Rf0 */
     uint64_t rng;
     int have_L;
     int algorithm, move_attempts;   /* 0 ermak_const, 1 ermak_var; rejections allowed */
@@ -699,12 +700,144 @@ static int sim_attempt(orc_sim* s, double dt, const double* X_in, double preDii)
    (positions and forces restored, load_coord / load_forces) and redrawn with the next
    Gaussians of the stream, ermak_var halving dt for the first 16 rejections; more than
    move_attempts rejections fail. */
+/* Rf = pre D F (hydro cholesky: D stored in Dm; none: diag_D) */
+static void sim_rf(orc_sim* s, const double* Dm, double pre, double preDii, double* out)
+{
+    const int n = 3 * s->N;
+    int i;
+    if (s->p.hydro == 1)
+    {
+        orc_symv_lower(n, Dm, n, s->F, out);
+        for (i = 0; i < n; ++i) out[i] *= pre;
+    }
+    else
+        for (i = 0; i < n; ++i) out[i] = pre * (preDii / s->coord[4 * (i / 3) + 3]) * s->F[i];
+}
+
+/* Rs from the next Gaussians of the stream: L sqrt(2 dt) X, or X sqrt(2 dt diag_D) */
+static void sim_rs(orc_sim* s, const double* Lm, double dt, double preDii)
+{
+    const int n = 3 * s->N;
+    int i;
+    orc_gauss(&s->rng, s->X, n);
+    if (s->p.hydro == 1)
+    {
+        double a = sqrt(2 * dt);
+        for (i = 0; i < n; ++i) s->X[i] *= a;
+        orc_trmv_lower(n, Lm, n, s->X, s->Rs);
+    }
+    else
+        for (i = 0; i < n; ++i) s->Rs[i] = s->X[i] * sqrt((preDii / s->coord[4 * (i / 3) + 3]) * 2 * dt);
+}
+
+/* coord = coord + (a + b), then the box wrap (diff_equation.c:254-264, :296-306) */
+static void sim_move(orc_sim* s, const double* a, const double* b)
+{
+    int i, k;
+    for (i = 0; i < s->N; ++i)
+        for (k = 0; k < 3; ++k)
+        {
+            s->coord[4 * i + k] = (a[3 * i + k] + b[3 * i + k]) + s->coord[4 * i + k];
+            if (s->p.bc == 1)
+            {
+                double inv = 1 / s->p.box;
+                s->coord[4 * i + k] -= s->p.box * floor(s->coord[4 * i + k] * inv + 0.5f);
+            }
+        }
+}
+
+/* One IG-T predictor-corrector step, comp_step_IGT_const / _var (diff_comp.c:76-144) over
+   compute_hydro_IGT and its second phase (diff_equation.c:288-374):
+     predictor  r* = r0 + Rf(r0) + L(r0) sqrt(2 dt) X1, forces at r*;
+     corrector  r  = r0 + (Rf(r0) + Rf(r*)) / 2 + L_avg sqrt(2 dt) X2,
+                L_avg the factor of (D(r0) + D(r*)) / 2, forces at r.
+   Rejections: a rejected predictor is redrawn (igt_var halves dt); a rejected corrector is
+   redrawn alone with L_avg kept (igt_const; its first rejection is not counted) or undoes
+   the whole step (igt_var, dt halved, L(r0) kept).  hydro none uses diag_D throughout. */
+static int sim_step_igt(orc_sim* s, double preDii)
+{
+    const int N = s->N, n = 3 * N;
+    const int var = s->algorithm == 4;
+    double dt = s->p.dt;
+    int count = 0, i;
+    memcpy(s->coord_prev, s->coord, sizeof(double) * 4 * N);
+    memcpy(s->F_prev, s->F, sizeof(double) * n);
+    if (s->p.hydro == 1)
+    {
+        orc_compute_D(N, s->coord, &s->p, s->D, n);
+        memcpy(s->L, s->D, sizeof(double) * (size_t)n * n);
+        if (orc_cholesky(n, s->L, n)) return 1;
+    }
+    for (;;)
+    {
+        const double pre = dt / (KB * s->p.T);
+        int rep, fails = 0;
+        /* predictor */
+        sim_rf(s, s->D, pre, preDii, s->Rf);
+        sim_rs(s, s->L, dt, preDii);
+        sim_move(s, s->Rf, s->Rs);
+        rep = orc_forces(N, s->coord, s->Q, s->LJ, s->nb, s->bonds, s->bond_parms, &s->p, s->F, NULL);
+        if (rep < 0) return 2;
+        if (rep && s->p.check_overlap)
+        {
+            memcpy(s->coord, s->coord_prev, sizeof(double) * 4 * N);
+            memcpy(s->F, s->F_prev, sizeof(double) * n);
+            if (var && count < 16) dt /= 2;
+            if (++count > s->move_attempts) return 3;
+            continue;
+        }
+        /* averaged deterministic move and tensor at (r0, r*) */
+        if (s->p.hydro == 1) orc_compute_D(N, s->coord, &s->p, s->D1, n);
+        sim_rf(s, s->D1, pre, preDii, s->Rs);
+        for (i = 0; i < n; ++i) s->Rf0[i] = 0.5 * (s->Rf[i] + s->Rs[i]);
+        if (s->p.hydro == 1)
+        {
+            size_t e;
+            for (e = 0; e < (size_t)n * n; ++e) s->L2[e] = (s->D[e] + s->D1[e]) / 2;
+            if (orc_cholesky(n, s->L2, n)) return 1;
+        }
+        memcpy(s->coord, s->coord_prev, sizeof(double) * 4 * N);
+        /* corrector */
+        for (;;)
+        {
+            sim_rs(s, s->L2, dt, preDii);
+            sim_move(s, s->Rf0, s->Rs);
+            rep = orc_forces(N, s->coord, s->Q, s->LJ, s->nb, s->bonds, s->bond_parms, &s->p, s->F, NULL);
+            if (rep < 0) return 2;
+            if (!(rep && s->p.check_overlap)) break;
+            memcpy(s->coord, s->coord_prev, sizeof(double) * 4 * N);
+            if (var) break;
+            if (fails++ > 0 && ++count > s->move_attempts) return 3;
+        }
+        if (!(rep && s->p.check_overlap)) break;
+        /* igt_var: the whole step is undone */
+        memcpy(s->F, s->F_prev, sizeof(double) * n);
+        if (count < 16) dt /= 2;
+        if (++count > s->move_attempts) return 3;
+    }
+    memcpy(s->Rf, s->Rf0, sizeof(double) * n);
+    s->last_dt = dt;
+    s->rejections += count;
+    s->steps_done++;
+    return 0;
+}
+
 int orc_sim_run(orc_sim* s, int steps, const double* xi)
 {
     const int N = s->N, n = 3 * N;
     double preDii, preDij;
     int st;
     orc_prefactors(&s->p, &preDii, &preDij);
+    if (s->algorithm >= 3)
+    {
+        if (xi || s->p.hydro == 2) return 4;   /* IG-T: stream noise, hydro none / cholesky */
+        for (st = 0; st < steps; ++st)
+        {
+            int rc = sim_step_igt(s, preDii);
+            if (rc) return rc;
+        }
+        return 0;
+    }
     for (st = 0; st < steps; ++st)
     {
         double dt = s->p.dt;
@@ -742,8 +875,18 @@ int orc_sim_run(orc_sim* s, int steps, const double* xi)
 
 void orc_sim_set_algorithm(orc_sim* s, int algorithm, int move_attempts)
 {
+    const size_t n = 3 * (size_t)s->N;
     s->algorithm = algorithm;
     s->move_attempts = move_attempts;
+    if (algorithm >= 3 && !s->Rf0)
+    {
+        s->Rf0 = (double*)calloc(n, sizeof(double));
+        if (s->p.hydro == 1)
+        {
+            s->D1 = (double*)malloc(sizeof(double) * n * n);
+            s->L2 = (double*)malloc(sizeof(double) * n * n);
+        }
+    }
 }
 
 double orc_sim_last_dt(const orc_sim* s) { return s->last_dt; }
@@ -764,5 +907,6 @@ void orc_sim_destroy(orc_sim* s)
     free(s->coord); free(s->Q); free(s->LJ); free(s->bonds); free(s->bond_parms);
     free(s->F); free(s->D); free(s->L); free(s->Rf); free(s->Rs); free(s->X);
     free(s->coord_prev); free(s->F_prev);
+    free(s->D1); free(s->L2); free(s->Rf0);
     free(s);
 }

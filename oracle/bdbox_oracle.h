@@ -96,10 +96,12 @@ orc_sim* orc_sim_create(int N, const double* coord, const double* Q, const doubl
 /* Advance `steps` steps; when xi != NULL the Gaussians of each step's first attempt are
    taken from xi[steps*3N] instead of the internal drand48 stream (redraws after a rejected
    attempt always come from the stream).  Returns 0; 1 on a failed factorization, 2 on an
-   overstretched bond, 3 after more than move_attempts rejections of one step. */
+   overstretched bond, 3 after more than move_attempts rejections of one step, 4 for an
+   unsupported IG-T request. */
 int  orc_sim_run(orc_sim* s, int steps, const double* xi);
-/* integration algorithm (0 ermak_const, 1 ermak_var) and the rejection limit (default
-   10000, diff_decl.c:71) */
+/* integration algorithm (0 ermak_const, 1 ermak_var, 3 igt_const, 4 igt_var; IG-T with
+   hydro none or cholesky and stream noise) and the rejection limit (default 10000,
+   diff_decl.c:71) */
 void orc_sim_set_algorithm(orc_sim* s, int algorithm, int move_attempts);
 double orc_sim_last_dt(const orc_sim* s);          /* dt of the last step taken */
 long long orc_sim_rejections(const orc_sim* s);    /* rejected attempts so far */

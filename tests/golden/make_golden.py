@@ -141,6 +141,20 @@ def main():
                                           check_overlap="yes", move_attempts=200,
                                           rand_seed=11),
              [["traj", "8"]])
+    # 7. IG-T predictor-corrector (comp_step_IGT_const / _var, diff_comp.c:76-144;
+    #    compute_hydro_IGT, diff_equation.c:288-374), with and without rejections
+    s16i = wl.chain(16, seed=3)
+    for name, alg, hydro in (("igt_const", "igt_const", "cholesky"),
+                             ("igt_var_none", "igt_var", "none")):
+        case(name, s16i, wl.chain_params(hydro=hydro, algorithm=alg, rand_seed=4242),
+             [["traj", "6"]])
+    for name, alg, dt, hydro in (("igt_const_retry", "igt_const", 0.5, "cholesky"),
+                                 ("igt_var_retry", "igt_var", 2.0, "cholesky"),
+                                 ("igt_var_retry_none", "igt_var", 2.0, "none")):
+        case(name, s24r, wl.chain_params(hydro=hydro, algorithm=alg, dt=dt,
+                                          check_overlap="yes", move_attempts=200,
+                                          rand_seed=11),
+             [["traj", "8"]])
 
 
 if __name__ == "__main__":

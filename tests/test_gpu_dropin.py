@@ -68,6 +68,10 @@ DROPIN_CASES = {
     "retry_const": (dict(lj_diam=3.3), dict(hydro="cholesky", algorithm="ermak_const", dt=0.5,
                                             check_overlap="yes", move_attempts=1000)),
     "geyer": ({}, dict(hydro="geyer")),
+    # IG-T predictor-corrector (diff_comp.c:76-144), rejections redrawn on the device
+    "igt_var_retry": (dict(lj_diam=3.3), dict(hydro="cholesky", algorithm="igt_var", dt=2.0,
+                                              check_overlap="yes", move_attempts=200)),
+    "igt_const": ({}, dict(hydro="cholesky", algorithm="igt_const")),
     "bucket_list": ({}, dict(hydro="cholesky", nb_list="bucket_list")),
     "bucket_space": ({}, dict(hydro="none", nb_list="bucket_space")),
 }

@@ -262,3 +262,34 @@ def test_rejection_loop_discriminates(golden):
     tight = po.Sim(sysm, p, seed=P.rand_seed, algorithm=P.algorithm, move_attempts=0)
     with pytest.raises(RuntimeError, match=r"\(3\)"):
         tight.run(8)
+
+
+IGT_CASES = [("igt_const", 6), ("igt_var_none", 6), ("igt_const_retry", 8),
+             ("igt_var_retry", 8), ("igt_var_retry_none", 8)]
+
+
+@pytest.mark.parametrize("case,steps", IGT_CASES)
+def test_igt_vs_reference(golden, case, steps):
+    """IG-T predictor-corrector (comp_step_IGT_const / _var, diff_comp.c:76-144) against the
+    reference's own trajectory and per-step curr_dt, rejections included."""
+    g, sysm, P, p = golden(case)
+    assert P.algorithm in (3, 4)
+    sim = po.Sim(sysm, p, seed=P.rand_seed, algorithm=P.algorithm,
+                 move_attempts=P.move_attempts)
+    traj = g["traj.coord"].reshape(steps, -1, 4)
+    for k in range(steps):
+        sim.run(1)
+        assert sim.last_dt() == g["traj.dt"][k], k
+        err = np.max(np.abs(sim.coords() - traj[k]))
+        assert err < 1e-11, (k, err)
+    if case.endswith("retry") or case.endswith("retry_none"):
+        assert sim.rejections() > 0
+
+
+def test_igt_is_not_ermak(golden):
+    """The same stream through the E-M step departs from the IG-T trajectory (the check
+    distinguishes the corrector from a plain step)."""
+    g, sysm, P, p = golden("igt_const")
+    sim = po.Sim(sysm, p, seed=P.rand_seed, algorithm=0)
+    sim.run(1)
+    assert np.max(np.abs(sim.coords() - g["traj.coord"].reshape(6, -1, 4)[0])) > 1e-6
