@@ -36,7 +36,7 @@ class Config(C.Structure):
         ("box", C.c_double), ("hydro", C.c_int), ("ewald_method", C.c_int),
         ("ewald_real", C.c_int), ("precision", C.c_int), ("refactor_every", C.c_int),
         ("rand_seed", C.c_uint32), ("nb_list", C.c_int), ("algorithm", C.c_int),
-        ("move_attempts", C.c_int),
+        ("move_attempts", C.c_int), ("ewald_alpha", C.c_double),
     ]
 
 

@@ -16,7 +16,7 @@
  * Supported: algorithm ermak_const / ermak_var (with the reference's rejection of steps that
  * overlap when check_overlap is set, diff_comp.c:152-179) and the IG-T predictor-corrector
  * igt_const / igt_var (diff_comp.c:76-144; hydro none or cholesky), hydro none / cholesky /
- * geyer, bc none or periodic (cube; Beenakker-Ewald tensor with hydrodynamics), nb_list
+ * geyer, bc none or periodic (cube; Smith- or Beenakker-Ewald tensor with hydrodynamics), nb_list
  * brute or either bucket method (both map to the backend's device cell list, which finds
  * the same interacting pairs), FP64 or mixed precision
  * (environment BDBOX_HIP_PRECISION=fp64|mixed|split, default fp64), refactor interval
@@ -126,8 +126,6 @@ void init_diff_tensor(void)
     if (bc != BC_NONE && bc != BC_BOX) UNERR("HIP backend: bc none or periodic");
     if (bc == BC_BOX && (box[0] != box[1] || box[0] != box[2]))
         UNERR("HIP backend: periodic box must be a cube");
-    if (bc == BC_BOX && hydro != DIFF_NONE && ewald_method != EWALD_METHOD_BEENAKKER)
-        UNERR("HIP backend: periodic hydrodynamics need ewald_method beenakker");
     memset(&cfg, 0, sizeof(cfg));
     cfg.dt = dt; cfg.T = T; cfg.visc = visc; cfg.vfactor = vfactor;
     cfg.alpha_lj = alpha_lj; cfg.cutoff_lj = cutoff_lj; cfg.lj_6_term = lj_6_term;
@@ -136,6 +134,7 @@ void init_diff_tensor(void)
     cfg.check_overlap = check_overlap; cfg.bc = bc == BC_BOX; cfg.box = box[0];
     cfg.nb_list = nb_list;
     cfg.algorithm = algorithm; cfg.move_attempts = move_attempts;
+    cfg.ewald_alpha = ewald_alpha;
     cfg.hydro = hydro; cfg.ewald_method = ewald_method; cfg.ewald_real = ewald_real;
     cfg.precision = !prec || !strcmp(prec, "fp64") ? BDH_FP64
                   : !strcmp(prec, "mixed")            ? BDH_MIXED

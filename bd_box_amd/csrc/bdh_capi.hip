@@ -19,6 +19,7 @@
 
 #include "../../include/bdbox_hip.h"
 #include "bdh_common.hpp"
+#include "bdh_ewald.hpp"
 #include "bdh_kernels.hpp"
 
 namespace bdh {
@@ -240,17 +241,22 @@ int compute_forces_dev(bdh_system* s, bool energies = false)
 
 // The Ewald tables live in constant memory of the module: reload them when the system
 // asks for a different box or lattice radius than the last one loaded.
-double g_ew_box = -1.0;
-int g_ew_R = -1;
+double g_ew_box = -1.0, g_ew_alpha = -1.0;
+int g_ew_R = -1, g_ew_method = -1;
 int g_ew_dev = -1;
 
 int ensure_ewald(bdh_system* s)
 {
     if (!s->ewald) return 0;
-    if (g_ew_box == s->cfg.box && g_ew_R == s->cfg.ewald_real && g_ew_dev == s->device) return 0;
-    if (ewald_upload(s->cfg.box, s->cfg.ewald_real) < 0) return -1;
-    g_ew_box = s->cfg.box;
-    g_ew_R = s->cfg.ewald_real;
+    const bdh_config& c = s->cfg;
+    if (g_ew_box == c.box && g_ew_R == c.ewald_real && g_ew_method == c.ewald_method &&
+        (c.ewald_method != EW_SMITH || g_ew_alpha == c.ewald_alpha) && g_ew_dev == s->device)
+        return 0;
+    if (ewald_upload(c.box, c.ewald_real, c.ewald_method, c.ewald_alpha) < 0) return -1;
+    g_ew_box = c.box;
+    g_ew_R = c.ewald_real;
+    g_ew_method = c.ewald_method;
+    g_ew_alpha = c.ewald_alpha;
     g_ew_dev = s->device;
     return 0;
 }
@@ -664,8 +670,11 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
               cfg->hydro == BDH_HYDRO_GEYER, "bdh_create: hydro must be none, cholesky or geyer");
     BDH_CHECK(cfg->bc == 0 || cfg->bc == 1, "bdh_create: bc must be none or periodic");
     // periodic boundaries with hydrodynamics mean an Ewald-summed tensor (input.c:390)
-    BDH_CHECK(!(cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE && cfg->ewald_method != 1),
-              "bdh_create: only Beenakker Ewald summation is available (ewald_method beenakker)");
+    BDH_CHECK(!(cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE &&
+                cfg->ewald_method != EW_SMITH && cfg->ewald_method != EW_BEENAKKER),
+              "bdh_create: ewald_method smith (0) or beenakker (1)");
+    BDH_CHECK(!(cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE && cfg->ewald_method == EW_SMITH &&
+                !(cfg->ewald_alpha > 0)), "bdh_create: ewald_alpha must be positive");
     BDH_CHECK(!(cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE) ||
               (cfg->ewald_real >= 0 && cfg->ewald_real <= 5), "bdh_create: ewald_real 0..5");
     BDH_CHECK(cfg->bc == 0 || cfg->box > 0, "bdh_create: box edge");

@@ -1,4 +1,5 @@
-// Beenakker-Ewald RPY blocks on the device (src/diff_alg/diff_tensor.c:265-422).
+// Ewald-summed RPY blocks on the device: Beenakker (src/diff_alg/diff_tensor.c:265-422)
+// and Smith (:102-263).
 //
 // alpha = sqrt(pi)/L (forced by init_tensor, diff_tensor.c:551-554); real and reciprocal
 // lattices share the sphere h^2 + f^2 + g^2 <= R^2 (ewald_real == ewald_recip,
@@ -19,20 +20,28 @@ namespace bdh {
 
 constexpr int EW_MAXV = 700;  // lattice vectors with |hfg| <= 5
 
+enum EwaldMethod { EW_SMITH = 0, EW_BEENAKKER = 1 };
+
 struct EwaldTables {
     int nv;                  // vectors in the sphere (including 0)
     int R;                   // lattice sphere radius
+    int method;              // EwaldMethod
     double box, inv_L, alpha, alpha2, alpha3, inv_alpha2, preQ;
-    double self_a, self_b;   // self term: -6 alpha/sqrt(pi) and 40 alpha^3/(3 sqrt(pi))
+    double self_a, self_b;   // Beenakker self term: -6 alpha/sqrt(pi), 40 alpha^3/(3 sqrt(pi))
+    double preO;             // Smith: 0.75 / L
 };
 
 extern __constant__ EwaldTables c_ew;
 // per lattice vector, h then f then g ascending over the sphere (the reference's order):
-extern __constant__ double4 c_recip[EW_MAXV];    // (G_k, Rk, e_x, e_y); e_z in c_recip_z
-extern __constant__ double c_recip_z[EW_MAXV];
+//   Beenakker: (G_k, Rk, e_x, e_y);  Smith: (2 exp(m k^2) / (pi k^2), -1 + m k^2, e_x, e_y) with
+//   m = -pi^2 / alpha^2 and k^2 = h^2 + f^2 + g^2, and 4 pi exp(m k^2) in c_recip_w
+extern __constant__ double4 c_recip[EW_MAXV];
+extern __constant__ double c_recip_z[EW_MAXV];   // e_z
+extern __constant__ double c_recip_w[EW_MAXV];
 
-// Upload the tables for box edge L and radius R; returns the vector count.
-int ewald_upload(double box, int R);
+// Upload the tables of one method for box edge L, lattice radius R and (Smith) the
+// splitting parameter alpha in units of 1/L; returns the vector count.
+int ewald_upload(double box, int R, int method, double alpha);
 
 struct Blk6 {
     double xx, yy, zz, xy, xz, yz;
@@ -60,6 +69,29 @@ __device__ __forceinline__ int isqrt_small(int m)
     return r;
 }
 
+// Overlapping beads: + overlapping RPY - non-overlapping RPY, which the lattice sums count
+// (diff_tensor.c:115-148 Smith, :279-313 Beenakker; the same terms).
+__device__ __forceinline__ void overlap_correction(double rx, double ry, double rz, double r2,
+                                                   double ai, double aj, double preDii,
+                                                   double preDij, Blk6& out)
+{
+    const double rn = sqrt(r2), inv = 1.0 / rn;
+    const double ex = rx * inv, ey = ry * inv, ez = rz * inv;
+    const double sig = 0.5 * (ai + aj);
+    const double pre1 = preDii / sig;
+    const double part1 = 1.0 - 9.0 * rn / (32.0 * sig);
+    const double c1 = 3.0 * rn / (32.0 * sig);
+    const double pre2 = preDij / rn;
+    const double sr = (ai * ai + aj * aj) / r2;
+    const double d2 = 1.0 + sr / 3.0, o2 = 1.0 - sr;
+    out.xx += pre1 * (part1 + c1 * ex * ex) - pre2 * (d2 + ex * ex * o2);
+    out.yy += pre1 * (part1 + c1 * ey * ey) - pre2 * (d2 + ey * ey * o2);
+    out.zz += pre1 * (part1 + c1 * ez * ez) - pre2 * (d2 + ez * ez * o2);
+    out.xy += pre1 * c1 * ex * ey - pre2 * ex * ey * o2;
+    out.xz += pre1 * c1 * ex * ez - pre2 * ex * ez * o2;
+    out.yz += pre1 * c1 * ey * ez - pre2 * ey * ez * o2;
+}
+
 // Beenakker block for the pair (i, j): r = r_j - r_i (no minimum image, dist_vec), radii
 // ai, aj; self = (i == j) adds the self term and clears the off-diagonal entries.
 template <typename T>
@@ -74,24 +106,7 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
     Blk6 out{0, 0, 0, 0, 0, 0};
     const double r2 = rx * rx + ry * ry + rz * rz;
     const double s2 = (ai + aj) * (ai + aj);
-    if (!self && r2 < s2) {
-        // overlap correction: + overlapping RPY - non-overlapping RPY (diff_tensor.c:279-313)
-        const double rn = sqrt(r2), inv = 1.0 / rn;
-        const double ex = rx * inv, ey = ry * inv, ez = rz * inv;
-        const double sig = 0.5 * (ai + aj);
-        const double pre1 = preDii / sig;
-        const double part1 = 1.0 - 9.0 * rn / (32.0 * sig);
-        const double c1 = 3.0 * rn / (32.0 * sig);
-        const double pre2 = preDij / rn;
-        const double sr = (ai * ai + aj * aj) / r2;
-        const double d2 = 1.0 + sr / 3.0, o2 = 1.0 - sr;
-        out.xx += pre1 * (part1 + c1 * ex * ex) - pre2 * (d2 + ex * ex * o2);
-        out.yy += pre1 * (part1 + c1 * ey * ey) - pre2 * (d2 + ey * ey * o2);
-        out.zz += pre1 * (part1 + c1 * ez * ez) - pre2 * (d2 + ez * ez * o2);
-        out.xy += pre1 * c1 * ex * ey - pre2 * ex * ey * o2;
-        out.xz += pre1 * c1 * ex * ez - pre2 * ex * ez * o2;
-        out.yz += pre1 * c1 * ey * ez - pre2 * ey * ez * o2;
-    }
+    if (!self && r2 < s2) overlap_correction(rx, ry, rz, r2, ai, aj, preDii, preDij, out);
     const double vx = rx * E.inv_L, vy = ry * E.inv_L, vz = rz * E.inv_L;
     const T al = (T)E.alpha, al2 = (T)E.alpha2, al3 = (T)E.alpha3, sg = (T)sigma_sq;
     const T inv_sqrtpi = (T)(1.0 / SQRTPI);
@@ -168,6 +183,124 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
         out.xx = preDii / ai + sc3 + out.xx;
         out.yy = preDii / ai + sc3 + out.yy;
         out.zz = preDii / ai + sc3 + out.zz;
+        out.xy = out.xz = out.yz = 0.0;
+    }
+    return out;
+}
+
+// Smith-Ewald block (src/diff_alg/diff_tensor.c:102-263) in reduced units v = r / L with
+// the .prm alpha (EwaldTables.alpha): per lattice vector (h, f, g) of the sphere,
+//   real space, x = v + (h, f, g), |x| = r, e = x / r (in T):
+//     Op += (erfc(a r)/r) I + (erfc(a r)/r + 2 a exp(-a^2 r^2)/sqrt(pi)) e e^T
+//     Qp += s I + (-3 s - 4 a^3 exp(-a^2 r^2)/sqrt(pi)) e e^T,
+//           s = (erfc(a r) + 2 a r exp(-a^2 r^2)/sqrt(pi)) / r^3
+//   reciprocal, (h, f, g) != 0, k^2 = h^2 + f^2 + g^2 (FP64, tabulated factors):
+//     Op += G1 cos I + (-1 + m k^2) G1 cos e e^T,  Qp += G2 cos e e^T,
+//     cos = cos 2 pi v.(h, f, g)
+//   D_ij += preDii (0.75/L Op + sigma^2/(2 L^3) Qp); self: + preDii/a - self term.
+// 1/sqrt(pi) is the reference's M1_SQRTPI = 0.5641895835 (src/data.h:228).
+template <typename T>
+__device__ __forceinline__ Blk6 smith_block(double rx, double ry, double rz, double ai,
+                                            double aj, bool self, double preDii, double preDij)
+{
+    const EwaldTables& E = c_ew;
+    const double sigma_sq = 0.5 * (ai * ai + aj * aj);
+    double Op[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz
+    double Qp[6] = {0, 0, 0, 0, 0, 0};
+    Blk6 out{0, 0, 0, 0, 0, 0};
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    const double s2 = (ai + aj) * (ai + aj);
+    if (!self && r2 < s2) overlap_correction(rx, ry, rz, r2, ai, aj, preDii, preDij, out);
+    const double vx = rx * E.inv_L, vy = ry * E.inv_L, vz = rz * E.inv_L;
+    const T al = (T)E.alpha, al2 = (T)E.alpha2, al3 = (T)E.alpha3;
+    const T m1sp = (T)0.5641895835;
+    double sx, cx, sy, cy, sz, cz;
+    sincos(2.0 * PI_F * vx, &sx, &cx);
+    sincos(2.0 * PI_F * vy, &sy, &cy);
+    sincos(2.0 * PI_F * vz, &sz, &cz);
+    const int R = E.R, RR = R * R;
+    double hr = 1.0, hi = 0.0;
+    for (int k = 0; k < R; ++k) cmul_conj_step(hr, hi, cx, sx);
+    int q = 0;
+    for (int h = -R; h <= R; ++h) {
+        const int nf = isqrt_small(RR - h * h);
+        double fr = 1.0, fi = 0.0;
+        for (int k = 0; k < nf; ++k) cmul_conj_step(fr, fi, cy, sy);
+        for (int f = -nf; f <= nf; ++f) {
+            const int ng = isqrt_small(RR - h * h - f * f);
+            const double hfr = hr * fr - hi * fi, hfi = hr * fi + hi * fr;
+            double gr = 1.0, gi = 0.0;
+            for (int k = 0; k < ng; ++k) cmul_conj_step(gr, gi, cz, sz);
+            for (int g = -ng; g <= ng; ++g, ++q) {
+                const T x = (T)(vx + h), y = (T)(vy + f), z = (T)(vz + g);
+                const T rr2 = x * x + y * y + z * z;
+                if (rr2 != (T)0) {
+                    const T rn = sqrt(rr2);
+                    const T inv_r = (T)1 / rn, inv_r3 = (T)1 / (rn * rr2);
+                    const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;
+                    const T eg = exp(-al2 * rr2);
+                    const T ec = erfc(rn * al);
+                    const T so = ec * inv_r;
+                    const T so2 = so + (T)2 * al * m1sp * eg;
+                    Op[0] += (double)(so + so2 * ex * ex);
+                    Op[1] += (double)(so + so2 * ey * ey);
+                    Op[2] += (double)(so + so2 * ez * ez);
+                    Op[3] += (double)(so2 * ex * ey);
+                    Op[4] += (double)(so2 * ex * ez);
+                    Op[5] += (double)(so2 * ey * ez);
+                    const T sq = inv_r3 * (ec + (T)2 * al * rn * m1sp * eg);
+                    const T sq2 = (T)-3 * sq - (T)4 * al3 * m1sp * eg;
+                    Qp[0] += (double)(sq + sq2 * ex * ex);
+                    Qp[1] += (double)(sq + sq2 * ey * ey);
+                    Qp[2] += (double)(sq + sq2 * ez * ez);
+                    Qp[3] += (double)(sq2 * ex * ey);
+                    Qp[4] += (double)(sq2 * ex * ez);
+                    Qp[5] += (double)(sq2 * ey * ez);
+                }
+                if (h != 0 || f != 0 || g != 0) {
+                    const double4 gk = c_recip[q];
+                    const double ekz = c_recip_z[q];
+                    const double cosv = hfr * gr - hfi * gi;
+                    const double so = gk.x * cosv;
+                    const double so2 = gk.y * so;
+                    Op[0] += so + so2 * gk.z * gk.z;
+                    Op[1] += so + so2 * gk.w * gk.w;
+                    Op[2] += so + so2 * ekz * ekz;
+                    Op[3] += so2 * gk.z * gk.w;
+                    Op[4] += so2 * gk.z * ekz;
+                    Op[5] += so2 * gk.w * ekz;
+                    const double sq = c_recip_w[q] * cosv;
+                    Qp[0] += sq * gk.z * gk.z;
+                    Qp[1] += sq * gk.w * gk.w;
+                    Qp[2] += sq * ekz * ekz;
+                    Qp[3] += sq * gk.z * gk.w;
+                    Qp[4] += sq * gk.z * ekz;
+                    Qp[5] += sq * gk.w * ekz;
+                }
+                cmul_step(gr, gi, cz, sz);
+            }
+            cmul_step(fr, fi, cy, sy);
+        }
+        cmul_step(hr, hi, cx, sx);
+    }
+    const double preQ = 0.5 * sigma_sq * E.inv_L * E.inv_L * E.inv_L;
+    out.xx += preDii * (E.preO * Op[0] + preQ * Qp[0]);
+    out.yy += preDii * (E.preO * Op[1] + preQ * Qp[1]);
+    out.zz += preDii * (E.preO * Op[2] + preQ * Qp[2]);
+    out.xy += preDii * (E.preO * Op[3] + preQ * Qp[3]);
+    out.xz += preDii * (E.preO * Op[4] + preQ * Qp[4]);
+    out.yz += preDii * (E.preO * Op[5] + preQ * Qp[5]);
+    if (self) {
+        // diff_tensor.c:241-263
+        const double m1 = 0.5641895835;
+        double sc3 = E.alpha * ai * E.inv_L;
+        sc3 = sc3 * sc3 * sc3 * m1 / 3;
+        sc3 = 0.75 * E.inv_L * 1.5 * E.alpha * ai * m1 * E.inv_L +
+              0.5 * ai * ai * E.inv_L * E.inv_L * E.inv_L * sc3;
+        sc3 *= preDii;
+        out.xx = preDii / ai - sc3 + out.xx;
+        out.yy = preDii / ai - sc3 + out.yy;
+        out.zz = preDii / ai - sc3 + out.zz;
         out.xy = out.xz = out.yz = 0.0;
     }
     return out;

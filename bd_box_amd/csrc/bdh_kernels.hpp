@@ -30,7 +30,7 @@ template <typename T>
 int launch_set_identity_pad(T* d_A, long long ld, int n, int n_pad, hipStream_t st);
 
 // ---- pair sums, Ewald, TEA (bdh_pairs.hip)
-int ewald_upload(double box, int R);
+int ewald_upload(double box, int R, int method, double alpha_smith);
 int pair_splits(int N, bool ewald);   // row partials per bead: one per 64-bead tile
 int pair_eparts(int N);                // eps partials of the TEA sums: one per tile pair
 template <typename T>

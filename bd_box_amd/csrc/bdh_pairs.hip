@@ -26,23 +26,40 @@ namespace bdh {
 __constant__ EwaldTables c_ew;
 __constant__ double4 c_recip[EW_MAXV];
 __constant__ double c_recip_z[EW_MAXV];
+__constant__ double c_recip_w[EW_MAXV];
 
-int ewald_upload(double box, int R)
+// Method of the tables last uploaded (the launchers pick the matching instantiation; every
+// launch is preceded by an upload check for the calling system).
+static int g_uploaded_method = EW_BEENAKKER;
+
+int ewald_upload(double box, int R, int method, double alpha_smith)
 {
     BDH_CHECK(R >= 0 && R <= 5, "Ewald lattice radius must be 0..5");
+    BDH_CHECK(method == EW_SMITH || method == EW_BEENAKKER, "Ewald method");
     EwaldTables t{};
     int nv = 0;
     std::vector<double4> rec;
-    std::vector<double> recz;
+    std::vector<double> recz, recw;
+    t.method = method;
     t.box = box;
     t.inv_L = 1.0 / box;
-    t.alpha = SQRTPI * t.inv_L;            // init_tensor, diff_tensor.c:551-554
+    if (method == EW_BEENAKKER) {
+        t.alpha = SQRTPI * t.inv_L;            // init_tensor, diff_tensor.c:551-554
+    } else {
+        t.alpha = alpha_smith;                  // reduced units (the .prm ewald_alpha)
+    }
     t.alpha2 = t.alpha * t.alpha;
     t.alpha3 = t.alpha * t.alpha2;
     t.inv_alpha2 = 1.0 / t.alpha2;
     t.preQ = t.inv_L * t.inv_L * t.inv_L;
+    t.preO = 0.75 * t.inv_L;
     t.self_a = -6.0 * t.alpha / SQRTPI;
     t.self_b = 40.0 * t.alpha3 / (3.0 * SQRTPI);
+    // M_PI_ALPHA_SQ = -M_PIF * M_PIF / alpha2 with the product of the float pis in float
+    // (diff_tensor.c:557)
+    const float pif = 3.14159265358979323846f;
+    const float mpp = -pif * pif;
+    const double m_pi_alpha_sq = (double)mpp / t.alpha2;
     const int rr = R * R;
     for (int h = -R; h <= R; ++h) {
         const int nf = (int)std::sqrt((double)(rr - h * h));
@@ -54,14 +71,26 @@ int ewald_upload(double box, int R)
                 if (s == 0.0) {
                     rec.push_back(make_double4(0, 0, 0, 0));
                     recz.push_back(0.0);
+                    recw.push_back(0.0);
                     continue;
                 }
                 const double nrm = std::sqrt(s);
-                const double k2 = s * 4.0 * PI_F * PI_F;
-                const double G = (1.0 + (t.inv_alpha2 * k2 / 4.0) * (1.0 + t.inv_alpha2 * k2 / 2.0)) *
-                                 (6.0 * PI_F * std::exp(-t.inv_alpha2 * k2 / 4.0) / k2);
-                rec.push_back(make_double4(G, k2 / 3.0, h / nrm, f / nrm));
-                recz.push_back(g / nrm);
+                if (method == EW_BEENAKKER) {
+                    const double k2 = s * 4.0 * PI_F * PI_F;
+                    const double G = (1.0 + (t.inv_alpha2 * k2 / 4.0) * (1.0 + t.inv_alpha2 * k2 / 2.0)) *
+                                     (6.0 * PI_F * std::exp(-t.inv_alpha2 * k2 / 4.0) / k2);
+                    rec.push_back(make_double4(G, k2 / 3.0, h / nrm, f / nrm));
+                    recz.push_back(g / nrm);
+                    recw.push_back(0.0);
+                } else {
+                    // diff_tensor.c:199-225; e = (h, f, g) / |.| as get_ev2_norms forms it
+                    const double inv = 1.0 / nrm;
+                    const double ex = h * inv, ey = f * inv, ez = g * inv;
+                    const double G1 = 2.0 * std::exp(m_pi_alpha_sq * s) / ((double)pif * s);
+                    rec.push_back(make_double4(G1, -1.0 + m_pi_alpha_sq * s, ex, ey));
+                    recz.push_back(ez);
+                    recw.push_back(4.0 * (double)pif * std::exp(m_pi_alpha_sq * s));
+                }
             }
         }
     }
@@ -71,6 +100,8 @@ int ewald_upload(double box, int R)
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ew), &t, sizeof(t)));
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip), rec.data(), sizeof(double4) * rec.size()));
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip_z), recz.data(), sizeof(double) * recz.size()));
+    BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip_w), recw.data(), sizeof(double) * recw.size()));
+    g_uploaded_method = method;
     return t.nv;
 }
 
@@ -113,13 +144,17 @@ __device__ __forceinline__ Blk6 free_block(double dx, double dy, double dz, doub
                 (double)(o * ex * ey), (double)(o * ex * ez), (double)(o * ey * ez)};
 }
 
-template <typename T, bool EW>
+// EW: 0 free space, 1 Beenakker-Ewald, 2 Smith-Ewald
+template <typename T, int EW>
 __device__ __forceinline__ Blk6 pair_block(const double4& pi, const double4& pj, bool self,
                                            double preDii, double preDij)
 {
-    if constexpr (EW)
+    if constexpr (EW == 1)
         return beenakker_block<T>(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, pi.w, pj.w, self, preDii,
                                   preDij);
+    else if constexpr (EW == 2)
+        return smith_block<T>(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z, pi.w, pj.w, self, preDii,
+                              preDij);
     else
         return free_block<T>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, pi.w, pj.w, self, preDii,
                              preDij);
@@ -143,8 +178,8 @@ constexpr int PS_TILE = 64;
 int pair_splits(int N, bool) { return ceil_div(N, PS_TILE); }
 int pair_eparts(int N) { const int t = ceil_div(N, PS_TILE); return t * (t + 1) / 2; }
 
-template <typename T, bool EW, bool TEA>
-__global__ void __launch_bounds__(256)
+template <typename T, int EW, bool TEA>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 pair_sums_kernel(const double4* __restrict__ pos, const double* __restrict__ F,
                  const double* __restrict__ xi, int N, double preDii, double preDij, double dt,
                  double* __restrict__ Pf, double* __restrict__ Px, double* __restrict__ Pc,
@@ -286,18 +321,18 @@ int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi
 {
     BDH_CHECK(splits == pair_splits(N, ewald), "launch_pair_sums: split count");
     const dim3 grid(pair_eparts(N)), blk(PS_TILE, 4);
-    if (ewald && tea)
-        hipLaunchKernelGGL((pair_sums_kernel<T, true, true>), grid, blk, 0, st, d_pos, d_F, d_xi, N,
-                           preDii, preDij, dt, d_Pf, d_Px, d_Pc, d_Pe);
-    else if (ewald)
-        hipLaunchKernelGGL((pair_sums_kernel<T, true, false>), grid, blk, 0, st, d_pos, d_F, d_xi, N,
-                           preDii, preDij, dt, d_Pf, d_Px, d_Pc, d_Pe);
-    else if (tea)
-        hipLaunchKernelGGL((pair_sums_kernel<T, false, true>), grid, blk, 0, st, d_pos, d_F, d_xi, N,
-                           preDii, preDij, dt, d_Pf, d_Px, d_Pc, d_Pe);
-    else
-        hipLaunchKernelGGL((pair_sums_kernel<T, false, false>), grid, blk, 0, st, d_pos, d_F, d_xi, N,
-                           preDii, preDij, dt, d_Pf, d_Px, d_Pc, d_Pe);
+    const int ew = ewald ? (g_uploaded_method == EW_SMITH ? 2 : 1) : 0;
+#define PS_LAUNCH(E, TE)                                                                          \
+    hipLaunchKernelGGL((pair_sums_kernel<T, E, TE>), grid, blk, 0, st, d_pos, d_F, d_xi, N, preDii, \
+                       preDij, dt, d_Pf, d_Px, d_Pc, d_Pe)
+    if (ew == 1) {
+        if (tea) PS_LAUNCH(1, true); else PS_LAUNCH(1, false);
+    } else if (ew == 2) {
+        if (tea) PS_LAUNCH(2, true); else PS_LAUNCH(2, false);
+    } else {
+        if (tea) PS_LAUNCH(0, true); else PS_LAUNCH(0, false);
+    }
+#undef PS_LAUNCH
     BDH_LAUNCH_CHECK();
     return 0;
 }
@@ -305,8 +340,8 @@ int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi
 // ------------------------------------------------------------------ Ewald assembly
 // Lower triangle of the Beenakker-Ewald D (rows of i, columns of j, i >= j), the same
 // tiling as the free-space assembly: (64 row beads) x (16 column beads) per workgroup.
-// With pos2, the mean of the blocks at pos and pos2 (the IG-T tensor).
-template <typename T>
+// With pos2, the mean of the blocks at pos and pos2 (the IG-T tensor).  EW as pair_block.
+template <typename T, int EW>
 __global__ void __launch_bounds__(256)
 ewald_assemble_kernel(const double4* __restrict__ pos, const double4* __restrict__ pos2, int N,
                       double preDii, double preDij, T* __restrict__ A, long long ld)
@@ -319,12 +354,10 @@ ewald_assemble_kernel(const double4* __restrict__ pos, const double4* __restrict
     for (int q = 0; q < 4; ++q) {
         const int j = j0 + threadIdx.y * 4 + q;
         if (j >= N || j > i) continue;
-        Blk6 B = beenakker_block<T>(pos[j].x - pi.x, pos[j].y - pi.y, pos[j].z - pi.z, pi.w,
-                                    pos[j].w, i == j, preDii, preDij);
+        Blk6 B = pair_block<T, EW>(pi, pos[j], i == j, preDii, preDij);
         if (pos2 && i != j) {
             const double4 qi = pos2[i], qj = pos2[j];
-            const Blk6 B2 = beenakker_block<T>(qj.x - qi.x, qj.y - qi.y, qj.z - qi.z, qi.w, qj.w, false,
-                                               preDii, preDij);
+            const Blk6 B2 = pair_block<T, EW>(qi, qj, false, preDii, preDij);
             B.xx = (B.xx + B2.xx) / 2; B.yy = (B.yy + B2.yy) / 2; B.zz = (B.zz + B2.zz) / 2;
             B.xy = (B.xy + B2.xy) / 2; B.xz = (B.xz + B2.xz) / 2; B.yz = (B.yz + B2.yz) / 2;
         }
@@ -340,8 +373,12 @@ int launch_ewald_assemble(const double4* d_pos, int N, double preDii, double pre
                           long long ld, hipStream_t st, const double4* d_pos2)
 {
     dim3 grid(ceil_div(N, 64), ceil_div(N, 16));
-    hipLaunchKernelGGL(ewald_assemble_kernel<T>, grid, dim3(64, 4), 0, st, d_pos, d_pos2, N, preDii,
-                       preDij, d_A, ld);
+    if (g_uploaded_method == EW_SMITH)
+        hipLaunchKernelGGL((ewald_assemble_kernel<T, 2>), grid, dim3(64, 4), 0, st, d_pos, d_pos2, N,
+                           preDii, preDij, d_A, ld);
+    else
+        hipLaunchKernelGGL((ewald_assemble_kernel<T, 1>), grid, dim3(64, 4), 0, st, d_pos, d_pos2, N,
+                           preDii, preDij, d_A, ld);
     BDH_LAUNCH_CHECK();
     return 0;
 }

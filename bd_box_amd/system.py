@@ -50,7 +50,7 @@ def make_config(P: Params, precision: int = MIXED, refactor_every: int = 1) -> _
         ewald_method=P.ewald_method, ewald_real=P.ewald_real, precision=precision,
         refactor_every=refactor_every, rand_seed=P.rand_seed & 0xFFFFFFFF, nb_list=P.nb_list,
         algorithm=_algorithm(P),
-        move_attempts=P.move_attempts)
+        move_attempts=P.move_attempts, ewald_alpha=P.ewald_alpha)
 
 
 class BDSystem:

@@ -56,7 +56,7 @@ typedef struct bdh_config {
     int bc;                                  /* 0 none, 1 periodic cube                  */
     double box;                              /* cube edge when bc == 1                   */
     int hydro;                               /* enum bdh_hydro                           */
-    int ewald_method;                        /* 1 Beenakker (bc periodic)                */
+    int ewald_method;                        /* bc periodic: 0 Smith, 1 Beenakker        */
     int ewald_real;                          /* lattice sphere radius 0..5               */
     int precision;                           /* enum bdh_precision                       */
     int refactor_every;                      /* refactor D every k steps (reference: 1)  */
@@ -66,6 +66,8 @@ typedef struct bdh_config {
     int algorithm;                           /* enum bdh_algorithm                       */
     int move_attempts;                       /* rejections allowed per step when
                                                 check_overlap (diff_decl.c:71: 10000)    */
+    double ewald_alpha;                      /* Smith splitting parameter in units of 1/L
+                                                (diff_decl.c:79; Beenakker forces sqrt(pi)/L) */
 } bdh_config;
 
 /* Last error of the calling thread ("" if none). */

@@ -234,13 +234,167 @@ void orc_beenakker_block(const double* coord, int i, int j, double box, int rr,
     memcpy(out, tD, sizeof(tD));
 }
 
+/* Smith-Ewald block (diff_tensor.c:102-239, self term :241-263), in reduced units v / L
+   with the .prm ewald_alpha (float-rounded) and the sphere h^2+f^2+g^2 <= rr^2 shared by
+   both lattices.  Constants as the reference spells them: M1_SQRTPI = 0.5641895835
+   (data.h:228), float pi, M_PI_ALPHA_SQ = -(pi_f pi_f in float) / alpha^2 (init_tensor,
+   diff_tensor.c:546-558). */
+void orc_smith_block(const double* coord, int i, int j, double box, int rr, double alpha,
+                     double preDii, double preDij, double* out)
+{
+    const double M1_SQRTPI_D = 0.5641895835;
+    const float pif = 3.14159265358979323846f;
+    const float mpp = -pif * pif;
+    const double inv_L = 1 / box;
+    const double preO = 0.75 * inv_L;
+    const double alpha2 = alpha * alpha;
+    const double alpha3 = alpha * alpha2;
+    const double m_pi_alpha_sq = mpp / alpha2;
+    const double si = coord[4 * j + 3], sj = coord[4 * i + 3];
+    const double sigma_sq = 0.5 * (si * si + sj * sj);
+    const double preQ = 0.5 * sigma_sq * inv_L * inv_L * inv_L;
+    const int rr_sq = rr * rr;
+    double Op[9] = {0}, Qp[9] = {0}, tD[9] = {0};
+    double v[3];
+    int k, l, h, f, g;
+    v[0] = coord[4 * i + 0] - coord[4 * j + 0];   /* dist_vec(c_j, c_i), trans.c:123-128 */
+    v[1] = coord[4 * i + 1] - coord[4 * j + 1];
+    v[2] = coord[4 * i + 2] - coord[4 * j + 2];
+    if ((v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) < (si + sj) * (si + sj) && i != j)
+    {
+        double r_norm, r_sq, e[3];
+        ev2_norms(v[0], v[1], v[2], e, &r_norm, &r_sq);
+        {   /* overlapping RPY added */
+            double sigma = (si + sj) / 2;
+            double pre = preDii / sigma;
+            double part1 = (1 - (9 * r_norm / (32 * sigma)));
+            for (k = 0; k < 3; ++k)
+            {
+                tD[k + 3 * k] += pre * (part1 + 3 * e[k] * e[k] * r_norm / (32 * sigma));
+                for (l = 0; l < k; ++l)
+                {
+                    double vval = pre * 3 * e[k] * e[l] * r_norm / (32 * sigma);
+                    tD[l + 3 * k] += vval;
+                    tD[k + 3 * l] += vval;
+                }
+            }
+        }
+        {   /* non-overlapping RPY removed */
+            double pre = preDij / r_norm;
+            double sigm_r = (si * si + sj * sj) / r_sq;
+            double part1 = 1 + sigm_r / 3;
+            for (k = 0; k < 3; ++k)
+            {
+                tD[k + 3 * k] -= pre * (part1 + e[k] * e[k] * (1 - sigm_r));
+                for (l = 0; l < k; ++l)
+                {
+                    double vval = pre * e[k] * e[l] * (1 - sigm_r);
+                    tD[l + 3 * k] -= vval;
+                    tD[k + 3 * l] -= vval;
+                }
+            }
+        }
+    }
+    v[0] *= inv_L; v[1] *= inv_L; v[2] *= inv_L;
+    for (h = -rr; h <= rr; ++h)
+    {
+        int next_f = (int)sqrt((double)(rr_sq - h * h));
+        for (f = -next_f; f <= next_f; ++f)
+        {
+            int next_g = (int)sqrt((double)(rr_sq - h * h - f * f));
+            for (g = -next_g; g <= next_g; ++g)
+            {
+                double r_norm, r_sq, e[3], sc, sc2;
+                ev2_norms(v[0] + h, v[1] + f, v[2] + g, e, &r_norm, &r_sq);
+                if (r_norm != 0.0f)
+                {
+                    double inv_r = 1 / r_norm;
+                    double inv_r3 = 1 / (r_norm * r_sq);
+                    sc = erfc(r_norm * alpha) * inv_r;
+                    sc2 = 2 * alpha * M1_SQRTPI_D * exp(-alpha2 * r_sq);
+                    for (k = 0; k < 3; ++k)
+                    {
+                        double sc_ek = e[k] * (sc + sc2);
+                        Op[k * 4] += sc + sc_ek * e[k];
+                        for (l = 0; l < k; ++l) Op[k + l * 3] += e[l] * sc_ek;
+                    }
+                    sc = inv_r3 * (erfc(r_norm * alpha) + 2 * alpha * r_norm * M1_SQRTPI_D * exp(-alpha2 * r_sq));
+                    sc2 = -4 * alpha3 * M1_SQRTPI_D * exp(-alpha2 * r_sq);
+                    for (k = 0; k < 3; ++k)
+                    {
+                        double sc_ek = (-3 * sc + sc2) * e[k];
+                        Qp[k * 4] += sc + sc_ek * e[k];
+                        for (l = 0; l < k; ++l) Qp[k + l * 3] += e[l] * sc_ek;
+                    }
+                }
+                ev2_norms(h, f, g, e, &r_norm, &r_sq);
+                if (r_norm != 0.0f)
+                {
+                    sc = 2 * exp(m_pi_alpha_sq * r_sq) / (pif * r_sq);
+                    sc2 = -1 + m_pi_alpha_sq * r_sq;
+                    sc *= cos(2 * pif * (v[0] * h + v[1] * f + v[2] * g));
+                    sc2 *= sc;
+                    for (k = 0; k < 3; ++k)
+                    {
+                        double sc_ek = sc2 * e[k];
+                        Op[k * 4] += sc + sc_ek * e[k];
+                        for (l = 0; l < k; ++l) Op[k + l * 3] += e[l] * sc_ek;
+                    }
+                    sc = 4 * pif * exp(m_pi_alpha_sq * r_sq);
+                    sc *= cos(2 * pif * (v[0] * h + v[1] * f + v[2] * g));
+                    for (k = 0; k < 3; ++k)
+                    {
+                        double sc_ek = sc * e[k];
+                        Qp[k * 4] += sc_ek * e[k];
+                        for (l = 0; l < k; ++l) Qp[k + l * 3] += e[l] * sc_ek;
+                    }
+                }
+            }
+        }
+    }
+    for (k = 0; k < 3; ++k)
+    {
+        tD[k * 4] += preDii * (preO * Op[k * 4] + preQ * Qp[k * 4]);
+        for (l = 0; l < k; ++l)
+        {
+            tD[k + l * 3] += preDii * (preO * Op[k + l * 3] + preQ * Qp[k + l * 3]);
+            tD[l + k * 3] += preDii * (preO * Op[k + l * 3] + preQ * Qp[k + l * 3]);
+        }
+    }
+    if (i == j)
+    {
+        double aw = coord[4 * i + 3];
+        double sc3 = alpha * aw * inv_L;
+        sc3 = sc3 * sc3 * sc3 * M1_SQRTPI_D / 3;
+        sc3 = 0.75f * inv_L * 1.5f * alpha * aw * M1_SQRTPI_D * inv_L +
+              0.5f * aw * aw * inv_L * inv_L * inv_L * sc3;
+        sc3 *= preDii;
+        for (k = 0; k < 3; ++k)
+        {
+            tD[k * 4] = preDii / aw - sc3 + tD[k * 4];
+            for (l = 0; l < k; ++l) { tD[l + 3 * k] = 0; tD[k + 3 * l] = 0; }
+        }
+    }
+    memcpy(out, tD, sizeof(tD));
+}
+
+/* the periodic block of the configured Ewald method */
+static void ewald_block(const double* coord, int i, int j, const orc_params* p, double preDii,
+                        double preDij, double* tD)
+{
+    if (p->ewald_method == 0)
+        orc_smith_block(coord, i, j, p->box, p->ewald_real, p->ewald_alpha, preDii, preDij, tD);
+    else
+        orc_beenakker_block(coord, i, j, p->box, p->ewald_real, preDii, preDij, tD);
+}
+
 /* block (i, j), j < i, in the orientation compute_D stores it (rows of j, columns of i) */
 static void pair_block(int N, const double* coord, const orc_params* p, double preDii,
                        double preDij, int i, int j, double* tD)
 {
     (void)N;
     if (p->ewald)
-        orc_beenakker_block(coord, i, j, p->box, p->ewald_real, preDii, preDij, tD);
+        ewald_block(coord, i, j, p, preDii, preDij, tD);
     else
         orc_rpy_block(coord + 4 * i, coord + 4 * j, preDii, preDij, tD);
 }
@@ -267,7 +421,7 @@ void orc_compute_D(int N, const double* coord, const orc_params* p, double* D, i
                 }
         }
         if (p->ewald)
-            orc_beenakker_block(coord, i, i, p->box, p->ewald_real, preDii, preDij, tD);
+            ewald_block(coord, i, i, p, preDii, preDij, tD);
         else
         {
             memset(tD, 0, sizeof(tD));
@@ -526,7 +680,7 @@ void orc_rf_onfly(int N, const double* coord, const orc_params* p, double dt,
                     Rf[k * 3 + b] += F[i * 3 + a] * tD[a * 3 + b];
                 }
         }
-        if (p->ewald) orc_beenakker_block(coord, i, i, p->box, p->ewald_real, preDii, preDij, tD);
+        if (p->ewald) ewald_block(coord, i, i, p, preDii, preDij, tD);
         else { memset(tD, 0, sizeof(tD)); tD[0] = tD[4] = tD[8] = preDii / coord[4 * i + 3]; }
         Rf[i * 3] += F[i * 3] * tD[0];
         Rf[i * 3 + 1] += F[i * 3 + 1] * tD[4];

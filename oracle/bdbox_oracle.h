@@ -33,8 +33,10 @@ typedef struct orc_params {
     int    bc;                        /* 0 none, 1 periodic cube */
     double box;                       /* cube edge for bc == 1 */
     int    hydro;                     /* 0 none, 1 cholesky, 2 geyer (TEA-HI) */
-    int    ewald;                     /* 1 -> Beenakker-Ewald RPY (bc periodic and hydro) */
+    int    ewald;                     /* 1 -> Ewald-summed RPY (bc periodic and hydro) */
     int    ewald_real;                /* lattice sphere radius (ewald_real == ewald_recip) */
+    int    ewald_method;              /* 0 Smith, 1 Beenakker */
+    double ewald_alpha;               /* Smith's splitting parameter (reduced units) */
 } orc_params;
 
 /* kB*T/(6 pi eta vfactor) and kB*T/(8 pi eta vfactor)  (diff_decl.c:93-101) */
@@ -46,6 +48,9 @@ void orc_rpy_block(const double* ci, const double* cj, double preDii, double pre
 
 /* Beenakker-Ewald block for pair (i, j), i != j or i == j (self term included)
    (diff_tensor.c:265-422, init_tensor :546-559 with alpha = sqrt(pi)/L).  out[9] row-major. */
+/* Smith-Ewald block (diff_tensor.c:102-263), the .prm ewald_alpha in reduced units */
+void orc_smith_block(const double* coord, int i, int j, double box, int rr, double alpha,
+                     double preDii, double preDij, double* out);
 void orc_beenakker_block(const double* coord, int i, int j, double box, int rr,
                          double preDii, double preDij, double* out);
 

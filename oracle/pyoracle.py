@@ -27,6 +27,7 @@ class OrcParams(C.Structure):
         ("gamma_c", C.c_double), ("cutoff_c", C.c_double), ("bond_lj_scale", C.c_double),
         ("bond_c_scale", C.c_double), ("check_overlap", C.c_int), ("bc", C.c_int),
         ("box", C.c_double), ("hydro", C.c_int), ("ewald", C.c_int), ("ewald_real", C.c_int),
+        ("ewald_method", C.c_int), ("ewald_alpha", C.c_double),
     ]
 
 
@@ -91,8 +92,8 @@ def params_from(P) -> OrcParams:
         kappa_c=P.kappa_c, gamma_c=P.gamma_c, cutoff_c=P.cutoff_c,
         bond_lj_scale=P.bond_lj_scale, bond_c_scale=P.bond_c_scale,
         check_overlap=P.check_overlap, bc=P.bc, box=P.box, hydro=P.hydro,
-        ewald=int(P.bc == 1 and P.hydro != 0 and P.ewald_method == 1),
-        ewald_real=P.ewald_real)
+        ewald=int(P.bc == 1 and P.hydro != 0),
+        ewald_real=P.ewald_real, ewald_method=P.ewald_method, ewald_alpha=P.ewald_alpha)
 
 
 def prefactors(p: OrcParams):

@@ -125,6 +125,22 @@ def main():
     case("beenakker12", hs, pe, [["params"], ["D"], ["step1"]])
     pc = dict(pe, hydro="cholesky")
     case("beenakker12_chol", hs, pc, [["D"], ["step1"]])
+    # 4b. periodic Smith-Ewald RPY (diff_tensor.c:102-263; the .prm ewald_alpha in reduced
+    #     units), the same spheres; and beads that overlap (the overlap correction)
+    ps = dict(pe, ewald_method="smith", ewald_alpha=2.5)
+    case("smith12", hs, dict(ps, hydro="cholesky"), [["params"], ["D"], ["step1"], ["traj", "3"]])
+    case("smith12_geyer", hs, ps, [["step1"], ["traj", "3"]])
+    rng = np.random.default_rng(17)
+    Lo = 150.0
+    Po = rng.uniform(-Lo / 2 + 20, Lo / 2 - 20, size=(8, 3))   # extent within the box
+    Po[0] = 0.0
+    Po[1] = np.array([30.0, 5.0, 0.0])                 # an overlapping pair
+    ao = rng.uniform(20.0, 30.0, size=8)
+    ov8 = wl.System(np.concatenate([Po, ao[:, None]], 1), np.zeros(8), np.tile([30.0, 1.0], (8, 1)),
+                    np.zeros((0, 2), np.int32), np.zeros((0, 3)))
+    po8 = dict(ps, xbox=Lo, ybox=Lo, zbox=Lo, hydro="cholesky", alpha_lj=0.0, ewald_alpha=1.77245385)
+    case("smith_ov8", ov8, po8, [["params"], ["D"]])
+
     # 5. the Gaussian stream alone, several seeds and an odd table length
     for seed in (1, 12345):
         case(f"gauss_seed{seed}", wl.chain(4, seed=1), wl.chain_params(rand_seed=seed),

@@ -340,3 +340,38 @@ def test_tea_vs_oracle_given_noise(api, N):
         d_gpu = s.coords()[:, :3] - sysm.coord[:, :3]
         d_ref = ref.coords()[:, :3] - sysm.coord[:, :3]
         assert np.linalg.norm(d_gpu - d_ref) / np.linalg.norm(d_ref) < tol, prec
+
+
+# ----------------------------------------------------------------------------- Smith-Ewald
+@pytest.mark.parametrize("case", ["smith12", "smith_ov8"])
+def test_smith_D_vs_reference_golden(api, golden, case):
+    """Smith-Ewald tensor (diff_tensor.c:102-263) against the reference's own D: FP64 to
+    1e-12 relative, the mixed path (FP32 real-space images) to 1e-5."""
+    S, _ = api
+    g, sysm, P, p = golden(case)
+    assert P.ewald_method == 0
+    coord = g["params.coord"].reshape(-1, 4)
+    print(f"smith D {case}", flush=True)
+    Dref = g["D.D"]
+    assert _rel(S.BDSystem(sysm, P, precision=S.FP64, coord=coord).compute_D(), Dref) < 1e-12
+    assert _rel(S.BDSystem(sysm, P, precision=S.MIXED, coord=coord).compute_D(), Dref) < 1e-5
+
+
+@pytest.mark.parametrize("case", ["smith12", "smith12_geyer"])
+def test_smith_steps_vs_reference_golden(api, golden, case):
+    """First step and a 3-step trajectory with the device stream (Cholesky and TEA noise
+    over the Smith-Ewald tensor) against the reference's own."""
+    S, _ = api
+    g, sysm, P, p = golden(case)
+    coord = golden("smith12")[0]["params.coord"].reshape(-1, 4)
+    print(f"smith steps {case}", flush=True)
+    for prec, tol in ((S.FP64, 1e-10), (S.MIXED, 2e-5)):
+        s = S.BDSystem(sysm, P, precision=prec, coord=coord)
+        s.compute_hydro(1)
+        Rf, Rs = s.last_displacement()
+        assert _rel(Rs, g["step1.Rs"]) < tol, (case, prec, _rel(Rs, g["step1.Rs"]))
+    s = S.BDSystem(sysm, P, precision=S.FP64, coord=coord)
+    traj = g["traj.coord"].reshape(3, -1, 4)
+    for k in range(3):
+        s.compute_hydro(1)
+        assert np.max(np.abs(s.coords() - traj[k])) < 1e-9, (case, k)

@@ -293,3 +293,41 @@ def test_igt_is_not_ermak(golden):
     sim = po.Sim(sysm, p, seed=P.rand_seed, algorithm=0)
     sim.run(1)
     assert np.max(np.abs(sim.coords() - g["traj.coord"].reshape(6, -1, 4)[0])) > 1e-6
+
+
+@pytest.mark.parametrize("case", ["smith12", "smith_ov8"])
+def test_smith_D_vs_reference(golden, case):
+    """Smith-Ewald tensor (diff_tensor.c:102-263): the reference's own D, including the
+    overlap correction (smith_ov8 has an overlapping pair)."""
+    g, sysm, P, p = golden(case)
+    D = po.compute_D(g["params.coord"].reshape(-1, 4), p)
+    Dref = g["D.D"]
+    assert np.max(np.abs(D - Dref)) <= 1e-14 * np.max(np.abs(Dref))
+    assert np.allclose(D, D.T, rtol=0, atol=1e-15 * np.abs(D).max())
+
+
+@pytest.mark.parametrize("case", ["smith12", "smith12_geyer"])
+def test_smith_steps_vs_reference(golden, case):
+    g, sysm, P, p = golden(case)
+    coord = golden("smith12")[0]["params.coord"].reshape(-1, 4)
+    sim = po.Sim(sysm, p, seed=P.rand_seed, coord=coord)
+    sim.run(1)
+    Rf, Rs = sim.last()
+    assert np.max(np.abs(Rs - g["step1.Rs"])) <= 1e-12 * np.abs(g["step1.Rs"]).max()
+    sim2 = po.Sim(sysm, p, seed=P.rand_seed, coord=coord)
+    traj = g["traj.coord"].reshape(3, -1, 4)
+    for k in range(3):
+        sim2.run(1)
+        assert np.max(np.abs(sim2.coords() - traj[k])) < 1e-11
+
+
+def test_smith_differs_from_beenakker_method(golden):
+    """The method switch reaches the tensor: the reference's Smith sums are not the Beenakker
+    sums for the same system (they also change with the splitting parameter, so a Smith
+    tensor computed with Beenakker's terms would fail the golden above)."""
+    g, sysm, P, p = golden("smith12")
+    coord = g["params.coord"].reshape(-1, 4)
+    Ds = po.compute_D(coord, p)
+    p.ewald_method = 1
+    Db = po.compute_D(coord, p)
+    assert np.max(np.abs(Ds - Db)) > 1e-3 * np.max(np.abs(Db))
