@@ -77,6 +77,11 @@ DROPIN_CASES = {
                                 zbox=100.0, ewald_method="smith", ewald_real=2, ewald_recip=2,
                                 ewald_alpha=2.5)),
     "bucket_list": ({}, dict(hydro="cholesky", nb_list="bucket_list")),
+    # periodic chain over the bucket lists (bucket.c:173-379, phantom images at the faces)
+    "bucket_list_periodic": ({}, dict(hydro="none", bc="periodic", xbox=60.0, ybox=60.0,
+                                      zbox=60.0, nb_list="bucket_list")),
+    "bucket_space_periodic": ({}, dict(hydro="none", bc="periodic", xbox=60.0, ybox=60.0,
+                                       zbox=60.0, nb_list="bucket_space")),
     "bucket_space": ({}, dict(hydro="none", nb_list="bucket_space")),
 }
 
@@ -95,3 +100,43 @@ def test_dropin_cases_match_reference(name):
         b = run(HIP, sysm, prm, t2, {"BDBOX_HIP_PRECISION": "fp64"})
     assert a.shape == b.shape == (8, N, 3)
     assert np.max(np.abs(a - b)) <= 1.5e-6, (name, np.max(np.abs(a - b)))
+
+
+def _run_dir(exe, d, prm_path, extra=(), env_extra=None):
+    env = dict(os.environ, OMP_NUM_THREADS="1", **(env_extra or {}))
+    r = subprocess.run([exe, prm_path, *extra], cwd=d, env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, (exe, r.stdout[-2000:], r.stderr[-2000:])
+
+
+@pytest.mark.skipif(not (os.path.exists(REF) and os.path.exists(HIP)),
+                    reason="reference / drop-in binaries not built here")
+@pytest.mark.parametrize("writer,reader", [("hip", "ref"), ("ref", "hip")])
+def test_dropin_restart_round_trip(writer, reader):
+    """Restart files cross between the backends (restart.c:126-359): a run of 3 steps by
+    one, continued from its restart file for 3 more by the other, writes the trajectory of
+    an uninterrupted 6-step reference run (positions and the random stream are synced from
+    the device on restart steps)."""
+    import shutil
+    exe = {"hip": HIP, "ref": REF}
+    envs = {"hip": {"BDBOX_HIP_PRECISION": "fp64"}, "ref": {}}
+    sysm = wl.chain(24, seed=5)
+    base = wl.chain_params(hydro="cholesky", rand_seed=11, save_xyz_freq=1, save_rst_freq=3)
+    print(f"restart {writer} -> {reader}", flush=True)
+    with tempfile.TemporaryDirectory() as t:
+        full, short, cont = (os.path.join(t, x) for x in ("full", "short", "cont"))
+        for d in (full, short):
+            os.makedirs(d)
+            wl.write_str(os.path.join(d, "c.str"), sysm)
+        wl.write_prm(os.path.join(full, "c.prm"), dict(base, bdsteps=7), os.path.join(full, "c.str"), full)
+        _run_dir(REF, full, os.path.join(full, "c.prm"))
+        a = read_xyz(os.path.join(full, "out.xyz"))
+        wl.write_prm(os.path.join(short, "c.prm"), dict(base, bdsteps=4), os.path.join(short, "c.str"), short)
+        _run_dir(exe[writer], short, os.path.join(short, "c.prm"), env_extra=envs[writer])
+        shutil.copytree(short, cont)
+        wl.write_prm(os.path.join(cont, "c2.prm"), dict(base, bdsteps=7), os.path.join(cont, "c.str"), cont)
+        _run_dir(exe[reader], cont, os.path.join(cont, "c2.prm"),
+                 ["--restart=" + os.path.join(cont, "out.rst1")], env_extra=envs[reader])
+        b = read_xyz(os.path.join(cont, "out.xyz"))
+    assert a.shape == b.shape == (7, 24, 3)
+    assert np.max(np.abs(a - b)) <= 1.5e-6, np.max(np.abs(a - b))
