@@ -43,10 +43,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def workload(N, seed):
+def workload(N, seed, nb_list="brute"):
     from bd_box_amd import workloads as wl
     sysm = wl.chain(N, seed=seed)
-    prm = wl.chain_params(rand_seed=20240 + seed)
+    prm = wl.chain_params(rand_seed=20240 + seed, nb_list=nb_list)
     return sysm, prm, wl.parse_params(prm)
 
 
@@ -200,6 +200,8 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed mixed-precision steps (for rocprofv3 runs), then a "
                          "256 MiB device copy that calibrates the HBM byte counters")
+    ap.add_argument("--nb-list", choices=("brute", "bucket_list"), default="brute",
+                    help="nonbonded pairs: all pairs, or the device cell list (the same pairs)")
     ap.add_argument("--workload", choices=("chain", "periodic"), default="chain",
                     help="chain: the N=3000 RPY + Cholesky step (default); periodic: N=8192 "
                          "spheres, Beenakker-Ewald RPY + TEA (its own JSON line)")
@@ -218,7 +220,7 @@ def main():
     if args.workload == "periodic":
         return bench_periodic(args)
 
-    sysm, prm, P = workload(args.N, args.seed)
+    sysm, prm, P = workload(args.N, args.seed, args.nb_list)
     n = 3 * sysm.N
     log(f"workload: N={sysm.N} n={n} refactor_every={args.refactor}")
     prec = MIXED_SPLIT if args.precision == "split" else MIXED
