@@ -440,12 +440,17 @@ __device__ __forceinline__ int* chol_flags(double* Lbuf, int n_pad)
     return reinterpret_cast<int*>(Lbuf + (long long)(n_pad / CHOL_NB) * LB_STRIDE);
 }
 
-// Bounded wait (about a second) for a flag set by another workgroup of the same grid.
+// Bounded wait (about a second) for a flag set by another workgroup of the same grid.  The
+// polls are relaxed; one agent-scope acquire follows the flag (an acquiring poll would
+// invalidate this XCD's L2 at every iteration, under the feet of the other workgroups).
 constexpr int CHOL_SPIN = 1 << 20;
 __device__ __forceinline__ bool wait_flag(int* flag)
 {
     for (int it = 0; it < CHOL_SPIN; ++it) {
-        if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+        if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return true;
+        }
         __builtin_amdgcn_s_sleep(2);
     }
     return false;
@@ -475,7 +480,8 @@ __device__ __forceinline__ void fused_trsm(T* __restrict__ A, long long ld, int 
         if (tid == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         store_diag64<T>(sD, A, ld, t0);
     }
-    __threadfence();
+    // this workgroup's own C stores are read back below: workgroup scope suffices (no L2
+    // write-back / invalidation, which an agent-scope fence would cost every such workgroup)
     __syncthreads();
     Trsm16 x0, x1;
     if (g0 < re) trsm16_load<T>(A, ld, t0, g0, re, lane, x0);
