@@ -475,7 +475,9 @@ __device__ __forceinline__ void fused_trsm(T* __restrict__ A, long long ld, int 
     const int g0 = rb + 16 * wave, g1 = g0 + 64;   // this wavefront's two 16-row groups
     if (diag) {
         // the FP64 block is in Lb (factor_diag64); the T copy in A is not read by this kernel
-        __threadfence();
+        // each wavefront's Lb stores written back (release only: no L2 invalidation), then
+        // the flag
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (tid == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         store_diag64<T>(sD, A, ld, t0);
@@ -717,10 +719,9 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
                 }
             }
         }
-    if (diag) {
-        __syncthreads();
-        factor_diag64(sD, t0, status, Lb_next);
-    }
+    // wavefront 0 wrote the whole diagonal block to LDS itself and starts factoring it while
+    // the other wavefronts finish their stores (the factor's first barrier joins them)
+    if (diag) factor_diag64(sD, t0, status, Lb_next);
     if (tj == 0) fused_trsm<float>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem, sD);
 }
 
@@ -824,10 +825,9 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int
                 *reinterpret_cast<double2*>(cp + 2) = c1v;
             }
         }
-    if (diag) {
-        __syncthreads();
-        factor_diag64(sD, t0, status, Lb_next);
-    }
+    // wavefront 0 wrote the whole diagonal block to LDS itself and starts factoring it while
+    // the other wavefronts finish their stores (the factor's first barrier joins them)
+    if (diag) factor_diag64(sD, t0, status, Lb_next);
     if (tj == 0) fused_trsm<double>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem, sD);
 }
 
