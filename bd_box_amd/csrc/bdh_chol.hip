@@ -954,12 +954,14 @@ trmv_partial_kernel(const T* __restrict__ L, long long ld, int n, const double* 
 {
     constexpr int V = 16 / sizeof(T);
     constexpr int ROWS = 64 * V;
-    constexpr int UNR = M >= 8 ? 2 : (M >= 4 ? 4 : 8);
+    constexpr int UNR = M >= 8 ? 4 : (M >= 4 ? 4 : 8);
     const int rb = blockIdx.x, cb = blockIdx.y;
     const int r0 = rb * ROWS, cbeg = cb * TRMV_COLS;
     if (cbeg > r0 + ROWS - 1 || cbeg >= n) return;  // tile entirely above the diagonal
     __shared__ double s_red[4][ROWS];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wavefront index is uniform: its columns, and so the X values, come through the
+    // scalar cache into SGPRs instead of one vector load per lane
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int rbase = r0 + lane * V;
     double acc[V][M];
 #pragma unroll
