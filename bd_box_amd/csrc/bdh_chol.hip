@@ -68,17 +68,24 @@ __device__ __forceinline__ void diag_subpanel(double* __restrict__ sD, int lane,
     double mydiag = 1.0;   // this lane's pivot, if it owns one in the sub-panel
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
+        // column t's entries of the later sub-panel rows, unscaled: final since column t - 1,
+        // so these broadcasts do not wait for the pivot's reciprocal square root
+        double a[16];
+#pragma unroll
+        for (int u = t + 1; u < 16; ++u) a[u] = readlane_f64(p[t], B0 + u);
         mydiag = (lane == B0 + t) ? piv : mydiag;
         const double rs = rsqrt_f64(piv);
         double nxt = 0.0;
         if (t < 15) {
-            const double le = readlane_f64(p[t], B0 + t + 1) * rs;
+            const double le = a[t + 1] * rs;
             nxt = __builtin_fma(-le, le, readlane_f64(p[t + 1], B0 + t + 1));
         }
         const double l = (lane == B0 + t) ? piv * rs : p[t] * rs;
         p[t] = l;
+        // p[u] -= l (a_u rs)
+        const double lr2 = l * rs;
 #pragma unroll
-        for (int u = t + 1; u < 16; ++u) p[u] = __builtin_fma(-l, readlane_f64(l, B0 + u), p[u]);
+        for (int u = t + 1; u < 16; ++u) p[u] = __builtin_fma(-lr2, a[u], p[u]);
         piv = nxt;
     }
     const unsigned long long fail = __ballot(lane >= B0 && lane < B0 + 16 && !(mydiag > 0.0));
