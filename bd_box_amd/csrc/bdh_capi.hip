@@ -724,6 +724,18 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     fp.check_overlap = cfg->check_overlap;
     fp.pbc = cfg->bc == 1;
     fp.box = cfg->box;
+    {
+        // reach of any pair term: the DH and LJ cutoffs and the overlap test R_i + R_j > r,
+        // padded well above the rounding of the T-precision comparisons in pair_term
+        double rmax = 0.0;
+        for (int i = 0; i < N; ++i) rmax = std::max(rmax, LJ[2 * i]);
+        double reach = 2.0 * rmax;
+        if (cfg->elec) reach = std::max(reach, cfg->cutoff_c);
+        if (cfg->alpha_lj != 0.0)
+            reach = std::max(reach, cfg->cutoff_lj > 0 ? cfg->cutoff_lj : 1.122462 * 2.0 * rmax);
+        reach = reach * (1.0 + 1e-5) + 1e-5;
+        fp.far_r2 = reach * reach;
+    }
 
     // per-bead adjacency (CSR), host-built
     std::vector<int> deg(N + 1, 0);

@@ -386,7 +386,8 @@ forces_cells_kernel(const double4* __restrict__ pos, const double* __restrict__ 
                 const double4 pj = pos[j];
                 double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
                 const double r2 = pair_r2(dx, dy, dz, fp.pbc, fp.box);
-                pair_term<T>(dx, dy, dz, r2, qi, Q[j], lji, LJ[j], bonded(topo, nb0, nb1, j), fp, a);
+                if (r2 <= fp.far_r2)
+                    pair_term<T>(dx, dy, dz, r2, qi, Q[j], lji, LJ[j], bonded(topo, nb0, nb1, j), fp, a);
             }
         }
         if (sub == 0) e_b = bond_terms(pos, i, pi, topo, fp, a);

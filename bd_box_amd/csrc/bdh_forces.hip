@@ -54,7 +54,8 @@ forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
             if (j == i) continue;
             double dx = s_pos[q].x - pi.x, dy = s_pos[q].y - pi.y, dz = s_pos[q].z - pi.z;
             const double r2 = pair_r2(dx, dy, dz, fp.pbc, fp.box);
-            pair_term<T>(dx, dy, dz, r2, qi, s_q[q], lji, s_lj[q], bonded(topo, nb0, nb1, j), fp, a);
+            if (r2 <= fp.far_r2)
+                pair_term<T>(dx, dy, dz, r2, qi, s_q[q], lji, s_lj[q], bonded(topo, nb0, nb1, j), fp, a);
         }
     }
     // bonds (only in the first column split, so each bond end is added once)

@@ -74,6 +74,8 @@ struct ForceParams {
     int check_overlap;
     int pbc;
     double box;
+    double far_r2;                // beyond this squared separation a pair adds nothing (no
+                                  // force, no overlap): pair_term is skipped
 };
 // Per-bead adjacency: conn_* lists bonded partners (pair exclusion / scaling), bond_* lists
 // each bead's bonds (partner, r0, rmax, H) so both ends are accumulated without atomics.
