@@ -61,6 +61,23 @@ __device__ __forceinline__ void cmul_conj_step(double& re, double& im, double c,
     re = r;
 }
 
+// erfc(x) for x >= 0 given ex = exp(-x^2).  FP64: the library erfc.  FP32 (the mixed
+// step's real-space images): Abramowitz & Stegun 7.1.26, erfc(x) = t (a1 + t (a2 + t (a3 +
+// t (a4 + t a5)))) exp(-x^2), t = 1 / (1 + p x), absolute error <= 1.5e-7 -- the exponential
+// is the one the image term needs anyway, so no second transcendental.
+template <typename T>
+__device__ __forceinline__ T erfc_with_exp(T x, T ex);
+template <>
+__device__ __forceinline__ double erfc_with_exp<double>(double x, double) { return erfc(x); }
+template <>
+__device__ __forceinline__ float erfc_with_exp<float>(float x, float ex)
+{
+    const float t = 1.0f / (1.0f + 0.3275911f * x);
+    const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f +
+                       t * (-1.453152027f + t * 1.061405429f))));
+    return poly * ex;
+}
+
 // floor(sqrt(m)) for 0 <= m <= 25 (lattice radius <= 5)
 __device__ __forceinline__ int isqrt_small(int m)
 {
@@ -138,8 +155,9 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
                     const T rn = sqrt(rr2);
                     const T inv_r = (T)1 / rn, inv_r2 = (T)1 / rr2, inv_r3 = inv_r * inv_r2;
                     const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;
-                    const T eg = exp(-al2 * rr2) * inv_sqrtpi;
-                    const T ec = erfc(rn * al);
+                    const T ex2 = exp(-al2 * rr2);
+                    const T eg = ex2 * inv_sqrtpi;
+                    const T ec = erfc_with_exp<T>(rn * al, ex2);
                     T sc = al * sg * inv_r2 + (T)14 * al3 * sg - (T)4.5 * al - (T)20 * al2 * al3 * sg * rr2 +
                            (T)3 * al3 * rr2 + (T)4 * al3 * al2 * al2 * sg * rr2 * rr2;
                     sc = sc * eg + ec * ((T)0.75 * inv_r + (T)0.5 * sg * inv_r3);
@@ -239,7 +257,7 @@ __device__ __forceinline__ Blk6 smith_block(double rx, double ry, double rz, dou
                     const T inv_r = (T)1 / rn, inv_r3 = (T)1 / (rn * rr2);
                     const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;
                     const T eg = exp(-al2 * rr2);
-                    const T ec = erfc(rn * al);
+                    const T ec = erfc_with_exp<T>(rn * al, eg);
                     const T so = ec * inv_r;
                     const T so2 = so + (T)2 * al * m1sp * eg;
                     Op[0] += (double)(so + so2 * ex * ex);
