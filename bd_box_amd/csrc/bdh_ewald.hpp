@@ -78,6 +78,23 @@ __device__ __forceinline__ float erfc_with_exp<float>(float x, float ex)
     return poly * ex;
 }
 
+// r, 1/r, 1/r^2 and 1/r^3 of an image at squared distance r2: FP64 with IEEE sqrt and
+// divisions; FP32 from one hardware reciprocal square root (about 1 ulp).
+template <typename T>
+struct InvPowers {
+    T r, inv_r, inv_r2, inv_r3;
+};
+__device__ __forceinline__ InvPowers<double> inv_powers(double r2)
+{
+    const double r = sqrt(r2), inv_r = 1.0 / r, inv_r2 = 1.0 / r2;
+    return {r, inv_r, inv_r2, inv_r * inv_r2};
+}
+__device__ __forceinline__ InvPowers<float> inv_powers(float r2)
+{
+    const float inv_r = __builtin_amdgcn_rsqf(r2), inv_r2 = inv_r * inv_r;
+    return {r2 * inv_r, inv_r, inv_r2, inv_r * inv_r2};
+}
+
 // floor(sqrt(m)) for 0 <= m <= 25 (lattice radius <= 5)
 __device__ __forceinline__ int isqrt_small(int m)
 {
@@ -152,8 +169,8 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
                 const T x = (T)(rx + E.box * h), y = (T)(ry + E.box * f), z = (T)(rz + E.box * g);
                 const T rr2 = x * x + y * y + z * z;
                 if (rr2 != (T)0) {
-                    const T rn = sqrt(rr2);
-                    const T inv_r = (T)1 / rn, inv_r2 = (T)1 / rr2, inv_r3 = inv_r * inv_r2;
+                    const InvPowers<T> ip = inv_powers(rr2);
+                    const T rn = ip.r, inv_r = ip.inv_r, inv_r2 = ip.inv_r2, inv_r3 = ip.inv_r3;
                     const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;
                     const T ex2 = exp(-al2 * rr2);
                     const T eg = ex2 * inv_sqrtpi;
@@ -253,8 +270,8 @@ __device__ __forceinline__ Blk6 smith_block(double rx, double ry, double rz, dou
                 const T x = (T)(vx + h), y = (T)(vy + f), z = (T)(vz + g);
                 const T rr2 = x * x + y * y + z * z;
                 if (rr2 != (T)0) {
-                    const T rn = sqrt(rr2);
-                    const T inv_r = (T)1 / rn, inv_r3 = (T)1 / (rn * rr2);
+                    const InvPowers<T> ip = inv_powers(rr2);
+                    const T rn = ip.r, inv_r = ip.inv_r, inv_r3 = ip.inv_r3;
                     const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;
                     const T eg = exp(-al2 * rr2);
                     const T ec = erfc_with_exp<T>(rn * al, eg);
