@@ -1294,3 +1294,504 @@ int bdh_sync(bdh_system* s)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ batches of chains
+// Independent systems of uneven sizes sharing one configuration, stepped together: each
+// phase of the step (assembly, every launch of the factorization, L.xi, D.F, move, forces,
+// the Gaussian streams) is one launch over all chains.  Per chain the launch shapes and the
+// arithmetic are those of a bdh_system of its own, so chain c's trajectory is the one
+// bdh_create / bdh_step give for its beads with rand_seed seeds[c].
+struct bdh_batch {
+    bdh_config cfg{};
+    int B = 0, Ntot = 0, ntot = 0;
+    bool mixed = false;
+    int syrk_mode = SYRK_F64;
+    int device = 0;
+    hipStream_t st = nullptr;
+    double preDii = 0, preDij = 0;
+    ForceParams fp{};
+    Topology topo{};
+    std::vector<BatchChain> hc;
+    std::vector<int> chol_blocks;   // workgroups of update launch k (host)
+    BatchPlan plan;
+    BatchChain* d_ch = nullptr;
+    int* d_tab = nullptr;           // every prefix table
+    double4* pos = nullptr;
+    double* Q = nullptr;
+    double2* LJ = nullptr;
+    int *conn_off = nullptr, *conn = nullptr, *bond_off = nullptr, *bond_p = nullptr;
+    double* bond_parm = nullptr;
+    double *F = nullptr, *Rf = nullptr, *RsB = nullptr, *xi = nullptr, *xi_user = nullptr;
+    size_t xi_user_cap = 0;
+    double *Pp = nullptr, *Pf = nullptr, *Pt = nullptr;
+    void* A = nullptr;
+    double* Lbuf = nullptr;
+    uint64_t* rng = nullptr;
+    long long* rng_off = nullptr;
+    int* rng_cnt = nullptr;         // [m][B]: m * ldx_c Gaussians for m steps
+    int* flags = nullptr;           // [0] force flags, [1] rng status
+    int* status = nullptr;          // [B] factorization status per chain
+    long long steps_done = 0;
+    bool have_L = false;
+    EventRecorder* rec = nullptr;
+};
+
+namespace {
+
+int batch_check(bdh_batch* b)
+{
+    std::vector<int> st(b->B);
+    int h[2];
+    BDH_HIP(hipMemcpyAsync(h, b->flags, sizeof(h), hipMemcpyDeviceToHost, b->st));
+    BDH_HIP(hipMemcpyAsync(st.data(), b->status, sizeof(int) * b->B, hipMemcpyDeviceToHost, b->st));
+    BDH_HIP(hipStreamSynchronize(b->st));
+    for (int c = 0; c < b->B; ++c) {
+        if (st[c] < 0) {
+            set_error("Cholesky decomposition error in chain " + std::to_string(c) +
+                      ": a panel did not receive its diagonal block");
+            return -3;
+        }
+        if (st[c]) {
+            set_error("Cholesky decomposition error in chain " + std::to_string(c) +
+                      ": non-positive pivot at row " + std::to_string(st[c]));
+            return -3;
+        }
+    }
+    if (h[0] & 2) {
+        set_error("Overstretched bond");
+        return -4;
+    }
+    if (h[1]) {
+        set_error("Gaussian stream did not complete");
+        return -5;
+    }
+    return 0;
+}
+
+#define BREC_BEGIN(ph) do { if (b->rec) b->rec->begin(ph, b->st); } while (0)
+#define BREC_END(ph) do { if (b->rec) b->rec->end(ph, b->st); } while (0)
+
+int batch_forces(bdh_batch* b)
+{
+    BREC_BEGIN(PH_FORCES);
+    const int rc = b->mixed
+        ? launch_forces_batch<float>(b->pos, b->Q, b->LJ, b->topo, b->fp, b->plan, b->Pf, b->flags, b->st)
+        : launch_forces_batch<double>(b->pos, b->Q, b->LJ, b->topo, b->fp, b->plan, b->Pf, b->flags, b->st);
+    if (rc || launch_reduce_batch(b->Pf, 1, 1, b->F, 0, 1.0, b->plan, b->st)) return -1;
+    BREC_END(PH_FORCES);
+    return 0;
+}
+
+int batch_factor(bdh_batch* b)
+{
+    BREC_BEGIN(PH_ASSEMBLE);
+    if (b->mixed) {
+        if (launch_rpy_assemble_batch<float>(b->pos, b->preDii, b->preDij, (float*)b->A, b->plan, b->st))
+            return -1;
+        BREC_END(PH_ASSEMBLE);
+        if (chol_factor_batch<float>((float*)b->A, b->Lbuf, b->plan, b->status, b->syrk_mode, b->st, b->rec))
+            return -1;
+    } else {
+        if (launch_rpy_assemble_batch<double>(b->pos, b->preDii, b->preDij, (double*)b->A, b->plan, b->st))
+            return -1;
+        BREC_END(PH_ASSEMBLE);
+        if (chol_factor_batch<double>((double*)b->A, b->Lbuf, b->plan, b->status, SYRK_F64, b->st, b->rec))
+            return -1;
+    }
+    b->have_L = true;
+    return 0;
+}
+
+// The single-system schedule of run_steps (no rejections): chunks of up to TRMV_MAX_RHS
+// steps that never cross a refactorization; X == nullptr draws the Gaussians of every chain.
+int batch_run(bdh_batch* b, int steps, const double* X_user)
+{
+    const int R = b->cfg.refactor_every;
+    const double dt = b->cfg.dt;
+    const double rf_scale = dt / (kB * b->cfg.T);
+    for (int k = 0; k < steps;) {
+        const int ph = (int)(b->steps_done % R);
+        const bool refactor = !b->have_L || ph == 0;
+        const int m = std::min(std::min(steps - k, TRMV_MAX_RHS), refactor ? R : R - ph);
+        const double* X = b->xi;
+        long long x_stride = 0;
+        if (X_user) {
+            X = X_user + (size_t)k * b->ntot;
+            x_stride = b->ntot;
+        } else {
+            BREC_BEGIN(PH_NOISE);
+            if (launch_gauss_streams(b->rng, b->xi, b->rng_off, b->rng_cnt + (size_t)m * b->B, b->B,
+                                     b->flags + 1, b->st))
+                return -1;
+            BREC_END(PH_NOISE);
+        }
+        if (refactor && batch_factor(b)) return -1;
+        BREC_BEGIN(PH_TRMV);
+        if (b->mixed ? launch_trmv_batch<float>((const float*)b->A, X, x_stride, m, b->plan, b->Pt, b->st)
+                     : launch_trmv_batch<double>((const double*)b->A, X, x_stride, m, b->plan, b->Pt, b->st))
+            return -1;
+        if (launch_reduce_batch(b->Pt, 2, m, b->RsB, b->ntot, std::sqrt(2.0 * dt), b->plan, b->st)) return -1;
+        BREC_END(PH_TRMV);
+        for (int j = 0; j < m; ++j) {
+            BREC_BEGIN(PH_MATVEC);
+            if (b->mixed ? launch_pair_sums_batch<float>(b->pos, b->F, b->preDii, b->preDij, b->plan, b->Pp, b->st)
+                         : launch_pair_sums_batch<double>(b->pos, b->F, b->preDii, b->preDij, b->plan, b->Pp, b->st))
+                return -1;
+            BREC_END(PH_MATVEC);
+            BREC_BEGIN(PH_MOVE);
+            if (launch_reduce_batch(b->Pp, 0, 1, b->Rf, 0, rf_scale, b->plan, b->st)) return -1;
+            if (launch_em_update(b->pos, b->Ntot, b->Rf, b->RsB + (size_t)j * b->ntot, 0, 0.0, b->st)) return -1;
+            BREC_END(PH_MOVE);
+            if (batch_forces(b)) return -1;
+            b->steps_done++;
+        }
+        k += m;
+    }
+    return 0;
+}
+
+int batch_create_impl(const bdh_config* cfg, int B, const int* N, const double* coord, const double* Q,
+                      const double* LJ, const int* nbonds, const int* bonds, const double* bond_parms,
+                      const uint32_t* seeds, bdh_batch* b)
+{
+    BDH_CHECK(cfg && N && coord && Q && LJ && nbonds && seeds, "bdh_batch_create: null argument");
+    BDH_CHECK(B > 0, "bdh_batch_create: no chains");
+    BDH_CHECK(cfg->hydro == BDH_HYDRO_CHOLESKY, "bdh_batch_create: hydro must be cholesky");
+    BDH_CHECK(cfg->bc == 0, "bdh_batch_create: free space only (bc none)");
+    BDH_CHECK(cfg->algorithm == BDH_ERMAK_CONST || cfg->algorithm == BDH_ERMAK_VAR,
+              "bdh_batch_create: algorithm ermak_const");
+    BDH_CHECK(!cfg->check_overlap, "bdh_batch_create: check_overlap is not supported in a batch");
+    BDH_CHECK(cfg->nb_list == 0, "bdh_batch_create: brute-force pairs only (nb_list brute)");
+    BDH_CHECK(cfg->precision == BDH_FP64 || cfg->precision == BDH_MIXED || cfg->precision == BDH_MIXED_SPLIT,
+              "bdh_batch_create: precision");
+    BDH_CHECK(cfg->dt > 0 && cfg->T > 0 && cfg->visc != 0 && cfg->vfactor != 0, "bdh_batch_create: dt/T/visc");
+    b->cfg = *cfg;
+    if (b->cfg.refactor_every < 1) b->cfg.refactor_every = 1;
+    BDH_HIP(hipGetDevice(&b->device));
+    BDH_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+    b->B = B;
+    b->mixed = cfg->precision != BDH_FP64;
+    b->syrk_mode = cfg->precision == BDH_MIXED_SPLIT ? SYRK_BF16X3 : (b->mixed ? SYRK_F32 : SYRK_F64);
+    b->preDii = kB_1_6PI * cfg->T / (cfg->visc * cfg->vfactor);
+    b->preDij = kB_1_8PI * cfg->T / (cfg->visc * cfg->vfactor);
+
+    // layout
+    b->hc.resize(B);
+    long long bead = 0, mat = 0, lb = 0, xo = 0, pp = 0, pf = 0, pt = 0, bonds_tot = 0;
+    int max_steps = 0;
+    for (int c = 0; c < B; ++c) {
+        BDH_CHECK(N[c] > 0 && N[c] <= (1 << 20), "bdh_batch_create: chain size");
+        BDH_CHECK(nbonds[c] >= 0 && (nbonds[c] == 0 || (bonds && bond_parms)), "bdh_batch_create: bonds");
+        BatchChain& h = b->hc[c];
+        h.N = N[c];
+        h.n = 3 * N[c];
+        h.n_pad = ceil_div(h.n, PAD_TO) * PAD_TO;
+        h.bead0 = (int)bead;
+        h.psplits = pair_splits(h.N, false);
+        h.fsplits = force_splits(h.N);
+        h.tcols = trmv_col_blocks(h.n);
+        h.ldx = h.n + (h.n & 1);
+        h.mat = mat;
+        h.lb = lb;
+        h.xi_off = xo;
+        h.pp = pp;
+        h.pf = pf;
+        h.pt = pt;
+        bead += h.N;
+        mat += (long long)h.n_pad * h.n_pad;
+        lb += ((long long)chol_lbuf_doubles(h.n_pad) + 1) / 2 * 2;   // 16-byte aligned workspaces
+        xo += (long long)TRMV_MAX_RHS * h.ldx;
+        pp += (long long)h.psplits * h.n;
+        pf += (long long)h.fsplits * h.n;
+        pt += (long long)h.tcols * TRMV_MAX_RHS * h.n;
+        bonds_tot += nbonds[c];
+        max_steps = std::max(max_steps, chol_update_steps(h.n_pad));
+    }
+    BDH_CHECK(bead <= (1 << 26), "bdh_batch_create: too many beads");
+    b->Ntot = (int)bead;
+    b->ntot = 3 * b->Ntot;
+    for (long long i = 0; i < bead; ++i)
+        BDH_CHECK(coord[4 * i + 3] > 0.0, "bdh_batch_create: bead radius must be positive");
+
+    // prefix tables: asm, pair, force, trmv, rows, trsm, then max_steps chol tables
+    const int W = B + 1;
+    const int ntab = 6 + max_steps;
+    std::vector<int> tab((size_t)ntab * W, 0);
+    b->chol_blocks.assign(max_steps, 0);
+    auto fill = [&](int t, auto count) -> int {
+        long long acc = 0;
+        for (int c = 0; c < B; ++c) {
+            tab[(size_t)t * W + c] = (int)acc;
+            acc += count(b->hc[c]);
+        }
+        tab[(size_t)t * W + B] = (int)acc;
+        return acc <= (long long)INT32_MAX ? (int)acc : -1;
+    };
+    b->plan.asm_blocks = fill(0, [](const BatchChain& h) { return rpy_blocks(h.N); });
+    b->plan.pair_blocks = fill(1, [](const BatchChain& h) { return pair_eparts(h.N); });
+    b->plan.force_blocks = fill(2, [](const BatchChain& h) { return ceil_div(h.N, 64) * h.fsplits; });
+    b->plan.trmv_blocks = b->mixed ? fill(3, [](const BatchChain& h) { return trmv_blocks<float>(h.n); })
+                                   : fill(3, [](const BatchChain& h) { return trmv_blocks<double>(h.n); });
+    b->plan.rows_blocks = fill(4, [](const BatchChain& h) { return ceil_div(h.n, 64); });
+    b->plan.trsm_blocks = fill(5, [](const BatchChain& h) { return chol_trsm0_blocks(h.n_pad); });
+    for (int k = 0; k < max_steps; ++k)
+        b->chol_blocks[k] = fill(6 + k, [k](const BatchChain& h) { return chol_update_blocks(h.n_pad, k); });
+    BDH_CHECK(b->plan.asm_blocks >= 0 && b->plan.pair_blocks >= 0 && b->plan.force_blocks >= 0 &&
+              b->plan.trmv_blocks >= 0, "bdh_batch_create: batch too large");
+    if (dmalloc(&b->d_tab, sizeof(int) * tab.size())) return -1;
+    BDH_HIP(hipMemcpy(b->d_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+    if (dmalloc(&b->d_ch, sizeof(BatchChain) * B)) return -1;
+    BDH_HIP(hipMemcpy(b->d_ch, b->hc.data(), sizeof(BatchChain) * B, hipMemcpyHostToDevice));
+    b->plan.nch = B;
+    b->plan.ch = b->d_ch;
+    b->plan.asm_pre = b->d_tab;
+    b->plan.pair_pre = b->d_tab + W;
+    b->plan.force_pre = b->d_tab + 2 * W;
+    b->plan.trmv_pre = b->d_tab + 3 * W;
+    b->plan.rows_pre = b->d_tab + 4 * W;
+    b->plan.trsm_pre = b->d_tab + 5 * W;
+    b->plan.chol_pre = b->d_tab + 6 * W;
+    b->plan.chol_steps = max_steps;
+    b->plan.chol_blocks = b->chol_blocks.data();
+
+    // forces (a single set of parameters; the pair-skip reach is the largest of any chain)
+    ForceParams& fp = b->fp;
+    fp.alpha_lj = cfg->alpha_lj;
+    fp.cutoff_lj = cfg->cutoff_lj;
+    fp.lj_6_term = cfg->lj_6_term ? 1.0 : 0.0;
+    fp.elec = cfg->elec;
+    fp.gam = cfg->gamma_c / cfg->epsilon_c;
+    fp.kappa_c = cfg->kappa_c;
+    fp.cutoff_c = cfg->cutoff_c;
+    fp.bond_lj_scale = cfg->bond_lj_scale;
+    fp.bond_c_scale = cfg->bond_c_scale;
+    fp.use_conn = bonds_tot && ((cfg->elec && cfg->bond_c_scale != 1) ||
+                                (cfg->alpha_lj != 0.0 && cfg->bond_lj_scale != 1));
+    fp.check_overlap = 0;
+    fp.pbc = 0;
+    fp.box = 0.0;
+    {
+        double rmax = 0.0;
+        for (long long i = 0; i < bead; ++i) rmax = std::max(rmax, LJ[2 * i]);
+        double reach = 2.0 * rmax;
+        if (cfg->elec) reach = std::max(reach, cfg->cutoff_c);
+        if (cfg->alpha_lj != 0.0)
+            reach = std::max(reach, cfg->cutoff_lj > 0 ? cfg->cutoff_lj : 1.122462 * 2.0 * rmax);
+        reach = reach * (1.0 + 1e-5) + 1e-5;
+        fp.far_r2 = reach * reach;
+    }
+
+    // bonds: per chain a CSR of N + 1 offsets (global positions in the concatenated lists)
+    // naming chain-local partners
+    std::vector<int> coff(b->Ntot + B), partner(std::max<long long>(1, 2 * bonds_tot));
+    std::vector<double> parm(3 * std::max<long long>(1, 2 * bonds_tot));
+    {
+        long long base = 0, bo = 0;
+        for (int c = 0; c < B; ++c) {
+            const BatchChain& h = b->hc[c];
+            const int* bd = bonds + 2 * bo;
+            const double* bp = bond_parms + 3 * bo;
+            std::vector<int> deg(h.N, 0);
+            for (int q = 0; q < nbonds[c]; ++q) {
+                const int i = bd[2 * q], j = bd[2 * q + 1];
+                BDH_CHECK(i >= 0 && i < h.N && j >= 0 && j < h.N && i != j,
+                          "bdh_batch_create: bond index out of range");
+                deg[i]++;
+                deg[j]++;
+            }
+            int* off = coff.data() + h.bead0 + c;
+            off[0] = (int)base;
+            for (int i = 0; i < h.N; ++i) off[i + 1] = off[i] + deg[i];
+            std::vector<int> fillp(off, off + h.N);
+            for (int q = 0; q < nbonds[c]; ++q) {
+                const int i = bd[2 * q], j = bd[2 * q + 1];
+                for (int e = 0; e < 2; ++e) {
+                    const int me = e ? j : i, other = e ? i : j;
+                    const int at = fillp[me]++;
+                    partner[at] = other;
+                    parm[3 * at] = bp[3 * q];
+                    parm[3 * at + 1] = bp[3 * q + 1];
+                    parm[3 * at + 2] = bp[3 * q + 2];
+                }
+            }
+            base = off[h.N];
+            bo += nbonds[c];
+        }
+    }
+    if (dmalloc(&b->conn_off, sizeof(int) * coff.size())) return -1;
+    if (dmalloc(&b->conn, sizeof(int) * partner.size())) return -1;
+    if (dmalloc(&b->bond_parm, sizeof(double) * parm.size())) return -1;
+    BDH_HIP(hipMemcpy(b->conn_off, coff.data(), sizeof(int) * coff.size(), hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(b->conn, partner.data(), sizeof(int) * partner.size(), hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(b->bond_parm, parm.data(), sizeof(double) * parm.size(), hipMemcpyHostToDevice));
+    // the bonded-partner table and the bond list are the same CSR
+    b->topo = Topology{b->conn_off, b->conn, b->conn_off, b->conn, b->bond_parm};
+
+    // beads and vectors
+    std::vector<double2> lj(b->Ntot);
+    for (int i = 0; i < b->Ntot; ++i) lj[i] = make_double2(LJ[2 * i], LJ[2 * i + 1]);
+    if (dmalloc(&b->pos, sizeof(double4) * b->Ntot)) return -1;
+    if (dmalloc(&b->Q, sizeof(double) * b->Ntot)) return -1;
+    if (dmalloc(&b->LJ, sizeof(double2) * b->Ntot)) return -1;
+    BDH_HIP(hipMemcpy(b->pos, coord, sizeof(double4) * b->Ntot, hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(b->Q, Q, sizeof(double) * b->Ntot, hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(b->LJ, lj.data(), sizeof(double2) * b->Ntot, hipMemcpyHostToDevice));
+    if (dmalloc(&b->F, sizeof(double) * b->ntot)) return -1;
+    if (dmalloc(&b->Rf, sizeof(double) * b->ntot)) return -1;
+    if (dmalloc(&b->RsB, sizeof(double) * b->ntot * TRMV_MAX_RHS)) return -1;
+    if (dmalloc(&b->xi, sizeof(double) * xo)) return -1;
+    if (dmalloc(&b->Pp, sizeof(double) * pp)) return -1;
+    if (dmalloc(&b->Pf, sizeof(double) * pf)) return -1;
+    if (dmalloc(&b->Pt, sizeof(double) * pt)) return -1;
+    const size_t esz = b->mixed ? sizeof(float) : sizeof(double);
+    if (dmalloc(&b->A, esz * mat)) return -1;
+    BDH_HIP(hipMemsetAsync(b->A, 0, esz * mat, b->st));
+    for (int c = 0; c < B; ++c) {
+        const BatchChain& h = b->hc[c];
+        const int rc = b->mixed ? launch_set_identity_pad<float>((float*)b->A + h.mat, h.n_pad, h.n, h.n_pad, b->st)
+                                : launch_set_identity_pad<double>((double*)b->A + h.mat, h.n_pad, h.n, h.n_pad, b->st);
+        if (rc) return -1;
+    }
+    if (dmalloc(&b->Lbuf, sizeof(double) * lb)) return -1;
+    BDH_HIP(hipMemsetAsync(b->Lbuf, 0, sizeof(double) * lb, b->st));
+
+    // Gaussian streams
+    std::vector<uint64_t> rs(B);
+    std::vector<long long> ro(B);
+    std::vector<int> cnt((size_t)(TRMV_MAX_RHS + 1) * B);
+    for (int c = 0; c < B; ++c) {
+        rs[c] = lcg_seed(seeds[c]);
+        ro[c] = b->hc[c].xi_off;
+        for (int m = 0; m <= TRMV_MAX_RHS; ++m) cnt[(size_t)m * B + c] = m * b->hc[c].ldx;
+    }
+    if (dmalloc(&b->rng, sizeof(uint64_t) * B)) return -1;
+    if (dmalloc(&b->rng_off, sizeof(long long) * B)) return -1;
+    if (dmalloc(&b->rng_cnt, sizeof(int) * cnt.size())) return -1;
+    BDH_HIP(hipMemcpy(b->rng, rs.data(), sizeof(uint64_t) * B, hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(b->rng_off, ro.data(), sizeof(long long) * B, hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(b->rng_cnt, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice));
+    if (dmalloc(&b->flags, sizeof(int) * 4)) return -1;
+    if (dmalloc(&b->status, sizeof(int) * B)) return -1;
+    BDH_HIP(hipMemsetAsync(b->flags, 0, sizeof(int) * 4, b->st));
+    BDH_HIP(hipMemsetAsync(b->status, 0, sizeof(int) * B, b->st));
+
+    // initial forces (main_loop.c:91)
+    if (batch_forces(b)) return -1;
+    return batch_check(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+void bdh_batch_destroy(bdh_batch* b)
+{
+    if (!b) return;
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    dfree(b->d_ch); dfree(b->d_tab);
+    dfree(b->pos); dfree(b->Q); dfree(b->LJ);
+    dfree(b->conn_off); dfree(b->conn); dfree(b->bond_parm);
+    dfree(b->F); dfree(b->Rf); dfree(b->RsB); dfree(b->xi); dfree(b->xi_user);
+    dfree(b->Pp); dfree(b->Pf); dfree(b->Pt); dfree(b->A); dfree(b->Lbuf);
+    dfree(b->rng); dfree(b->rng_off); dfree(b->rng_cnt); dfree(b->flags); dfree(b->status);
+    delete b->rec;
+    if (b->st) (void)hipStreamDestroy(b->st);
+    delete b;
+}
+
+int bdh_batch_create(const bdh_config* cfg, int nchains, const int* N, const double* coord,
+                     const double* Q, const double* LJ, const int* nbonds, const int* bonds,
+                     const double* bond_parms, const uint32_t* seeds, bdh_batch** out)
+{
+    BDH_CHECK(out, "bdh_batch_create: null output");
+    *out = nullptr;
+    bdh_batch* b = new bdh_batch();
+    const int rc = batch_create_impl(cfg, nchains, N, coord, Q, LJ, nbonds, bonds, bond_parms, seeds, b);
+    if (rc) {
+        const std::string msg = last_error();
+        bdh_batch_destroy(b);
+        set_error(msg);
+        return rc;
+    }
+    *out = b;
+    return 0;
+}
+
+int bdh_batch_step(bdh_batch* b, int steps, const double* xi)
+{
+    BDH_CHECK(b && steps >= 0, "bdh_batch_step: bad arguments");
+    if (steps == 0) return 0;
+    const double* X = nullptr;
+    if (xi) {
+        const size_t need = (size_t)steps * b->ntot;
+        if (need > b->xi_user_cap) {
+            dfree(b->xi_user);
+            if (dmalloc(&b->xi_user, sizeof(double) * need)) return -1;
+            b->xi_user_cap = need;
+        }
+        BDH_HIP(hipMemcpyAsync(b->xi_user, xi, sizeof(double) * need, hipMemcpyHostToDevice, b->st));
+        X = b->xi_user;
+    }
+    if (batch_run(b, steps, X)) return -1;
+    return batch_check(b);
+}
+
+int bdh_batch_get_coords(bdh_batch* b, double* coord)
+{
+    BDH_CHECK(b && coord, "bdh_batch_get_coords: bad arguments");
+    BDH_HIP(hipMemcpyAsync(coord, b->pos, sizeof(double4) * b->Ntot, hipMemcpyDeviceToHost, b->st));
+    BDH_HIP(hipStreamSynchronize(b->st));
+    return 0;
+}
+
+int bdh_batch_get_forces(bdh_batch* b, double* F)
+{
+    BDH_CHECK(b && F, "bdh_batch_get_forces: bad arguments");
+    BDH_HIP(hipMemcpyAsync(F, b->F, sizeof(double) * b->ntot, hipMemcpyDeviceToHost, b->st));
+    BDH_HIP(hipStreamSynchronize(b->st));
+    return 0;
+}
+
+int bdh_batch_time_steps(bdh_batch* b, int steps, double* ms)
+{
+    BDH_CHECK(b && steps > 0 && ms, "bdh_batch_time_steps: bad arguments");
+    hipEvent_t e0, e1;
+    BDH_HIP(hipEventCreate(&e0));
+    BDH_HIP(hipEventCreate(&e1));
+    BDH_HIP(hipEventRecord(e0, b->st));
+    if (batch_run(b, steps, nullptr)) return -1;
+    BDH_HIP(hipEventRecord(e1, b->st));
+    BDH_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BDH_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return batch_check(b);
+}
+
+int bdh_batch_phase_timing(bdh_batch* b, int enable)
+{
+    BDH_CHECK(b, "bdh_batch_phase_timing: null batch");
+    BDH_HIP(hipStreamSynchronize(b->st));
+    delete b->rec;
+    b->rec = enable ? new EventRecorder() : nullptr;
+    return 0;
+}
+
+int bdh_batch_phase_times(bdh_batch* b, double* ms, long long* counts)
+{
+    BDH_CHECK(b && b->rec && ms && counts, "bdh_batch_phase_times: timing not enabled");
+    BDH_HIP(hipStreamSynchronize(b->st));
+    for (int p = 0; p < PH_COUNT; ++p) { ms[p] = 0.0; counts[p] = 0; }
+    for (auto& r : b->rec->recs) {
+        float t = 0.f;
+        BDH_HIP(hipEventElapsedTime(&t, r.e0, r.e1));
+        ms[r.phase] += t;
+        counts[r.phase] += 1;
+    }
+    b->rec->clear();
+    return 0;
+}
+
+}  // extern "C"

@@ -215,9 +215,9 @@ __device__ __forceinline__ void store_diag64(const double* __restrict__ sD, T* _
 
 // Panel 0's diagonal block (later ones are factored inside the SYRK kernel).
 template <typename T>
-__global__ void __launch_bounds__(256)
-chol_diag_kernel(T* __restrict__ A, long long ld, int c0, double* __restrict__ Lb,
-                 int* __restrict__ flags, int nflags, int* __restrict__ status)
+__device__ __forceinline__ void chol_diag_body(T* __restrict__ A, long long ld, int c0,
+                                               double* __restrict__ Lb, int* __restrict__ flags,
+                                               int nflags, int* __restrict__ status)
 {
     __shared__ double sD[CHOL_NB * DG_LD];
     for (int k = threadIdx.x; k < nflags; k += 256) flags[k] = 0;
@@ -235,6 +235,14 @@ chol_diag_kernel(T* __restrict__ A, long long ld, int c0, double* __restrict__ L
     __syncthreads();
     factor_diag64(sD, c0, status, Lb);
     store_diag64<T>(sD, A, ld, c0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+chol_diag_kernel(T* __restrict__ A, long long ld, int c0, double* __restrict__ Lb,
+                 int* __restrict__ flags, int nflags, int* __restrict__ status)
+{
+    chol_diag_body<T>(A, ld, c0, Lb, flags, nflags, status);
 }
 
 // Broadcast within each quad of lanes from quad lane Q (DPP quad_perm).
@@ -425,19 +433,26 @@ constexpr int TRSM_LDS_DOUBLES = CHOL_NB * CHOL_NB + 4 * 256 + 4 * 16 * SC_PITCH
 // Panel c0's TRSM as its own launch (panel 0 only; the others run inside the SYRK kernel):
 // 64 rows per workgroup, 16 per wavefront.
 template <typename T>
-__global__ void __launch_bounds__(256)
-chol_trsm_kernel(T* __restrict__ A, long long ld, int c0, int n_pad, const double* __restrict__ Lb)
+__device__ __forceinline__ void chol_trsm_body(T* __restrict__ A, long long ld, int c0, int n_pad,
+                                               const double* __restrict__ Lb, int bid)
 {
     __shared__ double smem[TRSM_LDS_DOUBLES];
     double* sLT = smem;
     double* sInv = sLT + CHOL_NB * CHOL_NB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double* sc = sInv + 4 * 256 + wave * 16 * SC_PITCH;
-    const int g0 = c0 + CHOL_NB + blockIdx.x * TRSM_ROWS + 16 * wave;
+    const int g0 = c0 + CHOL_NB + bid * TRSM_ROWS + 16 * wave;
     Trsm16 g;
     trsm16_load<T>(A, ld, c0, g0, n_pad, lane, g);
     load_lkk(Lb, sLT, sInv);
     if (g0 < n_pad) trsm16_solve_store<T>(A, ld, c0, g0, n_pad, lane, g, sLT, sInv, sc);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+chol_trsm_kernel(T* __restrict__ A, long long ld, int c0, int n_pad, const double* __restrict__ Lb)
+{
+    chol_trsm_body<T>(A, ld, c0, n_pad, Lb, blockIdx.x);
 }
 
 // Flags of the diagonal blocks, after the FP64 blocks in the workspace: flag k is set once
@@ -559,9 +574,10 @@ __device__ __forceinline__ void split_bf16(float x, unsigned& hi, unsigned& lo)
 
 // Three waves per SIMD: the accumulators live in VGPRs next to the fused TRSM's registers.
 template <bool SPLIT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
-syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int t1, int n_pad,
-                double* __restrict__ Lb_next, int* __restrict__ flag_next, int* __restrict__ status)
+__device__ __forceinline__ void syrk_f32_body(float* __restrict__ A, long long ld, int kb, int K, int t0,
+                                              int t1, int n_pad, double* __restrict__ Lb_next,
+                                              int* __restrict__ flag_next, int* __restrict__ status,
+                                              int bid)
 {
     __shared__ __align__(16) unsigned char smem[SY_SMEM];
     float (*sA)[SY_TILE + 4] = reinterpret_cast<float (*)[SY_TILE + 4]>(smem);
@@ -571,9 +587,9 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
     double* sD = reinterpret_cast<double*>(smem);
     // workgroup 0 holds tile (0, 0), whose top-left 64x64 block (wave 0) is the next
     // panel's diagonal block
-    const bool diag = blockIdx.x == 0;
+    const bool diag = bid == 0;
     int ti, tj;
-    trap_tile(blockIdx.x, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
+    trap_tile(bid, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
     const int ri0 = t0 + ti * SY_TILE;
     const int rj0 = t0 + tj * SY_TILE;
     const int tid = threadIdx.x;
@@ -732,6 +748,14 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
     if (tj == 0) fused_trsm<float>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem, sD);
 }
 
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int t1, int n_pad,
+                double* __restrict__ Lb_next, int* __restrict__ flag_next, int* __restrict__ status)
+{
+    syrk_f32_body<SPLIT>(A, ld, kb, K, t0, t1, n_pad, Lb_next, flag_next, status, blockIdx.x);
+}
+
 // ------------------------------------------------------------------ FP64 SYRK (MFMA)
 // Same tiling; each wave 64x64 = 4x4 blocks of 16x16 (v_mfma_f64_16x16x4_f64).  The f64
 // MFMA returns row 4*r + l/16 in accumulator r of lane l; feeding the A operand rows in the
@@ -739,18 +763,19 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
 // lane owns 4 consecutive rows of one column (two 16-byte stores).
 constexpr int SY_KC64 = 16;
 
-__global__ void __launch_bounds__(256)
-syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int t1, int n_pad,
-                double* __restrict__ Lb_next, int* __restrict__ flag_next, int* __restrict__ status)
+__device__ __forceinline__ void syrk_f64_body(double* __restrict__ A, long long ld, int kb, int K, int t0,
+                                              int t1, int n_pad, double* __restrict__ Lb_next,
+                                              int* __restrict__ flag_next, int* __restrict__ status,
+                                              int bid)
 {
     static_assert(2 * SY_KC64 * (SY_TILE + 2) * 8 <= SY_SMEM, "SYRK LDS");
     __shared__ __align__(16) unsigned char smem[SY_SMEM];
     double (*sA)[SY_TILE + 2] = reinterpret_cast<double (*)[SY_TILE + 2]>(smem);
     double (*sB)[SY_TILE + 2] = reinterpret_cast<double (*)[SY_TILE + 2]>(smem + SY_KC64 * (SY_TILE + 2) * 8);
     double* sD = reinterpret_cast<double*>(smem);
-    const bool diag = blockIdx.x == 0;
+    const bool diag = bid == 0;
     int ti, tj;
-    trap_tile(blockIdx.x, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
+    trap_tile(bid, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
     const int ri0 = t0 + ti * SY_TILE;
     const int rj0 = t0 + tj * SY_TILE;
     const int tid = threadIdx.x;
@@ -838,20 +863,56 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int
     if (tj == 0) fused_trsm<double>(A, ld, t0, ri0, n_pad, Lb_next, flag_next, status, diag, smem, sD);
 }
 
+__global__ void __launch_bounds__(256)
+syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int t1, int n_pad,
+                double* __restrict__ Lb_next, int* __restrict__ flag_next, int* __restrict__ status)
+{
+    syrk_f64_body(A, ld, kb, K, t0, t1, n_pad, Lb_next, flag_next, status, blockIdx.x);
+}
+
 // Two-level blocking: outer panels of CHOL_OUTER columns are factored panel by panel
 // (64 columns) with updates restricted to the outer panel; the trailing matrix beyond it is
 // updated once per outer panel with K = CHOL_OUTER, which divides its HBM traffic (read +
 // write of the trailing lower triangle) by CHOL_OUTER / 64.
 constexpr int CHOL_OUTER = 256;
 
+// Update launch k (k = 0, 1, ...) of an n_pad matrix, after panel k = columns 64 k ..:
+// inside an outer panel, panel k's update of the outer panel's later columns; at the end of
+// an outer panel, the whole outer panel's update of the trailing matrix.  False once no
+// columns are left.
+__host__ __device__ inline bool chol_step(int n_pad, int k, int& kb, int& K, int& t0, int& t1)
+{
+    const int c0 = k * CHOL_NB;
+    t0 = c0 + CHOL_NB;
+    if (t0 >= n_pad) return false;
+    const int P0 = c0 / CHOL_OUTER * CHOL_OUTER;
+    const int P1 = n_pad < P0 + CHOL_OUTER ? n_pad : P0 + CHOL_OUTER;
+    if (t0 < P1) {
+        kb = c0;
+        K = CHOL_NB;
+        t1 = P1;
+    } else {
+        kb = P0;
+        K = P1 - P0;
+        t1 = n_pad;
+    }
+    return true;
+}
+
+// Workgroups of an update launch: the lower-trapezoid tiles of rows t0..n_pad, columns
+// t0..t1.
+__host__ __device__ inline int update_tiles(int n_pad, int t0, int t1)
+{
+    const int ni = (n_pad - t0 + SY_TILE - 1) / SY_TILE;
+    const int nj = (t1 - t0 + SY_TILE - 1) / SY_TILE;
+    return nj * ni - nj * (nj - 1) / 2;
+}
+
 template <typename T>
 int launch_update(T* d_A, long long ld, int kb, int K, int t0, int t1, int n_pad, double* d_Lb_next,
                   int* d_flag_next, int* d_status, int syrk_mode, hipStream_t st)
 {
-    const int ni = ceil_div(n_pad - t0, SY_TILE);
-    const int nj = ceil_div(t1 - t0, SY_TILE);
-    int ntiles = 0;
-    for (int j = 0; j < nj; ++j) ntiles += ni - j;
+    const int ntiles = update_tiles(n_pad, t0, t1);
     if constexpr (sizeof(T) == 4) {
         if (syrk_mode == SYRK_BF16X3)
             hipLaunchKernelGGL(syrk_f32_kernel<true>, dim3(ntiles), dim3(256), 0, st,
@@ -892,23 +953,13 @@ int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, 
         BDH_LAUNCH_CHECK();
     }
     if (rec) rec->end(PH_PANEL, st);
-    for (int P0 = 0; P0 < n_pad; P0 += CHOL_OUTER) {
-        const int P1 = std::min(n_pad, P0 + CHOL_OUTER);
-        for (int c0 = P0; c0 < P1; c0 += CHOL_NB) {
-            const int t0 = c0 + CHOL_NB;
-            if (t0 >= n_pad) return 0;
-            const long long k = c0 / CHOL_NB;
-            if (rec) rec->begin(PH_SYRK, st);
-            // inside the outer panel: this panel's update of the panel's later columns;
-            // at its end: the whole outer panel's update of the trailing matrix
-            const int rc = (t0 < P1)
-                ? launch_update<T>(d_A, ld, c0, CHOL_NB, t0, P1, n_pad, d_Lbuf + (k + 1) * LB,
-                                   d_flags + k + 1, d_status, syrk_mode, st)
-                : launch_update<T>(d_A, ld, P0, P1 - P0, t0, n_pad, n_pad, d_Lbuf + (k + 1) * LB,
-                                   d_flags + k + 1, d_status, syrk_mode, st);
-            if (rc) return rc;
-            if (rec) rec->end(PH_SYRK, st);
-        }
+    int kb, K, t0, t1;
+    for (int k = 0; chol_step(n_pad, k, kb, K, t0, t1); ++k) {
+        if (rec) rec->begin(PH_SYRK, st);
+        const int rc = launch_update<T>(d_A, ld, kb, K, t0, t1, n_pad, d_Lbuf + (k + 1) * LB,
+                                        d_flags + k + 1, d_status, syrk_mode, st);
+        if (rc) return rc;
+        if (rec) rec->end(PH_SYRK, st);
     }
     return 0;
 }
@@ -955,14 +1006,13 @@ __device__ __forceinline__ void trmv_accum(const T* __restrict__ col, int rbase,
 }
 
 template <typename T, int M>
-__global__ void __launch_bounds__(256)
-trmv_partial_kernel(const T* __restrict__ L, long long ld, int n, const double* __restrict__ X,
-                    long long ldx, int m, double* __restrict__ P)
+__device__ __forceinline__ void trmv_partial_body(const T* __restrict__ L, long long ld, int n,
+                                                  const double* __restrict__ X, long long ldx, int m,
+                                                  double* __restrict__ P, int rb, int cb)
 {
     constexpr int V = 16 / sizeof(T);
     constexpr int ROWS = 64 * V;
     constexpr int UNR = M >= 8 ? 4 : (M >= 4 ? 4 : 8);
-    const int rb = blockIdx.x, cb = blockIdx.y;
     const int r0 = rb * ROWS, cbeg = cb * TRMV_COLS;
     if (cbeg > r0 + ROWS - 1 || cbeg >= n) return;  // tile entirely above the diagonal
     __shared__ double s_red[4][ROWS];
@@ -1018,6 +1068,14 @@ trmv_partial_kernel(const T* __restrict__ L, long long ld, int n, const double* 
     }
 }
 
+template <typename T, int M>
+__global__ void __launch_bounds__(256)
+trmv_partial_kernel(const T* __restrict__ L, long long ld, int n, const double* __restrict__ X,
+                    long long ldx, int m, double* __restrict__ P)
+{
+    trmv_partial_body<T, M>(L, ld, n, X, ldx, m, P, blockIdx.x, blockIdx.y);
+}
+
 int trmv_col_blocks(int n) { return ceil_div(n, TRMV_COLS); }
 
 template <typename T>
@@ -1043,6 +1101,164 @@ int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_X, l
 }
 
 int trmv_blk_cols() { return TRMV_COLS; }
+
+// ------------------------------------------------------------------ batches of matrices
+// Launch k of every chain's factorization runs as one grid: workgroup x maps to its chain
+// through the prefix table of launch k and then does exactly what workgroup
+// x - prefix[c] of the chain's own launch k does (chol_factor), on the chain's matrix,
+// workspace, flags and status.  A chain's diagonal workgroup (local index 0) has the lowest
+// index of the chain's workgroups, so it is dispatched before any workgroup that waits for
+// its flag, as in the single-matrix launch.  Chains whose factorization has fewer launches
+// drop out of the later grids.
+int chol_update_steps(int n_pad)
+{
+    int kb, K, t0, t1, k = 0;
+    while (chol_step(n_pad, k, kb, K, t0, t1)) ++k;
+    return k;
+}
+
+int chol_update_blocks(int n_pad, int k)
+{
+    int kb, K, t0, t1;
+    return chol_step(n_pad, k, kb, K, t0, t1) ? update_tiles(n_pad, t0, t1) : 0;
+}
+
+int chol_trsm0_blocks(int n_pad) { return n_pad > CHOL_NB ? ceil_div(n_pad - CHOL_NB, TRSM_ROWS) : 0; }
+
+template <typename T>
+int trmv_blocks(int n)
+{
+    constexpr int ROWS = 64 * (16 / sizeof(T));
+    return ceil_div(n, ROWS) * trmv_col_blocks(n);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+chol_diag_batch_kernel(T* __restrict__ A, double* __restrict__ Lbuf, const BatchChain* __restrict__ ch,
+                       int* __restrict__ status)
+{
+    const int c = blockIdx.x;
+    const BatchChain b = ch[c];
+    double* Lb = Lbuf + b.lb;
+    chol_diag_body<T>(A + b.mat, b.n_pad, 0, Lb, chol_flags(Lb, b.n_pad), b.n_pad / CHOL_NB, status + c);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+chol_trsm_batch_kernel(T* __restrict__ A, const double* __restrict__ Lbuf,
+                       const BatchChain* __restrict__ ch, const int* __restrict__ prefix, int nch)
+{
+    const int c = batch_find(prefix, nch, blockIdx.x);
+    const BatchChain b = ch[c];
+    chol_trsm_body<T>(A + b.mat, b.n_pad, 0, b.n_pad, Lbuf + b.lb, blockIdx.x - prefix[c]);
+}
+
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+syrk_f32_batch_kernel(float* __restrict__ A, double* __restrict__ Lbuf, const BatchChain* __restrict__ ch,
+                      const int* __restrict__ prefix, int nch, int k, int* __restrict__ status)
+{
+    const int c = batch_find(prefix, nch, blockIdx.x);
+    const BatchChain b = ch[c];
+    int kb, K, t0, t1;
+    chol_step(b.n_pad, k, kb, K, t0, t1);
+    double* Lb = Lbuf + b.lb;
+    syrk_f32_body<SPLIT>(A + b.mat, b.n_pad, kb, K, t0, t1, b.n_pad, Lb + (long long)(k + 1) * LB_STRIDE,
+                         chol_flags(Lb, b.n_pad) + k + 1, status + c, blockIdx.x - prefix[c]);
+}
+
+__global__ void __launch_bounds__(256)
+syrk_f64_batch_kernel(double* __restrict__ A, double* __restrict__ Lbuf, const BatchChain* __restrict__ ch,
+                      const int* __restrict__ prefix, int nch, int k, int* __restrict__ status)
+{
+    const int c = batch_find(prefix, nch, blockIdx.x);
+    const BatchChain b = ch[c];
+    int kb, K, t0, t1;
+    chol_step(b.n_pad, k, kb, K, t0, t1);
+    double* Lb = Lbuf + b.lb;
+    syrk_f64_body(A + b.mat, b.n_pad, kb, K, t0, t1, b.n_pad, Lb + (long long)(k + 1) * LB_STRIDE,
+                  chol_flags(Lb, b.n_pad) + k + 1, status + c, blockIdx.x - prefix[c]);
+}
+
+template <typename T>
+int chol_factor_batch(T* d_A, double* d_Lbuf, const BatchPlan& plan, int* d_status, int syrk_mode,
+                      hipStream_t st, PhaseRecorder* rec)
+{
+    BDH_CHECK(syrk_mode != SYRK_BF16X3 || sizeof(T) == 4, "chol_factor_batch: split SYRK needs FP32 storage");
+    BDH_CHECK(sizeof(T) == 8 || syrk_mode != SYRK_F64, "chol_factor_batch: FP32 storage needs an FP32 SYRK");
+    if (rec) rec->begin(PH_PANEL, st);
+    hipLaunchKernelGGL(chol_diag_batch_kernel<T>, dim3(plan.nch), dim3(256), 0, st, d_A, d_Lbuf, plan.ch,
+                       d_status);
+    BDH_LAUNCH_CHECK();
+    if (plan.trsm_blocks > 0) {
+        hipLaunchKernelGGL(chol_trsm_batch_kernel<T>, dim3(plan.trsm_blocks), dim3(256), 0, st, d_A,
+                           (const double*)d_Lbuf, plan.ch, plan.trsm_pre, plan.nch);
+        BDH_LAUNCH_CHECK();
+    }
+    if (rec) rec->end(PH_PANEL, st);
+    for (int k = 0; k < plan.chol_steps; ++k) {
+        const int blocks = plan.chol_blocks[k];
+        if (blocks == 0) continue;
+        const int* pre = plan.chol_pre + (long long)k * (plan.nch + 1);
+        if (rec) rec->begin(PH_SYRK, st);
+        if constexpr (sizeof(T) == 4) {
+            if (syrk_mode == SYRK_BF16X3)
+                hipLaunchKernelGGL(syrk_f32_batch_kernel<true>, dim3(blocks), dim3(256), 0, st,
+                                   reinterpret_cast<float*>(d_A), d_Lbuf, plan.ch, pre, plan.nch, k, d_status);
+            else
+                hipLaunchKernelGGL(syrk_f32_batch_kernel<false>, dim3(blocks), dim3(256), 0, st,
+                                   reinterpret_cast<float*>(d_A), d_Lbuf, plan.ch, pre, plan.nch, k, d_status);
+        } else {
+            hipLaunchKernelGGL(syrk_f64_batch_kernel, dim3(blocks), dim3(256), 0, st,
+                               reinterpret_cast<double*>(d_A), d_Lbuf, plan.ch, pre, plan.nch, k, d_status);
+        }
+        BDH_LAUNCH_CHECK();
+        if (rec) rec->end(PH_SYRK, st);
+    }
+    return 0;
+}
+
+template <typename T, int M>
+__global__ void __launch_bounds__(256)
+trmv_batch_kernel(const T* __restrict__ A, const double* __restrict__ X, long long x_stride, int m,
+                  const BatchChain* __restrict__ ch, const int* __restrict__ prefix, int nch,
+                  double* __restrict__ Pt)
+{
+    constexpr int ROWS = 64 * (16 / sizeof(T));
+    const int c = batch_find(prefix, nch, blockIdx.x);
+    const BatchChain b = ch[c];
+    const int x = blockIdx.x - prefix[c];
+    const int gx = (b.n + ROWS - 1) / ROWS;
+    const double* Xc = x_stride > 0 ? X + 3LL * b.bead0 : X + b.xi_off;
+    const long long ldx = x_stride > 0 ? x_stride : b.ldx;
+    trmv_partial_body<T, M>(A + b.mat, b.n_pad, b.n, Xc, ldx, m, Pt + b.pt, x % gx, x / gx);
+}
+
+template <typename T>
+int launch_trmv_batch(const T* d_A, const double* d_X, long long x_stride, int m, const BatchPlan& plan,
+                      double* d_Pt, hipStream_t st)
+{
+    BDH_CHECK(m >= 1 && m <= TRMV_MAX_RHS, "trmv: 1..16 right-hand sides");
+    const dim3 grid(plan.trmv_blocks);
+#define TB_LAUNCH(MM)                                                                              \
+    hipLaunchKernelGGL((trmv_batch_kernel<T, MM>), grid, dim3(256), 0, st, d_A, d_X, x_stride, m, \
+                       plan.ch, plan.trmv_pre, plan.nch, d_Pt)
+    if (m == 1) TB_LAUNCH(1);
+    else if (m == 2) TB_LAUNCH(2);
+    else if (m <= 4) TB_LAUNCH(4);
+    else if (m <= 8) TB_LAUNCH(8);
+    else TB_LAUNCH(16);
+#undef TB_LAUNCH
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+template int chol_factor_batch<float>(float*, double*, const BatchPlan&, int*, int, hipStream_t, PhaseRecorder*);
+template int chol_factor_batch<double>(double*, double*, const BatchPlan&, int*, int, hipStream_t, PhaseRecorder*);
+template int launch_trmv_batch<float>(const float*, const double*, long long, int, const BatchPlan&, double*, hipStream_t);
+template int launch_trmv_batch<double>(const double*, const double*, long long, int, const BatchPlan&, double*, hipStream_t);
+template int trmv_blocks<float>(int);
+template int trmv_blocks<double>(int);
 
 template int chol_factor<float>(float*, long long, int, double*, int*, int, hipStream_t, PhaseRecorder*);
 template int chol_factor<double>(double*, long long, int, double*, int*, int, hipStream_t, PhaseRecorder*);

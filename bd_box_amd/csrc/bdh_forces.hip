@@ -18,20 +18,22 @@ namespace bdh {
 constexpr int FR_ROWS = 64;
 constexpr int FR_CHUNK = 256;
 
+// Workgroup (bx, by): row beads 64 bx .., column split by; PE (when given) receives the
+// workgroup's energies at 3 pe_idx.
 template <typename T>
-__global__ void __launch_bounds__(256)
-forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
-              const double2* __restrict__ LJ, int N, Topology topo, ForceParams fp,
-              int cols_per_split, double* __restrict__ PF, double* __restrict__ PE,
-              int* __restrict__ flags)
+__device__ __forceinline__ void forces_body(const double4* __restrict__ pos, const double* __restrict__ Q,
+                                            const double2* __restrict__ LJ, int N, const Topology& topo,
+                                            const ForceParams& fp, int cols_per_split,
+                                            double* __restrict__ PF, double* __restrict__ PE,
+                                            int* __restrict__ flags, int bx, int by, int pe_idx)
 {
     __shared__ double4 s_pos[FR_CHUNK];
     __shared__ double s_q[FR_CHUNK];
     __shared__ double2 s_lj[FR_CHUNK];
     __shared__ double s_red[6][4][FR_ROWS];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * FR_ROWS + tx;
-    const int i = blockIdx.x * FR_ROWS + tx;
-    const int jb = blockIdx.y * cols_per_split;
+    const int i = bx * FR_ROWS + tx;
+    const int jb = by * cols_per_split;
     const int je = min(N, jb + cols_per_split);
     const double4 pi = (i < N) ? pos[i] : make_double4(0, 0, 0, 1);
     const double qi = (i < N) ? Q[i] : 0.0;
@@ -60,7 +62,7 @@ forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
     }
     // bonds (only in the first column split, so each bond end is added once)
     double e_b = 0.0;
-    if (i < N && blockIdx.y == 0 && ty == 0) e_b = bond_terms(pos, i, pi, topo, fp, a);
+    if (i < N && by == 0 && ty == 0) e_b = bond_terms(pos, i, pi, topo, fp, a);
     s_red[0][ty][tx] = a.fx;
     s_red[1][ty][tx] = a.fy;
     s_red[2][ty][tx] = a.fz;
@@ -69,21 +71,64 @@ forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
     s_red[5][ty][tx] = e_b;
     __syncthreads();
     if (ty == 0 && i < N) {
-        double* out = PF + (long long)blockIdx.y * 3 * N + 3 * i;
+        double* out = PF + (long long)by * 3 * N + 3 * i;
         out[0] = s_red[0][0][tx] + s_red[0][1][tx] + s_red[0][2][tx] + s_red[0][3][tx];
         out[1] = s_red[1][0][tx] + s_red[1][1][tx] + s_red[1][2][tx] + s_red[1][3][tx];
         out[2] = s_red[2][0][tx] + s_red[2][1][tx] + s_red[2][2][tx] + s_red[2][3][tx];
     }
     // energies: pair terms are seen from both ends (x 1/2), bonds likewise
     __syncthreads();
-    if (tid < 3) {
+    if (PE && tid < 3) {
         double e = 0;
         for (int y = 0; y < 4; ++y)
             for (int x = 0; x < FR_ROWS; ++x) e += s_red[3 + tid][y][x];
-        PE[3 * (blockIdx.y * gridDim.x + blockIdx.x) + tid] = 0.5 * e;
+        PE[3 * pe_idx + tid] = 0.5 * e;
     }
     if (a.rep && fp.check_overlap) atomicOr(flags, 1);
     if (a.rep & 2) atomicOr(flags, 2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
+              const double2* __restrict__ LJ, int N, Topology topo, ForceParams fp,
+              int cols_per_split, double* __restrict__ PF, double* __restrict__ PE,
+              int* __restrict__ flags)
+{
+    forces_body<T>(pos, Q, LJ, N, topo, fp, cols_per_split, PF, PE, flags, blockIdx.x, blockIdx.y,
+                   blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// Every chain of a batch: workgroup x of chain c is (x % gx, x / gx) of the chain's own grid
+// (gx row blocks, fsplits column splits).  The bond tables of chain c start at entry
+// bead0 + c of the concatenated CSR offsets (N + 1 per chain) and name chain-local beads.
+template <typename T>
+__global__ void __launch_bounds__(256)
+forces_batch_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
+                    const double2* __restrict__ LJ, Topology topo, ForceParams fp,
+                    const BatchChain* __restrict__ ch, const int* __restrict__ prefix, int nch,
+                    double* __restrict__ Pf, int* __restrict__ flags)
+{
+    const int c = batch_find(prefix, nch, blockIdx.x);
+    const BatchChain b = ch[c];
+    const int x = blockIdx.x - prefix[c];
+    const int gx = (b.N + FR_ROWS - 1) / FR_ROWS;
+    Topology t = topo;
+    t.conn_off += b.bead0 + c;
+    t.bond_off += b.bead0 + c;
+    forces_body<T>(pos + b.bead0, Q + b.bead0, LJ + b.bead0, b.N, t, fp, (b.N + b.fsplits - 1) / b.fsplits,
+                   Pf + b.pf, nullptr, flags, x % gx, x / gx, 0);
+}
+
+template <typename T>
+int launch_forces_batch(const double4* d_pos, const double* d_Q, const double2* d_LJ,
+                        const Topology& topo, const ForceParams& fp, const BatchPlan& plan,
+                        double* d_Pf, int* d_flags, hipStream_t st)
+{
+    hipLaunchKernelGGL(forces_batch_kernel<T>, dim3(plan.force_blocks), dim3(FR_ROWS, 4), 0, st, d_pos,
+                       d_Q, d_LJ, topo, fp, plan.ch, plan.force_pre, plan.nch, d_Pf, d_flags);
+    BDH_LAUNCH_CHECK();
+    return 0;
 }
 
 int force_splits(int N)
@@ -166,6 +211,55 @@ reduce_partials_kernel(ReduceJobs jobs)
         J.out[r] = J.scale * ((s_red[0][lane] + s_red[1][lane]) + (s_red[2][lane] + s_red[3][lane]));
 }
 
+// The sums of reduce_partials_kernel, in the same order, per chain of a batch: workgroup x
+// of chain c covers rows 64 x' .. of the chain's 3N, right-hand side blockIdx.y.
+__global__ void __launch_bounds__(256)
+reduce_batch_kernel(const double* __restrict__ P, int kind, int m, double* __restrict__ out,
+                    long long out_stride, double scale, const BatchChain* __restrict__ ch,
+                    const int* __restrict__ prefix, int nch, int blk_cols)
+{
+    __shared__ double s_red[4][64];
+    const int c = batch_find(prefix, nch, blockIdx.x);
+    const BatchChain b = ch[c];
+    const int k = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = (blockIdx.x - prefix[c]) * 64 + lane;
+    const int n = b.n;
+    const int S = kind == 0 ? b.psplits : (kind == 1 ? b.fsplits : b.tcols);
+    // split s of right-hand side k: row (s m + k) of length n (m = 1 but for L.x)
+    const double* base = P + (kind == 0 ? b.pp : (kind == 1 ? b.pf : b.pt)) + (long long)k * n;
+    const long long stride = (long long)m * n;
+    double acc = 0.0;
+    if (r < n) {
+        const int last = kind == 2 ? min(S - 1, r / blk_cols) : S - 1;
+        const double* p = base + r;
+        int sidx = wave;
+        for (; sidx + 12 <= last; sidx += 16) {
+            const double v0 = p[sidx * stride];
+            const double v1 = p[(sidx + 4) * stride];
+            const double v2 = p[(sidx + 8) * stride];
+            const double v3 = p[(sidx + 12) * stride];
+            acc += (v0 + v1) + (v2 + v3);
+        }
+        for (; sidx <= last; sidx += 4) acc += p[sidx * stride];
+    }
+    s_red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && r < n)
+        out[k * out_stride + 3LL * b.bead0 + r] =
+            scale * ((s_red[0][lane] + s_red[1][lane]) + (s_red[2][lane] + s_red[3][lane]));
+}
+
+int launch_reduce_batch(const double* d_P, int kind, int m, double* d_out, long long out_stride,
+                        double scale, const BatchPlan& plan, hipStream_t st)
+{
+    BDH_CHECK(kind >= 0 && kind <= 2 && m >= 1 && (kind == 2 || m == 1), "launch_reduce_batch: kind");
+    hipLaunchKernelGGL(reduce_batch_kernel, dim3(plan.rows_blocks, m), dim3(256), 0, st, d_P, kind, m, d_out,
+                       out_stride, scale, plan.ch, plan.rows_pre, plan.nch, trmv_blk_cols());
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
 int launch_reduce_partials(const ReduceJobs& jobs, int njobs, int max_n, hipStream_t st)
 {
     if (njobs <= 0) return 0;
@@ -205,6 +299,8 @@ int launch_em_update(double4* d_pos, int N, const double* d_Rf, const double* d_
     return 0;
 }
 
+template int launch_forces_batch<float>(const double4*, const double*, const double2*, const Topology&, const ForceParams&, const BatchPlan&, double*, int*, hipStream_t);
+template int launch_forces_batch<double>(const double4*, const double*, const double2*, const Topology&, const ForceParams&, const BatchPlan&, double*, int*, hipStream_t);
 template int launch_forces<float>(const double4*, const double*, const double2*, int, const Topology&, const ForceParams&, double*, double*, int*, int, hipStream_t);
 template int launch_forces<double>(const double4*, const double*, const double2*, int, const Topology&, const ForceParams&, double*, double*, int*, int, hipStream_t);
 

@@ -17,6 +17,45 @@ struct PhaseRecorder {
     virtual void end(int phase, hipStream_t st) = 0;
 };
 
+// ---- batches of independent systems
+// Chain c of a batch owns beads bead0 .. bead0 + N - 1 of the concatenated bead arrays,
+// entries 3 bead0 .. of the concatenated 3N-vectors, an n_pad x n_pad column-major matrix
+// (ld = n_pad) at A + mat, FP64 factor workspace at Lbuf + lb, its Gaussians at xi + xi_off
+// (rows of ldx), and partial-sum rows at pp (pair sums: psplits rows), pf (forces: fsplits
+// rows) and pt (L.x: tcols x 16 rows).  Every per-chain launch shape is the one the
+// single-system launchers use, so a chain's arithmetic is the same as on its own.
+struct BatchChain {
+    int N, n, n_pad, bead0;
+    int psplits, fsplits, tcols, ldx;
+    long long mat, lb, xi_off, pp, pf, pt;
+};
+
+// Workgroup x of a batched launch belongs to chain c with prefix[c] <= x < prefix[c + 1]
+// (chains without work have equal bounds and are never chosen).  x is uniform, so the
+// search runs on scalar loads.
+__device__ __forceinline__ int batch_find(const int* __restrict__ prefix, int nch, int x)
+{
+    int lo = 0, hi = nch;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Prefix tables (nch + 1 ints on the device) and workgroup totals of one batch's launches.
+struct BatchPlan {
+    int nch = 0;
+    const BatchChain* ch = nullptr;
+    const int *asm_pre = nullptr, *pair_pre = nullptr, *force_pre = nullptr, *trmv_pre = nullptr;
+    const int *rows_pre = nullptr, *diag_pre = nullptr, *trsm_pre = nullptr;
+    int asm_blocks = 0, pair_blocks = 0, force_blocks = 0, trmv_blocks = 0, rows_blocks = 0,
+        trsm_blocks = 0;
+    const int* chol_pre = nullptr;   // steps x (nch + 1): launch k of every chain's factorization
+    int chol_steps = 0;
+    const int* chol_blocks = nullptr;   // host: workgroups of launch k
+};
+
 // ---- random stream (bdh_rng.hip)
 uint64_t lcg_seed(uint32_t seed);
 int launch_gauss_streams(uint64_t* d_states, double* d_out, const long long* d_offsets,
@@ -28,6 +67,10 @@ int launch_rpy_assemble(const double4* d_pos, int N, double preDii, double preDi
                         long long ld, hipStream_t st, const double4* d_pos2 = nullptr);
 template <typename T>
 int launch_set_identity_pad(T* d_A, long long ld, int n, int n_pad, hipStream_t st);
+int rpy_blocks(int N);   // workgroups of one assembly
+template <typename T>
+int launch_rpy_assemble_batch(const double4* d_pos, double preDii, double preDij, T* d_A,
+                              const BatchPlan& plan, hipStream_t st);
 
 // ---- pair sums, Ewald, TEA (bdh_pairs.hip)
 int ewald_upload(double box, int R, int method, double alpha_smith);
@@ -37,6 +80,10 @@ template <typename T>
 int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi, int N,
                      double preDii, double preDij, double dt, bool ewald, bool tea, int splits,
                      double* d_Pf, double* d_Px, double* d_Pc, double* d_Pe, hipStream_t st);
+// Free-space D.F partials of every chain (pair_sums_kernel's arithmetic, no TEA).
+template <typename T>
+int launch_pair_sums_batch(const double4* d_pos, const double* d_F, double preDii, double preDij,
+                           const BatchPlan& plan, double* d_Pp, hipStream_t st);
 template <typename T>
 int launch_ewald_assemble(const double4* d_pos, int N, double preDii, double preDij, T* d_A,
                           long long ld, hipStream_t st, const double4* d_pos2 = nullptr);
@@ -61,6 +108,21 @@ template <typename T>
 int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_X, long long ldx, int m,
                          double* d_P, hipStream_t st);
 int trmv_blk_cols();   // columns per partial of the L.x kernel
+// Per-chain workgroup counts of the batched launches (host).
+int chol_update_steps(int n_pad);                 // update launches of one factorization
+int chol_update_blocks(int n_pad, int k);         // workgroups of launch k (0 when none)
+int chol_trsm0_blocks(int n_pad);                 // panel-0 solve workgroups
+template <typename T>
+int trmv_blocks(int n);                           // row x column blocks of the L.x partials
+// Factorization of every chain's matrix, launch k of all chains together.
+template <typename T>
+int chol_factor_batch(T* d_A, double* d_Lbuf, const BatchPlan& plan, int* d_status, int syrk_mode,
+                      hipStream_t st, PhaseRecorder* rec = nullptr);
+// L.x partials of every chain: X of chain c at X + xi_off (rows of ldx), or, with
+// x_stride > 0, at X + 3 bead0 (rows of x_stride).
+template <typename T>
+int launch_trmv_batch(const T* d_A, const double* d_X, long long x_stride, int m, const BatchPlan& plan,
+                      double* d_Pt, hipStream_t st);
 
 // ---- forces (bdh_forces.hip)
 struct ForceParams {
@@ -128,6 +190,15 @@ struct ReduceJobs {
     ReduceJob job[3];
 };
 int launch_reduce_partials(const ReduceJobs& jobs, int njobs, int max_n, hipStream_t st);
+// The same sums per chain of a batch: kind 0 pair-sum partials (pp, psplits), 1 force
+// partials (pf, fsplits), 2 L.x partials of m right-hand sides (pt, tcols, triangular);
+// out[k * out_stride + 3 bead0 + r] = scale * sum.
+int launch_reduce_batch(const double* d_P, int kind, int m, double* d_out, long long out_stride,
+                        double scale, const BatchPlan& plan, hipStream_t st);
+template <typename T>
+int launch_forces_batch(const double4* d_pos, const double* d_Q, const double2* d_LJ,
+                        const Topology& topo, const ForceParams& fp, const BatchPlan& plan,
+                        double* d_Pf, int* d_flags, hipStream_t st);
 
 // ---- integrator (bdh_forces.hip)
 int launch_em_update(double4* d_pos, int N, const double* d_Rf, const double* d_Rs, int pbc,

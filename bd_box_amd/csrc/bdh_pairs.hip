@@ -179,11 +179,11 @@ int pair_splits(int N, bool) { return ceil_div(N, PS_TILE); }
 int pair_eparts(int N) { const int t = ceil_div(N, PS_TILE); return t * (t + 1) / 2; }
 
 template <typename T, int EW, bool TEA>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
-pair_sums_kernel(const double4* __restrict__ pos, const double* __restrict__ F,
-                 const double* __restrict__ xi, int N, double preDii, double preDij, double dt,
-                 double* __restrict__ Pf, double* __restrict__ Px, double* __restrict__ Pc,
-                 double* __restrict__ Pe)
+__device__ __forceinline__ void pair_sums_body(const double4* __restrict__ pos, const double* __restrict__ F,
+                                               const double* __restrict__ xi, int N, double preDii,
+                                               double preDij, double dt, double* __restrict__ Pf,
+                                               double* __restrict__ Px, double* __restrict__ Pc,
+                                               double* __restrict__ Pe, int b)
 {
     constexpr int NV = TEA ? 10 : 3;
     __shared__ double4 s_pos[PS_TILE];
@@ -192,7 +192,6 @@ pair_sums_kernel(const double4* __restrict__ pos, const double* __restrict__ F,
     __shared__ double s_col[4][NV][PS_TILE];
     __shared__ double s_row[4][NV][PS_TILE];
     const int t = threadIdx.x, w = threadIdx.y, tid = w * PS_TILE + t;
-    const int b = blockIdx.x;
     int J = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
     while (J * (J + 1) / 2 > b) --J;
     while ((J + 1) * (J + 2) / 2 <= b) ++J;
@@ -312,6 +311,39 @@ pair_sums_kernel(const double4* __restrict__ pos, const double* __restrict__ F,
         }
     }
 #undef sum4
+}
+
+template <typename T, int EW, bool TEA>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+pair_sums_kernel(const double4* __restrict__ pos, const double* __restrict__ F,
+                 const double* __restrict__ xi, int N, double preDii, double preDij, double dt,
+                 double* __restrict__ Pf, double* __restrict__ Px, double* __restrict__ Pc,
+                 double* __restrict__ Pe)
+{
+    pair_sums_body<T, EW, TEA>(pos, F, xi, N, preDii, preDij, dt, Pf, Px, Pc, Pe, blockIdx.x);
+}
+
+// Free-space D.F of every chain of a batch: workgroup x of chain c is the chain's tile pair x.
+template <typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+pair_sums_batch_kernel(const double4* __restrict__ pos, const double* __restrict__ F, double preDii,
+                       double preDij, const BatchChain* __restrict__ ch, const int* __restrict__ prefix,
+                       int nch, double* __restrict__ Pp)
+{
+    const int c = batch_find(prefix, nch, blockIdx.x);
+    const BatchChain bc = ch[c];
+    pair_sums_body<T, 0, false>(pos + bc.bead0, F + 3LL * bc.bead0, nullptr, bc.N, preDii, preDij, 0.0,
+                                Pp + bc.pp, nullptr, nullptr, nullptr, blockIdx.x - prefix[c]);
+}
+
+template <typename T>
+int launch_pair_sums_batch(const double4* d_pos, const double* d_F, double preDii, double preDij,
+                           const BatchPlan& plan, double* d_Pp, hipStream_t st)
+{
+    hipLaunchKernelGGL(pair_sums_batch_kernel<T>, dim3(plan.pair_blocks), dim3(PS_TILE, 4), 0, st, d_pos,
+                       d_F, preDii, preDij, plan.ch, plan.pair_pre, plan.nch, d_Pp);
+    BDH_LAUNCH_CHECK();
+    return 0;
 }
 
 template <typename T>
@@ -441,6 +473,8 @@ int launch_tea_close(const double4* d_pos, const double* d_xi, const double* d_F
 
 template int launch_pair_sums<float>(const double4*, const double*, const double*, int, double, double, double, bool, bool, int, double*, double*, double*, double*, hipStream_t);
 template int launch_pair_sums<double>(const double4*, const double*, const double*, int, double, double, double, bool, bool, int, double*, double*, double*, double*, hipStream_t);
+template int launch_pair_sums_batch<float>(const double4*, const double*, double, double, const BatchPlan&, double*, hipStream_t);
+template int launch_pair_sums_batch<double>(const double4*, const double*, double, double, const BatchPlan&, double*, hipStream_t);
 template int launch_ewald_assemble<float>(const double4*, int, double, double, float*, long long, hipStream_t,
                                           const double4*);
 template int launch_ewald_assemble<double>(const double4*, int, double, double, double*, long long, hipStream_t,

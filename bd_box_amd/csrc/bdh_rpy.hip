@@ -73,12 +73,13 @@ constexpr int ASM_COLS = 16;
 // (D(r0) + D(r*)) / 2 of compute_hydro_IGT, diff_equation.c:327-352; the diagonal blocks do
 // not depend on the positions).
 template <typename T>
-__global__ void __launch_bounds__(256)
-rpy_assemble_kernel(const double4* __restrict__ pos, const double4* __restrict__ pos2, int N,
-                    T preDii, T preDij, T* __restrict__ A, long long ld)
+__device__ __forceinline__ void rpy_assemble_body(const double4* __restrict__ pos,
+                                                  const double4* __restrict__ pos2, int N, T preDii,
+                                                  T preDij, T* __restrict__ A, long long ld, int bx,
+                                                  int by)
 {
-    const int i0 = blockIdx.x * ASM_ROWS;
-    const int j0 = blockIdx.y * ASM_COLS;
+    const int i0 = bx * ASM_ROWS;
+    const int j0 = by * ASM_COLS;
     if (j0 > i0 + ASM_ROWS - 1) return;  // tile above the diagonal
     __shared__ double4 s_col[ASM_COLS], s_col2[ASM_COLS];
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -122,6 +123,39 @@ rpy_assemble_kernel(const double4* __restrict__ pos, const double4* __restrict__
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256)
+rpy_assemble_kernel(const double4* __restrict__ pos, const double4* __restrict__ pos2, int N,
+                    T preDii, T preDij, T* __restrict__ A, long long ld)
+{
+    rpy_assemble_body<T>(pos, pos2, N, preDii, preDij, A, ld, blockIdx.x, blockIdx.y);
+}
+
+// Every chain of a batch: workgroup x of chain c is tile (x % gx, x / gx) of its own grid.
+template <typename T>
+__global__ void __launch_bounds__(256)
+rpy_assemble_batch_kernel(const double4* __restrict__ pos, T preDii, T preDij, T* __restrict__ A,
+                          const BatchChain* __restrict__ ch, const int* __restrict__ prefix, int nch)
+{
+    const int c = batch_find(prefix, nch, blockIdx.x);
+    const BatchChain b = ch[c];
+    const int x = blockIdx.x - prefix[c];
+    const int gx = (b.N + ASM_ROWS - 1) / ASM_ROWS;
+    rpy_assemble_body<T>(pos + b.bead0, nullptr, b.N, preDii, preDij, A + b.mat, b.n_pad, x % gx, x / gx);
+}
+
+int rpy_blocks(int N) { return ceil_div(N, ASM_ROWS) * ceil_div(N, ASM_COLS); }
+
+template <typename T>
+int launch_rpy_assemble_batch(const double4* d_pos, double preDii, double preDij, T* d_A,
+                              const BatchPlan& plan, hipStream_t st)
+{
+    hipLaunchKernelGGL(rpy_assemble_batch_kernel<T>, dim3(plan.asm_blocks), dim3(ASM_ROWS, 4), 0, st,
+                       d_pos, (T)preDii, (T)preDij, d_A, plan.ch, plan.asm_pre, plan.nch);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
 // Padding rows/columns n..n_pad-1 form an identity block so the blocked factorization
 // runs on whole tiles; everything else of the matrix starts at zero.
 template <typename T>
@@ -156,6 +190,8 @@ template int launch_rpy_assemble<float>(const double4*, int, double, double, flo
                                         const double4*);
 template int launch_rpy_assemble<double>(const double4*, int, double, double, double*, long long, hipStream_t,
                                          const double4*);
+template int launch_rpy_assemble_batch<float>(const double4*, double, double, float*, const BatchPlan&, hipStream_t);
+template int launch_rpy_assemble_batch<double>(const double4*, double, double, double*, const BatchPlan&, hipStream_t);
 template int launch_set_identity_pad<float>(float*, long long, int, int, hipStream_t);
 template int launch_set_identity_pad<double>(double*, long long, int, int, hipStream_t);
 

@@ -195,6 +195,92 @@ class BDSystem:
         return ms.value
 
 
+class BDBatch:
+    """Independent chains of uneven sizes stepped together (one launch per phase over all of
+    them).  Chain c follows the trajectory BDSystem(systems[c], P with rand_seed seeds[c])
+    takes on its own; all chains share P (hydro cholesky, bc none, ermak_const, no
+    check_overlap, brute-force pairs)."""
+
+    def __init__(self, systems, P: Params, seeds, precision: int = MIXED,
+                 refactor_every: int = 1, device: int = 0):
+        L = _lib.load()
+        _lib.check(L.bdh_init(device), "bdh_init")
+        self.sizes = [s.N for s in systems]
+        self.B = len(systems)
+        self.Ntot = int(sum(self.sizes))
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(int)
+        self.cfg = make_config(P, precision, refactor_every)
+        N = np.array(self.sizes, dtype=np.int32)
+        c = np.ascontiguousarray(np.concatenate([s.coord for s in systems]), float).reshape(-1)
+        Q = np.ascontiguousarray(np.concatenate([s.Q for s in systems]), float)
+        LJ = np.ascontiguousarray(np.concatenate([s.LJ for s in systems]), float).reshape(-1)
+        nb = np.array([len(s.bonds) for s in systems], dtype=np.int32)
+        bl = [np.asarray(s.bonds, np.int32).reshape(-1, 2) for s in systems]
+        pl = [np.asarray(s.bond_parms, float).reshape(-1, 3) for s in systems]
+        bonds = np.ascontiguousarray(np.concatenate(bl), np.int32).reshape(-1)
+        parms = np.ascontiguousarray(np.concatenate(pl), float).reshape(-1)
+        sd = np.array([int(x) & 0xFFFFFFFF for x in seeds], dtype=np.uint32)
+        assert len(sd) == self.B
+        h = C.c_void_p()
+        _lib.check(L.bdh_batch_create(C.byref(self.cfg), self.B,
+                                      N.ctypes.data_as(C.POINTER(C.c_int)), _lib.dptr(c),
+                                      _lib.dptr(Q), _lib.dptr(LJ),
+                                      nb.ctypes.data_as(C.POINTER(C.c_int)),
+                                      bonds.ctypes.data_as(C.POINTER(C.c_int)), _lib.dptr(parms),
+                                      sd.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(h)),
+                   "bdh_batch_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().bdh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute_hydro(self, steps: int = 1, xi: Optional[np.ndarray] = None):
+        """steps steps of every chain; xi: (steps, 3 * sum N) Gaussians, or None for each
+        chain's own stream."""
+        p = None
+        if xi is not None:
+            xi = np.ascontiguousarray(xi, float).reshape(steps, 3 * self.Ntot)
+            p = _lib.dptr(xi)
+        _lib.check(_lib.load().bdh_batch_step(self._h, steps, p), "bdh_batch_step")
+
+    def coords(self) -> np.ndarray:
+        out = np.zeros((self.Ntot, 4))
+        _lib.check(_lib.load().bdh_batch_get_coords(self._h, _lib.dptr(out)), "batch coords")
+        return out
+
+    def chain_coords(self, c: int) -> np.ndarray:
+        return self.coords()[self.offsets[c]:self.offsets[c + 1]]
+
+    def forces(self) -> np.ndarray:
+        out = np.zeros(3 * self.Ntot)
+        _lib.check(_lib.load().bdh_batch_get_forces(self._h, _lib.dptr(out)), "batch forces")
+        return out
+
+    def time_steps(self, steps: int) -> float:
+        ms = C.c_double()
+        _lib.check(_lib.load().bdh_batch_time_steps(self._h, steps, C.byref(ms)),
+                   "batch time_steps")
+        return ms.value
+
+    def phase_timing(self, enable: bool = True):
+        _lib.check(_lib.load().bdh_batch_phase_timing(self._h, int(enable)), "phase_timing")
+
+    def phase_times(self):
+        """{phase: (total_ms, launches)} since the previous call."""
+        ms = np.zeros(8)
+        cnt = (C.c_longlong * 8)()
+        _lib.check(_lib.load().bdh_batch_phase_times(self._h, _lib.dptr(ms), cnt), "phase_times")
+        return {p: (float(ms[i]), int(cnt[i])) for i, p in enumerate(BDSystem.PHASES)}
+
+
 def gauss(state: int, count: int):
     """`count` Gaussians of the drand48 polar Box-Muller stream from 48-bit `state`."""
     st = C.c_uint64(state)

@@ -152,6 +152,17 @@ def chain(N: int, seed: int, a: float = 1.9, lj_diam: float = 4.0, lj_eps: float
     return System(coord, Q, LJ, bonds, parms)
 
 
+def batch_sizes(count: int, seed: int, lo: int = 64, hi: int = 1024) -> List[int]:
+    """Chain lengths drawn log-uniformly in [lo, hi] (256 chains average about 346 beads)."""
+    rng = np.random.default_rng(seed)
+    return [int(round(math.exp(u))) for u in rng.uniform(math.log(lo), math.log(hi), count)]
+
+
+def chains(sizes, seed: int) -> List[System]:
+    """One self-avoiding chain per size, chain k from seed + k."""
+    return [chain(int(N), seed=seed + k) for k, N in enumerate(sizes)]
+
+
 def spheres(N: int, seed: int, radius: float = 25.0, phi: float = 0.2,
             lj_diam: float = 50.0, lj_eps: float = 1.0) -> Tuple[System, float]:
     """Random sequential addition of N non-overlapping spheres in a cubic periodic box of

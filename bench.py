@@ -64,6 +64,17 @@ def periodic_workload(N, seed):
     return sysm, prm, wl.parse_params(prm), L
 
 
+def batch_workload(count, seed):
+    """`count` independent chains (the chain recipe above) with lengths log-uniform in
+    [64, 1024] beads; chain k has its own drand48 seed."""
+    from bd_box_amd import workloads as wl
+    sizes = wl.batch_sizes(count, seed=seed)
+    systems = wl.chains(sizes, seed=1000 * seed)
+    prm = wl.chain_params(rand_seed=20240 + seed)
+    seeds = [20240 + seed + 7919 * k for k in range(count)]
+    return systems, prm, wl.parse_params(prm), seeds
+
+
 # ----------------------------------------------------------------------------- roofline
 def syrk_flops_per_factor(n, n_pad, nb=64):
     """Algorithmic flops of the trailing updates of one factorization (lower triangle
@@ -202,9 +213,11 @@ def main():
                          "256 MiB device copy that calibrates the HBM byte counters")
     ap.add_argument("--nb-list", choices=("brute", "bucket_list"), default="brute",
                     help="nonbonded pairs: all pairs, or the device cell list (the same pairs)")
-    ap.add_argument("--workload", choices=("chain", "periodic"), default="chain",
+    ap.add_argument("--workload", choices=("chain", "periodic", "batch"), default="chain",
                     help="chain: the N=3000 RPY + Cholesky step (default); periodic: N=8192 "
-                         "spheres, Beenakker-Ewald RPY + TEA (its own JSON line)")
+                         "spheres, Beenakker-Ewald RPY + TEA; batch: 256 independent chains "
+                         "of 64-1024 beads stepped together (each its own JSON line)")
+    ap.add_argument("--chains", type=int, default=256, help="chains of the batch workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step of a "
@@ -219,6 +232,8 @@ def main():
 
     if args.workload == "periodic":
         return bench_periodic(args)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     sysm, prm, P = workload(args.N, args.seed, args.nb_list)
     n = 3 * sysm.N
@@ -361,6 +376,84 @@ def bench_periodic(args):
         "phases_ms_per_step": out["mixed"][1],
         "fp64_phases_ms_per_step": out["fp64"][1],
         "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
+def bench_batch(args):
+    """Steps/s of a batch of independent chains (every chain advances one step per step),
+    mixed (split-bf16 trailing updates) and FP64, with each phase one launch over all
+    chains."""
+    import torch
+    from bd_box_amd.system import BDBatch, FP64, MIXED_SPLIT
+    systems, prm, P, seeds = batch_workload(args.chains, args.seed)
+    sizes = [s.N for s in systems]
+    Ntot = int(sum(sizes))
+    R = max(args.refactor, 1)
+    steps = min(args.steps, 10 * R)
+    warm = min(args.warmup, R)
+    log(f"batch workload: {len(sizes)} chains, {Ntot} beads (N {min(sizes)}..{max(sizes)}), "
+        f"{steps} timed steps")
+    out = {}
+    for name, prec in (("mixed", MIXED_SPLIT), ("fp64", FP64)):
+        b = BDBatch(systems, P, seeds, precision=prec, refactor_every=R)
+        b.compute_hydro(warm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.compute_hydro(steps)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        b.phase_timing(True)
+        b.compute_hydro(2 * R)
+        ph = b.phase_times()
+        b.phase_timing(False)
+        out[name] = (steps / t, ph)
+        log(f"{name}: {steps / t:.2f} steps/s phases {json.dumps(ph)}")
+        b.close()
+    # accuracy: R steps of both precisions from the same state with the same streams
+    disp = {}
+    for name, prec in (("mixed", MIXED_SPLIT), ("fp64", FP64)):
+        b = BDBatch(systems, P, seeds, precision=prec, refactor_every=R)
+        b.compute_hydro(R)
+        disp[name] = b.coords()[:, :3]
+        b.close()
+    x0 = np.concatenate([s.coord for s in systems])[:, :3]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    errs = []
+    for c in range(len(sizes)):
+        dm = disp["mixed"][off[c]:off[c + 1]] - x0[off[c]:off[c + 1]]
+        dd = disp["fp64"][off[c]:off[c + 1]] - x0[off[c]:off[c + 1]]
+        errs.append(float(np.linalg.norm(dm - dd) / np.linalg.norm(dd)))
+    ph = out["mixed"][1]
+    cyc = 2 * R
+    ms, cnt = ph["syrk"]
+    fl = sum(syrk_flops_per_factor(3 * N, -(-3 * N // 128) * 128) for N in sizes) * 2
+    ach = fl / (ms * 1e-3) / 1e12
+    roof = {"kernel": "syrk", "launches": cnt, "avg_ms": ms / max(cnt, 1), "bound": "mfma",
+            "achieved": ach, "peak": PEAK_TFLOPS["bf16_mfma"], "unit": "TFLOP/s",
+            "frac": ach / PEAK_TFLOPS["bf16_mfma"], "traffic": None,
+            "basis": "sum over chains of the lower-triangle trailing-update flops, "
+                     "2 factorizations; each FP32 product costs 3 bf16 MFMA products"}
+    value = out["mixed"][0]
+    line = {
+        "metric": f"BD steps/s, batch of {len(sizes)} chains N~logU[64,1024] RPY+Cholesky",
+        "value": value, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": 1e3 / value, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{len(sizes)} bead-spring chains, lengths log-uniform in "
+                               f"[64, 1024] (sum {Ntot} beads), free-space RPY + Cholesky "
+                               f"refactored every {R} steps, split-bf16 trailing updates",
+                   "chains": len(sizes), "beads": Ntot, "refactor_every": R,
+                   "parallelism": "1 GPU"},
+        "chain_steps_per_s": value * len(sizes),
+        "bead_steps_per_s": value * Ntot,
+        "fp64_steps_per_s": out["fp64"][0],
+        "speedup_vs_fp64": value / out["fp64"][0],
+        "rel_disp_err_max": max(errs),
+        "phases_ms_per_step": {k: v[0] / cyc for k, v in ph.items()},
+        "fp64_phases_ms_per_step": {k: v[0] / cyc for k, v in out["fp64"][1].items()},
+        "roofline": roof,
         "cpu_baseline": None,
     }
     print(json.dumps(line))

@@ -169,6 +169,30 @@ int bdh_phase_timing(bdh_system* sys, int enable);
 int bdh_phase_times(bdh_system* sys, double* ms, long long* counts);
 int bdh_sync(bdh_system* sys);
 
+/* ---- Batches of independent chains.  No counterpart in the reference, which integrates one
+ * system per process: nchains systems of uneven sizes that share one bdh_config advance
+ * together, every phase of the step (assembly, each launch of the blocked factorization,
+ * L.xi, D.F, move, forces, the Gaussian streams) running as one launch over all chains.
+ * Chain c follows exactly the trajectory bdh_create + bdh_step give for its own beads with
+ * rand_seed = seeds[c]: per chain the launch shapes and the arithmetic are the same.
+ * Supported: hydro cholesky, bc none, ermak_const, check_overlap off, brute-force pairs.
+ * Beads are concatenated chain after chain: coord[4 * sum N], Q, LJ[2 * sum N]; chain c has
+ * nbonds[c] bonds, listed after those of chains 0..c-1 in bonds / bond_parms with
+ * chain-local bead indices. */
+typedef struct bdh_batch bdh_batch;
+int bdh_batch_create(const bdh_config* cfg, int nchains, const int* N, const double* coord,
+                     const double* Q, const double* LJ, const int* nbonds, const int* bonds,
+                     const double* bond_parms, const uint32_t* seeds, bdh_batch** out);
+void bdh_batch_destroy(bdh_batch* batch);
+/* xi == NULL: each chain draws from its own stream; otherwise xi[steps][3 * sum N]. */
+int bdh_batch_step(bdh_batch* batch, int steps, const double* xi);
+int bdh_batch_get_coords(bdh_batch* batch, double* coord);   /* 4 * sum N */
+int bdh_batch_get_forces(bdh_batch* batch, double* F);       /* 3 * sum N */
+/* Measurement, as bdh_time_steps / bdh_phase_timing / bdh_phase_times. */
+int bdh_batch_time_steps(bdh_batch* batch, int steps, double* ms_total);
+int bdh_batch_phase_timing(bdh_batch* batch, int enable);
+int bdh_batch_phase_times(bdh_batch* batch, double* ms, long long* counts);
+
 #ifdef __cplusplus
 }
 #endif
