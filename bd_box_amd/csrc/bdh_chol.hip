@@ -880,6 +880,9 @@ constexpr int CHOL_OUTER = 256;
 // inside an outer panel, panel k's update of the outer panel's later columns; at the end of
 // an outer panel, the whole outer panel's update of the trailing matrix.  False once no
 // columns are left.
+#ifndef CHOL_LEFT_MAX
+#define CHOL_LEFT_MAX 4096
+#endif
 __host__ __device__ inline bool chol_step(int n_pad, int k, int& kb, int& K, int& t0, int& t1)
 {
     const int c0 = k * CHOL_NB;
@@ -887,7 +890,11 @@ __host__ __device__ inline bool chol_step(int n_pad, int k, int& kb, int& K, int
     if (t0 >= n_pad) return false;
     const int P0 = c0 / CHOL_OUTER * CHOL_OUTER;
     const int P1 = n_pad < P0 + CHOL_OUTER ? n_pad : P0 + CHOL_OUTER;
-    if (t0 < P1) {
+    if (t0 < P1 && n_pad <= CHOL_LEFT_MAX) {
+        kb = P0;
+        K = c0 + CHOL_NB - P0;
+        t1 = t0 + CHOL_NB;
+    } else if (t0 < P1) {
         kb = c0;
         K = CHOL_NB;
         t1 = P1;
