@@ -141,6 +141,22 @@ def test_factor_vs_oracle(api, N):
     assert res < 5e-6, res
 
 
+def test_factor_large_right_looking(api):
+    """n_pad above 4096 takes the right-looking inner updates (smaller matrices update
+    left-looking within the outer panel); LAPACK's factor of the oracle's D is the check."""
+    S, po = api
+    sysm, P = _system(1400, seed=1400)
+    D = po.compute_D(sysm.coord, po.params_from(P))
+    Lref = np.linalg.cholesky(D)
+    print("factor N=1400", flush=True)
+    L64 = S.BDSystem(sysm, P, precision=S.FP64).factor()
+    assert _rel(L64, Lref) < 1e-12
+    Ls = S.BDSystem(sysm, P, precision=S.MIXED_SPLIT).factor()
+    v = np.random.default_rng(0).normal(size=D.shape[0])
+    res = np.linalg.norm(Ls @ (Ls.T @ v) - D @ v) / np.linalg.norm(D @ v)
+    assert res < 5e-6, res
+
+
 def test_factor_vs_reference_golden(api, golden):
     S, _ = api
     for case in ("chain16", "overlap10"):
