@@ -55,6 +55,24 @@ def test_batch_chains_match_own_systems_device_stream(api, prec):
         s.close()
 
 
+def test_batch_edge_sizes(api):
+    """Chains of 1, 2 and 3 beads (no bonds or one tile) beside a long one, and a batch of
+    one chain: each equals its own system."""
+    S, _ = api
+    for sizes in ([1, 2, 3, 200], [57]):
+        systems = wl.chains(sizes, seed=800)
+        seeds = [31 + k for k in range(len(sizes))]
+        print(f"batch edge sizes {sizes}", flush=True)
+        b = S.BDBatch(systems, _params(), seeds, precision=S.MIXED_SPLIT, refactor_every=2)
+        b.compute_hydro(5)
+        for c, (sysm, seed) in enumerate(zip(systems, seeds)):
+            s = S.BDSystem(sysm, _params(seed), precision=S.MIXED_SPLIT, refactor_every=2)
+            s.compute_hydro(5)
+            assert np.array_equal(b.chain_coords(c), s.coords()), (sizes, c)
+            s.close()
+        b.close()
+
+
 def test_batch_given_noise_vs_oracle(api):
     S, po = api
     sizes = [12, 33, 8]
