@@ -1101,6 +1101,8 @@ int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_X, l
         hipLaunchKernelGGL((trmv_partial_kernel<T, 4>), grid, dim3(256), 0, st, d_L, ld, n, d_X, ldx, m, d_P);
     else if (m <= 8)
         hipLaunchKernelGGL((trmv_partial_kernel<T, 8>), grid, dim3(256), 0, st, d_L, ld, n, d_X, ldx, m, d_P);
+    else if (m <= 10)
+        hipLaunchKernelGGL((trmv_partial_kernel<T, 10>), grid, dim3(256), 0, st, d_L, ld, n, d_X, ldx, m, d_P);
     else
         hipLaunchKernelGGL((trmv_partial_kernel<T, 16>), grid, dim3(256), 0, st, d_L, ld, n, d_X, ldx, m, d_P);
     BDH_LAUNCH_CHECK();
@@ -1254,6 +1256,7 @@ int launch_trmv_batch(const T* d_A, const double* d_X, long long x_stride, int m
     else if (m == 2) TB_LAUNCH(2);
     else if (m <= 4) TB_LAUNCH(4);
     else if (m <= 8) TB_LAUNCH(8);
+    else if (m <= 10) TB_LAUNCH(10);
     else TB_LAUNCH(16);
 #undef TB_LAUNCH
     BDH_LAUNCH_CHECK();
