@@ -151,6 +151,7 @@ struct bdh_system {
     long long steps_done = 0;
     bool have_L = false;
     EventRecorder* rec = nullptr;   // non-null while phase timing is enabled
+    hipGraphExec_t factor_exec = nullptr;   // assembly + factorization, captured once
 };
 
 namespace {
@@ -261,11 +262,43 @@ int ensure_ewald(bdh_system* s)
     return 0;
 }
 
+#ifndef BDH_FACTOR_GRAPH
+#define BDH_FACTOR_GRAPH 1
+#endif
+
+int assemble_and_factor_launches(bdh_system* s, void* Abuf, const double4* pos2);
+
 // D at the current positions (with pos2: the mean of the tensors at both sets of positions)
-// into Abuf (default: the system's matrix), factored in place.
+// into Abuf (default: the system's matrix), factored in place.  The refactor of the system's
+// own matrix (the same ~150 launches with the same arguments every time) is captured into a
+// HIP graph on first use and replayed afterwards; phase timing runs the launches directly.
 int assemble_and_factor(bdh_system* s, void* Abuf = nullptr, const double4* pos2 = nullptr)
 {
     if (ensure_ewald(s)) return -1;
+    const bool graph = BDH_FACTOR_GRAPH && !Abuf && !pos2 && !s->rec;
+    if (!graph) return assemble_and_factor_launches(s, Abuf, pos2);
+    if (!s->factor_exec) {
+        BDH_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+        const int rc = assemble_and_factor_launches(s, nullptr, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s->st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        BDH_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&s->factor_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) s->factor_exec = nullptr;
+        BDH_HIP(ei);
+    }
+    BDH_HIP(hipGraphLaunch(s->factor_exec, s->st));
+    s->have_L = true;
+    return 0;
+}
+
+int assemble_and_factor_launches(bdh_system* s, void* Abuf, const double4* pos2)
+{
     void* M = Abuf ? Abuf : s->A;
     if (s->mixed) {
         float* A = (float*)M;
@@ -656,6 +689,7 @@ void bdh_destroy(bdh_system* s)
     dfree(s->Px); dfree(s->Pc); dfree(s->Pe); dfree(s->Feff); dfree(s->Cs); dfree(s->bt);
     dfree(s->cl.grid); dfree(s->cl.key); dfree(s->cl.count); dfree(s->cl.fill);
     dfree(s->cl.start); dfree(s->cl.sorted);
+    if (s->factor_exec) (void)hipGraphExecDestroy(s->factor_exec);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
