@@ -36,6 +36,7 @@ METRIC = "BD steps/s, N=3000 RPY+Cholesky: mixed vs FP64 GPU at rel err≤1e-5"
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f32_mfma": 157.3, "f64_mfma": 78.6, "bf16_mfma": 2500.0}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_N3000.json")
+TRAFFIC_FILE_BATCH = os.path.join(ROOT, "profiles", "traffic_batch256.json")
 KB = 0.001986248532617
 
 
@@ -87,14 +88,50 @@ def syrk_flops_per_factor(n, n_pad, nb=64):
     return f
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, path=TRAFFIC_FILE):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)
         return t["kernels"][kernel]["hbm_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def update_steps(n_pad, nb=64, outer=256, left_max=4096):
+    """(kb, K, t0, t1) of each update launch of one factorization: inside an outer panel
+    the next panel's columns (n_pad <= left_max: left-looking, all of the outer panel's
+    panels so far) or the outer panel's remaining columns (one panel); at its end the whole
+    trailing matrix with the outer panel."""
+    out = []
+    k = 0
+    while True:
+        c0 = k * nb
+        t0 = c0 + nb
+        if t0 >= n_pad:
+            return out
+        P0 = c0 // outer * outer
+        P1 = min(n_pad, P0 + outer)
+        if t0 < P1 and n_pad <= left_max:
+            out.append((P0, c0 + nb - P0, t0, t0 + nb))
+        elif t0 < P1:
+            out.append((c0, nb, t0, P1))
+        else:
+            out.append((P0, P1 - P0, t0, n_pad))
+        k += 1
+
+
+def update_bytes_per_factor(n_pad, word=4):
+    """Algorithmic HBM bytes of one factorization's update launches: every updated C entry
+    (rows t0..n_pad, columns t0..t1, lower part) read and written once, and the operand rows
+    t0..n_pad x K read once per launch."""
+    b = 0
+    for kb, K, t0, t1 in update_steps(n_pad):
+        rows = n_pad - t0
+        w = t1 - t0
+        c_entries = w * rows - w * (w - 1) // 2
+        b += 2 * word * c_entries + word * rows * K
+    return b
 
 
 def roofline(phases, n, mode, refactors):
@@ -429,12 +466,15 @@ def bench_batch(args):
     cyc = 2 * R
     ms, cnt = ph["syrk"]
     fl = sum(syrk_flops_per_factor(3 * N, -(-3 * N // 128) * 128) for N in sizes) * 2
-    ach = fl / (ms * 1e-3) / 1e12
-    roof = {"kernel": "syrk", "launches": cnt, "avg_ms": ms / max(cnt, 1), "bound": "mfma",
-            "achieved": ach, "peak": PEAK_TFLOPS["bf16_mfma"], "unit": "TFLOP/s",
-            "frac": ach / PEAK_TFLOPS["bf16_mfma"], "traffic": None,
-            "basis": "sum over chains of the lower-triangle trailing-update flops, "
-                     "2 factorizations; each FP32 product costs 3 bf16 MFMA products"}
+    by = sum(update_bytes_per_factor(-(-3 * N // 128) * 128) for N in sizes) * 2
+    ach = by / (ms * 1e-3) / 1e9
+    tr = measured_traffic("syrk", TRAFFIC_FILE_BATCH)
+    roof = {"kernel": "syrk", "launches": cnt, "avg_ms": ms / max(cnt, 1), "bound": "hbm",
+            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": tr, "mfma_tflops": fl / (ms * 1e-3) / 1e12,
+            "basis": "sum over chains and update launches of the updated lower-trapezoid C "
+                     "entries read and written (FP32) plus the K-column operand rows read once, "
+                     "2 factorizations; traffic = measured HBM bytes per launch"}
     value = out["mixed"][0]
     line = {
         "metric": f"BD steps/s, batch of {len(sizes)} chains N~logU[64,1024] RPY+Cholesky",
