@@ -1571,7 +1571,10 @@ int batch_create_impl(const bdh_config* cfg, int B, const int* N, const double* 
     for (int k = 0; k < max_steps; ++k)
         b->chol_blocks[k] = fill(6 + k, [k](const BatchChain& h) { return chol_update_blocks(h.n_pad, k); });
     BDH_CHECK(b->plan.asm_blocks >= 0 && b->plan.pair_blocks >= 0 && b->plan.force_blocks >= 0 &&
-              b->plan.trmv_blocks >= 0, "bdh_batch_create: batch too large");
+              b->plan.trmv_blocks >= 0 && b->plan.rows_blocks >= 0 && b->plan.trsm_blocks >= 0,
+              "bdh_batch_create: batch too large");
+    for (int k = 0; k < max_steps; ++k)
+        BDH_CHECK(b->chol_blocks[k] >= 0, "bdh_batch_create: batch too large");
     if (dmalloc(&b->d_tab, sizeof(int) * tab.size())) return -1;
     BDH_HIP(hipMemcpy(b->d_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
     if (dmalloc(&b->d_ch, sizeof(BatchChain) * B)) return -1;
