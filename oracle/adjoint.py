@@ -1,0 +1,172 @@
+"""FP64 reverse-mode derivative (vector-Jacobian product) of one Ermak-McCammon step with
+Cholesky noise, in numpy: test infrastructure, the checker of the device adjoint.
+
+The step (comp_step_ermak / compute_hydro_ermak, src/diff_alg/diff_comp.c:152-179,
+diff_equation.c:216-286, diff_rand.c:64-85), refactored at every step, with given
+Gaussians xi:
+
+    r' = r + c D(r) F(r) + s L(r) xi,   c = dt / kT,  s = sqrt(2 dt),  D = L L^T
+
+For a cotangent g of r' the cotangent of r is
+
+    rbar = g + grad_r <U, D(r)> - c H(r) (D g),
+    U    = c sym(g F^T) + sym(L^-T Phi(L^T Lbar) L^-1),  Lbar = s tril(g xi^T)
+
+with H the Hessian of the potential energy (dF/dr = -H), Phi the lower triangle with the
+diagonal halved (the derivative of the Cholesky factor, Murray 2016, "Differentiation of
+the Cholesky decomposition"), and grad_r <U, D> summed over the RPY blocks
+D_kj = B(r_k - r_j) (diff_tensor.c:45-92).  Cutoffs enter as the hard truncations the forward
+pass applies (no derivative at the cutoff itself).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import pyoracle as po
+
+KB = 0.001986248532617
+
+
+def _sym(X):
+    return 0.5 * (X + X.T)
+
+
+def step(sysm, p, coord, xi):
+    """r' of one step from coord with Gaussians xi (numpy FP64; refactor at coord)."""
+    D = po.compute_D(coord, p)
+    L = po.cholesky(D)
+    F, _, _ = po.forces(sysm, p, coord=coord)
+    c = p.dt / (KB * p.T)
+    s = math.sqrt(2.0 * p.dt)
+    out = np.array(coord, dtype=float).reshape(-1, 4).copy()
+    out[:, :3] += (c * (D @ F) + s * (L @ xi)).reshape(-1, 3)
+    return out
+
+
+def _pair_geometry(xyz):
+    d = xyz[:, None, :] - xyz[None, :, :]
+    r = np.sqrt(np.sum(d * d, axis=-1))
+    return d, r
+
+
+def rpy_fg(r, ai, aj, preDii, preDij):
+    """f, f', g, g' of B = f I + g e e^T (both branches of diff_tensor.c:45-92)."""
+    sig2 = ai * ai + aj * aj
+    abar = 0.5 * (ai + aj)
+    over = r < ai + aj
+    rs = np.where(r > 0, r, 1.0)
+    f = np.where(over, preDii / abar * (1.0 - 9.0 * rs / (32.0 * abar)),
+                 preDij * (1.0 / rs + sig2 / (3.0 * rs ** 3)))
+    fp = np.where(over, -9.0 * preDii / (32.0 * abar * abar), preDij * (-1.0 / rs ** 2 - sig2 / rs ** 4))
+    g = np.where(over, preDii / abar * (3.0 * rs / (32.0 * abar)), preDij * (1.0 / rs - sig2 / rs ** 3))
+    gp = np.where(over, 3.0 * preDii / (32.0 * abar * abar), preDij * (-1.0 / rs ** 2 + 3.0 * sig2 / rs ** 4))
+    return f, fp, g, gp
+
+
+def rpy_contract_grad(coord, U, p):
+    """grad over r of <U, D(r)> (U n x n; only the off-diagonal blocks depend on r)."""
+    coord = np.asarray(coord, float).reshape(-1, 4)
+    N = coord.shape[0]
+    preDii, preDij = po.prefactors(p)
+    xyz, a = coord[:, :3], coord[:, 3]
+    d, r = _pair_geometry(xyz)
+    f, fp, g, gp = rpy_fg(r, a[:, None], a[None, :], preDii, preDij)
+    Ub = U.reshape(N, 3, N, 3).transpose(0, 2, 1, 3)        # Ub[k, j] = block (k, j)
+    W = Ub + Ub.transpose(1, 0, 3, 2)                        # U_kj + U_jk^T
+    rs = np.where(r > 0, r, 1.0)
+    e = d / rs[..., None]
+    trW = np.einsum("kjaa->kj", W)
+    eWe = np.einsum("kja,kjab,kjb->kj", e, W, e)
+    WWe = np.einsum("kjab,kjb->kja", W + W.transpose(0, 1, 3, 2), e)
+    perp = WWe - np.einsum("kja,kj->kja", e, np.einsum("kja,kja->kj", e, WWe))
+    grad = (fp * trW)[..., None] * e + (gp * eWe)[..., None] * e + (g / rs)[..., None] * perp
+    mask = ~np.eye(N, dtype=bool)
+    return np.sum(grad * mask[..., None], axis=1).reshape(-1)
+
+
+def _pair_derivs(sysm, p, xyz):
+    """phi'(r), phi''(r) of every pair (LJ + Debye-Hueckel with the forward pass's cutoffs,
+    bonded-partner scaling) as N x N arrays."""
+    N = xyz.shape[0]
+    d, r = _pair_geometry(xyz)
+    rs = np.where(r > 0, r, 1.0)
+    LJ = np.asarray(sysm.LJ, float)
+    Q = np.asarray(sysm.Q, float)
+    conn = np.zeros((N, N), dtype=bool)
+    bonds = np.asarray(sysm.bonds, int).reshape(-1, 2)
+    use_conn = len(bonds) > 0 and ((p.elec and p.bond_c_scale != 1.0) or
+                                   (p.alpha_lj != 0.0 and p.bond_lj_scale != 1.0))
+    if use_conn:
+        conn[bonds[:, 0], bonds[:, 1]] = conn[bonds[:, 1], bonds[:, 0]] = True
+    d1 = np.zeros((N, N))
+    d2 = np.zeros((N, N))
+    if p.alpha_lj != 0.0:
+        sig = LJ[:, None, 0] + LJ[None, :, 0]
+        eps = np.sqrt(LJ[:, None, 1] * LJ[None, :, 1])
+        cut = p.cutoff_lj if p.cutoff_lj > 0 else 1.122462 * sig
+        s6 = (sig / rs) ** 6
+        lj6 = float(p.lj_6_term)
+        scale = np.where(conn, p.bond_lj_scale, 1.0)
+        on = (r <= cut) & (scale != 0.0)
+        d1 += np.where(on, scale * p.alpha_lj * eps * (-12.0 * s6 * s6 + 6.0 * lj6 * s6) / rs, 0.0)
+        d2 += np.where(on, scale * p.alpha_lj * eps * (156.0 * s6 * s6 - 42.0 * lj6 * s6) / rs ** 2, 0.0)
+    if p.elec:
+        gam = p.gamma_c / p.epsilon_c
+        k = p.kappa_c
+        qq = gam * Q[:, None] * Q[None, :]
+        scale = np.where(conn, p.bond_c_scale, 1.0)
+        on = (r <= p.cutoff_c) & (scale != 0.0)
+        ex = np.exp(-k * rs)
+        d1 += np.where(on, -scale * qq * ex * (1.0 / rs ** 2 + k / rs), 0.0)
+        d2 += np.where(on, scale * qq * ex * (2.0 / rs ** 3 + 2.0 * k / rs ** 2 + k * k / rs), 0.0)
+    np.fill_diagonal(d1, 0.0)
+    np.fill_diagonal(d2, 0.0)
+    return d, rs, d1, d2
+
+
+def hessian_vec(sysm, p, coord, y):
+    """H y, H the Hessian of the potential energy (pairs and bonds, bond.c:43-85)."""
+    coord = np.asarray(coord, float).reshape(-1, 4)
+    xyz = coord[:, :3]
+    N = xyz.shape[0]
+    y = np.asarray(y, float).reshape(N, 3)
+    d, rs, d1, d2 = _pair_derivs(sysm, p, xyz)
+    e = d / rs[..., None]
+    dy = y[:, None, :] - y[None, :, :]
+    ed = np.einsum("kja,kja->kj", e, dy)
+    out = np.einsum("kj,kj,kja->ka", d2, ed, e) + np.einsum("kj,kja->ka", d1 / rs, dy - ed[..., None] * e)
+    for (i, j), (r0, rmax, H) in zip(np.asarray(sysm.bonds, int).reshape(-1, 2),
+                                     np.asarray(sysm.bond_parms, float).reshape(-1, 3)):
+        dv = xyz[i] - xyz[j]
+        r = math.sqrt(dv @ dv)
+        ev = dv / r
+        den = 1.0 - r * r / (rmax * rmax)
+        b1 = H * (r - r0) / den
+        b2 = H * (den + (r - r0) * 2.0 * r / (rmax * rmax)) / (den * den)
+        M = b2 * np.outer(ev, ev) + (b1 / r) * (np.eye(3) - np.outer(ev, ev))
+        v = M @ (y[i] - y[j])
+        out[i] += v
+        out[j] -= v
+    return out.reshape(-1)
+
+
+def chol_vjp(L, Lbar):
+    """Symmetric U with <U, dD> = <Lbar, dL> for L = chol(D) (Lbar lower)."""
+    P = L.T @ np.tril(Lbar)
+    P = np.tril(P) - 0.5 * np.diag(np.diag(P))
+    Linv = np.linalg.inv(L)
+    return _sym(Linv.T @ P @ Linv)
+
+
+def step_vjp(sysm, p, coord, xi, gbar):
+    """Cotangent of r given the cotangent gbar (3N) of r' = step(r)."""
+    D = po.compute_D(coord, p)
+    L = po.cholesky(D)
+    F, _, _ = po.forces(sysm, p, coord=coord)
+    c = p.dt / (KB * p.T)
+    s = math.sqrt(2.0 * p.dt)
+    g = np.asarray(gbar, float).reshape(-1)
+    U = c * _sym(np.outer(g, F)) + chol_vjp(L, s * np.outer(g, xi))
+    return g + rpy_contract_grad(coord, U, p) - c * hessian_vec(sysm, p, coord, D @ g)
