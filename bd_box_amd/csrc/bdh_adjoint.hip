@@ -1,0 +1,250 @@
+// Reverse-mode derivative (vector-Jacobian product) of one FP64 Ermak-McCammon step with
+// Cholesky noise refactored at the step's positions (the step of comp_step_ermak /
+// compute_hydro_ermak, src/diff_alg/diff_comp.c:152-179, diff_equation.c:216-286,
+// diff_rand.c:64-85):
+//
+//   r' = r + c D(r) F(r) + s L(r) xi,        c = dt / kT,  s = sqrt(2 dt),  D = L L^T
+//
+// For a cotangent g of r' the cotangent of r is
+//
+//   rbar = g + grad_r <U, D(r)> - c H(r) (D g)
+//   U    = c sym(g F^T) + sym(S),  S = L^-T Phi(L^T Lbar) L^-1,  Lbar = s tril(g xi^T)
+//
+// H is the Hessian of the potential energy (dF/dr = -H), Phi the lower triangle with the
+// diagonal halved (derivative of the Cholesky factor, Murray 2016).  With a = L^T g,
+// Phi(L^T Lbar) = s Phi(a xi^T), so S comes from two triangular solves with n right-hand
+// sides: Y1 = L^-T (s Phi(a xi^T)), Y2 = L^-T Y1^T = S^T.
+//
+// Kernels (FP64, written for correctness first; the solves are O(n^3) with one thread per
+// right-hand side):
+//   adj_lt_g_kernel         a = L^T g
+//   adj_upper_solve_kernel  L^T X = B by back substitution, B = s Phi(a xi^T) or Y1^T
+//   adj_contract_kernel     grad_r <U, D(r)> over the RPY blocks (diff_tensor.c:45-92)
+//   adj_hvp_kernel          g + contraction - c H y  (pair and bond Hessians, the forward
+//                           pass's cutoffs and bonded-partner scaling; bond.c:43-85)
+#include "bdh_common.hpp"
+#include "bdh_kernels.hpp"
+#include "bdh_pairterm.hpp"
+
+namespace bdh {
+
+__global__ void __launch_bounds__(256)
+adj_lt_g_kernel(const double* __restrict__ L, long long ld, int n, const double* __restrict__ g,
+                double* __restrict__ a)
+{
+    __shared__ double s_red[256];
+    const int i = blockIdx.x;
+    const double* col = L + (long long)i * ld;
+    double acc = 0.0;
+    for (int k = i + threadIdx.x; k < n; k += 256) acc += col[k] * g[k];
+    s_red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a[i] = s_red[0];
+}
+
+// Right-hand side q per thread; L[k][i] = L[i * ld + k] (column-major lower factor).
+__global__ void __launch_bounds__(64)
+adj_upper_solve_kernel(const double* __restrict__ L, long long ld, int n, int mode,
+                       const double* __restrict__ a, const double* __restrict__ xi, double s,
+                       const double* __restrict__ B, double* __restrict__ X)
+{
+    const int q = blockIdx.x * 64 + threadIdx.x;
+    if (q >= n) return;
+    double* x = X + (long long)q * n;
+    const double sxq = mode == 0 ? s * xi[q] : 0.0;
+    for (int i = n - 1; i >= 0; --i) {
+        double b;
+        if (mode == 0) b = i > q ? a[i] * sxq : (i == q ? 0.5 * a[i] * sxq : 0.0);
+        else b = B[(long long)i * n + q];
+        const double* col = L + (long long)i * ld;
+        for (int k = i + 1; k < n; ++k) b -= col[k] * x[k];
+        x[i] = b / col[i];
+    }
+}
+
+// f, f', g, g' of the RPY block B = f I + g e e^T at separation r (radii ai, aj).
+__device__ __forceinline__ void rpy_fg(double r, double ai, double aj, double preDii, double preDij,
+                                       double& fp, double& gv, double& gp)
+{
+    if (r < ai + aj) {
+        const double abar = 0.5 * (ai + aj);
+        fp = -9.0 * preDii / (32.0 * abar * abar);
+        gv = preDii / abar * (3.0 * r / (32.0 * abar));
+        gp = 3.0 * preDii / (32.0 * abar * abar);
+    } else {
+        const double sig2 = ai * ai + aj * aj, r2 = r * r;
+        fp = preDij * (-1.0 / r2 - sig2 / (r2 * r2));
+        gv = preDij * (1.0 / r - sig2 / (r2 * r));
+        gp = preDij * (-1.0 / r2 + 3.0 * sig2 / (r2 * r2));
+    }
+}
+
+// out_k = sum_{j != k} grad_d <W_kj, B(d)> at d = r_k - r_j, with
+// W_kj = c (g_k F_j^T + F_k g_j^T) + Y2(3k.., 3j..) + Y2(3j.., 3k..)^T  (= 2 U_kj).
+__global__ void __launch_bounds__(128)
+adj_contract_kernel(const double4* __restrict__ pos, int N, double preDii, double preDij,
+                    const double* __restrict__ Y2, const double* __restrict__ g,
+                    const double* __restrict__ F, double c, double* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const long long n = 3LL * N;
+    const double4 pk = pos[k];
+    const double gk[3] = {g[3 * k], g[3 * k + 1], g[3 * k + 2]};
+    const double Fk[3] = {F[3 * k], F[3 * k + 1], F[3 * k + 2]};
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < N; ++j) {
+        if (j == k) continue;
+        const double4 pj = pos[j];
+        const double d[3] = {pk.x - pj.x, pk.y - pj.y, pk.z - pj.z};
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double e[3] = {d[0] / r, d[1] / r, d[2] / r};
+        double fp, gv, gp;
+        rpy_fg(r, pk.w, pj.w, preDii, preDij, fp, gv, gp);
+        double W[3][3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const long long row = 3LL * k + u, col = 3LL * j + v;
+                W[u][v] = c * (gk[u] * F[3 * j + v] + Fk[u] * g[3 * j + v]) + Y2[col * n + row] +
+                          Y2[row * n + col];
+            }
+        const double trW = W[0][0] + W[1][1] + W[2][2];
+        double We[3], WTe[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            We[u] = W[u][0] * e[0] + W[u][1] * e[1] + W[u][2] * e[2];
+            WTe[u] = W[0][u] * e[0] + W[1][u] * e[1] + W[2][u] * e[2];
+        }
+        const double eWe = e[0] * We[0] + e[1] * We[1] + e[2] * We[2];
+        double v[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) v[u] = We[u] + WTe[u];
+        const double ev = e[0] * v[0] + e[1] * v[1] + e[2] * v[2];
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+            acc[u] += (fp * trW + gp * eWe) * e[u] + (gv / r) * (v[u] - ev * e[u]);
+    }
+    out[3 * k] = acc[0];
+    out[3 * k + 1] = acc[1];
+    out[3 * k + 2] = acc[2];
+}
+
+// out_k = g_k + contraction_k - c (H y)_k
+__global__ void __launch_bounds__(128)
+adj_hvp_kernel(const double4* __restrict__ pos, const double* __restrict__ Q, const double2* __restrict__ LJ,
+               int N, Topology topo, ForceParams fp, const double* __restrict__ y,
+               const double* __restrict__ contract, const double* __restrict__ g, double c,
+               double* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const double4 pk = pos[k];
+    const double2 ljk = LJ[k];
+    const double qk = Q[k];
+    const double yk[3] = {y[3 * k], y[3 * k + 1], y[3 * k + 2]};
+    const int nb0 = fp.use_conn ? topo.conn_off[k] : 0;
+    const int nb1 = fp.use_conn ? topo.conn_off[k + 1] : 0;
+    const double lj6 = fp.lj_6_term;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < N; ++j) {
+        if (j == k) continue;
+        const double4 pj = pos[j];
+        const double d[3] = {pk.x - pj.x, pk.y - pj.y, pk.z - pj.z};
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (r2 > fp.far_r2) continue;
+        const double r = sqrt(r2);
+        const bool conn = bonded(topo, nb0, nb1, j);
+        double d1 = 0.0, d2 = 0.0;
+        if (fp.alpha_lj != 0.0) {
+            const double2 ljj = LJ[j];
+            const double sig = ljk.x + ljj.x;
+            const double cut = fp.cutoff_lj > 0.0 ? fp.cutoff_lj : (double)1.122462f * sig;
+            const double scale = conn ? fp.bond_lj_scale : 1.0;
+            if (r <= cut && scale != 0.0) {
+                const double eps = sqrt(ljk.y * ljj.y);
+                double s6 = sig / r;
+                s6 *= s6;
+                s6 *= s6 * s6;
+                d1 += scale * fp.alpha_lj * eps * (-12.0 * s6 * s6 + 6.0 * lj6 * s6) / r;
+                d2 += scale * fp.alpha_lj * eps * (156.0 * s6 * s6 - 42.0 * lj6 * s6) / r2;
+            }
+        }
+        if (fp.elec) {
+            const double scale = conn ? fp.bond_c_scale : 1.0;
+            if (r <= fp.cutoff_c && scale != 0.0) {
+                const double qq = fp.gam * qk * Q[j];
+                const double ex = exp(-fp.kappa_c * r);
+                const double kap = fp.kappa_c;
+                d1 += -scale * qq * ex * (1.0 / r2 + kap / r);
+                d2 += scale * qq * ex * (2.0 / (r2 * r) + 2.0 * kap / r2 + kap * kap / r);
+            }
+        }
+        if (d1 == 0.0 && d2 == 0.0) continue;
+        const double e[3] = {d[0] / r, d[1] / r, d[2] / r};
+        const double dy[3] = {yk[0] - y[3 * j], yk[1] - y[3 * j + 1], yk[2] - y[3 * j + 2]};
+        const double ed = e[0] * dy[0] + e[1] * dy[1] + e[2] * dy[2];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) acc[u] += d2 * ed * e[u] + (d1 / r) * (dy[u] - ed * e[u]);
+    }
+    for (int b = topo.bond_off[k]; b < topo.bond_off[k + 1]; ++b) {
+        const int p = topo.bond_p[b];
+        const double r0 = topo.bond_parm[3 * b], rmax = topo.bond_parm[3 * b + 1];
+        const double H = topo.bond_parm[3 * b + 2];
+        const double4 pp = pos[p];
+        const double d[3] = {pk.x - pp.x, pk.y - pp.y, pk.z - pp.z};
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double e[3] = {d[0] / r, d[1] / r, d[2] / r};
+        const double den = 1.0 - r * r / (rmax * rmax);
+        const double b1 = H * (r - r0) / den;
+        const double b2 = H * (den + (r - r0) * 2.0 * r / (rmax * rmax)) / (den * den);
+        const double dy[3] = {yk[0] - y[3 * p], yk[1] - y[3 * p + 1], yk[2] - y[3 * p + 2]};
+        const double ed = e[0] * dy[0] + e[1] * dy[1] + e[2] * dy[2];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) acc[u] += b2 * ed * e[u] + (b1 / r) * (dy[u] - ed * e[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 3; ++u) out[3 * k + u] = g[3 * k + u] + contract[3 * k + u] - c * acc[u];
+}
+
+int launch_adj_lt_g(const double* d_L, long long ld, int n, const double* d_g, double* d_a, hipStream_t st)
+{
+    hipLaunchKernelGGL(adj_lt_g_kernel, dim3(n), dim3(256), 0, st, d_L, ld, n, d_g, d_a);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_adj_upper_solve(const double* d_L, long long ld, int n, int mode, const double* d_a,
+                           const double* d_xi, double s, const double* d_B, double* d_X, hipStream_t st)
+{
+    hipLaunchKernelGGL(adj_upper_solve_kernel, dim3(ceil_div(n, 64)), dim3(64), 0, st, d_L, ld, n, mode,
+                       d_a, d_xi, s, d_B, d_X);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_adj_contract(const double4* d_pos, int N, double preDii, double preDij, const double* d_Y2,
+                        const double* d_g, const double* d_F, double c, double* d_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(adj_contract_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, st, d_pos, N, preDii,
+                       preDij, d_Y2, d_g, d_F, c, d_out);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_adj_hvp(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
+                   const Topology& topo, const ForceParams& fp, const double* d_y,
+                   const double* d_contract, const double* d_g, double c, double* d_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(adj_hvp_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, st, d_pos, d_Q, d_LJ, N, topo,
+                       fp, d_y, d_contract, d_g, c, d_out);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace bdh

@@ -152,6 +152,7 @@ struct bdh_system {
     bool have_L = false;
     EventRecorder* rec = nullptr;   // non-null while phase timing is enabled
     hipGraphExec_t factor_exec = nullptr;   // assembly + factorization, captured once
+    double* adj = nullptr;          // reverse-mode step: Y1, Y2 (n x n), then vectors
 };
 
 namespace {
@@ -690,6 +691,7 @@ void bdh_destroy(bdh_system* s)
     dfree(s->cl.grid); dfree(s->cl.key); dfree(s->cl.count); dfree(s->cl.fill);
     dfree(s->cl.start); dfree(s->cl.sorted);
     if (s->factor_exec) (void)hipGraphExecDestroy(s->factor_exec);
+    dfree(s->adj);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
@@ -1095,6 +1097,51 @@ int bdh_set_coords(bdh_system* s, const double* coord)
     if (compute_forces_dev(s)) return -1;
     BDH_HIP(hipStreamSynchronize(s->st));
     return 0;
+}
+
+int bdh_step_vjp(bdh_system* s, const double* coord, const double* xi, const double* gbar, double* rbar)
+{
+    BDH_CHECK(s && coord && xi && gbar && rbar, "bdh_step_vjp: bad arguments");
+    BDH_CHECK(!s->mixed, "bdh_step_vjp: FP64 systems only");
+    BDH_CHECK(s->cfg.hydro == BDH_HYDRO_CHOLESKY && !s->ewald, "bdh_step_vjp: free-space Cholesky noise only");
+    BDH_HIP(hipSetDevice(s->device));
+    const int n = s->n;
+    const size_t nn = (size_t)n * n;
+    if (!s->adj && dmalloc(&s->adj, sizeof(double) * (2 * nn + 7 * (size_t)n))) return -1;
+    double* Y1 = s->adj;
+    double* Y2 = Y1 + nn;
+    double* xi_d = Y2 + nn;
+    double* g_d = xi_d + n;
+    double* a_d = g_d + n;
+    double* Dg = a_d + n;
+    double* ctr = Dg + n;
+    double* out = ctr + n;
+    // the forward step's state at coord: forces and the factor of D
+    BDH_HIP(hipMemcpyAsync(s->pos, coord, sizeof(double) * 4 * s->N, hipMemcpyHostToDevice, s->st));
+    if (compute_forces_dev(s)) return -1;
+    if (assemble_and_factor(s)) return -1;
+    BDH_HIP(hipMemcpyAsync(xi_d, xi, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
+    BDH_HIP(hipMemcpyAsync(g_d, gbar, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
+    const double dt = s->cfg.dt;
+    const double c = dt / (kB * s->cfg.T);
+    const double sq = std::sqrt(2.0 * dt);
+    const double* L = (const double*)s->A;
+    // S^T = L^-T (L^-T s Phi(a xi^T))^T, a = L^T g
+    if (launch_adj_lt_g(L, s->ld, n, g_d, a_d, s->st)) return -1;
+    if (launch_adj_upper_solve(L, s->ld, n, 0, a_d, xi_d, sq, nullptr, Y1, s->st)) return -1;
+    if (launch_adj_upper_solve(L, s->ld, n, 1, nullptr, nullptr, 0.0, Y1, Y2, s->st)) return -1;
+    if (launch_adj_contract(s->pos, s->N, s->preDii, s->preDij, Y2, g_d, s->F, c, ctr, s->st)) return -1;
+    // D g from blocks on the fly, then g + contraction - c H (D g)
+    if (launch_pair_sums<double>(s->pos, g_d, nullptr, s->N, s->preDii, s->preDij, dt, false, false,
+                                 s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st))
+        return -1;
+    ReduceJobs jobs{};
+    jobs.job[0] = ReduceJob{s->Pf, Dg, n, s->msplits, 0, 1.0};
+    if (launch_reduce_partials(jobs, 1, n, s->st)) return -1;
+    if (launch_adj_hvp(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, Dg, ctr, g_d, c, out, s->st)) return -1;
+    BDH_HIP(hipMemcpyAsync(rbar, out, sizeof(double) * n, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return check_flags(s);
 }
 
 int bdh_get_forces(bdh_system* s, double* F)

@@ -147,6 +147,17 @@ class BDSystem:
         c = np.ascontiguousarray(coord, float).reshape(-1)
         _lib.check(_lib.load().bdh_set_coords(self._h, _lib.dptr(c)), "set_coords")
 
+    def step_vjp(self, coord, xi, gbar) -> np.ndarray:
+        """Cotangent of the positions before one FP64 step from coord with Gaussians xi,
+        given the cotangent gbar (3N) of the positions after it (bdh_step_vjp)."""
+        c = np.ascontiguousarray(coord, float).reshape(-1)
+        x = np.ascontiguousarray(xi, float).reshape(-1)
+        g = np.ascontiguousarray(gbar, float).reshape(-1)
+        out = np.zeros(self.n)
+        _lib.check(_lib.load().bdh_step_vjp(self._h, _lib.dptr(c), _lib.dptr(x), _lib.dptr(g),
+                                            _lib.dptr(out)), "step_vjp")
+        return out
+
     def forces(self) -> np.ndarray:
         F = np.zeros(self.n)
         _lib.check(_lib.load().bdh_get_forces(self._h, _lib.dptr(F)), "get_forces")

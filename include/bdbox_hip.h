@@ -169,6 +169,15 @@ int bdh_phase_timing(bdh_system* sys, int enable);
 int bdh_phase_times(bdh_system* sys, double* ms, long long* counts);
 int bdh_sync(bdh_system* sys);
 
+/* Reverse-mode derivative of one step (no counterpart in the reference, which has no
+ * derivatives): for the FP64 Ermak-McCammon step with Cholesky noise refactored at coord,
+ * r' = r + (dt/kT) D(r) F(r) + sqrt(2 dt) L(r) xi with the given xi[3N], rbar[3N] receives
+ * the cotangent of r for the cotangent gbar[3N] of r' (forces, RPY assembly and the
+ * Cholesky factor all differentiated).  FP64 systems with hydro cholesky and bc none.  The
+ * system is left at coord with its forces and factor there. */
+int bdh_step_vjp(bdh_system* sys, const double* coord, const double* xi, const double* gbar,
+                 double* rbar);
+
 /* ---- Batches of independent chains.  No counterpart in the reference, which integrates one
  * system per process: nchains systems of uneven sizes that share one bdh_config advance
  * together, every phase of the step (assembly, each launch of the blocked factorization,
