@@ -15,10 +15,10 @@
 // Phi(L^T Lbar) = s Phi(a xi^T), so S comes from two triangular solves with n right-hand
 // sides: Y1 = L^-T (s Phi(a xi^T)), Y2 = L^-T Y1^T = S^T.
 //
-// Kernels (FP64, written for correctness first; the solves are O(n^3) with one thread per
-// right-hand side):
+// Kernels (FP64; the two solves are the O(n^3) part):
 //   adj_lt_g_kernel         a = L^T g
-//   adj_upper_solve_kernel  L^T X = B by back substitution, B = s Phi(a xi^T) or Y1^T
+//   adj_rhs_kernel, adj_trsm_diag_kernel, adj_trsm_update_kernel
+//                           L^T X = B blocked by 64 rows, B = s Phi(a xi^T) or Y1^T
 //   adj_contract_kernel     grad_r <U, D(r)> over the RPY blocks (diff_tensor.c:45-92)
 //   adj_hvp_kernel          g + contraction - c H y  (pair and bond Hessians, the forward
 //                           pass's cutoffs and bonded-partner scaling; bond.c:43-85)
@@ -46,23 +46,94 @@ adj_lt_g_kernel(const double* __restrict__ L, long long ld, int n, const double*
     if (threadIdx.x == 0) a[i] = s_red[0];
 }
 
-// Right-hand side q per thread; L[k][i] = L[i * ld + k] (column-major lower factor).
-__global__ void __launch_bounds__(64)
-adj_upper_solve_kernel(const double* __restrict__ L, long long ld, int n, int mode,
-                       const double* __restrict__ a, const double* __restrict__ xi, double s,
-                       const double* __restrict__ B, double* __restrict__ X)
+// ---- L^T X = B for np right-hand sides (np = the padded order: L's padding is an identity
+// block, so the padded rows and columns of X stay zero), blocked by 64 rows from the last
+// block up: X_J <- L_JJ^-T X_J, then X_I -= L_JI^T X_J for every I < J.  X is column-major
+// (ld np), solved in place after the right-hand sides are written into it.
+constexpr int ADJ_NB = 64;
+
+// mode 0: X(i, q) = s a_i xi_q (i > q), s a_q xi_q / 2 (i = q), 0 (i < q or beyond n)
+// mode 1: X = Y^T (64 x 64 tiles through LDS)
+__global__ void __launch_bounds__(256)
+adj_rhs_kernel(int mode, int np, int n, const double* __restrict__ a, const double* __restrict__ xi,
+               double s, const double* __restrict__ Y, double* __restrict__ X)
 {
-    const int q = blockIdx.x * 64 + threadIdx.x;
-    if (q >= n) return;
-    double* x = X + (long long)q * n;
-    const double sxq = mode == 0 ? s * xi[q] : 0.0;
-    for (int i = n - 1; i >= 0; --i) {
-        double b;
-        if (mode == 0) b = i > q ? a[i] * sxq : (i == q ? 0.5 * a[i] * sxq : 0.0);
-        else b = B[(long long)i * n + q];
-        const double* col = L + (long long)i * ld;
-        for (int k = i + 1; k < n; ++k) b -= col[k] * x[k];
-        x[i] = b / col[i];
+    __shared__ double t[ADJ_NB][ADJ_NB + 1];
+    const int i0 = blockIdx.x * ADJ_NB, q0 = blockIdx.y * ADJ_NB;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    if (mode == 0) {
+        for (int qq = ty; qq < ADJ_NB; qq += 4) {
+            const int i = i0 + tx, q = q0 + qq;
+            double v = 0.0;
+            if (i < n && q < n) v = i > q ? s * a[i] * xi[q] : (i == q ? 0.5 * s * a[i] * xi[q] : 0.0);
+            X[(long long)q * np + i] = v;
+        }
+        return;
+    }
+    // X(i, q) = Y(q, i): read the tile of Y with rows q0.., columns i0.. (column-major)
+    for (int ii = ty; ii < ADJ_NB; ii += 4) t[ii][tx] = Y[(long long)(i0 + ii) * np + q0 + tx];
+    __syncthreads();
+    for (int qq = ty; qq < ADJ_NB; qq += 4) X[(long long)(q0 + qq) * np + i0 + tx] = t[tx][qq];
+}
+
+// X_J <- L_JJ^-T X_J for the 256 right-hand sides of this workgroup (one per thread).
+__global__ void __launch_bounds__(256)
+adj_trsm_diag_kernel(const double* __restrict__ L, long long ld, int np, int J, double* __restrict__ X)
+{
+    __shared__ double sL[ADJ_NB][ADJ_NB + 1];   // sL[k][i] = L(64 J + k, 64 J + i)
+    const int j0 = J * ADJ_NB;
+    for (int e = threadIdx.x; e < ADJ_NB * ADJ_NB; e += 256) {
+        const int k = e % ADJ_NB, i = e / ADJ_NB;
+        sL[k][i] = L[(long long)(j0 + i) * ld + j0 + k];
+    }
+    __syncthreads();
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= np) return;
+    double* xc = X + (long long)q * np + j0;
+    double x[ADJ_NB];
+#pragma unroll
+    for (int i = 0; i < ADJ_NB; ++i) x[i] = xc[i];
+#pragma unroll
+    for (int i = ADJ_NB - 1; i >= 0; --i) {
+        double b = x[i];
+#pragma unroll
+        for (int k = i + 1; k < ADJ_NB; ++k) b -= sL[k][i] * x[k];
+        x[i] = b / sL[i][i];
+    }
+#pragma unroll
+    for (int i = 0; i < ADJ_NB; ++i) xc[i] = x[i];
+}
+
+// X_I -= L_JI^T X_J: workgroup (I, column tile); 4 x 4 outputs per thread, K = 64.
+__global__ void __launch_bounds__(256)
+adj_trsm_update_kernel(const double* __restrict__ L, long long ld, int np, int J, double* __restrict__ X)
+{
+    __shared__ double sA[ADJ_NB][ADJ_NB + 1];   // sA[k][i] = L(64 J + k, 64 I + i)
+    __shared__ double sB[ADJ_NB][ADJ_NB + 1];   // sB[k][q] = X(64 J + k, q0 + q)
+    const int I = blockIdx.x, q0 = blockIdx.y * ADJ_NB;
+    const int i0 = I * ADJ_NB, j0 = J * ADJ_NB;
+    for (int e = threadIdx.x; e < ADJ_NB * ADJ_NB; e += 256) {
+        const int k = e % ADJ_NB, c = e / ADJ_NB;
+        sA[k][c] = L[(long long)(i0 + c) * ld + j0 + k];
+        sB[k][c] = X[(long long)(q0 + c) * np + j0 + k];
+    }
+    __syncthreads();
+    const int ti = (threadIdx.x >> 4) * 4, tq = (threadIdx.x & 15) * 4;
+    double acc[4][4] = {};
+    for (int k = 0; k < ADJ_NB; ++k) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { av[u] = sA[k][ti + u]; bv[u] = sB[k][tq + u]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double* xc = X + (long long)(q0 + tq + v) * np + i0 + ti;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xc[u] -= acc[u][v];
     }
 }
 
@@ -87,12 +158,11 @@ __device__ __forceinline__ void rpy_fg(double r, double ai, double aj, double pr
 // W_kj = c (g_k F_j^T + F_k g_j^T) + Y2(3k.., 3j..) + Y2(3j.., 3k..)^T  (= 2 U_kj).
 __global__ void __launch_bounds__(128)
 adj_contract_kernel(const double4* __restrict__ pos, int N, double preDii, double preDij,
-                    const double* __restrict__ Y2, const double* __restrict__ g,
+                    const double* __restrict__ Y2, long long n, const double* __restrict__ g,
                     const double* __restrict__ F, double c, double* __restrict__ out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
-    const long long n = 3LL * N;
     const double4 pk = pos[k];
     const double gk[3] = {g[3 * k], g[3 * k + 1], g[3 * k + 2]};
     const double Fk[3] = {F[3 * k], F[3 * k + 1], F[3 * k + 2]};
@@ -219,20 +289,30 @@ int launch_adj_lt_g(const double* d_L, long long ld, int n, const double* d_g, d
     return 0;
 }
 
-int launch_adj_upper_solve(const double* d_L, long long ld, int n, int mode, const double* d_a,
+int launch_adj_upper_solve(const double* d_L, long long ld, int np, int n, int mode, const double* d_a,
                            const double* d_xi, double s, const double* d_B, double* d_X, hipStream_t st)
 {
-    hipLaunchKernelGGL(adj_upper_solve_kernel, dim3(ceil_div(n, 64)), dim3(64), 0, st, d_L, ld, n, mode,
-                       d_a, d_xi, s, d_B, d_X);
+    BDH_CHECK(np % ADJ_NB == 0 && ld >= np, "adjoint solve: padded order");
+    const int nb = np / ADJ_NB;
+    hipLaunchKernelGGL(adj_rhs_kernel, dim3(nb, nb), dim3(256), 0, st, mode, np, n, d_a, d_xi, s, d_B, d_X);
     BDH_LAUNCH_CHECK();
+    for (int J = nb - 1; J >= 0; --J) {
+        hipLaunchKernelGGL(adj_trsm_diag_kernel, dim3(ceil_div(np, 256)), dim3(256), 0, st, d_L, ld, np, J, d_X);
+        BDH_LAUNCH_CHECK();
+        if (J > 0) {
+            hipLaunchKernelGGL(adj_trsm_update_kernel, dim3(J, nb), dim3(256), 0, st, d_L, ld, np, J, d_X);
+            BDH_LAUNCH_CHECK();
+        }
+    }
     return 0;
 }
 
 int launch_adj_contract(const double4* d_pos, int N, double preDii, double preDij, const double* d_Y2,
-                        const double* d_g, const double* d_F, double c, double* d_out, hipStream_t st)
+                        long long ldy, const double* d_g, const double* d_F, double c, double* d_out,
+                        hipStream_t st)
 {
     hipLaunchKernelGGL(adj_contract_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, st, d_pos, N, preDii,
-                       preDij, d_Y2, d_g, d_F, c, d_out);
+                       preDij, d_Y2, ldy, d_g, d_F, c, d_out);
     BDH_LAUNCH_CHECK();
     return 0;
 }

@@ -1105,8 +1105,8 @@ int bdh_step_vjp(bdh_system* s, const double* coord, const double* xi, const dou
     BDH_CHECK(!s->mixed, "bdh_step_vjp: FP64 systems only");
     BDH_CHECK(s->cfg.hydro == BDH_HYDRO_CHOLESKY && !s->ewald, "bdh_step_vjp: free-space Cholesky noise only");
     BDH_HIP(hipSetDevice(s->device));
-    const int n = s->n;
-    const size_t nn = (size_t)n * n;
+    const int n = s->n, np = s->n_pad;
+    const size_t nn = (size_t)np * np;
     if (!s->adj && dmalloc(&s->adj, sizeof(double) * (2 * nn + 7 * (size_t)n))) return -1;
     double* Y1 = s->adj;
     double* Y2 = Y1 + nn;
@@ -1128,9 +1128,9 @@ int bdh_step_vjp(bdh_system* s, const double* coord, const double* xi, const dou
     const double* L = (const double*)s->A;
     // S^T = L^-T (L^-T s Phi(a xi^T))^T, a = L^T g
     if (launch_adj_lt_g(L, s->ld, n, g_d, a_d, s->st)) return -1;
-    if (launch_adj_upper_solve(L, s->ld, n, 0, a_d, xi_d, sq, nullptr, Y1, s->st)) return -1;
-    if (launch_adj_upper_solve(L, s->ld, n, 1, nullptr, nullptr, 0.0, Y1, Y2, s->st)) return -1;
-    if (launch_adj_contract(s->pos, s->N, s->preDii, s->preDij, Y2, g_d, s->F, c, ctr, s->st)) return -1;
+    if (launch_adj_upper_solve(L, s->ld, np, n, 0, a_d, xi_d, sq, nullptr, Y1, s->st)) return -1;
+    if (launch_adj_upper_solve(L, s->ld, np, n, 1, nullptr, nullptr, 0.0, Y1, Y2, s->st)) return -1;
+    if (launch_adj_contract(s->pos, s->N, s->preDii, s->preDij, Y2, np, g_d, s->F, c, ctr, s->st)) return -1;
     // D g from blocks on the fly, then g + contraction - c H (D g)
     if (launch_pair_sums<double>(s->pos, g_d, nullptr, s->N, s->preDii, s->preDij, dt, false, false,
                                  s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st))

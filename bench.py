@@ -250,10 +250,11 @@ def main():
                          "256 MiB device copy that calibrates the HBM byte counters")
     ap.add_argument("--nb-list", choices=("brute", "bucket_list"), default="brute",
                     help="nonbonded pairs: all pairs, or the device cell list (the same pairs)")
-    ap.add_argument("--workload", choices=("chain", "periodic", "batch"), default="chain",
+    ap.add_argument("--workload", choices=("chain", "periodic", "batch", "adjoint"), default="chain",
                     help="chain: the N=3000 RPY + Cholesky step (default); periodic: N=8192 "
                          "spheres, Beenakker-Ewald RPY + TEA; batch: 256 independent chains "
-                         "of 64-1024 beads stepped together (each its own JSON line)")
+                         "of 64-1024 beads stepped together; adjoint: gradient of an N=1024 "
+                         "FP64 trajectory (each its own JSON line)")
     ap.add_argument("--chains", type=int, default=256, help="chains of the batch workload")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -271,6 +272,8 @@ def main():
         return bench_periodic(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     sysm, prm, P = workload(args.N, args.seed, args.nb_list)
     n = 3 * sysm.N
@@ -495,6 +498,47 @@ def bench_batch(args):
         "fp64_phases_ms_per_step": {k: v[0] / cyc for k, v in out["fp64"][1].items()},
         "roofline": roof,
         "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
+def bench_adjoint(args):
+    """Gradient of the positions after K FP64 steps (N = 1024 chain, given Gaussians,
+    refactor every step) with respect to the initial positions: forward, then the
+    checkpointed reverse pass (every 10 steps), timed together."""
+    import torch
+    from bd_box_amd import adjoint as adj
+    from bd_box_amd import workloads as wl
+    from bd_box_amd.system import BDSystem, FP64
+    N = 1024 if args.N == 3000 else args.N
+    K = min(args.steps, 100)
+    sysm = wl.chain(N, seed=args.seed)
+    sysm.coord[:, 3] = 1.6
+    P = wl.parse_params(wl.chain_params(rand_seed=20240 + args.seed))
+    rng = np.random.default_rng(args.seed)
+    xi = rng.normal(size=(K, 3 * N))
+    w = torch.from_numpy(rng.normal(size=(N, 3)))
+    log(f"adjoint workload: N={N}, {K} steps, checkpoints every 10")
+    s = BDSystem(sysm, P, precision=FP64, refactor_every=1)
+    x0 = torch.tensor(sysm.coord[:, :3].copy(), dtype=torch.float64, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = adj.trajectory(s, x0, xi, sysm.coord[:, 3], checkpoint=10)
+    t1 = time.perf_counter()
+    (out * w).sum().backward()
+    t2 = time.perf_counter()
+    g = x0.grad.numpy()
+    log(f"forward {t1 - t0:.3f} s, backward {t2 - t1:.3f} s, |grad| {np.linalg.norm(g):.4e}")
+    line = {
+        "metric": f"gradient steps/s, N={N} FP64 trajectory (forward + reverse)",
+        "value": K / (t2 - t0), "unit": "steps/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": 1e3 * (t2 - t0) / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"N={N} bead-spring chain (a = 1.6 A), free-space RPY + Cholesky "
+                               f"refactored every step, {K} steps with given Gaussians, "
+                               "checkpoints every 10 steps", "N": N, "parallelism": "1 GPU"},
+        "forward_s": t1 - t0, "backward_s": t2 - t1,
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line))
 
