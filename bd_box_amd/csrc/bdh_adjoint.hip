@@ -17,6 +17,7 @@
 //
 // Kernels (FP64; the two solves are the O(n^3) part):
 //   adj_lt_g_kernel         a = L^T g
+//   adj_diag_inv_kernel     inverses of L's 64 x 64 diagonal blocks
 //   adj_rhs_kernel, adj_trsm_diag_kernel, adj_trsm_update_kernel
 //                           L^T X = B blocked by 64 rows, B = s Phi(a xi^T) or Y1^T
 //   adj_contract_kernel     grad_r <U, D(r)> over the RPY blocks (diff_tensor.c:45-92)
@@ -76,32 +77,59 @@ adj_rhs_kernel(int mode, int np, int n, const double* __restrict__ a, const doub
     for (int qq = ty; qq < ADJ_NB; qq += 4) X[(long long)(q0 + qq) * np + i0 + tx] = t[tx][qq];
 }
 
-// X_J <- L_JJ^-T X_J for the 256 right-hand sides of this workgroup (one per thread).
-__global__ void __launch_bounds__(256)
-adj_trsm_diag_kernel(const double* __restrict__ L, long long ld, int np, int J, double* __restrict__ X)
+// Inverses of the 64 x 64 diagonal blocks of L (workgroup J, thread c: column c of
+// L_JJ^-1 by forward substitution), column-major into Minv + 4096 J.
+__global__ void __launch_bounds__(64)
+adj_diag_inv_kernel(const double* __restrict__ L, long long ld, double* __restrict__ Minv)
 {
-    __shared__ double sL[ADJ_NB][ADJ_NB + 1];   // sL[k][i] = L(64 J + k, 64 J + i)
-    const int j0 = J * ADJ_NB;
+    __shared__ double sL[ADJ_NB][ADJ_NB + 1];   // sL[i][k] = L(64 J + i, 64 J + k)
+    const int j0 = blockIdx.x * ADJ_NB, c = threadIdx.x;
+    for (int k = 0; k < ADJ_NB; ++k) sL[c][k] = L[(long long)(j0 + k) * ld + j0 + c];
+    __syncthreads();
+    double m[ADJ_NB];
+#pragma unroll
+    for (int i = 0; i < ADJ_NB; ++i) {
+        double b = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k) b -= sL[i][k] * m[k];
+        m[i] = (i >= c) ? b / sL[i][i] : 0.0;
+    }
+    double* out = Minv + (long long)blockIdx.x * ADJ_NB * ADJ_NB + (long long)c * ADJ_NB;
+#pragma unroll
+    for (int i = 0; i < ADJ_NB; ++i) out[i] = m[i];
+}
+
+// X_J <- L_JJ^-T X_J = M_J^T X_J for one 64-column tile of X (workgroup = column tile).
+__global__ void __launch_bounds__(256)
+adj_trsm_diag_kernel(const double* __restrict__ Minv, int np, int J, double* __restrict__ X)
+{
+    __shared__ double sA[ADJ_NB][ADJ_NB + 1];   // sA[k][i] = M_J(k, i)
+    __shared__ double sB[ADJ_NB][ADJ_NB + 1];   // sB[k][q] = X(64 J + k, q0 + q)
+    const int q0 = blockIdx.x * ADJ_NB, j0 = J * ADJ_NB;
+    const double* M = Minv + (long long)J * ADJ_NB * ADJ_NB;
     for (int e = threadIdx.x; e < ADJ_NB * ADJ_NB; e += 256) {
-        const int k = e % ADJ_NB, i = e / ADJ_NB;
-        sL[k][i] = L[(long long)(j0 + i) * ld + j0 + k];
+        const int k = e % ADJ_NB, c = e / ADJ_NB;
+        sA[k][c] = M[c * ADJ_NB + k];
+        sB[k][c] = X[(long long)(q0 + c) * np + j0 + k];
     }
     __syncthreads();
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= np) return;
-    double* xc = X + (long long)q * np + j0;
-    double x[ADJ_NB];
+    const int ti = (threadIdx.x >> 4) * 4, tq = (threadIdx.x & 15) * 4;
+    double acc[4][4] = {};
+    for (int k = 0; k < ADJ_NB; ++k) {
+        double av[4], bv[4];
 #pragma unroll
-    for (int i = 0; i < ADJ_NB; ++i) x[i] = xc[i];
+        for (int u = 0; u < 4; ++u) { av[u] = sA[k][ti + u]; bv[u] = sB[k][tq + u]; }
 #pragma unroll
-    for (int i = ADJ_NB - 1; i >= 0; --i) {
-        double b = x[i];
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int k = i + 1; k < ADJ_NB; ++k) b -= sL[k][i] * x[k];
-        x[i] = b / sL[i][i];
+            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
     }
 #pragma unroll
-    for (int i = 0; i < ADJ_NB; ++i) xc[i] = x[i];
+    for (int v = 0; v < 4; ++v) {
+        double* xc = X + (long long)(q0 + tq + v) * np + j0 + ti;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xc[u] = acc[u][v];
+    }
 }
 
 // X_I -= L_JI^T X_J: workgroup (I, column tile); 4 x 4 outputs per thread, K = 64.
@@ -289,15 +317,24 @@ int launch_adj_lt_g(const double* d_L, long long ld, int n, const double* d_g, d
     return 0;
 }
 
+int launch_adj_diag_inverses(const double* d_L, long long ld, int np, double* d_Minv, hipStream_t st)
+{
+    BDH_CHECK(np % ADJ_NB == 0 && ld >= np, "adjoint solve: padded order");
+    hipLaunchKernelGGL(adj_diag_inv_kernel, dim3(np / ADJ_NB), dim3(ADJ_NB), 0, st, d_L, ld, d_Minv);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
 int launch_adj_upper_solve(const double* d_L, long long ld, int np, int n, int mode, const double* d_a,
-                           const double* d_xi, double s, const double* d_B, double* d_X, hipStream_t st)
+                           const double* d_xi, double s, const double* d_B, const double* d_Minv,
+                           double* d_X, hipStream_t st)
 {
     BDH_CHECK(np % ADJ_NB == 0 && ld >= np, "adjoint solve: padded order");
     const int nb = np / ADJ_NB;
     hipLaunchKernelGGL(adj_rhs_kernel, dim3(nb, nb), dim3(256), 0, st, mode, np, n, d_a, d_xi, s, d_B, d_X);
     BDH_LAUNCH_CHECK();
     for (int J = nb - 1; J >= 0; --J) {
-        hipLaunchKernelGGL(adj_trsm_diag_kernel, dim3(ceil_div(np, 256)), dim3(256), 0, st, d_L, ld, np, J, d_X);
+        hipLaunchKernelGGL(adj_trsm_diag_kernel, dim3(nb), dim3(256), 0, st, d_Minv, np, J, d_X);
         BDH_LAUNCH_CHECK();
         if (J > 0) {
             hipLaunchKernelGGL(adj_trsm_update_kernel, dim3(J, nb), dim3(256), 0, st, d_L, ld, np, J, d_X);

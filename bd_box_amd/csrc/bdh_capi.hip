@@ -1107,7 +1107,8 @@ int bdh_step_vjp(bdh_system* s, const double* coord, const double* xi, const dou
     BDH_HIP(hipSetDevice(s->device));
     const int n = s->n, np = s->n_pad;
     const size_t nn = (size_t)np * np;
-    if (!s->adj && dmalloc(&s->adj, sizeof(double) * (2 * nn + 7 * (size_t)n))) return -1;
+    const size_t ninv = (size_t)(np / 64) * 4096;
+    if (!s->adj && dmalloc(&s->adj, sizeof(double) * (2 * nn + 7 * (size_t)n + ninv))) return -1;
     double* Y1 = s->adj;
     double* Y2 = Y1 + nn;
     double* xi_d = Y2 + nn;
@@ -1116,6 +1117,7 @@ int bdh_step_vjp(bdh_system* s, const double* coord, const double* xi, const dou
     double* Dg = a_d + n;
     double* ctr = Dg + n;
     double* out = ctr + n;
+    double* Minv = out + n;
     // the forward step's state at coord: forces and the factor of D
     BDH_HIP(hipMemcpyAsync(s->pos, coord, sizeof(double) * 4 * s->N, hipMemcpyHostToDevice, s->st));
     if (compute_forces_dev(s)) return -1;
@@ -1128,8 +1130,9 @@ int bdh_step_vjp(bdh_system* s, const double* coord, const double* xi, const dou
     const double* L = (const double*)s->A;
     // S^T = L^-T (L^-T s Phi(a xi^T))^T, a = L^T g
     if (launch_adj_lt_g(L, s->ld, n, g_d, a_d, s->st)) return -1;
-    if (launch_adj_upper_solve(L, s->ld, np, n, 0, a_d, xi_d, sq, nullptr, Y1, s->st)) return -1;
-    if (launch_adj_upper_solve(L, s->ld, np, n, 1, nullptr, nullptr, 0.0, Y1, Y2, s->st)) return -1;
+    if (launch_adj_diag_inverses(L, s->ld, np, Minv, s->st)) return -1;
+    if (launch_adj_upper_solve(L, s->ld, np, n, 0, a_d, xi_d, sq, nullptr, Minv, Y1, s->st)) return -1;
+    if (launch_adj_upper_solve(L, s->ld, np, n, 1, nullptr, nullptr, 0.0, Y1, Minv, Y2, s->st)) return -1;
     if (launch_adj_contract(s->pos, s->N, s->preDii, s->preDij, Y2, np, g_d, s->F, c, ctr, s->st)) return -1;
     // D g from blocks on the fly, then g + contraction - c H (D g)
     if (launch_pair_sums<double>(s->pos, g_d, nullptr, s->N, s->preDii, s->preDij, dt, false, false,

@@ -208,8 +208,11 @@ int launch_em_update(double4* d_pos, int N, const double* d_Rf, const double* d_
 int launch_adj_lt_g(const double* d_L, long long ld, int n, const double* d_g, double* d_a, hipStream_t st);
 // L^T X = B for n right-hand sides: mode 0 B = s Phi(a xi^T), mode 1 B = Y^T (d_B = Y)
 // (np = padded order, a multiple of 64; X and B are np x np column-major)
+// inverses of the 64 x 64 diagonal blocks (np / 64 x 4096 doubles)
+int launch_adj_diag_inverses(const double* d_L, long long ld, int np, double* d_Minv, hipStream_t st);
 int launch_adj_upper_solve(const double* d_L, long long ld, int np, int n, int mode, const double* d_a,
-                           const double* d_xi, double s, const double* d_B, double* d_X, hipStream_t st);
+                           const double* d_xi, double s, const double* d_B, const double* d_Minv,
+                           double* d_X, hipStream_t st);
 int launch_adj_contract(const double4* d_pos, int N, double preDii, double preDij, const double* d_Y2,
                         long long ldy, const double* d_g, const double* d_F, double c, double* d_out,
                         hipStream_t st);
