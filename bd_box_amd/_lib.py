@@ -25,7 +25,8 @@ EXPORTS = [
     "bdh_phase_timing", "bdh_phase_times", "bdh_neighbour_pairs", "bdh_last_dt",
     "bdh_batch_create", "bdh_batch_destroy", "bdh_batch_step", "bdh_batch_get_coords",
     "bdh_batch_get_forces", "bdh_batch_time_steps", "bdh_batch_phase_timing",
-    "bdh_batch_phase_times", "bdh_step_vjp",
+    "bdh_batch_phase_times", "bdh_step_vjp", "bdh_factor_cache", "bdh_save_factor",
+    "bdh_step_vjp_cached",
 ]
 
 
@@ -91,6 +92,9 @@ def load():
     L.bdh_phase_timing.argtypes = [C.c_void_p, C.c_int]
     L.bdh_phase_times.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_longlong)]
     L.bdh_step_vjp.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.bdh_factor_cache.argtypes = [C.c_void_p, C.c_int]
+    L.bdh_save_factor.argtypes = [C.c_void_p, C.c_int]
+    L.bdh_step_vjp_cached.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
     L.bdh_batch_create.argtypes = [C.POINTER(Config), C.c_int, _ip, _dp, _dp, _dp, _ip, _ip, _dp,
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]
     L.bdh_batch_destroy.argtypes = [C.c_void_p]

@@ -5,7 +5,8 @@ comp_step_ermak with Cholesky noise (src/diff_alg/diff_comp.c:152-179), refactor
 every step.
 
 The forward pass keeps the positions of every `checkpoint`-th step only; the backward pass
-re-runs each segment from its checkpoint and walks it back step by step.
+re-runs each segment from its checkpoint, saving the factor of each step on the device, and
+walks it back step by step with those factors.
 """
 from __future__ import annotations
 
@@ -40,17 +41,21 @@ class Trajectory(torch.autograd.Function):
         system, xi, every = ctx.system, ctx.xi, ctx.every
         K = xi.shape[0]
         g = g_out.detach().cpu().numpy().astype(np.float64).reshape(-1)
+        system.factor_cache(every)
         for seg in range(len(ctx.checkpoints) - 1, -1, -1):
             t0 = seg * every
             t1 = min(K, t0 + every)
-            # positions at the start of each step of the segment
+            # positions at the start of each step of the segment, and the step's factor
             starts = [ctx.checkpoints[seg]]
             system.set_coords(starts[0])
-            for t in range(t0, t1 - 1):
+            for t in range(t0, t1):
                 system.compute_hydro(1, xi[t])
-                starts.append(system.coords())
+                system.save_factor(t - t0)
+                if t + 1 < t1:
+                    starts.append(system.coords())
             for t in range(t1 - 1, t0 - 1, -1):
-                g = system.step_vjp(starts[t - t0], xi[t], g)
+                g = system.step_vjp(starts[t - t0], xi[t], g, slot=t - t0)
+        system.factor_cache(0)
         return torch.from_numpy(g.reshape(-1, 3)), None, None, None, None
 
 

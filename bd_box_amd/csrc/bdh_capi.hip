@@ -153,6 +153,8 @@ struct bdh_system {
     EventRecorder* rec = nullptr;   // non-null while phase timing is enabled
     hipGraphExec_t factor_exec = nullptr;   // assembly + factorization, captured once
     double* adj = nullptr;          // reverse-mode step: Y1, Y2 (n x n), then vectors
+    double* fcache = nullptr;       // saved factors (fcache_slots x n_pad^2, FP64)
+    int fcache_slots = 0;
 };
 
 namespace {
@@ -692,6 +694,7 @@ void bdh_destroy(bdh_system* s)
     dfree(s->cl.start); dfree(s->cl.sorted);
     if (s->factor_exec) (void)hipGraphExecDestroy(s->factor_exec);
     dfree(s->adj);
+    dfree(s->fcache);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
@@ -1099,7 +1102,47 @@ int bdh_set_coords(bdh_system* s, const double* coord)
     return 0;
 }
 
+int bdh_factor_cache(bdh_system* s, int slots)
+{
+    BDH_CHECK(s && slots >= 0, "bdh_factor_cache: bad arguments");
+    BDH_CHECK(!s->mixed, "bdh_factor_cache: FP64 systems only");
+    BDH_HIP(hipSetDevice(s->device));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    dfree(s->fcache);
+    s->fcache_slots = 0;
+    if (slots == 0) return 0;
+    if (dmalloc(&s->fcache, sizeof(double) * (size_t)s->n_pad * s->n_pad * slots)) return -1;
+    s->fcache_slots = slots;
+    return 0;
+}
+
+int bdh_save_factor(bdh_system* s, int slot)
+{
+    BDH_CHECK(s && s->have_L, "bdh_save_factor: no factor");
+    BDH_CHECK(slot >= 0 && slot < s->fcache_slots, "bdh_save_factor: slot out of range");
+    BDH_HIP(hipSetDevice(s->device));
+    const size_t nn = (size_t)s->n_pad * s->n_pad;
+    BDH_HIP(hipMemcpyAsync(s->fcache + nn * slot, s->A, sizeof(double) * nn, hipMemcpyDeviceToDevice, s->st));
+    return 0;
+}
+
+static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const double* xi,
+                         const double* gbar, double* rbar);
+
 int bdh_step_vjp(bdh_system* s, const double* coord, const double* xi, const double* gbar, double* rbar)
+{
+    return step_vjp_impl(s, -1, coord, xi, gbar, rbar);
+}
+
+int bdh_step_vjp_cached(bdh_system* s, int slot, const double* coord, const double* xi,
+                        const double* gbar, double* rbar)
+{
+    BDH_CHECK(s && slot >= 0 && slot < s->fcache_slots, "bdh_step_vjp_cached: slot out of range");
+    return step_vjp_impl(s, slot, coord, xi, gbar, rbar);
+}
+
+static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const double* xi,
+                         const double* gbar, double* rbar)
 {
     BDH_CHECK(s && coord && xi && gbar && rbar, "bdh_step_vjp: bad arguments");
     BDH_CHECK(!s->mixed, "bdh_step_vjp: FP64 systems only");
@@ -1121,7 +1164,14 @@ int bdh_step_vjp(bdh_system* s, const double* coord, const double* xi, const dou
     // the forward step's state at coord: forces and the factor of D
     BDH_HIP(hipMemcpyAsync(s->pos, coord, sizeof(double) * 4 * s->N, hipMemcpyHostToDevice, s->st));
     if (compute_forces_dev(s)) return -1;
-    if (assemble_and_factor(s)) return -1;
+    if (slot >= 0) {
+        // the factor at coord, saved when the forward pass stepped from there
+        BDH_HIP(hipMemcpyAsync(s->A, s->fcache + nn * slot, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                               s->st));
+        s->have_L = true;
+    } else if (assemble_and_factor(s)) {
+        return -1;
+    }
     BDH_HIP(hipMemcpyAsync(xi_d, xi, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
     BDH_HIP(hipMemcpyAsync(g_d, gbar, sizeof(double) * n, hipMemcpyHostToDevice, s->st));
     const double dt = s->cfg.dt;

@@ -147,16 +147,28 @@ class BDSystem:
         c = np.ascontiguousarray(coord, float).reshape(-1)
         _lib.check(_lib.load().bdh_set_coords(self._h, _lib.dptr(c)), "set_coords")
 
-    def step_vjp(self, coord, xi, gbar) -> np.ndarray:
+    def step_vjp(self, coord, xi, gbar, slot: Optional[int] = None) -> np.ndarray:
         """Cotangent of the positions before one FP64 step from coord with Gaussians xi,
-        given the cotangent gbar (3N) of the positions after it (bdh_step_vjp)."""
+        given the cotangent gbar (3N) of the positions after it (bdh_step_vjp); with slot,
+        the factor at coord comes from that factor-cache slot (bdh_step_vjp_cached)."""
         c = np.ascontiguousarray(coord, float).reshape(-1)
         x = np.ascontiguousarray(xi, float).reshape(-1)
         g = np.ascontiguousarray(gbar, float).reshape(-1)
         out = np.zeros(self.n)
-        _lib.check(_lib.load().bdh_step_vjp(self._h, _lib.dptr(c), _lib.dptr(x), _lib.dptr(g),
-                                            _lib.dptr(out)), "step_vjp")
+        L = _lib.load()
+        if slot is None:
+            rc = L.bdh_step_vjp(self._h, _lib.dptr(c), _lib.dptr(x), _lib.dptr(g), _lib.dptr(out))
+        else:
+            rc = L.bdh_step_vjp_cached(self._h, int(slot), _lib.dptr(c), _lib.dptr(x), _lib.dptr(g),
+                                       _lib.dptr(out))
+        _lib.check(rc, "step_vjp")
         return out
+
+    def factor_cache(self, slots: int):
+        _lib.check(_lib.load().bdh_factor_cache(self._h, int(slots)), "factor_cache")
+
+    def save_factor(self, slot: int):
+        _lib.check(_lib.load().bdh_save_factor(self._h, int(slot)), "save_factor")
 
     def forces(self) -> np.ndarray:
         F = np.zeros(self.n)

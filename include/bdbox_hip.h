@@ -177,6 +177,14 @@ int bdh_sync(bdh_system* sys);
  * system is left at coord with its forces and factor there. */
 int bdh_step_vjp(bdh_system* sys, const double* coord, const double* xi, const double* gbar,
                  double* rbar);
+/* Factor reuse for the reverse pass (no counterpart in the reference): bdh_factor_cache
+ * allocates `slots` FP64 factors on the device; bdh_save_factor copies the factor of the
+ * last step taken into a slot; bdh_step_vjp_cached is bdh_step_vjp with the factor at coord
+ * taken from a slot instead of refactoring (the caller saved it when stepping from coord). */
+int bdh_factor_cache(bdh_system* sys, int slots);
+int bdh_save_factor(bdh_system* sys, int slot);
+int bdh_step_vjp_cached(bdh_system* sys, int slot, const double* coord, const double* xi,
+                        const double* gbar, double* rbar);
 
 /* ---- Batches of independent chains.  No counterpart in the reference, which integrates one
  * system per process: nchains systems of uneven sizes that share one bdh_config advance

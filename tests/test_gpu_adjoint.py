@@ -45,6 +45,24 @@ def test_step_vjp_vs_oracle(api, N):
     assert np.linalg.norm(rbar - ref) < 1e-9 * np.linalg.norm(ref)
 
 
+def test_cached_factor_gives_the_same_vjp(api):
+    """The factor saved when stepping from coord is the one a refactor at coord gives."""
+    S, ad, po = api
+    sysm, P = _chain(30, seed=5)
+    rng = np.random.default_rng(30)
+    xi = rng.normal(size=3 * sysm.N)
+    g = rng.normal(size=3 * sysm.N)
+    print("cached factor vjp", flush=True)
+    s = S.BDSystem(sysm, P, precision=S.FP64, refactor_every=1)
+    ref = s.step_vjp(sysm.coord, xi, g)
+    s.factor_cache(2)
+    s.set_coords(sysm.coord)
+    s.compute_hydro(1, xi)
+    s.save_factor(1)
+    assert np.array_equal(s.step_vjp(sysm.coord, xi, g, slot=1), ref)
+    s.factor_cache(0)
+
+
 def test_trajectory_gradient(api):
     import torch
     from bd_box_amd import adjoint as adj
