@@ -20,6 +20,7 @@
 //   adj_diag_inv_kernel     inverses of L's 64 x 64 diagonal blocks
 //   adj_rhs_kernel, adj_trsm_diag_kernel, adj_trsm_update_kernel
 //                           L^T X = B blocked by 64 rows, B = s Phi(a xi^T) or Y1^T
+//   adj_symmetrize_kernel   Z = Y2 + Y2^T
 //   adj_contract_kernel     grad_r <U, D(r)> over the RPY blocks (diff_tensor.c:45-92)
 //   adj_hvp_kernel          g + contraction - c H y  (pair and bond Hessians, the forward
 //                           pass's cutoffs and bonded-partner scaling; bond.c:43-85)
@@ -165,6 +166,23 @@ adj_trsm_update_kernel(const double* __restrict__ L, long long ld, int np, int J
     }
 }
 
+// Z = Y + Y^T (np x np, column-major; 64 x 64 tiles through LDS), so that the contraction
+// reads one coalesced matrix.
+__global__ void __launch_bounds__(256)
+adj_symmetrize_kernel(int np, const double* __restrict__ Y, double* __restrict__ Z)
+{
+    __shared__ double t[ADJ_NB][ADJ_NB + 1];
+    const int i0 = blockIdx.x * ADJ_NB, q0 = blockIdx.y * ADJ_NB;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // t[ii][tx] = Y(q0 + tx, i0 + ii) = Y^T(i0 + ii, q0 + tx)
+    for (int ii = ty; ii < ADJ_NB; ii += 4) t[ii][tx] = Y[(long long)(i0 + ii) * np + q0 + tx];
+    __syncthreads();
+    for (int qq = ty; qq < ADJ_NB; qq += 4) {
+        const long long e = (long long)(q0 + qq) * np + i0 + tx;
+        Z[e] = Y[e] + t[tx][qq];
+    }
+}
+
 // f, f', g, g' of the RPY block B = f I + g e e^T at separation r (radii ai, aj).
 __device__ __forceinline__ void rpy_fg(double r, double ai, double aj, double preDii, double preDij,
                                        double& fp, double& gv, double& gp)
@@ -183,7 +201,7 @@ __device__ __forceinline__ void rpy_fg(double r, double ai, double aj, double pr
 }
 
 // out_k = sum_{j != k} grad_d <W_kj, B(d)> at d = r_k - r_j, with
-// W_kj = c (g_k F_j^T + F_k g_j^T) + Y2(3k.., 3j..) + Y2(3j.., 3k..)^T  (= 2 U_kj).
+// W_kj = c (g_k F_j^T + F_k g_j^T) + Z(3k.., 3j..),  Z = Y2 + Y2^T  (W = 2 U_kj).
 __global__ void __launch_bounds__(128)
 adj_contract_kernel(const double4* __restrict__ pos, int N, double preDii, double preDij,
                     const double* __restrict__ Y2, long long n, const double* __restrict__ g,
@@ -209,8 +227,7 @@ adj_contract_kernel(const double4* __restrict__ pos, int N, double preDii, doubl
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
                 const long long row = 3LL * k + u, col = 3LL * j + v;
-                W[u][v] = c * (gk[u] * F[3 * j + v] + Fk[u] * g[3 * j + v]) + Y2[col * n + row] +
-                          Y2[row * n + col];
+                W[u][v] = c * (gk[u] * F[3 * j + v] + Fk[u] * g[3 * j + v]) + Y2[col * n + row];
             }
         const double trW = W[0][0] + W[1][1] + W[2][2];
         double We[3], WTe[3];
@@ -341,6 +358,14 @@ int launch_adj_upper_solve(const double* d_L, long long ld, int np, int n, int m
             BDH_LAUNCH_CHECK();
         }
     }
+    return 0;
+}
+
+int launch_adj_symmetrize(int np, const double* d_Y, double* d_Z, hipStream_t st)
+{
+    BDH_CHECK(np % ADJ_NB == 0, "adjoint: padded order");
+    hipLaunchKernelGGL(adj_symmetrize_kernel, dim3(np / ADJ_NB, np / ADJ_NB), dim3(256), 0, st, np, d_Y, d_Z);
+    BDH_LAUNCH_CHECK();
     return 0;
 }
 

@@ -213,6 +213,8 @@ int launch_adj_diag_inverses(const double* d_L, long long ld, int np, double* d_
 int launch_adj_upper_solve(const double* d_L, long long ld, int np, int n, int mode, const double* d_a,
                            const double* d_xi, double s, const double* d_B, const double* d_Minv,
                            double* d_X, hipStream_t st);
+int launch_adj_symmetrize(int np, const double* d_Y, double* d_Z, hipStream_t st);
+// d_Y2 = Z = Y2 + Y2^T (ld ldy)
 int launch_adj_contract(const double4* d_pos, int N, double preDii, double preDij, const double* d_Y2,
                         long long ldy, const double* d_g, const double* d_F, double c, double* d_out,
                         hipStream_t st);
