@@ -26,6 +26,7 @@
 //                           pass's cutoffs and bonded-partner scaling; bond.c:43-85)
 #include "bdh_common.hpp"
 #include "bdh_kernels.hpp"
+#include "bdh_mfma.hpp"
 #include "bdh_pairterm.hpp"
 
 namespace bdh {
@@ -78,6 +79,54 @@ adj_rhs_kernel(int mode, int np, int n, const double* __restrict__ a, const doub
     for (int qq = ty; qq < ADJ_NB; qq += 4) X[(long long)(q0 + qq) * np + i0 + tx] = t[tx][qq];
 }
 
+// P = sA^T sB for 64 x 64 LDS tiles (sA[k][i], sB[k][q], k < 64) on v_mfma_f64_16x16x4_f64:
+// wavefront w owns the 32 x 32 quarter (rows 32 (w / 2), columns 32 (w % 2)) as 2 x 2 blocks;
+// accumulator r of lane l holds row 4 r + l / 16, column l % 16 of its block.
+__device__ __forceinline__ void tile_tn_mfma(const double (*sA)[ADJ_NB + 1], const double (*sB)[ADJ_NB + 1],
+                                             doublex4 acc[2][2])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = 32 * (wave >> 1), c0 = 32 * (wave & 1);
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k = 0; k < ADJ_NB; k += 4) {
+        double av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) av[a] = sA[k + lk][r0 + 16 * a + lr];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bv[b] = sB[k + lk][c0 + 16 * b + lr];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f64_16x16x4(av[a], bv[b], acc[a][b]);
+    }
+}
+
+// X(i0 + i, q0 + q) = P(i, q) (sub = false) or -= P(i, q) (sub = true) from tile_tn_mfma's
+// accumulators.
+__device__ __forceinline__ void tile_store(double* __restrict__ X, int np, int i0, int q0,
+                                           const doublex4 acc[2][2], bool sub)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = 32 * (wave >> 1), c0 = 32 * (wave & 1);
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            double* xc = X + (long long)(q0 + c0 + 16 * b + lr) * np + i0 + r0 + 16 * a + lk;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (sub) xc[4 * r] -= acc[a][b][r];
+                else xc[4 * r] = acc[a][b][r];
+            }
+        }
+}
+
 // Inverses of the 64 x 64 diagonal blocks of L (workgroup J, thread c: column c of
 // L_JJ^-1 by forward substitution), column-major into Minv + 4096 J.
 __global__ void __launch_bounds__(64)
@@ -114,26 +163,12 @@ adj_trsm_diag_kernel(const double* __restrict__ Minv, int np, int J, double* __r
         sB[k][c] = X[(long long)(q0 + c) * np + j0 + k];
     }
     __syncthreads();
-    const int ti = (threadIdx.x >> 4) * 4, tq = (threadIdx.x & 15) * 4;
-    double acc[4][4] = {};
-    for (int k = 0; k < ADJ_NB; ++k) {
-        double av[4], bv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) { av[u] = sA[k][ti + u]; bv[u] = sB[k][tq + u]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        double* xc = X + (long long)(q0 + tq + v) * np + j0 + ti;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) xc[u] = acc[u][v];
-    }
+    doublex4 acc[2][2];
+    tile_tn_mfma(sA, sB, acc);
+    tile_store(X, np, j0, q0, acc, false);
 }
 
-// X_I -= L_JI^T X_J: workgroup (I, column tile); 4 x 4 outputs per thread, K = 64.
+// X_I -= L_JI^T X_J: workgroup (I, column tile), K = 64 on MFMA.
 __global__ void __launch_bounds__(256)
 adj_trsm_update_kernel(const double* __restrict__ L, long long ld, int np, int J, double* __restrict__ X)
 {
@@ -147,23 +182,9 @@ adj_trsm_update_kernel(const double* __restrict__ L, long long ld, int np, int J
         sB[k][c] = X[(long long)(q0 + c) * np + j0 + k];
     }
     __syncthreads();
-    const int ti = (threadIdx.x >> 4) * 4, tq = (threadIdx.x & 15) * 4;
-    double acc[4][4] = {};
-    for (int k = 0; k < ADJ_NB; ++k) {
-        double av[4], bv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) { av[u] = sA[k][ti + u]; bv[u] = sB[k][tq + u]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        double* xc = X + (long long)(q0 + tq + v) * np + i0 + ti;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) xc[u] -= acc[u][v];
-    }
+    doublex4 acc[2][2];
+    tile_tn_mfma(sA, sB, acc);
+    tile_store(X, np, i0, q0, acc, true);
 }
 
 // Z = Y + Y^T (np x np, column-major; 64 x 64 tiles through LDS), so that the contraction
