@@ -1470,6 +1470,7 @@ struct bdh_batch {
     long long steps_done = 0;
     bool have_L = false;
     EventRecorder* rec = nullptr;
+    hipGraphExec_t factor_exec = nullptr;   // assembly + factorization of every chain, captured once
 };
 
 namespace {
@@ -1518,7 +1519,34 @@ int batch_forces(bdh_batch* b)
     return 0;
 }
 
+int batch_factor_launches(bdh_batch* b);
+
+// The refactor of every chain (the same ~50 launches with the same arguments each time) is
+// captured into a HIP graph on first use and replayed; phase timing runs the launches.
 int batch_factor(bdh_batch* b)
+{
+    if (b->rec || !BDH_FACTOR_GRAPH) return batch_factor_launches(b);
+    if (!b->factor_exec) {
+        BDH_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_factor_launches(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        BDH_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&b->factor_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) b->factor_exec = nullptr;
+        BDH_HIP(ei);
+    }
+    BDH_HIP(hipGraphLaunch(b->factor_exec, b->st));
+    b->have_L = true;
+    return 0;
+}
+
+int batch_factor_launches(bdh_batch* b)
 {
     BREC_BEGIN(PH_ASSEMBLE);
     if (b->mixed) {
@@ -1834,6 +1862,7 @@ void bdh_batch_destroy(bdh_batch* b)
     dfree(b->Pp); dfree(b->Pf); dfree(b->Pt); dfree(b->A); dfree(b->Lbuf);
     dfree(b->rng); dfree(b->rng_off); dfree(b->rng_cnt); dfree(b->flags); dfree(b->status);
     delete b->rec;
+    if (b->factor_exec) (void)hipGraphExecDestroy(b->factor_exec);
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
 }
