@@ -259,7 +259,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step of a "
-              "single chain runs on one MI355X; sharding is described in DESIGN.md)",
+              "single chain runs on one MI355X, and this build has no multi-GPU path)",
               file=sys.stderr)
         sys.exit(2)
 
