@@ -63,6 +63,11 @@ def trajectory(system: BDSystem, xyz0: torch.Tensor, xi, radii, checkpoint: int 
     """Positions (N, 3) after len(xi) steps from xyz0 (N, 3, float64) with Gaussians
     xi (K, 3N); differentiable with respect to xyz0.  `system` must be an FP64 BDSystem
     created with refactor_every=1 (its state is overwritten)."""
+    if system.cfg.precision != 0 or system.cfg.refactor_every != 1:
+        raise ValueError("trajectory: the system must be FP64 with refactor_every=1 (the "
+                         "derivative is that of a step refactored at its own positions)")
+    if checkpoint < 1:
+        raise ValueError("trajectory: checkpoint must be positive")
     if not torch.is_tensor(xi):
         xi = torch.as_tensor(np.asarray(xi, dtype=np.float64))
     return Trajectory.apply(xyz0, xi, system, radii, checkpoint)
