@@ -98,3 +98,19 @@ def test_trajectory_gradient(api):
 
     fd = (f(h) - f(-h)) / (2 * h)
     assert abs(grad @ v.reshape(-1) - fd) < 1e-5 * abs(fd)
+
+
+def test_trajectory_refuses_other_steps(api):
+    """Mixed precision or a factor reused over several steps is not the step differentiated."""
+    import torch
+    from bd_box_amd import adjoint as adj
+    S, ad, po = api
+    sysm, P = _chain(10, seed=3)
+    x0 = torch.tensor(sysm.coord[:, :3].copy(), dtype=torch.float64, requires_grad=True)
+    xi = np.zeros((2, 30))
+    for prec, refactor in ((S.MIXED, 1), (S.FP64, 10)):
+        s = S.BDSystem(sysm, P, precision=prec, refactor_every=refactor)
+        with pytest.raises(ValueError):
+            adj.trajectory(s, x0, xi, sysm.coord[:, 3])
+        s.close()
+
