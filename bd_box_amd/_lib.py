@@ -26,7 +26,8 @@ EXPORTS = [
     "bdh_batch_create", "bdh_batch_destroy", "bdh_batch_step", "bdh_batch_get_coords",
     "bdh_batch_get_forces", "bdh_batch_time_steps", "bdh_batch_phase_timing",
     "bdh_batch_phase_times", "bdh_step_vjp", "bdh_factor_cache", "bdh_save_factor",
-    "bdh_step_vjp_cached",
+    "bdh_step_vjp_cached", "bdh_batch_set_schedule", "bdh_batch_get_schedule",
+    "bdh_batch_autotune",
 ]
 
 
@@ -105,6 +106,9 @@ def load():
     L.bdh_batch_time_steps.argtypes = [C.c_void_p, C.c_int, _dp]
     L.bdh_batch_phase_timing.argtypes = [C.c_void_p, C.c_int]
     L.bdh_batch_phase_times.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_longlong)]
+    L.bdh_batch_set_schedule.argtypes = [C.c_void_p, C.c_int]
+    L.bdh_batch_get_schedule.argtypes = [C.c_void_p, _ip]
+    L.bdh_batch_autotune.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _ip, _ip]
     _lib = L
     return L
 

@@ -1449,6 +1449,8 @@ struct bdh_batch {
     Topology topo{};
     std::vector<BatchChain> hc;
     std::vector<int> chol_blocks;   // workgroups of update launch k (host)
+    std::vector<int> tab;           // host copy of every prefix table
+    int left_max = -1;              // update schedule (batch_set_schedule)
     BatchPlan plan;
     BatchChain* d_ch = nullptr;
     int* d_tab = nullptr;           // every prefix table
@@ -1543,6 +1545,95 @@ int batch_factor(bdh_batch* b)
     }
     BDH_HIP(hipGraphLaunch(b->factor_exec, b->st));
     b->have_L = true;
+    return 0;
+}
+
+// Update schedule of every chain's factorization: chains with n_pad <= left_max update
+// left-looking inside each outer panel, the others right-looking (chol_step); left_max < 0
+// gives each chain the schedule a single system of its size uses.  The update launches'
+// prefix tables are rewritten, and the captured refactor graph, whose grids they size, is
+// dropped.
+int batch_set_schedule(bdh_batch* b, int left_max)
+{
+    const int B = b->B, W = B + 1;
+    for (auto& h : b->hc) h.left = left_max < 0 ? chol_left_looking(h.n_pad) : h.n_pad <= left_max;
+    for (int k = 0; k < b->plan.chol_steps; ++k) {
+        int* t = b->tab.data() + (size_t)(6 + k) * W;
+        long long acc = 0;
+        for (int c = 0; c < B; ++c) {
+            t[c] = (int)acc;
+            acc += chol_update_blocks(b->hc[c].n_pad, b->hc[c].left != 0, k);
+        }
+        BDH_CHECK(acc <= (long long)INT32_MAX, "bdh_batch_set_schedule: batch too large");
+        t[B] = (int)acc;
+        b->chol_blocks[k] = (int)acc;
+    }
+    BDH_HIP(hipStreamSynchronize(b->st));
+    BDH_HIP(hipMemcpy(b->d_tab, b->tab.data(), sizeof(int) * b->tab.size(), hipMemcpyHostToDevice));
+    BDH_HIP(hipMemcpy(b->d_ch, b->hc.data(), sizeof(BatchChain) * B, hipMemcpyHostToDevice));
+    if (b->factor_exec) {
+        BDH_HIP(hipGraphExecDestroy(b->factor_exec));
+        b->factor_exec = nullptr;
+    }
+    b->left_max = left_max;
+    return 0;
+}
+
+// Candidate schedules: all right-looking (0), then left-looking for the chains up to each
+// of at most BDH_TUNE_MAX - 1 of the batch's distinct padded sizes (evenly spaced in sorted
+// order, the largest always included, which makes every chain left-looking).
+std::vector<int> batch_tune_candidates(const bdh_batch* b)
+{
+    std::vector<int> sizes;
+    for (const auto& h : b->hc) sizes.push_back(h.n_pad);
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    std::vector<int> cand{0};
+    const int S = (int)sizes.size(), M = std::min(S, BDH_TUNE_MAX - 1);
+    for (int i = 1; i <= M; ++i) cand.push_back(sizes[(size_t)((long long)i * S / M) - 1]);
+    return cand;
+}
+
+// Each candidate: one refactor to capture its graph, then `reps` timed replays (HIP events
+// on the batch's stream); its time is the fastest replay.  Every replay recomputes the same
+// matrices from the unchanged positions, so the candidates see identical work.
+int batch_autotune(bdh_batch* b, int reps, int* cand_out, double* ms_out, int* ncand, int* best)
+{
+    const std::vector<int> cand = batch_tune_candidates(b);
+    hipEvent_t e0, e1;
+    BDH_HIP(hipEventCreate(&e0));
+    BDH_HIP(hipEventCreate(&e1));
+    int rc = 0, ib = 0;
+    std::vector<double> t(cand.size(), 0.0);
+    for (size_t i = 0; i < cand.size() && !rc; ++i) {
+        rc = batch_set_schedule(b, cand[i]) || batch_factor(b) || batch_check(b) ? -1 : 0;
+        double tmin = 1e30;
+        for (int r = 0; r < reps && !rc; ++r) {
+            float ms = 0.f;
+            const bool ok = hipEventRecord(e0, b->st) == hipSuccess && batch_factor(b) == 0 &&
+                            hipEventRecord(e1, b->st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                            hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (!ok) {
+                set_error("bdh_batch_autotune: a timed refactor failed");
+                rc = -1;
+            }
+            tmin = std::min(tmin, (double)ms);
+        }
+        if (!rc) rc = batch_check(b);
+        t[i] = tmin;
+        if (t[i] < t[ib]) ib = (int)i;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (batch_set_schedule(b, cand[ib])) return -1;
+    b->have_L = false;
+    *ncand = (int)cand.size();
+    *best = cand[ib];
+    for (size_t i = 0; i < cand.size(); ++i) {
+        if (cand_out) cand_out[i] = cand[i];
+        if (ms_out) ms_out[i] = t[i];
+    }
     return 0;
 }
 
@@ -1655,6 +1746,7 @@ int batch_create_impl(const bdh_config* cfg, int B, const int* N, const double* 
         h.fsplits = force_splits(h.N);
         h.tcols = trmv_col_blocks(h.n);
         h.ldx = h.n + (h.n & 1);
+        h.left = chol_left_looking(h.n_pad);
         h.mat = mat;
         h.lb = lb;
         h.xi_off = xo;
@@ -1699,7 +1791,7 @@ int batch_create_impl(const bdh_config* cfg, int B, const int* N, const double* 
     b->plan.rows_blocks = fill(4, [](const BatchChain& h) { return ceil_div(h.n, 64); });
     b->plan.trsm_blocks = fill(5, [](const BatchChain& h) { return chol_trsm0_blocks(h.n_pad); });
     for (int k = 0; k < max_steps; ++k)
-        b->chol_blocks[k] = fill(6 + k, [k](const BatchChain& h) { return chol_update_blocks(h.n_pad, k); });
+        b->chol_blocks[k] = fill(6 + k, [k](const BatchChain& h) { return chol_update_blocks(h.n_pad, h.left, k); });
     BDH_CHECK(b->plan.asm_blocks >= 0 && b->plan.pair_blocks >= 0 && b->plan.force_blocks >= 0 &&
               b->plan.trmv_blocks >= 0 && b->plan.rows_blocks >= 0 && b->plan.trsm_blocks >= 0,
               "bdh_batch_create: batch too large");
@@ -1709,6 +1801,7 @@ int batch_create_impl(const bdh_config* cfg, int B, const int* N, const double* 
     BDH_HIP(hipMemcpy(b->d_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
     if (dmalloc(&b->d_ch, sizeof(BatchChain) * B)) return -1;
     BDH_HIP(hipMemcpy(b->d_ch, b->hc.data(), sizeof(BatchChain) * B, hipMemcpyHostToDevice));
+    b->tab = std::move(tab);
     b->plan.nch = B;
     b->plan.ch = b->d_ch;
     b->plan.asm_pre = b->d_tab;
@@ -1902,6 +1995,27 @@ int bdh_batch_step(bdh_batch* b, int steps, const double* xi)
     }
     if (batch_run(b, steps, X)) return -1;
     return batch_check(b);
+}
+
+int bdh_batch_set_schedule(bdh_batch* b, int left_max)
+{
+    BDH_CHECK(b, "bdh_batch_set_schedule: null batch");
+    return batch_set_schedule(b, left_max);
+}
+
+int bdh_batch_get_schedule(bdh_batch* b, int* left_max)
+{
+    BDH_CHECK(b && left_max, "bdh_batch_get_schedule: bad arguments");
+    *left_max = b->left_max;
+    return 0;
+}
+
+int bdh_batch_autotune(bdh_batch* b, int reps, int* cand, double* ms, int* ncand, int* best)
+{
+    BDH_CHECK(b && reps > 0 && ncand && best, "bdh_batch_autotune: bad arguments");
+    BDH_CHECK(b->steps_done == 0, "bdh_batch_autotune: call before the first step (the tuning "
+                                  "refactors at the current positions)");
+    return batch_autotune(b, reps, cand, ms, ncand, best);
 }
 
 int bdh_batch_get_coords(bdh_batch* b, double* coord)

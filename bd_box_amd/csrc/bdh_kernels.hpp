@@ -23,10 +23,13 @@ struct PhaseRecorder {
 // (ld = n_pad) at A + mat, FP64 factor workspace at Lbuf + lb, its Gaussians at xi + xi_off
 // (rows of ldx), and partial-sum rows at pp (pair sums: psplits rows), pf (forces: fsplits
 // rows) and pt (L.x: tcols x 16 rows).  Every per-chain launch shape is the one the
-// single-system launchers use, so a chain's arithmetic is the same as on its own.
+// single-system launchers use, so a chain's arithmetic is the same as on its own, as long as
+// `left` (the update schedule of its factorization, chol_left_looking) is the one the single
+// system picks for its n_pad.
 struct BatchChain {
     int N, n, n_pad, bead0;
     int psplits, fsplits, tcols, ldx;
+    int left;
     long long mat, lb, xi_off, pp, pf, pt;
 };
 
@@ -110,7 +113,8 @@ int launch_trmv_partials(const T* d_L, long long ld, int n, const double* d_X, l
 int trmv_blk_cols();   // columns per partial of the L.x kernel
 // Per-chain workgroup counts of the batched launches (host).
 int chol_update_steps(int n_pad);                 // update launches of one factorization
-int chol_update_blocks(int n_pad, int k);         // workgroups of launch k (0 when none)
+bool chol_left_looking(int n_pad);                // the single-system update schedule
+int chol_update_blocks(int n_pad, bool left, int k);   // workgroups of launch k (0 when none)
 int chol_trsm0_blocks(int n_pad);                 // panel-0 solve workgroups
 template <typename T>
 int trmv_blocks(int n);                           // row x column blocks of the L.x partials

@@ -303,6 +303,26 @@ class BDBatch:
         _lib.check(_lib.load().bdh_batch_phase_times(self._h, _lib.dptr(ms), cnt), "phase_times")
         return {p: (float(ms[i]), int(cnt[i])) for i, p in enumerate(BDSystem.PHASES)}
 
+    def set_schedule(self, left_max: int):
+        """Chains with padded size n_pad <= left_max factor with left-looking updates inside
+        each outer panel, the others right-looking; -1 restores the single-system choice."""
+        _lib.check(_lib.load().bdh_batch_set_schedule(self._h, int(left_max)), "set_schedule")
+
+    def schedule(self) -> int:
+        v = C.c_int()
+        _lib.check(_lib.load().bdh_batch_get_schedule(self._h, C.byref(v)), "schedule")
+        return v.value
+
+    def autotune(self, reps: int = 3):
+        """Times the refactor under each candidate schedule and keeps the fastest (before the
+        first step).  Returns (chosen left_max, {left_max: ms})."""
+        cand = (C.c_int * 8)()
+        ms = np.zeros(8)
+        n, best = C.c_int(), C.c_int()
+        _lib.check(_lib.load().bdh_batch_autotune(self._h, int(reps), cand, _lib.dptr(ms),
+                                                  C.byref(n), C.byref(best)), "autotune")
+        return best.value, {int(cand[i]): float(ms[i]) for i in range(n.value)}
+
 
 def gauss(state: int, count: int):
     """`count` Gaussians of the drand48 polar Box-Muller stream from 48-bit `state`."""

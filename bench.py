@@ -121,12 +121,12 @@ def update_steps(n_pad, nb=64, outer=256, left_max=4096):
         k += 1
 
 
-def update_bytes_per_factor(n_pad, word=4):
+def update_bytes_per_factor(n_pad, word=4, left_max=4096):
     """Algorithmic HBM bytes of one factorization's update launches: every updated C entry
     (rows t0..n_pad, columns t0..t1, lower part) read and written once, and the operand rows
     t0..n_pad x K read once per launch."""
     b = 0
-    for kb, K, t0, t1 in update_steps(n_pad):
+    for kb, K, t0, t1 in update_steps(n_pad, left_max=left_max):
         rows = n_pad - t0
         w = t1 - t0
         c_entries = w * rows - w * (w - 1) // 2
@@ -256,6 +256,9 @@ def main():
                          "of 64-1024 beads stepped together; adjoint: gradient of an N=1024 "
                          "FP64 trajectory (each its own JSON line)")
     ap.add_argument("--chains", type=int, default=256, help="chains of the batch workload")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="batch workload: keep the single-system update schedule instead of "
+                         "timing the candidates before the run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step of a "
@@ -435,9 +438,13 @@ def bench_batch(args):
     warm = min(args.warmup, R)
     log(f"batch workload: {len(sizes)} chains, {Ntot} beads (N {min(sizes)}..{max(sizes)}), "
         f"{steps} timed steps")
-    out = {}
+    out, sched = {}, {}
     for name, prec in (("mixed", MIXED_SPLIT), ("fp64", FP64)):
         b = BDBatch(systems, P, seeds, precision=prec, refactor_every=R)
+        if args.autotune:
+            best, times = b.autotune(reps=3)
+            sched[name] = {"left_max": best, "refactor_ms": {str(k): v for k, v in times.items()}}
+            log(f"{name}: autotuned update schedule left_max={best} (refactor ms {times})")
         b.compute_hydro(warm)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -455,6 +462,8 @@ def bench_batch(args):
     disp = {}
     for name, prec in (("mixed", MIXED_SPLIT), ("fp64", FP64)):
         b = BDBatch(systems, P, seeds, precision=prec, refactor_every=R)
+        if name in sched:
+            b.set_schedule(sched[name]["left_max"])
         b.compute_hydro(R)
         disp[name] = b.coords()[:, :3]
         b.close()
@@ -469,7 +478,8 @@ def bench_batch(args):
     cyc = 2 * R
     ms, cnt = ph["syrk"]
     fl = sum(syrk_flops_per_factor(3 * N, -(-3 * N // 128) * 128) for N in sizes) * 2
-    by = sum(update_bytes_per_factor(-(-3 * N // 128) * 128) for N in sizes) * 2
+    lm = sched["mixed"]["left_max"] if "mixed" in sched else 4096
+    by = sum(update_bytes_per_factor(-(-3 * N // 128) * 128, left_max=lm) for N in sizes) * 2
     ach = by / (ms * 1e-3) / 1e9
     tr = measured_traffic("syrk", TRAFFIC_FILE_BATCH)
     roof = {"kernel": "syrk", "launches": cnt, "avg_ms": ms / max(cnt, 1), "bound": "hbm",
@@ -488,6 +498,7 @@ def bench_batch(args):
                                f"[64, 1024] (sum {Ntot} beads), free-space RPY + Cholesky "
                                f"refactored every {R} steps, split-bf16 trailing updates",
                    "chains": len(sizes), "beads": Ntot, "refactor_every": R,
+                   "update_schedule": sched or "single-system default (left_max 4096)",
                    "parallelism": "1 GPU"},
         "chain_steps_per_s": value * len(sizes),
         "bead_steps_per_s": value * Ntot,

@@ -210,6 +210,24 @@ int bdh_batch_time_steps(bdh_batch* batch, int steps, double* ms_total);
 int bdh_batch_phase_timing(bdh_batch* batch, int enable);
 int bdh_batch_phase_times(bdh_batch* batch, double* ms, long long* counts);
 
+/* Update schedule of the batch's factorizations (no counterpart in the reference, whose
+ * CPU path calls LAPACK dpotrf).  Inside each 256-column outer panel, a chain whose padded
+ * size n_pad is at most left_max updates left-looking (every panel of the outer panel so
+ * far updates the next 64 columns); the others update right-looking (each panel updates all
+ * later columns of the outer panel).  left_max < 0 (the default) gives every chain the
+ * schedule a single system of its size uses: left-looking up to n_pad 4096, so chain c's
+ * trajectory is then bitwise the single system's.  Schedules differ in rounding only. */
+int bdh_batch_set_schedule(bdh_batch* batch, int left_max);
+int bdh_batch_get_schedule(bdh_batch* batch, int* left_max);
+/* Tile-schedule autotuner (no counterpart in the reference): times the refactor of every
+ * chain (assembly + factorization, graph replays, fastest of `reps`) under each candidate
+ * left_max -- 0, then up to BDH_TUNE_MAX - 1 of the batch's padded sizes, the largest
+ * included -- and keeps the fastest.  cand / ms (BDH_TUNE_MAX entries each, may be NULL)
+ * receive the candidates and their times, *ncand their count, *best the choice.  Call it
+ * before the first step; the first step then refactors as usual. */
+#define BDH_TUNE_MAX 8
+int bdh_batch_autotune(bdh_batch* batch, int reps, int* cand, double* ms, int* ncand, int* best);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,3 +105,72 @@ def test_batch_config_checks(api):
         S.BDBatch(systems, _params(hydro="geyer"), [1, 2], precision=S.FP64)
     with pytest.raises(BackendError, match="check_overlap"):
         S.BDBatch(systems, _params(check_overlap="yes"), [1, 2], precision=S.FP64)
+
+
+@pytest.mark.parametrize("left_max", [0, 512, 8192])
+def test_batch_update_schedules(api, left_max):
+    """Every update schedule leaves the factor up to rounding: all right-looking (0), mixed
+    (512: n_pad 128 and 512 chains left-looking, the rest right-looking) and all left-looking
+    (8192, including a chain with n_pad 4224 that a single system factors right-looking).
+    FP64 vs the oracle (small chains) and vs the chain's own FP64 system (N = 1400):
+    <= 1e-10 relative displacement; split-bf16 vs the oracle: <= 1e-5."""
+    S, po = api
+    sizes = [12, 130, 45, 1400]
+    systems = wl.chains(sizes, seed=610)
+    P = _params()
+    steps, refactor = 3, 2
+    xi = np.random.default_rng(6).normal(size=(steps, 3 * sum(sizes)))
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    print(f"batch update schedule left_max={left_max}", flush=True)
+    bb = {}
+    for prec in ("FP64", "MIXED_SPLIT"):
+        b = S.BDBatch(systems, P, [1, 2, 3, 4], precision=getattr(S, prec), refactor_every=refactor)
+        b.set_schedule(left_max)
+        assert b.schedule() == left_max
+        b.compute_hydro(steps, xi)
+        bb[prec] = b
+    for c, sysm in enumerate(systems):
+        xc = xi[:, 3 * off[c]:3 * off[c + 1]]
+        if sysm.N <= 200:
+            ref = po.Sim(sysm, po.params_from(P), seed=1, refactor_every=refactor)
+            ref.run(steps, xc)
+            tols = (("FP64", 1e-10), ("MIXED_SPLIT", 1e-5))
+        else:
+            ref = S.BDSystem(sysm, P, precision=S.FP64, refactor_every=refactor)
+            ref.compute_hydro(steps, np.ascontiguousarray(xc))
+            tols = (("FP64", 1e-10),)
+        d_ref = _disp(ref.coords(), sysm.coord)
+        for prec, tol in tols:
+            d = _disp(bb[prec].chain_coords(c), sysm.coord)
+            err = np.linalg.norm(d - d_ref) / np.linalg.norm(d_ref)
+            assert err < tol, (c, prec, err)
+    for b in bb.values():
+        b.close()
+
+
+def test_batch_autotune(api):
+    """The autotuner times each candidate schedule, keeps the fastest, and the batch then
+    steps exactly as one created with that schedule set by hand."""
+    S, _ = api
+    from bd_box_amd._lib import BackendError
+    systems = wl.chains(SIZES, seed=500)
+    P = _params()
+    xi = np.random.default_rng(7).normal(size=(4, 3 * sum(SIZES)))
+    print("batch autotune", flush=True)
+    b = S.BDBatch(systems, P, [1] * len(SIZES), precision=S.MIXED_SPLIT, refactor_every=2)
+    best, times = b.autotune(reps=2)
+    n_pads = sorted({-(-3 * n // 128) * 128 for n in SIZES})
+    assert 0 in times and n_pads[-1] in times and best in times
+    assert set(times) <= {0, *n_pads}
+    assert all(t > 0 for t in times.values())
+    assert times[best] == min(times.values())
+    assert b.schedule() == best
+    b.compute_hydro(4, xi)
+    r = S.BDBatch(systems, P, [1] * len(SIZES), precision=S.MIXED_SPLIT, refactor_every=2)
+    r.set_schedule(best)
+    r.compute_hydro(4, xi)
+    assert np.array_equal(b.coords(), r.coords())
+    with pytest.raises(BackendError, match="before the first step"):
+        b.autotune()
+    b.close()
+    r.close()
