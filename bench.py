@@ -242,9 +242,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--precision", choices=("split", "f32"), default="split",
+    ap.add_argument("--precision", choices=("split", "f32", "fp64"), default="split",
                     help="mixed-precision variant timed as `value`: trailing updates as "
-                         "split-bf16 MFMA products (split) or FP32 MFMA (f32)")
+                         "split-bf16 MFMA products (split) or FP32 MFMA (f32); fp64 (with "
+                         "--profile-only) runs the FP64 step instead")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the timed mixed-precision steps (for rocprofv3 runs), then a "
                          "256 MiB device copy that calibrates the HBM byte counters")
@@ -281,7 +282,10 @@ def main():
     sysm, prm, P = workload(args.N, args.seed, args.nb_list)
     n = 3 * sysm.N
     log(f"workload: N={sysm.N} n={n} refactor_every={args.refactor}")
-    prec = MIXED_SPLIT if args.precision == "split" else MIXED
+    if args.precision == "fp64" and not args.profile_only:
+        sys.exit("bench.py: --precision fp64 is for --profile-only (the bench times FP64 beside "
+                 "the mixed step itself)")
+    prec = {"split": MIXED_SPLIT, "f32": MIXED, "fp64": FP64}[args.precision]
     mixed = BDSystem(sysm, P, precision=prec, refactor_every=args.refactor)
     mixed.compute_hydro(args.warmup)
     torch.cuda.synchronize()
