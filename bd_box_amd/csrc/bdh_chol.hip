@@ -874,7 +874,11 @@ syrk_f64_kernel(double* __restrict__ A, long long ld, int kb, int K, int t0, int
 // (64 columns) with updates restricted to the outer panel; the trailing matrix beyond it is
 // updated once per outer panel with K = CHOL_OUTER, which divides its HBM traffic (read +
 // write of the trailing lower triangle) by CHOL_OUTER / 64.
-constexpr int CHOL_OUTER = 256;
+#ifndef CHOL_OUTER_COLS
+#define CHOL_OUTER_COLS 256
+#endif
+constexpr int CHOL_OUTER = CHOL_OUTER_COLS;
+static_assert(CHOL_OUTER % CHOL_NB == 0 && CHOL_OUTER >= CHOL_NB, "outer panel width");
 
 // Update launch k (k = 0, 1, ...) of an n_pad matrix, after panel k = columns 64 k ..:
 // inside an outer panel, the update of the outer panel's later columns; at the end of an
