@@ -21,9 +21,11 @@
 //   adj_rhs_kernel, adj_trsm_diag_kernel, adj_trsm_update_kernel
 //                           L^T X = B blocked by 64 rows, B = s Phi(a xi^T) or Y1^T
 //   adj_symmetrize_kernel   Z = Y2 + Y2^T
-//   adj_contract_kernel     grad_r <U, D(r)> over the RPY blocks (diff_tensor.c:45-92)
-//   adj_hvp_kernel          g + contraction - c H y  (pair and bond Hessians, the forward
-//                           pass's cutoffs and bonded-partner scaling; bond.c:43-85)
+//   adj_contract_kernel     grad_r <U, D(r)> over the RPY blocks (diff_tensor.c:45-92), column splits
+//   adj_hvp_pairs_kernel,   g + contraction - c H y  (pair and bond Hessians, the forward
+//   adj_hvp_finish_kernel   pass's cutoffs and bonded-partner scaling; bond.c:43-85)
+#include <algorithm>
+
 #include "bdh_common.hpp"
 #include "bdh_kernels.hpp"
 #include "bdh_mfma.hpp"
@@ -221,112 +223,170 @@ __device__ __forceinline__ void rpy_fg(double r, double ai, double aj, double pr
     }
 }
 
-// out_k = sum_{j != k} grad_d <W_kj, B(d)> at d = r_k - r_j, with
+// Term of pair (k, j) of out_k = sum_{j != k} grad_d <W_kj, B(d)> at d = r_k - r_j, with
 // W_kj = c (g_k F_j^T + F_k g_j^T) + Z(3k.., 3j..),  Z = Y2 + Y2^T  (W = 2 U_kj).
-__global__ void __launch_bounds__(128)
-adj_contract_kernel(const double4* __restrict__ pos, int N, double preDii, double preDij,
-                    const double* __restrict__ Y2, long long n, const double* __restrict__ g,
-                    const double* __restrict__ F, double c, double* __restrict__ out)
+__device__ __forceinline__ void contract_pair(const double4& pk, const double4& pj, int k, int j,
+                                              double preDii, double preDij, const double* __restrict__ Z,
+                                              long long ldz, const double* gk, const double* Fk,
+                                              const double* __restrict__ g, const double* __restrict__ F,
+                                              double c, double* acc)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    const double4 pk = pos[k];
-    const double gk[3] = {g[3 * k], g[3 * k + 1], g[3 * k + 2]};
-    const double Fk[3] = {F[3 * k], F[3 * k + 1], F[3 * k + 2]};
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int j = 0; j < N; ++j) {
-        if (j == k) continue;
-        const double4 pj = pos[j];
-        const double d[3] = {pk.x - pj.x, pk.y - pj.y, pk.z - pj.z};
-        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        const double e[3] = {d[0] / r, d[1] / r, d[2] / r};
-        double fp, gv, gp;
-        rpy_fg(r, pk.w, pj.w, preDii, preDij, fp, gv, gp);
-        double W[3][3];
+    const double d[3] = {pk.x - pj.x, pk.y - pj.y, pk.z - pj.z};
+    const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double e[3] = {d[0] / r, d[1] / r, d[2] / r};
+    double fp, gv, gp;
+    rpy_fg(r, pk.w, pj.w, preDii, preDij, fp, gv, gp);
+    double W[3][3];
 #pragma unroll
-        for (int u = 0; u < 3; ++u)
+    for (int u = 0; u < 3; ++u)
 #pragma unroll
-            for (int v = 0; v < 3; ++v) {
-                const long long row = 3LL * k + u, col = 3LL * j + v;
-                W[u][v] = c * (gk[u] * F[3 * j + v] + Fk[u] * g[3 * j + v]) + Y2[col * n + row];
-            }
-        const double trW = W[0][0] + W[1][1] + W[2][2];
-        double We[3], WTe[3];
-#pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            We[u] = W[u][0] * e[0] + W[u][1] * e[1] + W[u][2] * e[2];
-            WTe[u] = W[0][u] * e[0] + W[1][u] * e[1] + W[2][u] * e[2];
+        for (int v = 0; v < 3; ++v) {
+            const long long row = 3LL * k + u, col = 3LL * j + v;
+            W[u][v] = c * (gk[u] * F[3 * j + v] + Fk[u] * g[3 * j + v]) + Z[col * ldz + row];
         }
-        const double eWe = e[0] * We[0] + e[1] * We[1] + e[2] * We[2];
-        double v[3];
+    const double trW = W[0][0] + W[1][1] + W[2][2];
+    double We[3], WTe[3];
 #pragma unroll
-        for (int u = 0; u < 3; ++u) v[u] = We[u] + WTe[u];
-        const double ev = e[0] * v[0] + e[1] * v[1] + e[2] * v[2];
-#pragma unroll
-        for (int u = 0; u < 3; ++u)
-            acc[u] += (fp * trW + gp * eWe) * e[u] + (gv / r) * (v[u] - ev * e[u]);
+    for (int u = 0; u < 3; ++u) {
+        We[u] = W[u][0] * e[0] + W[u][1] * e[1] + W[u][2] * e[2];
+        WTe[u] = W[0][u] * e[0] + W[1][u] * e[1] + W[2][u] * e[2];
     }
-    out[3 * k] = acc[0];
-    out[3 * k + 1] = acc[1];
-    out[3 * k + 2] = acc[2];
+    const double eWe = e[0] * We[0] + e[1] * We[1] + e[2] * We[2];
+    double v[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) v[u] = We[u] + WTe[u];
+    const double ev = e[0] * v[0] + e[1] * v[1] + e[2] * v[2];
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+        acc[u] += (fp * trW + gp * eWe) * e[u] + (gv / r) * (v[u] - ev * e[u]);
 }
 
-// out_k = g_k + contraction_k - c (H y)_k
-__global__ void __launch_bounds__(128)
-adj_hvp_kernel(const double4* __restrict__ pos, const double* __restrict__ Q, const double2* __restrict__ LJ,
-               int N, Topology topo, ForceParams fp, const double* __restrict__ y,
-               const double* __restrict__ contract, const double* __restrict__ g, double c,
-               double* __restrict__ out)
+// Workgroup (x, s): row beads k = 64 x + lane, column beads j of split s
+// (js beads from s js); wave w takes j = s js + w, + 4, ...  The four waves' sums meet in
+// LDS in wave order and land in P[s][3k + u]; the splits are summed afterwards in split
+// order, so the result does not depend on scheduling.
+__global__ void __launch_bounds__(256)
+adj_contract_kernel(const double4* __restrict__ pos, int N, double preDii, double preDij,
+                    const double* __restrict__ Z, long long ldz, const double* __restrict__ g,
+                    const double* __restrict__ F, double c, int js, double* __restrict__ P)
+{
+    __shared__ double red[3][3][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    const int j0 = blockIdx.y * js, j1 = min(N, j0 + js);
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (k < N) {
+        const double4 pk = pos[k];
+        const double gk[3] = {g[3 * k], g[3 * k + 1], g[3 * k + 2]};
+        const double Fk[3] = {F[3 * k], F[3 * k + 1], F[3 * k + 2]};
+        for (int j = j0 + wave; j < j1; j += 4) {
+            if (j == k) continue;
+            contract_pair(pk, pos[j], k, j, preDii, preDij, Z, ldz, gk, Fk, g, F, c, acc);
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int u = 0; u < 3; ++u) red[wave - 1][u][lane] = acc[u];
+    }
+    __syncthreads();
+    if (wave == 0 && k < N) {
+        double* out = P + (long long)blockIdx.y * 3 * N;
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+            out[3 * k + u] = ((acc[u] + red[0][u][lane]) + red[1][u][lane]) + red[2][u][lane];
+    }
+}
+
+// Pair part of (H y)_k over the column beads of split s (the forward pass's cutoffs and
+// bonded-partner scaling); workgroups, waves and the wave-order LDS sum as in
+// adj_contract_kernel, partials in P[s][3k + u].
+__global__ void __launch_bounds__(256)
+adj_hvp_pairs_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
+                     const double2* __restrict__ LJ, int N, Topology topo, ForceParams fp,
+                     const double* __restrict__ y, int js, double* __restrict__ P)
+{
+    __shared__ double red[3][3][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    const int j0 = blockIdx.y * js, j1 = min(N, j0 + js);
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (k < N) {
+        const double4 pk = pos[k];
+        const double2 ljk = LJ[k];
+        const double qk = Q[k];
+        const double yk[3] = {y[3 * k], y[3 * k + 1], y[3 * k + 2]};
+        const int nb0 = fp.use_conn ? topo.conn_off[k] : 0;
+        const int nb1 = fp.use_conn ? topo.conn_off[k + 1] : 0;
+        const double lj6 = fp.lj_6_term;
+        for (int j = j0 + wave; j < j1; j += 4) {
+            if (j == k) continue;
+            const double4 pj = pos[j];
+            const double d[3] = {pk.x - pj.x, pk.y - pj.y, pk.z - pj.z};
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (r2 > fp.far_r2) continue;
+            const double r = sqrt(r2);
+            const bool conn = bonded(topo, nb0, nb1, j);
+            double d1 = 0.0, d2 = 0.0;
+            if (fp.alpha_lj != 0.0) {
+                const double2 ljj = LJ[j];
+                const double sig = ljk.x + ljj.x;
+                const double cut = fp.cutoff_lj > 0.0 ? fp.cutoff_lj : (double)1.122462f * sig;
+                const double scale = conn ? fp.bond_lj_scale : 1.0;
+                if (r <= cut && scale != 0.0) {
+                    const double eps = sqrt(ljk.y * ljj.y);
+                    double s6 = sig / r;
+                    s6 *= s6;
+                    s6 *= s6 * s6;
+                    d1 += scale * fp.alpha_lj * eps * (-12.0 * s6 * s6 + 6.0 * lj6 * s6) / r;
+                    d2 += scale * fp.alpha_lj * eps * (156.0 * s6 * s6 - 42.0 * lj6 * s6) / r2;
+                }
+            }
+            if (fp.elec) {
+                const double scale = conn ? fp.bond_c_scale : 1.0;
+                if (r <= fp.cutoff_c && scale != 0.0) {
+                    const double qq = fp.gam * qk * Q[j];
+                    const double ex = exp(-fp.kappa_c * r);
+                    const double kap = fp.kappa_c;
+                    d1 += -scale * qq * ex * (1.0 / r2 + kap / r);
+                    d2 += scale * qq * ex * (2.0 / (r2 * r) + 2.0 * kap / r2 + kap * kap / r);
+                }
+            }
+            if (d1 == 0.0 && d2 == 0.0) continue;
+            const double e[3] = {d[0] / r, d[1] / r, d[2] / r};
+            const double dy[3] = {yk[0] - y[3 * j], yk[1] - y[3 * j + 1], yk[2] - y[3 * j + 2]};
+            const double ed = e[0] * dy[0] + e[1] * dy[1] + e[2] * dy[2];
+    #pragma unroll
+            for (int u = 0; u < 3; ++u) acc[u] += d2 * ed * e[u] + (d1 / r) * (dy[u] - ed * e[u]);
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int u = 0; u < 3; ++u) red[wave - 1][u][lane] = acc[u];
+    }
+    __syncthreads();
+    if (wave == 0 && k < N) {
+        double* out = P + (long long)blockIdx.y * 3 * N;
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+            out[3 * k + u] = ((acc[u] + red[0][u][lane]) + red[1][u][lane]) + red[2][u][lane];
+    }
+}
+
+// out_k = g_k + contraction_k - c (H y)_k: the pair partials summed in split order, then the
+// bond Hessians (bond.c:43-85).
+__global__ void __launch_bounds__(256)
+adj_hvp_finish_kernel(const double4* __restrict__ pos, int N, Topology topo, const double* __restrict__ y,
+                      const double* __restrict__ P, int S, const double* __restrict__ contract,
+                      const double* __restrict__ g, double c, double* __restrict__ out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
     const double4 pk = pos[k];
-    const double2 ljk = LJ[k];
-    const double qk = Q[k];
     const double yk[3] = {y[3 * k], y[3 * k + 1], y[3 * k + 2]};
-    const int nb0 = fp.use_conn ? topo.conn_off[k] : 0;
-    const int nb1 = fp.use_conn ? topo.conn_off[k + 1] : 0;
-    const double lj6 = fp.lj_6_term;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int j = 0; j < N; ++j) {
-        if (j == k) continue;
-        const double4 pj = pos[j];
-        const double d[3] = {pk.x - pj.x, pk.y - pj.y, pk.z - pj.z};
-        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        if (r2 > fp.far_r2) continue;
-        const double r = sqrt(r2);
-        const bool conn = bonded(topo, nb0, nb1, j);
-        double d1 = 0.0, d2 = 0.0;
-        if (fp.alpha_lj != 0.0) {
-            const double2 ljj = LJ[j];
-            const double sig = ljk.x + ljj.x;
-            const double cut = fp.cutoff_lj > 0.0 ? fp.cutoff_lj : (double)1.122462f * sig;
-            const double scale = conn ? fp.bond_lj_scale : 1.0;
-            if (r <= cut && scale != 0.0) {
-                const double eps = sqrt(ljk.y * ljj.y);
-                double s6 = sig / r;
-                s6 *= s6;
-                s6 *= s6 * s6;
-                d1 += scale * fp.alpha_lj * eps * (-12.0 * s6 * s6 + 6.0 * lj6 * s6) / r;
-                d2 += scale * fp.alpha_lj * eps * (156.0 * s6 * s6 - 42.0 * lj6 * s6) / r2;
-            }
-        }
-        if (fp.elec) {
-            const double scale = conn ? fp.bond_c_scale : 1.0;
-            if (r <= fp.cutoff_c && scale != 0.0) {
-                const double qq = fp.gam * qk * Q[j];
-                const double ex = exp(-fp.kappa_c * r);
-                const double kap = fp.kappa_c;
-                d1 += -scale * qq * ex * (1.0 / r2 + kap / r);
-                d2 += scale * qq * ex * (2.0 / (r2 * r) + 2.0 * kap / r2 + kap * kap / r);
-            }
-        }
-        if (d1 == 0.0 && d2 == 0.0) continue;
-        const double e[3] = {d[0] / r, d[1] / r, d[2] / r};
-        const double dy[3] = {yk[0] - y[3 * j], yk[1] - y[3 * j + 1], yk[2] - y[3 * j + 2]};
-        const double ed = e[0] * dy[0] + e[1] * dy[1] + e[2] * dy[2];
+    for (int sp = 0; sp < S; ++sp) {
 #pragma unroll
-        for (int u = 0; u < 3; ++u) acc[u] += d2 * ed * e[u] + (d1 / r) * (dy[u] - ed * e[u]);
+        for (int u = 0; u < 3; ++u) acc[u] += P[(long long)sp * 3 * N + 3 * k + u];
     }
     for (int b = topo.bond_off[k]; b < topo.bond_off[k + 1]; ++b) {
         const int p = topo.bond_p[b];
@@ -390,22 +450,38 @@ int launch_adj_symmetrize(int np, const double* d_Y, double* d_Z, hipStream_t st
     return 0;
 }
 
-int launch_adj_contract(const double4* d_pos, int N, double preDii, double preDij, const double* d_Y2,
-                        long long ldy, const double* d_g, const double* d_F, double c, double* d_out,
-                        hipStream_t st)
+int adj_contract_splits(int N)
 {
-    hipLaunchKernelGGL(adj_contract_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, st, d_pos, N, preDii,
-                       preDij, d_Y2, ldy, d_g, d_F, c, d_out);
+    const int rows = ceil_div(N, 64);
+    return std::max(1, std::min({ADJ_CONTRACT_MAX_SPLITS, ceil_div(1024, rows), ceil_div(N, 4)}));
+}
+
+int launch_adj_contract(const double4* d_pos, int N, double preDii, double preDij, const double* d_Y2,
+                        long long ldy, const double* d_g, const double* d_F, double c, double* d_part,
+                        double* d_out, hipStream_t st)
+{
+    const int S = adj_contract_splits(N);
+    const int js = ceil_div(N, S);
+    hipLaunchKernelGGL(adj_contract_kernel, dim3(ceil_div(N, 64), S), dim3(256), 0, st, d_pos, N, preDii,
+                       preDij, d_Y2, ldy, d_g, d_F, c, js, d_part);
     BDH_LAUNCH_CHECK();
-    return 0;
+    ReduceJobs jobs{};
+    jobs.job[0] = ReduceJob{d_part, d_out, 3 * N, S, 0, 1.0};
+    return launch_reduce_partials(jobs, 1, 3 * N, st);
 }
 
 int launch_adj_hvp(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
                    const Topology& topo, const ForceParams& fp, const double* d_y,
-                   const double* d_contract, const double* d_g, double c, double* d_out, hipStream_t st)
+                   const double* d_contract, const double* d_g, double c, double* d_part, double* d_out,
+                   hipStream_t st)
 {
-    hipLaunchKernelGGL(adj_hvp_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, st, d_pos, d_Q, d_LJ, N, topo,
-                       fp, d_y, d_contract, d_g, c, d_out);
+    const int S = adj_contract_splits(N);
+    const int js = ceil_div(N, S);
+    hipLaunchKernelGGL(adj_hvp_pairs_kernel, dim3(ceil_div(N, 64), S), dim3(256), 0, st, d_pos, d_Q, d_LJ,
+                       N, topo, fp, d_y, js, d_part);
+    BDH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(adj_hvp_finish_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, d_pos, N, topo, d_y,
+                       (const double*)d_part, S, d_contract, d_g, c, d_out);
     BDH_LAUNCH_CHECK();
     return 0;
 }

@@ -1183,9 +1183,11 @@ static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const dou
     if (launch_adj_diag_inverses(L, s->ld, np, Minv, s->st)) return -1;
     if (launch_adj_upper_solve(L, s->ld, np, n, 0, a_d, xi_d, sq, nullptr, Minv, Y1, s->st)) return -1;
     if (launch_adj_upper_solve(L, s->ld, np, n, 1, nullptr, nullptr, 0.0, Y1, Minv, Y2, s->st)) return -1;
-    // Z = Y2 + Y2^T into Y1's storage (Y1 is no longer needed)
+    // Z = Y2 + Y2^T into Y1's storage (Y1 is no longer needed); Y2's storage then holds the
+    // contraction's partial sums (at most 64 x n doubles <= n_pad^2)
     if (launch_adj_symmetrize(np, Y2, Y1, s->st)) return -1;
-    if (launch_adj_contract(s->pos, s->N, s->preDii, s->preDij, Y1, np, g_d, s->F, c, ctr, s->st)) return -1;
+    if (launch_adj_contract(s->pos, s->N, s->preDii, s->preDij, Y1, np, g_d, s->F, c, Y2, ctr, s->st))
+        return -1;
     // D g from blocks on the fly, then g + contraction - c H (D g)
     if (launch_pair_sums<double>(s->pos, g_d, nullptr, s->N, s->preDii, s->preDij, dt, false, false,
                                  s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st))
@@ -1193,7 +1195,7 @@ static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const dou
     ReduceJobs jobs{};
     jobs.job[0] = ReduceJob{s->Pf, Dg, n, s->msplits, 0, 1.0};
     if (launch_reduce_partials(jobs, 1, n, s->st)) return -1;
-    if (launch_adj_hvp(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, Dg, ctr, g_d, c, out, s->st)) return -1;
+    if (launch_adj_hvp(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, Dg, ctr, g_d, c, Y2, out, s->st)) return -1;
     BDH_HIP(hipMemcpyAsync(rbar, out, sizeof(double) * n, hipMemcpyDeviceToHost, s->st));
     BDH_HIP(hipStreamSynchronize(s->st));
     return check_flags(s);

@@ -218,13 +218,18 @@ int launch_adj_upper_solve(const double* d_L, long long ld, int np, int n, int m
                            const double* d_xi, double s, const double* d_B, const double* d_Minv,
                            double* d_X, hipStream_t st);
 int launch_adj_symmetrize(int np, const double* d_Y, double* d_Z, hipStream_t st);
-// d_Y2 = Z = Y2 + Y2^T (ld ldy)
+// d_Y2 = Z = Y2 + Y2^T (ld ldy).  The column splits' partial sums go to d_part
+// (adj_contract_splits(N) rows of 3N doubles) and are summed in split order into d_out.
+constexpr int ADJ_CONTRACT_MAX_SPLITS = 64;
+int adj_contract_splits(int N);
 int launch_adj_contract(const double4* d_pos, int N, double preDii, double preDij, const double* d_Y2,
-                        long long ldy, const double* d_g, const double* d_F, double c, double* d_out,
-                        hipStream_t st);
+                        long long ldy, const double* d_g, const double* d_F, double c, double* d_part,
+                        double* d_out, hipStream_t st);
+// d_part: adj_contract_splits(N) rows of 3N doubles for the pair partials.
 int launch_adj_hvp(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
                    const Topology& topo, const ForceParams& fp, const double* d_y,
-                   const double* d_contract, const double* d_g, double c, double* d_out, hipStream_t st);
+                   const double* d_contract, const double* d_g, double c, double* d_part, double* d_out,
+                   hipStream_t st);
 
 // ---- probes (bdh_probe.hip)
 int probe_mfma(int kind, const void* A, const void* B, void* C, hipStream_t st);
