@@ -368,10 +368,9 @@ int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev, double d
         if (matvec_dev(s)) return -1;
         REC_END(PH_MATVEC);
         REC_BEGIN(PH_MOVE);
-        ReduceJobs jobs{};
-        jobs.job[0] = ReduceJob{s->Pf, s->Rf, s->n, s->msplits, 0, rf_scale};
-        if (launch_reduce_partials(jobs, 1, s->n, s->st)) return -1;
-        if (launch_em_update(s->pos, s->N, s->Rf, Rs_dev, c.bc == 1, c.box, s->st)) return -1;
+        if (launch_reduce_move(s->Pf, s->msplits, s->n, rf_scale, s->Rf, s->pos, Rs_dev, c.bc == 1, c.box,
+                               s->st))
+            return -1;
         s->last_Rs = Rs_dev;
         REC_END(PH_MOVE);
     } else if (c.hydro == BDH_HYDRO_GEYER) {

@@ -290,6 +290,61 @@ __global__ void em_update_kernel(double4* __restrict__ pos, int N, const double*
     pos[i] = p;
 }
 
+// reduce_partials_kernel's sum of S partial rows (same order, same scaling) into Rf, and
+// em_update_kernel's move of the coordinate each entry belongs to, in one launch: entry r of
+// the 3N-vector updates component r % 3 of bead r / 3 with the same expression, so positions
+// and Rf are bitwise those of the two launches.
+__global__ void __launch_bounds__(256)
+reduce_move_kernel(const double* __restrict__ P, int S, int n, double scale, double* __restrict__ Rf,
+                   double* __restrict__ pos, const double* __restrict__ Rs, int pbc, double box)
+{
+    __shared__ double s_red[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = blockIdx.x * 64 + lane;
+    double acc = 0.0;
+    if (r < n) {
+        const double* p = P + r;
+        int sidx = wave;
+        for (; sidx + 12 <= S - 1; sidx += 16) {
+            const double v0 = p[(long long)sidx * n];
+            const double v1 = p[(long long)(sidx + 4) * n];
+            const double v2 = p[(long long)(sidx + 8) * n];
+            const double v3 = p[(long long)(sidx + 12) * n];
+            acc += (v0 + v1) + (v2 + v3);
+        }
+        for (; sidx <= S - 1; sidx += 4) acc += p[(long long)sidx * n];
+    }
+    s_red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && r < n) {
+        double rf;
+        {
+            // rounded on its own, as the stored Rf that em_update_kernel reads (no FMA with
+            // the add below)
+#pragma clang fp contract(off)
+            rf = scale * ((s_red[0][lane] + s_red[1][lane]) + (s_red[2][lane] + s_red[3][lane]));
+        }
+        Rf[r] = rf;
+        double* q = pos + 4LL * (r / 3) + r % 3;
+        double x = (rf + Rs[r]) + *q;
+        if (pbc) {
+            const double inv = 1.0 / box;
+            x -= box * floor(x * inv + 0.5);
+        }
+        *q = x;
+    }
+}
+
+int launch_reduce_move(const double* d_P, int S, int n, double scale, double* d_Rf, double4* d_pos,
+                       const double* d_Rs, int pbc, double box, hipStream_t st)
+{
+    BDH_CHECK(S > 0 && n % 3 == 0, "reduce_move: bad shape");
+    hipLaunchKernelGGL(reduce_move_kernel, dim3(ceil_div(n, 64)), dim3(256), 0, st, d_P, S, n, scale, d_Rf,
+                       reinterpret_cast<double*>(d_pos), d_Rs, pbc, box);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
 int launch_em_update(double4* d_pos, int N, const double* d_Rf, const double* d_Rs, int pbc,
                      double box, hipStream_t st)
 {

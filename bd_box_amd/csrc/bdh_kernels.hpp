@@ -205,6 +205,10 @@ int launch_forces_batch(const double4* d_pos, const double* d_Q, const double2* 
                         double* d_Pf, int* d_flags, hipStream_t st);
 
 // ---- integrator (bdh_forces.hip)
+// Rf = scale * (sum of S partial rows of length n, reduce_partials order), then the move of
+// launch_em_update with that Rf; one launch, bitwise the same as the two.
+int launch_reduce_move(const double* d_P, int S, int n, double scale, double* d_Rf, double4* d_pos,
+                       const double* d_Rs, int pbc, double box, hipStream_t st);
 int launch_em_update(double4* d_pos, int N, const double* d_Rf, const double* d_Rs, int pbc,
                      double box, hipStream_t st);
 
