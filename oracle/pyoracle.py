@@ -109,10 +109,16 @@ def rpy_block(ci, cj, preDii, preDij):
     return out.reshape(3, 3)
 
 
-def compute_D(coord, p: OrcParams):
+def compute_D(coord, p: OrcParams, out=None):
+    """Full symmetric D (n x n).  orc_compute_D writes every entry, both triangles, so a
+    C-contiguous `out` of that shape is filled in place (no zeroing, no transpose)."""
     coord = np.ascontiguousarray(coord, float).reshape(-1)
     N = coord.size // 4
     n = 3 * N
+    if out is not None:
+        assert out.shape == (n, n) and out.dtype == np.float64 and out.flags.c_contiguous
+        lib().orc_compute_D(N, _d(coord), C.byref(p), _d(out), n)
+        return out
     D = np.zeros((n, n))  # column-major view below: D_cm[r + c*n] -> D.T row-major
     lib().orc_compute_D(N, _d(coord), C.byref(p), _d(D), n)
     return D.T.copy()

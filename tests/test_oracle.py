@@ -231,6 +231,32 @@ def test_refactor_interval_one_is_reference_semantics():
     assert d < 1e-2                                      # but stays close
 
 
+@pytest.mark.parametrize("R,K", [(3, 10), (10, 25)])
+def test_refactor_schedule_vs_host_composition(R, K):
+    """The stale-factor schedule over two or more cycles: the oracle's Sim(refactor_every=R)
+    equals the step composed on the host from pinned pieces (compute_D, numpy cholesky,
+    forces) with the factor refreshed on steps 0, R, 2R, ...; a schedule shifted by one
+    (refactor at 0, R-1, 2R-1, ... or at 0, R+1, ...) is told apart at the same tolerance."""
+    from compose import compose_steps
+    sysm = wl.chain(24, seed=31)
+    P = wl.parse_params(wl.chain_params())
+    p = po.params_from(P)
+    xi = np.random.default_rng(R).normal(size=(K, 3 * sysm.N))
+    traj, _ = compose_steps(sysm, P, xi, R)
+    sim = po.Sim(sysm, p, seed=1, refactor_every=R)
+    x0 = sysm.coord[:, :3]
+    for k in range(K):
+        sim.run(1, xi[k])
+        d_ref = traj[k][:, :3] - x0
+        err = np.linalg.norm(sim.coords()[:, :3] - traj[k][:, :3]) / np.linalg.norm(d_ref)
+        assert err < 1e-12, (k, err)
+    for shifted in ({k for k in range(K) if k == 0 or (k + 1) % R == 0},
+                    {k for k in range(K) if k == 0 or (k % R == 1 and k > 1)}):
+        bad, _ = compose_steps(sysm, P, xi, R, schedule=shifted)
+        d_ref = traj[-1][:, :3] - x0
+        assert np.linalg.norm(bad[-1][:, :3] - traj[-1][:, :3]) / np.linalg.norm(d_ref) > 1e-9
+
+
 @pytest.mark.parametrize("case", ["retry_const", "retry_var", "retry_var_none", "retry_var_geyer"])
 def test_rejected_steps_vs_reference(golden, case):
     """comp_step_ermak's rejection loop (diff_comp.c:152-179) against the reference's own
