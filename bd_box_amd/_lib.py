@@ -27,7 +27,7 @@ EXPORTS = [
     "bdh_batch_get_forces", "bdh_batch_time_steps", "bdh_batch_phase_timing",
     "bdh_batch_phase_times", "bdh_step_vjp", "bdh_factor_cache", "bdh_save_factor",
     "bdh_step_vjp_cached", "bdh_batch_set_schedule", "bdh_batch_get_schedule",
-    "bdh_batch_autotune",
+    "bdh_batch_autotune", "bdh_step_vjp_params", "bdh_set_params",
 ]
 
 
@@ -96,6 +96,8 @@ def load():
     L.bdh_factor_cache.argtypes = [C.c_void_p, C.c_int]
     L.bdh_save_factor.argtypes = [C.c_void_p, C.c_int]
     L.bdh_step_vjp_cached.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
+    L.bdh_step_vjp_params.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.bdh_set_params.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.bdh_batch_create.argtypes = [C.POINTER(Config), C.c_int, _ip, _dp, _dp, _dp, _ip, _ip, _dp,
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]
     L.bdh_batch_destroy.argtypes = [C.c_void_p]

@@ -126,6 +126,16 @@ void init_diff_tensor(void)
     if (bc != BC_NONE && bc != BC_BOX) UNERR("HIP backend: bc none or periodic");
     if (bc == BC_BOX && (box[0] != box[1] || box[0] != box[2]))
         UNERR("HIP backend: periodic box must be a cube");
+    /* The reference's on-the-fly TEA path with an Ewald tensor (diff_rand.c:100-130) keeps
+       one tD per bead i across all its partners k, and the Ewald block functions add into
+       it (diff_tensor.c:265-400), so its C, Feff and eps are not the tensor's sums.  This
+       backend always forms the sums of the tensor itself, which is what the reference's
+       stored-D path (geyer_on_the_fly no, diff_rand.c:136-164) computes. */
+    if (hydro == DIFF_GEYER && bc == BC_BOX && geyer_on_the_fly)
+        warning("HIP backend: geyer_on_the_fly with an Ewald tensor gives the TEA sums of the "
+                "tensor (those of geyer_on_the_fly no); the reference's on-the-fly Ewald path "
+                "accumulates the blocks of a bead into one another, so trajectories differ",
+                __FILE__, __LINE__);
     memset(&cfg, 0, sizeof(cfg));
     cfg.dt = dt; cfg.T = T; cfg.visc = visc; cfg.vfactor = vfactor;
     cfg.alpha_lj = alpha_lj; cfg.cutoff_lj = cutoff_lj; cfg.lj_6_term = lj_6_term;

@@ -486,4 +486,130 @@ int launch_adj_hvp(const double4* d_pos, const double* d_Q, const double2* d_LJ,
     return 0;
 }
 
+
+// ---- parameter cotangents of one step: theta_bar = (c D g)^T dF/dtheta for the per-bead LJ
+// epsilon_k and radius R_k, the charges Q_k and the per-bond constants H_b.  With
+// F_k = -sum_j phi'(r_kj) e_kj, pair (k, j) contributes -(y_k - y_j).e_kj dphi'/dtheta:
+//   LJ  (LJ.c:37-64, sigma_kj = R_k + R_j, eps_kj = sqrt(eps_k eps_j), data.c:97-102)
+//        phi' = a eps_kj (-12 s6^2 + 6 lj6 s6) / r                 -> d/deps_k = phi' / (2 eps_k)
+//        d phi' / d sigma = a eps_kj (-144 s6^2 + 36 lj6 s6) / (sigma r)  (= d / dR_k)
+//   DH  (electro.c:36-56)  phi' = -(gamma/eps_r) Q_k Q_j e^{-kappa r} (1/r^2 + kappa/r)
+//        d phi' / dQ_k = -(gamma/eps_r) Q_j e^{-kappa r} (1/r^2 + kappa/r)
+//   bond (bond.c:43-85)   phi' = H (r - r0) / (1 - r^2/rmax^2)     -> d/dH = (r - r0) / den
+// Row k sums its pairs over one column split (four wavefronts interleave the columns and
+// meet in LDS in wave order); P[split][3][N] holds (eps, R, Q) sums, combined in split order.
+__global__ void __launch_bounds__(256)
+adj_param_pairs_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
+                       const double2* __restrict__ LJ, int N, Topology topo, ForceParams fp,
+                       const double* __restrict__ y, int js, double* __restrict__ P)
+{
+    __shared__ double red[3][3][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    const int j0 = blockIdx.y * js, j1 = min(N, j0 + js);
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (k < N) {
+        const double4 pk = pos[k];
+        const double2 ljk = LJ[k];
+        const double yk[3] = {y[3 * k], y[3 * k + 1], y[3 * k + 2]};
+        const int nb0 = fp.use_conn ? topo.conn_off[k] : 0;
+        const int nb1 = fp.use_conn ? topo.conn_off[k + 1] : 0;
+        const double lj6 = fp.lj_6_term;
+        for (int j = j0 + wave; j < j1; j += 4) {
+            if (j == k) continue;
+            const double4 pj = pos[j];
+            const double d[3] = {pk.x - pj.x, pk.y - pj.y, pk.z - pj.z};
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (r2 > fp.far_r2) continue;
+            const double r = sqrt(r2);
+            const bool conn = bonded(topo, nb0, nb1, j);
+            const double ed = (d[0] * (yk[0] - y[3 * j]) + d[1] * (yk[1] - y[3 * j + 1]) +
+                               d[2] * (yk[2] - y[3 * j + 2])) / r;
+            if (fp.alpha_lj != 0.0) {
+                const double2 ljj = LJ[j];
+                const double sig = ljk.x + ljj.x;
+                const double cut = fp.cutoff_lj > 0.0 ? fp.cutoff_lj : (double)1.122462f * sig;
+                const double scale = conn ? fp.bond_lj_scale : 1.0;
+                if (r <= cut && scale != 0.0) {
+                    const double eps = sqrt(ljk.y * ljj.y);
+                    double s6 = sig / r;
+                    s6 *= s6;
+                    s6 *= s6 * s6;
+                    const double a = scale * fp.alpha_lj * eps;
+                    acc[0] += ed * (a * (-12.0 * s6 * s6 + 6.0 * lj6 * s6) / r);
+                    acc[1] += ed * (a * (-144.0 * s6 * s6 + 36.0 * lj6 * s6) / (sig * r));
+                }
+            }
+            if (fp.elec) {
+                const double scale = conn ? fp.bond_c_scale : 1.0;
+                if (r <= fp.cutoff_c && scale != 0.0) {
+                    const double ex = exp(-fp.kappa_c * r);
+                    acc[2] += ed * (-scale * fp.gam * Q[j] * ex * (1.0 / r2 + fp.kappa_c / r));
+                }
+            }
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int u = 0; u < 3; ++u) red[wave - 1][u][lane] = acc[u];
+    }
+    __syncthreads();
+    if (wave == 0 && k < N) {
+        double* out = P + (long long)blockIdx.y * 3 * N;
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+            out[u * N + k] = ((acc[u] + red[0][u][lane]) + red[1][u][lane]) + red[2][u][lane];
+    }
+}
+
+// Per bead: the split partials summed in split order, scaled by -c (and 1 / (2 eps_k) for
+// epsilon); per bond, at its lower-index end, -c (y_k - y_p).e (r - r0) / den into
+// H_bar[bond_id] (each bond written once).
+__global__ void __launch_bounds__(256)
+adj_param_finish_kernel(const double4* __restrict__ pos, const double2* __restrict__ LJ, int N,
+                        Topology topo, const int* __restrict__ bond_id, const double* __restrict__ y,
+                        const double* __restrict__ P, int S, double c, double* __restrict__ eps_bar,
+                        double* __restrict__ R_bar, double* __restrict__ Q_bar, double* __restrict__ H_bar)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int sp = 0; sp < S; ++sp) {
+#pragma unroll
+        for (int u = 0; u < 3; ++u) acc[u] += P[(long long)sp * 3 * N + u * N + k];
+    }
+    eps_bar[k] = -c * acc[0] / (2.0 * LJ[k].y);
+    R_bar[k] = -c * acc[1];
+    Q_bar[k] = -c * acc[2];
+    const double4 pk = pos[k];
+    for (int b = topo.bond_off[k]; b < topo.bond_off[k + 1]; ++b) {
+        const int p = topo.bond_p[b];
+        if (p < k) continue;
+        const double r0 = topo.bond_parm[3 * b], rmax = topo.bond_parm[3 * b + 1];
+        const double4 pp = pos[p];
+        const double d[3] = {pk.x - pp.x, pk.y - pp.y, pk.z - pp.z};
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double den = 1.0 - r * r / (rmax * rmax);
+        const double ed = (d[0] * (y[3 * k] - y[3 * p]) + d[1] * (y[3 * k + 1] - y[3 * p + 1]) +
+                           d[2] * (y[3 * k + 2] - y[3 * p + 2])) / r;
+        H_bar[bond_id[b]] = -c * ed * (r - r0) / den;
+    }
+}
+
+int launch_adj_params(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
+                      const Topology& topo, const int* d_bond_id, const ForceParams& fp,
+                      const double* d_y, double c, double* d_part, double* d_eps_bar,
+                      double* d_R_bar, double* d_Q_bar, double* d_H_bar, hipStream_t st)
+{
+    const int S = adj_contract_splits(N);
+    const int js = ceil_div(N, S);
+    hipLaunchKernelGGL(adj_param_pairs_kernel, dim3(ceil_div(N, 64), S), dim3(256), 0, st, d_pos, d_Q, d_LJ,
+                       N, topo, fp, d_y, js, d_part);
+    BDH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(adj_param_finish_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, d_pos, d_LJ, N, topo,
+                       d_bond_id, d_y, (const double*)d_part, S, c, d_eps_bar, d_R_bar, d_Q_bar, d_H_bar);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
 }  // namespace bdh

@@ -155,6 +155,10 @@ struct bdh_system {
     double* adj = nullptr;          // reverse-mode step: Y1, Y2 (n x n), then vectors
     double* fcache = nullptr;       // saved factors (fcache_slots x n_pad^2, FP64)
     int fcache_slots = 0;
+    int nbonds = 0;
+    int* bond_id = nullptr;         // bond index of each per-bead bond entry
+    std::vector<int> csr_bond;      // the same on the host
+    double* pgrad = nullptr;        // parameter cotangents: eps, R, Q (N each), H (nbonds)
 };
 
 namespace {
@@ -694,8 +698,32 @@ void bdh_destroy(bdh_system* s)
     if (s->factor_exec) (void)hipGraphExecDestroy(s->factor_exec);
     dfree(s->adj);
     dfree(s->fcache);
+    dfree(s->bond_id);
+    dfree(s->pgrad);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
+}
+
+// Reach of any pair term (the DH and LJ cutoffs and the overlap test R_i + R_j > r, padded
+// well above the rounding of the T-precision comparisons in pair_term), and the cell edge of
+// init_cutoff (pre_func.c:197-231): every potential's cutoff and twice the largest radius,
+// plus 0.01.  Both follow the LJ radii.
+static void set_reach(bdh_system* s, const double* LJ)
+{
+    const bdh_config* cfg = &s->cfg;
+    double rmax = 0.0;
+    for (int i = 0; i < s->N; ++i) rmax = std::max(rmax, LJ[2 * i]);
+    double reach = 2.0 * rmax;
+    if (cfg->elec) reach = std::max(reach, cfg->cutoff_c);
+    if (cfg->alpha_lj != 0.0)
+        reach = std::max(reach, cfg->cutoff_lj > 0 ? cfg->cutoff_lj : 1.122462 * 2.0 * rmax);
+    reach = reach * (1.0 + 1e-5) + 1e-5;
+    s->fp.far_r2 = reach * reach;
+    double cut = 2.0 * rmax;
+    if (cfg->elec) cut = std::max(cut, cfg->cutoff_c);
+    if (cfg->alpha_lj != 0.0)
+        cut = std::max(cut, cfg->cutoff_lj > 0 ? cfg->cutoff_lj : 1.1225 * 2.0 * rmax);
+    s->cell_cut = cut + 0.01;
 }
 
 static int create_impl(const bdh_config* cfg, int N, const double* coord, const double* Q,
@@ -762,18 +790,7 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     fp.check_overlap = cfg->check_overlap;
     fp.pbc = cfg->bc == 1;
     fp.box = cfg->box;
-    {
-        // reach of any pair term: the DH and LJ cutoffs and the overlap test R_i + R_j > r,
-        // padded well above the rounding of the T-precision comparisons in pair_term
-        double rmax = 0.0;
-        for (int i = 0; i < N; ++i) rmax = std::max(rmax, LJ[2 * i]);
-        double reach = 2.0 * rmax;
-        if (cfg->elec) reach = std::max(reach, cfg->cutoff_c);
-        if (cfg->alpha_lj != 0.0)
-            reach = std::max(reach, cfg->cutoff_lj > 0 ? cfg->cutoff_lj : 1.122462 * 2.0 * rmax);
-        reach = reach * (1.0 + 1e-5) + 1e-5;
-        fp.far_r2 = reach * reach;
-    }
+    set_reach(s, LJ);
 
     // per-bead adjacency (CSR), host-built
     std::vector<int> deg(N + 1, 0);
@@ -782,12 +799,15 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     for (int i = 0; i < N; ++i) off[i + 1] = off[i] + deg[i];
     std::vector<int> fill(off.begin(), off.end() - 1), partner(std::max(1, off[N]));
     std::vector<double> parm(3 * std::max(1, off[N]));
+    s->nbonds = nbonds;
+    s->csr_bond.assign(std::max(1, off[N]), 0);
     for (int b = 0; b < nbonds; ++b) {
         const int i = bonds[2 * b], j = bonds[2 * b + 1];
         for (int e = 0; e < 2; ++e) {
             const int me = e ? j : i, other = e ? i : j;
             const int k = fill[me]++;
             partner[k] = other;
+            s->csr_bond[k] = b;
             parm[3 * k] = bond_parms[3 * b];
             parm[3 * k + 1] = bond_parms[3 * b + 1];
             parm[3 * k + 2] = bond_parms[3 * b + 2];
@@ -798,6 +818,8 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     if (dmalloc(&s->conn, sizeof(int) * partner.size())) return -1;
     if (dmalloc(&s->bond_p, sizeof(int) * partner.size())) return -1;
     if (dmalloc(&s->bond_parm, sizeof(double) * parm.size())) return -1;
+    if (dmalloc(&s->bond_id, sizeof(int) * s->csr_bond.size())) return -1;
+    BDH_HIP(hipMemcpy(s->bond_id, s->csr_bond.data(), sizeof(int) * s->csr_bond.size(), hipMemcpyHostToDevice));
     BDH_HIP(hipMemcpy(s->conn_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
     BDH_HIP(hipMemcpy(s->bond_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
     BDH_HIP(hipMemcpy(s->conn, partner.data(), sizeof(int) * partner.size(), hipMemcpyHostToDevice));
@@ -818,15 +840,6 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     s->fsplits = force_splits(N);
     s->fblocks = s->cells ? cell_force_blocks(N) : ceil_div(N, 64) * s->fsplits;
     if (s->cells) {
-        // cell edge of init_cutoff (pre_func.c:197-231): every potential's cutoff and
-        // twice the largest radius (overlap test), plus 0.01
-        double rmax = 0;
-        for (int i = 0; i < N; ++i) rmax = std::max(rmax, LJ[2 * i]);
-        double cut = 2.0 * rmax;
-        if (cfg->elec) cut = std::max(cut, cfg->cutoff_c);
-        if (cfg->alpha_lj != 0.0)
-            cut = std::max(cut, cfg->cutoff_lj > 0 ? cfg->cutoff_lj : 1.1225 * 2.0 * rmax);
-        s->cell_cut = cut + 0.01;
         if (alloc_cells(s)) return -1;
     }
     s->ewald = cfg->bc == 1 && cfg->hydro != BDH_HYDRO_NONE;
@@ -1125,8 +1138,12 @@ int bdh_save_factor(bdh_system* s, int slot)
     return 0;
 }
 
+struct ParamBars {
+    double *eps, *R, *Q, *H;
+};
+
 static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const double* xi,
-                         const double* gbar, double* rbar);
+                         const double* gbar, double* rbar, const ParamBars* pb = nullptr);
 
 int bdh_step_vjp(bdh_system* s, const double* coord, const double* xi, const double* gbar, double* rbar)
 {
@@ -1140,12 +1157,51 @@ int bdh_step_vjp_cached(bdh_system* s, int slot, const double* coord, const doub
     return step_vjp_impl(s, slot, coord, xi, gbar, rbar);
 }
 
+int bdh_step_vjp_params(bdh_system* s, int slot, const double* coord, const double* xi,
+                        const double* gbar, double* rbar, double* eps_bar, double* R_bar,
+                        double* Q_bar, double* H_bar)
+{
+    BDH_CHECK(s && (slot < 0 || slot < s->fcache_slots), "bdh_step_vjp_params: slot out of range");
+    const ParamBars pb{eps_bar, R_bar, Q_bar, H_bar};
+    return step_vjp_impl(s, slot, coord, xi, gbar, rbar, &pb);
+}
+
+int bdh_set_params(bdh_system* s, const double* Q, const double* LJ, const double* bond_parms)
+{
+    BDH_CHECK(s, "bdh_set_params: null system");
+    BDH_HIP(hipSetDevice(s->device));
+    if (Q) BDH_HIP(hipMemcpyAsync(s->Q, Q, sizeof(double) * s->N, hipMemcpyHostToDevice, s->st));
+    if (LJ) {
+        for (int i = 0; i < s->N; ++i)
+            BDH_CHECK(LJ[2 * i] >= 0.0 && LJ[2 * i + 1] >= 0.0, "bdh_set_params: LJ radius and epsilon must be >= 0");
+        BDH_HIP(hipMemcpyAsync(s->LJ, LJ, sizeof(double) * 2 * s->N, hipMemcpyHostToDevice, s->st));
+        set_reach(s, LJ);
+    }
+    if (bond_parms && s->nbonds) {
+        std::vector<double> parm(3 * s->csr_bond.size());
+        for (size_t k = 0; k < s->csr_bond.size(); ++k)
+            for (int u = 0; u < 3; ++u) parm[3 * k + u] = bond_parms[3 * s->csr_bond[k] + u];
+        BDH_HIP(hipMemcpyAsync(s->bond_parm, parm.data(), sizeof(double) * parm.size(), hipMemcpyHostToDevice,
+                               s->st));
+        BDH_HIP(hipStreamSynchronize(s->st));   // parm is a host temporary
+    }
+    // the forces of the current positions under the new parameters (the next step uses them)
+    if (compute_forces_dev(s)) return -1;
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return check_flags(s);
+}
+
 static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const double* xi,
-                         const double* gbar, double* rbar)
+                         const double* gbar, double* rbar, const ParamBars* pb)
 {
     BDH_CHECK(s && coord && xi && gbar && rbar, "bdh_step_vjp: bad arguments");
     BDH_CHECK(!s->mixed, "bdh_step_vjp: FP64 systems only");
     BDH_CHECK(s->cfg.hydro == BDH_HYDRO_CHOLESKY && !s->ewald, "bdh_step_vjp: free-space Cholesky noise only");
+    // the derivative is that of one accepted ermak_const step with the given Gaussians: a
+    // rejected step (check_overlap) is redrawn from the stream with the same or a halved dt,
+    // which the given-xi step does not follow
+    BDH_CHECK(!s->cfg.check_overlap && s->cfg.algorithm == BDH_ERMAK_CONST,
+              "bdh_step_vjp: ermak_const steps without check_overlap only");
     BDH_HIP(hipSetDevice(s->device));
     const int n = s->n, np = s->n_pad;
     const size_t nn = (size_t)np * np;
@@ -1196,6 +1252,23 @@ static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const dou
     if (launch_reduce_partials(jobs, 1, n, s->st)) return -1;
     if (launch_adj_hvp(s->pos, s->Q, s->LJ, s->N, s->topo, s->fp, Dg, ctr, g_d, c, Y2, out, s->st)) return -1;
     BDH_HIP(hipMemcpyAsync(rbar, out, sizeof(double) * n, hipMemcpyDeviceToHost, s->st));
+    if (pb) {
+        // (c D g)^T dF/dtheta; the pair partials reuse Y2's storage, free again after the
+        // Hessian-vector product (stream order)
+        const int N = s->N;
+        if (!s->pgrad && dmalloc(&s->pgrad, sizeof(double) * (3 * (size_t)N + std::max(1, s->nbonds)))) return -1;
+        double* pe = s->pgrad;
+        if (launch_adj_params(s->pos, s->Q, s->LJ, N, s->topo, s->bond_id, s->fp, Dg, c, Y2, pe, pe + N,
+                              pe + 2 * N, pe + 3 * N, s->st))
+            return -1;
+        double* outs[4] = {pb->eps, pb->R, pb->Q, pb->H};
+        for (int q = 0; q < 4; ++q) {
+            const size_t cnt = q < 3 ? (size_t)N : (size_t)s->nbonds;
+            if (outs[q] && cnt)
+                BDH_HIP(hipMemcpyAsync(outs[q], pe + (size_t)q * N, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                                       s->st));
+        }
+    }
     BDH_HIP(hipStreamSynchronize(s->st));
     return check_flags(s);
 }

@@ -234,6 +234,13 @@ int launch_adj_hvp(const double4* d_pos, const double* d_Q, const double2* d_LJ,
                    const Topology& topo, const ForceParams& fp, const double* d_y,
                    const double* d_contract, const double* d_g, double c, double* d_part, double* d_out,
                    hipStream_t st);
+// Parameter cotangents of one step for y = D g (bdh_adjoint.hip): eps_bar, R_bar, Q_bar (N)
+// and H_bar (one per bond; bond_id maps each per-bead bond entry to its bond).  d_part:
+// adj_contract_splits(N) rows of 3N doubles.
+int launch_adj_params(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
+                      const Topology& topo, const int* d_bond_id, const ForceParams& fp,
+                      const double* d_y, double c, double* d_part, double* d_eps_bar,
+                      double* d_R_bar, double* d_Q_bar, double* d_H_bar, hipStream_t st);
 
 // ---- probes (bdh_probe.hip)
 int probe_mfma(int kind, const void* A, const void* B, void* C, hipStream_t st);

@@ -60,6 +60,7 @@ class BDSystem:
         _lib.check(L.bdh_init(device), "bdh_init")
         self.N = sysm.N
         self.n = 3 * self.N
+        self.nbonds = len(sysm.bonds)
         self.cfg = make_config(P, precision, refactor_every)
         c = np.ascontiguousarray(sysm.coord if coord is None else coord, float).reshape(-1)
         Q = np.ascontiguousarray(sysm.Q, float)
@@ -163,6 +164,36 @@ class BDSystem:
                                        _lib.dptr(out))
         _lib.check(rc, "step_vjp")
         return out
+
+    def step_vjp_params(self, coord, xi, gbar, slot: Optional[int] = None):
+        """step_vjp plus the cotangents of the force-field parameters for the same gbar
+        (bdh_step_vjp_params): (rbar[3N], eps_bar[N], R_bar[N], Q_bar[N], H_bar[nbonds])."""
+        c = np.ascontiguousarray(coord, float).reshape(-1)
+        x = np.ascontiguousarray(xi, float).reshape(-1)
+        g = np.ascontiguousarray(gbar, float).reshape(-1)
+        out = np.zeros(self.n)
+        eb, Rb, Qb = np.zeros(self.N), np.zeros(self.N), np.zeros(self.N)
+        Hb = np.zeros(max(self.nbonds, 1))
+        _lib.check(_lib.load().bdh_step_vjp_params(
+            self._h, -1 if slot is None else int(slot), _lib.dptr(c), _lib.dptr(x), _lib.dptr(g),
+            _lib.dptr(out), _lib.dptr(eb), _lib.dptr(Rb), _lib.dptr(Qb), _lib.dptr(Hb)),
+            "step_vjp_params")
+        return out, eb, Rb, Qb, Hb[:self.nbonds]
+
+    def set_params(self, Q=None, LJ=None, bond_parms=None):
+        """Replace the charges, LJ (R, eps) rows and per-bond (r0, r_max, H) rows
+        (bdh_set_params); None keeps the current values."""
+        def arr(a, shape):
+            return None if a is None else np.ascontiguousarray(np.asarray(a, float).reshape(shape))
+        q, lj, bp = arr(Q, -1), arr(LJ, -1), arr(bond_parms, -1)
+        if q is not None and q.size != self.N or lj is not None and lj.size != 2 * self.N or \
+                bp is not None and bp.size != 3 * self.nbonds:
+            raise ValueError("set_params: Q[N], LJ[N, 2], bond_parms[nbonds, 3]")
+        self._params = (q, lj, bp)   # kept alive across the call
+        _lib.check(_lib.load().bdh_set_params(self._h, None if q is None else _lib.dptr(q),
+                                              None if lj is None else _lib.dptr(lj),
+                                              None if bp is None else _lib.dptr(bp)),
+                   "set_params")
 
     def factor_cache(self, slots: int):
         _lib.check(_lib.load().bdh_factor_cache(self._h, int(slots)), "factor_cache")

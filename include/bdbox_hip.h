@@ -185,6 +185,22 @@ int bdh_factor_cache(bdh_system* sys, int slots);
 int bdh_save_factor(bdh_system* sys, int slot);
 int bdh_step_vjp_cached(bdh_system* sys, int slot, const double* coord, const double* xi,
                         const double* gbar, double* rbar);
+/* bdh_step_vjp (slot < 0) or bdh_step_vjp_cached (slot >= 0) that also returns the
+ * cotangents of the force-field parameters for the same gbar (no counterpart in the
+ * reference): eps_bar[N], R_bar[N] of the per-bead LJ epsilon and radius (LJ[2i+1], LJ[2i];
+ * they enter at src/potentials/LJ.c:37-64 as sigma_ij = R_i + R_j, eps_ij = sqrt(eps_i
+ * eps_j), set from the .str file at src/data.c:97-102), Q_bar[N] of the charges
+ * (electro.c:36-56) and H_bar[nbonds] of each bond's force constant (bond.c:43-85).  Any
+ * of the four may be NULL. */
+int bdh_step_vjp_params(bdh_system* sys, int slot, const double* coord, const double* xi,
+                        const double* gbar, double* rbar, double* eps_bar, double* R_bar,
+                        double* Q_bar, double* H_bar);
+/* Replace the per-bead charges Q[N], LJ (R, eps)[2N] and the per-bond parameters
+ * bond_parms[3 * nbonds] (r0, r_max, H; the bond list given to bdh_create) of a system, any
+ * of them NULL to keep; the forces at the current positions are recomputed.  The reference's
+ * driver owns these host arrays (src/data.c:85-109) and its backend reads them at every
+ * force evaluation, so a caller that changes them between steps sees the new values. */
+int bdh_set_params(bdh_system* sys, const double* Q, const double* LJ, const double* bond_parms);
 
 /* ---- Batches of independent chains.  No counterpart in the reference, which integrates one
  * system per process: nchains systems of uneven sizes that share one bdh_config advance
