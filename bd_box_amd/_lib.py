@@ -27,7 +27,8 @@ EXPORTS = [
     "bdh_batch_get_forces", "bdh_batch_time_steps", "bdh_batch_phase_timing",
     "bdh_batch_phase_times", "bdh_step_vjp", "bdh_factor_cache", "bdh_save_factor",
     "bdh_step_vjp_cached", "bdh_batch_set_schedule", "bdh_batch_get_schedule",
-    "bdh_batch_autotune", "bdh_step_vjp_params", "bdh_set_params",
+    "bdh_batch_autotune", "bdh_step_vjp_params", "bdh_set_params", "bdh_compute_Dij",
+    "bdh_compute_r", "bdh_ermak_move", "bdh_igt_phase", "bdh_set_forces",
 ]
 
 
@@ -98,6 +99,11 @@ def load():
     L.bdh_step_vjp_cached.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
     L.bdh_step_vjp_params.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.bdh_set_params.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.bdh_compute_Dij.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp]
+    L.bdh_compute_r.argtypes = [C.c_void_p, _dp]
+    L.bdh_ermak_move.argtypes = [C.c_void_p, C.c_double, _dp]
+    L.bdh_igt_phase.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, _ip]
+    L.bdh_set_forces.argtypes = [C.c_void_p, _dp]
     L.bdh_batch_create.argtypes = [C.POINTER(Config), C.c_int, _ip, _dp, _dp, _dp, _ip, _ip, _dp,
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]
     L.bdh_batch_destroy.argtypes = [C.c_void_p]

@@ -44,6 +44,7 @@
 #include "potentials/electro.h"
 #include "potentials/LJ.h"
 #include "potentials/bucket.h"
+#include "diff_alg/newton.h"
 
 #include "../../include/bdbox_hip.h"
 
@@ -58,6 +59,15 @@ DOUBLE preDii;
 INT save_iter = -1;
 INT curr_iter = 0;
 DOUBLE* E;
+/* work vectors of diff_tensor.h:126-145; the backend keeps its own on the device, these hold
+   the last move's Rf / Rs (Rf[0], Rs[0]) for callers of compute_hydro_ermak / compute_r */
+DOUBLE* X = NULL;
+DOUBLE** Rf = NULL;
+DOUBLE** Rs = NULL;
+DOUBLE* epsilon = NULL;
+DOUBLE* FS = NULL;
+DOUBLE* Rf0 = NULL;
+DOUBLE* S[2] = {NULL, NULL};
 
 MAKE_STR_IN( DOUBLE, visc, 0.01, "viscovity" )
 MAKE_STR_IN( DOUBLE, vfactor, 14.4, "factor for viscovity units conversion" )
@@ -161,6 +171,12 @@ void init_diff_tensor(void)
     /* diag_D as init_diff_tensor builds it (diff_decl.c:93-98) */
     preDii = kB_1_6PI * T / (visc * vfactor);
     preDij = kB_1_8PI * T / (visc * vfactor);
+    Rf = (DOUBLE**)malloc(sizeof(DOUBLE*)); CHMEM(Rf);
+    Rs = (DOUBLE**)malloc(sizeof(DOUBLE*)); CHMEM(Rs);
+    Rf[0] = (DOUBLE*)calloc(size * DIMS0, sizeof(DOUBLE)); CHMEM(Rf[0]);
+    Rs[0] = (DOUBLE*)calloc(size * DIMS0, sizeof(DOUBLE)); CHMEM(Rs[0]);
+    Rf0 = (DOUBLE*)calloc(size * DIMS1, sizeof(DOUBLE)); CHMEM(Rf0);
+    init_tensor();
     diag_D = (DOUBLE*)malloc(sizeof(DOUBLE) * size * DIMS0); CHMEM(diag_D);
     inv_diag_D = (DOUBLE*)malloc(sizeof(DOUBLE) * size * DIMS0); CHMEM(inv_diag_D);
     for (i = 0; i < size * DIMS0; ++i)
@@ -177,6 +193,9 @@ void free_diff_tensor(void)
     free(diag_D); diag_D = NULL;
     free(inv_diag_D); inv_diag_D = NULL;
     free(Ds); Ds = NULL;
+    if (Rf) { free(Rf[0]); free(Rf); Rf = NULL; }
+    if (Rs) { free(Rs[0]); free(Rs); Rs = NULL; }
+    free(Rf0); Rf0 = NULL;
 }
 
 void init_forces(void)
@@ -210,8 +229,30 @@ INT compute_forces(DOUBLE* _E, DOUBLE t)
     return rep;
 }
 
-void save_curr_coord(void) {}
-void save_curr_forces(void) {}
+/* save_curr_coord / save_curr_forces / load_coord / load_forces (diff_decl.c:219-237) on
+   the driver's save_coord / save_forces (data.c:57-58); host coord and F[0] are current after
+   every compute_hydro, and a load puts the backend back at the restored state */
+void save_curr_coord(void)
+{
+    memcpy(save_coord, coord, sizeof(DOUBLE) * DIMS1 * size);
+}
+
+void save_curr_forces(void)
+{
+    memcpy(save_forces, F[0], sizeof(DOUBLE) * DIMS0 * size);
+}
+
+void load_coord(void)
+{
+    memcpy(coord, save_coord, sizeof(DOUBLE) * DIMS1 * size);
+    if (bdh_set_coords(g_sys, coord)) fail("load_coord");
+}
+
+void load_forces(void)
+{
+    memcpy(F[0], save_forces, sizeof(DOUBLE) * DIMS0 * size);
+    if (bdh_set_forces(g_sys, F[0])) fail("load_forces");
+}
 
 static int output_step(INT s)
 {
@@ -231,10 +272,10 @@ void compute_hydro(DOUBLE* _E)
     if (bdh_last_dt(g_sys, &step_dt)) fail("step dt");
     curr_dt = step_dt;
     curr_time += curr_dt;
+    if (bdh_get_coords(g_sys, coord)) fail("get_coords");
+    if (bdh_get_forces(g_sys, F[0])) fail("get_forces");
     if (output_step(step) || step + 1 >= bdsteps)
     {
-        if (bdh_get_coords(g_sys, coord)) fail("get_coords");
-        if (bdh_get_forces(g_sys, F[0])) fail("get_forces");
         if (bdh_get_rng_state(g_sys, &st)) fail("rng state");
         set_host_rng_state(st);
         if (_E)
@@ -255,3 +296,142 @@ void compute_single_D(DOUBLE* _D)
 {
     if (bdh_compute_D(g_sys, _D, size * DIMS0)) fail("compute_D");
 }
+
+/* ---- the rest of the backend interface of diff_tensor.h:166-253 and myblas.h:37-51.  The
+   driver's own loop reaches the backend through compute_hydro / compute_forces only; these
+   serve other callers of the same interface.  Blocks and tensors are those of the backend's
+   current positions: the coordinates of the last compute_forces, compute_hydro, load_coord
+   or compute_D. */
+
+static void sync_host_forces(DOUBLE* _E)
+{
+    if (bdh_get_forces(g_sys, F[0])) fail("get_forces");
+    if (_E)
+    {
+        double Eb[3];
+        int rep;
+        if (bdh_compute_forces(g_sys, NULL, Eb, &rep)) fail("energies");
+        memset(_E, 0, sizeof(DOUBLE) * (ENERGY_LAST + 1));
+        _E[ENERGY_LJ] = Eb[0];
+        _E[ENERGY_COULOMB] = Eb[1];
+        _E[ENERGY_BOND] = Eb[2];
+    }
+}
+
+/* init_tensor (diff_tensor.c:546-559): the device's Ewald tables are built with the system;
+   what the driver can see is the Beenakker splitting parameter, forced to sqrt(pi)/L */
+void init_tensor(void)
+{
+    if (bc == BC_BOX && hydro && ewald_method == EWALD_METHOD_BEENAKKER)
+        ewald_alpha = SQRTPI / box[0];
+}
+
+/* compute_D (diff_tensor.c:472-544): the whole symmetric tensor at coord (the reference
+   fills the upper triangle; the lower one is its mirror) */
+void compute_D(DOUBLE* _D)
+{
+    if (!hydro) return;
+    if (bdh_set_coords(g_sys, coord)) fail("set_coords");
+    if (bdh_compute_D(g_sys, _D, size * DIMS0)) fail("compute_D");
+}
+
+void compute_Dij(INT i, INT j, DOUBLE* tD)
+{
+    if (bdh_compute_Dij(g_sys, i, j, 0, tD)) fail("compute_Dij");
+}
+
+void diff_tensor_calc_pos(INT i, INT j, DOUBLE* tD)
+{
+    if (bdh_compute_Dij(g_sys, i, j, 1, tD)) fail("diff_tensor_calc_pos");
+}
+
+void diff_tensor_calc_ewald_smith(INT i, INT j, DOUBLE* tD)
+{
+    if (bdh_compute_Dij(g_sys, i, j, 2, tD)) fail("diff_tensor_calc_ewald_smith");
+}
+
+void diff_tensor_calc_ewald_smith_diag(INT i, DOUBLE* tD)
+{
+    if (bdh_compute_Dij(g_sys, i, i, 2, tD)) fail("diff_tensor_calc_ewald_smith_diag");
+}
+
+void diff_tensor_calc_ewald_beenakker(INT i, INT j, DOUBLE* tD)
+{
+    if (bdh_compute_Dij(g_sys, i, j, 3, tD)) fail("diff_tensor_calc_ewald_beenakker");
+}
+
+void diff_tensor_calc_ewald_beenakker_diag(INT i, DOUBLE* tD)
+{
+    if (bdh_compute_Dij(g_sys, i, i, 3, tD)) fail("diff_tensor_calc_ewald_beenakker_diag");
+}
+
+/* compute_r (diff_rand.c:47-236): Rs[0] at coord from the next Gaussians of the stream.
+   The factor of D stays on the device; _D is not overwritten with it. */
+void compute_r(DOUBLE* _D)
+{
+    (void)_D;
+    if (bdh_set_coords(g_sys, coord)) fail("set_coords");
+    if (bdh_compute_r(g_sys, Rs[0])) fail("compute_r");
+}
+
+/* compute_hydro_ermak (diff_equation.c:216-264): coord_out = coord + Rf + Rs, Rf[0] and Rs[0]
+   the move's parts; the caller evaluates forces at coord_out */
+void compute_hydro_ermak(DOUBLE* coord_out)
+{
+    if (vel_grad_tensor) UNERR("HIP backend: no flow field");
+    if (bdh_ermak_move(g_sys, curr_dt > 0 ? curr_dt : dt, coord_out)) fail("compute_hydro_ermak");
+    if (bdh_get_last_displacement(g_sys, Rf[0], Rs[0])) fail("displacement");
+}
+
+/* compute_hydro_IGT (diff_equation.c:317-374): predictor, mean tensor, then the corrector;
+   1 if the predictor overlaps (coord unchanged, F[0] the predictor's forces), 2 if the
+   corrector does (coord the corrector's positions) */
+INT compute_hydro_IGT(DOUBLE* _E)
+{
+    int res = 0;
+    curr_iter++;
+    if (bdh_igt_phase(g_sys, 0, curr_dt > 0 ? curr_dt : dt, coord, &res)) fail("compute_hydro_IGT");
+    sync_host_forces(res == 1 ? NULL : _E);
+    return res;
+}
+
+/* compute_hydro_IGT_second_phase (diff_equation.c:288-315): the corrector from coord */
+INT compute_hydro_IGT_second_phase(DOUBLE* _E)
+{
+    int res = 0;
+    if (bdh_igt_phase(g_sys, 1, curr_dt > 0 ? curr_dt : dt, coord, &res)) fail("compute_hydro_IGT_second_phase");
+    sync_host_forces(_E);
+    return res;
+}
+
+/* my_dpotrf / my_dtrmv (myblas.c:45-118) on the device, LAPACK's 'U' convention: A = U^T U
+   with U in the upper triangle; INFO as LAPACK returns it */
+INT my_dpotrf(DOUBLE* A, INT n, INT LD)
+{
+    const int info = bdh_dpotrf('U', A, n, LD, BDH_FP64);
+    if (info < 0) fail("my_dpotrf");
+    return info;
+}
+
+void my_dtrmv(DOUBLE* A, DOUBLE* v, DOUBLE* o, INT n, INT LD)
+{
+    if (bdh_dtrmv('U', A, v, o, n, LD, BDH_FP64)) fail("my_dtrmv");
+}
+
+/* init_cholesky / free_cholesky (myblas.c:120-132): the factorization's workspaces live on
+   the device and belong to each call, so there is no host vector to allocate */
+void init_cholesky(void) {}
+void free_cholesky(void) {}
+
+/* newton.h:33-37: the elastic-collision moves of ermak_newton are not part of this backend
+   (init_diff_tensor accepts no algorithm that reaches them) */
+INT newton_moves(INT* count)
+{
+    (void)count;
+    UNERR("HIP backend: elastic collisions (ermak_newton) are not implemented");
+    return 1;
+}
+
+void newton_init_vel(void) { UNERR("HIP backend: elastic collisions (ermak_newton) are not implemented"); }
+void init_newton(void) {}
+void free_newton(void) {}

@@ -1097,6 +1097,162 @@ int bdh_compute_Rf(bdh_system* s, double* Rf)
     return 0;
 }
 
+// Rf = (dt / kT) D F at the current positions and forces into out, for every hydro mode
+// (compute_Rf, diff_equation.c:119-214).
+static int rf_any(bdh_system* s, double* out, double dt)
+{
+    const double scale = dt / (kB * s->cfg.T);
+    if (s->cfg.hydro != BDH_HYDRO_NONE) {
+        if (matvec_dev(s)) return -1;
+        ReduceJobs jobs{};
+        jobs.job[0] = ReduceJob{s->Pf, out, s->n, s->msplits, 0, scale};
+        return launch_reduce_partials(jobs, 1, s->n, s->st);
+    }
+    hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st, s->pos, s->F,
+                       (const double*)nullptr, s->N, s->preDii, dt, scale, out, (double*)nullptr);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
+// Rs of compute_r (diff_rand.c:47-236) at the current positions from the next 3N Gaussians
+// of the device stream: sqrt(2 dt) L xi (the factor refreshed when `refactor`), the TEA
+// displacement, or sqrt(2 dt diag_D) xi.  Returns the device vector, nullptr on error.
+static const double* noise_at(bdh_system* s, double dt, bool refactor)
+{
+    if (s->cfg.hydro == BDH_HYDRO_CHOLESKY && (refactor || !s->have_L) && assemble_and_factor(s)) return nullptr;
+    if (s->cfg.hydro != BDH_HYDRO_GEYER) return rs_dev(s, dt, s->A);
+    if (draw_noise(s, 1)) return nullptr;
+    if (matvec_dev(s, s->xi, true, dt)) return nullptr;
+    ReduceJobs jobs{};
+    jobs.job[0] = ReduceJob{s->Px, s->Feff, s->n, s->msplits, 0, 1.0};
+    jobs.job[1] = ReduceJob{s->Pc, s->Cs, s->n, s->msplits, 0, 1.0};
+    if (launch_reduce_partials(jobs, 2, s->n, s->st)) return nullptr;
+    if (launch_tea_close(s->pos, s->xi, s->Feff, s->Cs, s->Pe, pair_eparts(s->N), s->N, s->preDii, dt, s->bt,
+                         s->Rs, s->st))
+        return nullptr;
+    return s->Rs;
+}
+
+static int ensure_prev(bdh_system* s)
+{
+    if (!s->pos_prev && dmalloc(&s->pos_prev, sizeof(double4) * s->N)) return -1;
+    if (!s->F_prev && dmalloc(&s->F_prev, sizeof(double) * s->n)) return -1;
+    return 0;
+}
+
+int bdh_compute_Dij(bdh_system* s, int i, int j, int kind, double* tD)
+{
+    BDH_CHECK(s && tD && i >= 0 && i < s->N && j >= 0 && j < s->N, "bdh_compute_Dij: bad arguments");
+    BDH_CHECK(kind >= 0 && kind <= 3, "bdh_compute_Dij: kind 0 (the system's tensor), 1 free space, "
+                                      "2 Smith-Ewald, 3 Beenakker-Ewald");
+    BDH_HIP(hipSetDevice(s->device));
+    int ew = 0;
+    if (kind == 0) ew = s->ewald ? (s->cfg.ewald_method == EW_SMITH ? 2 : 1) : 0;
+    else if (kind >= 2) {
+        BDH_CHECK(s->ewald && s->cfg.ewald_method == (kind == 2 ? EW_SMITH : EW_BEENAKKER),
+                  "bdh_compute_Dij: the Ewald block of a method the system was not created with");
+        ew = kind == 2 ? 2 : 1;
+    }
+    BDH_CHECK(ew != 0 || kind == 0 || i != j, "bdh_compute_Dij: the free-space block needs i != j");
+    if (ensure_ewald(s)) return -1;
+    if (launch_single_block(s->pos, i, j, ew, s->preDii, s->preDij, s->noise, s->st)) return -1;
+    BDH_HIP(hipMemcpyAsync(tD, s->noise, sizeof(double) * 9, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+int bdh_compute_r(bdh_system* s, double* Rs)
+{
+    BDH_CHECK(s && Rs, "bdh_compute_r: bad arguments");
+    BDH_HIP(hipSetDevice(s->device));
+    const double* d = noise_at(s, s->cfg.dt, s->cfg.refactor_every == 1);
+    if (!d) return -1;
+    BDH_HIP(hipMemcpyAsync(Rs, d, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return check_flags(s);
+}
+
+int bdh_ermak_move(bdh_system* s, double dt, double* coord_out)
+{
+    BDH_CHECK(s && coord_out && dt > 0, "bdh_ermak_move: bad arguments");
+    BDH_CHECK(s->cfg.algorithm == BDH_ERMAK_CONST || s->cfg.algorithm == BDH_ERMAK_VAR,
+              "bdh_ermak_move: an Ermak-McCammon system");
+    BDH_HIP(hipSetDevice(s->device));
+    if (ensure_prev(s)) return -1;
+    const bool refactor = !s->have_L || s->steps_done % s->cfg.refactor_every == 0;
+    BDH_HIP(hipMemcpyAsync(s->pos_prev, s->pos, sizeof(double4) * s->N, hipMemcpyDeviceToDevice, s->st));
+    if (rf_any(s, s->Rf, dt)) return -1;
+    const double* Rs = noise_at(s, dt, refactor);
+    if (!Rs) return -1;
+    if (launch_em_update(s->pos, s->N, s->Rf, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+    BDH_HIP(hipMemcpyAsync(coord_out, s->pos, sizeof(double4) * s->N, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, sizeof(double4) * s->N, hipMemcpyDeviceToDevice, s->st));
+    if (Rs != s->Rs) BDH_HIP(hipMemcpyAsync(s->Rs, Rs, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->st));
+    s->last_Rs = s->Rs;
+    s->last_dt = dt;
+    s->steps_done++;
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return check_flags(s);
+}
+
+int bdh_igt_phase(bdh_system* s, int phase, double dt, double* coord_out, int* result)
+{
+    BDH_CHECK(s && coord_out && result && dt > 0 && (phase == 0 || phase == 1), "bdh_igt_phase: bad arguments");
+    BDH_CHECK(s->cfg.algorithm == BDH_IGT_CONST || s->cfg.algorithm == BDH_IGT_VAR,
+              "bdh_igt_phase: an IG-T system");
+    BDH_HIP(hipSetDevice(s->device));
+    const bool chol = s->cfg.hydro == BDH_HYDRO_CHOLESKY;
+    const size_t pb = sizeof(double4) * s->N, fb = sizeof(double) * s->n;
+    int fl = 0;
+    *result = 0;
+    if (phase == 0) {
+        // predictor r* = r0 + Rf(r0) + sqrt(2 dt) L(r0) xi1 and forces at r*
+        BDH_HIP(hipMemcpyAsync(s->pos_prev, s->pos, pb, hipMemcpyDeviceToDevice, s->st));
+        BDH_HIP(hipMemcpyAsync(s->F_prev, s->F, fb, hipMemcpyDeviceToDevice, s->st));
+        if (chol && assemble_and_factor(s)) return -1;
+        if (rf_dev(s, s->Rf, dt)) return -1;
+        const double* Rs = rs_dev(s, dt, s->A);
+        if (!Rs) return -1;
+        if (launch_em_update(s->pos, s->N, s->Rf, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+        if (forces_flags(s, fl)) return -1;
+        if (fl & 1) {
+            // positions back at r0; the forces stay those of r* (the caller restores them)
+            BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, pb, hipMemcpyDeviceToDevice, s->st));
+            BDH_HIP(hipMemcpyAsync(coord_out, s->pos, pb, hipMemcpyDeviceToHost, s->st));
+            BDH_HIP(hipStreamSynchronize(s->st));
+            *result = 1;
+            return 0;
+        }
+        // mean deterministic move and the factor of the mean tensor of (r0, r*)
+        if (rf_dev(s, s->Rfs, dt)) return -1;
+        hipLaunchKernelGGL(mean2_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st, s->Rf, s->Rfs, s->n,
+                           s->Rf0);
+        BDH_LAUNCH_CHECK();
+        if (chol && assemble_and_factor(s, s->A2, s->pos_prev)) return -1;
+        BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, pb, hipMemcpyDeviceToDevice, s->st));
+    }
+    // corrector r = r0 + Rf0 + sqrt(2 dt) L_avg xi2 and forces at r
+    const double* Rs = rs_dev(s, dt, s->A2);
+    if (!Rs) return -1;
+    if (launch_em_update(s->pos, s->N, s->Rf0, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+    if (forces_flags(s, fl)) return -1;
+    BDH_HIP(hipMemcpyAsync(coord_out, s->pos, pb, hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    s->last_Rs = Rs;
+    s->last_dt = dt;
+    *result = (fl & 1) ? 2 : 0;
+    return 0;
+}
+
+int bdh_set_forces(bdh_system* s, const double* F)
+{
+    BDH_CHECK(s && F, "bdh_set_forces: bad arguments");
+    BDH_HIP(hipSetDevice(s->device));
+    BDH_HIP(hipMemcpyAsync(s->F, F, sizeof(double) * s->n, hipMemcpyHostToDevice, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
 int bdh_get_coords(bdh_system* s, double* coord)
 {
     BDH_CHECK(s && coord, "bdh_get_coords: bad arguments");

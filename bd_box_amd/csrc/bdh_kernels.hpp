@@ -237,6 +237,10 @@ int launch_adj_hvp(const double4* d_pos, const double* d_Q, const double2* d_LJ,
 // Parameter cotangents of one step for y = D g (bdh_adjoint.hip): eps_bar, R_bar, Q_bar (N)
 // and H_bar (one per bond; bond_id maps each per-bead bond entry to its bond).  d_part:
 // adj_contract_splits(N) rows of 3N doubles.
+// One FP64 block of D (bdh_pairs.hip): ew 0 free space, 1 Beenakker-Ewald, 2 Smith-Ewald;
+// nine column-major values into d_out.
+int launch_single_block(const double4* d_pos, int i, int j, int ew, double preDii, double preDij,
+                        double* d_out, hipStream_t st);
 int launch_adj_params(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
                       const Topology& topo, const int* d_bond_id, const ForceParams& fp,
                       const double* d_y, double c, double* d_part, double* d_eps_bar,

@@ -369,6 +369,34 @@ int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi
     return 0;
 }
 
+// ------------------------------------------------------------------ single block
+// One 3x3 block of D in FP64 (compute_Dij, diff_tensor.c:437-470, and the block functions it
+// dispatches to: diff_tensor_calc_pos :45-92, the Smith and Beenakker blocks :102-422), EW as
+// pair_block, as nine column-major values (the block is symmetric).
+template <int EW>
+__global__ void single_block_kernel(const double4* __restrict__ pos, int i, int j, double preDii,
+                                    double preDij, double* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    const Blk6 B = pair_block<double, EW>(pos[i], pos[j], i == j, preDii, preDij);
+    out[0] = B.xx; out[1] = B.xy; out[2] = B.xz;
+    out[3] = B.xy; out[4] = B.yy; out[5] = B.yz;
+    out[6] = B.xz; out[7] = B.yz; out[8] = B.zz;
+}
+
+int launch_single_block(const double4* d_pos, int i, int j, int ew, double preDii, double preDij,
+                        double* d_out, hipStream_t st)
+{
+    if (ew == 1)
+        hipLaunchKernelGGL(single_block_kernel<1>, dim3(1), dim3(64), 0, st, d_pos, i, j, preDii, preDij, d_out);
+    else if (ew == 2)
+        hipLaunchKernelGGL(single_block_kernel<2>, dim3(1), dim3(64), 0, st, d_pos, i, j, preDii, preDij, d_out);
+    else
+        hipLaunchKernelGGL(single_block_kernel<0>, dim3(1), dim3(64), 0, st, d_pos, i, j, preDii, preDij, d_out);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
 // ------------------------------------------------------------------ Ewald assembly
 // Lower triangle of the Beenakker-Ewald D (rows of i, columns of j, i >= j), the same
 // tiling as the free-space assembly: (64 row beads) x (16 column beads) per workgroup.

@@ -138,6 +138,41 @@ class BDSystem:
         _lib.check(_lib.load().bdh_step(self._h, steps, None if x is None else _lib.dptr(x)),
                    "compute_hydro")
 
+    def compute_Dij(self, i: int, j: int, kind: int = 0) -> np.ndarray:
+        """3x3 block (rows of bead j, columns of bead i) at the current positions
+        (bdh_compute_Dij): kind 0 the system's tensor, 1 free space, 2 Smith, 3 Beenakker."""
+        t = np.zeros(9)
+        _lib.check(_lib.load().bdh_compute_Dij(self._h, int(i), int(j), int(kind), _lib.dptr(t)),
+                   "compute_Dij")
+        return t.reshape(3, 3).T.copy()   # column-major -> row-major
+
+    def compute_r(self) -> np.ndarray:
+        """Rs of compute_r at the current positions from the next Gaussians of the stream."""
+        Rs = np.zeros(self.n)
+        _lib.check(_lib.load().bdh_compute_r(self._h, _lib.dptr(Rs)), "compute_r")
+        return Rs
+
+    def ermak_move(self, dt: Optional[float] = None) -> np.ndarray:
+        """Positions (N, 4) of one Ermak-McCammon move from the current state; the system
+        stays at its positions (bdh_ermak_move, compute_hydro_ermak)."""
+        out = np.zeros(4 * self.N)
+        _lib.check(_lib.load().bdh_ermak_move(self._h, float(self.cfg.dt if dt is None else dt),
+                                              _lib.dptr(out)), "ermak_move")
+        return out.reshape(-1, 4)
+
+    def igt_phase(self, phase: int, dt: Optional[float] = None):
+        """(positions (N, 4), result) of an IG-T phase (bdh_igt_phase): phase 0 predictor and
+        corrector, 1 corrector only; result 1 / 2 when the predictor / corrector overlaps."""
+        out = np.zeros(4 * self.N)
+        res = C.c_int()
+        _lib.check(_lib.load().bdh_igt_phase(self._h, int(phase), float(self.cfg.dt if dt is None else dt),
+                                             _lib.dptr(out), C.byref(res)), "igt_phase")
+        return out.reshape(-1, 4), res.value
+
+    def set_forces(self, F):
+        f = np.ascontiguousarray(F, float).reshape(-1)
+        _lib.check(_lib.load().bdh_set_forces(self._h, _lib.dptr(f)), "set_forces")
+
     # ------------------------------------------------------------------ state access
     def coords(self) -> np.ndarray:
         c = np.zeros(4 * self.N)

@@ -129,6 +129,42 @@ int bdh_noise_product(bdh_system* sys, const double* x, double* y);
  * Stands in for compute_Rf (src/diff_alg/diff_equation.c:119-214). */
 int bdh_compute_Rf(bdh_system* sys, double* Rf);
 
+/* One 3x3 block of D at the current positions, tD[k + 3 l] = D(3 j + k, 3 i + l) (the
+ * orientation of insert9, src/diff_alg/diff_tensor.c:425-435; the block is symmetric).
+ * kind 0: the system's own tensor, compute_Dij (diff_tensor.c:437-470), i == j the diagonal
+ * block; 1: free-space RPY, diff_tensor_calc_pos (:45-92, no minimum image, i != j);
+ * 2 / 3: the Smith / Beenakker Ewald block and its diagonal (:102-263 / :265-422), for a
+ * periodic system created with that method. */
+int bdh_compute_Dij(bdh_system* sys, int i, int j, int kind, double* tD);
+
+/* Rs[3N] of compute_r (src/diff_alg/diff_rand.c:47-236) at the current positions, from the
+ * next 3N Gaussians of the device stream: sqrt(2 dt) L xi (L refreshed at the current
+ * positions when refactor_every is 1, the reference's schedule), the Geyer-Winter TEA
+ * displacement, or sqrt(2 dt diag_D) xi without hydrodynamics. */
+int bdh_compute_r(bdh_system* sys, double* Rs);
+
+/* One Ermak-McCammon move of length dt from the current positions and forces,
+ * compute_hydro_ermak (src/diff_alg/diff_equation.c:216-264): coord_out[4N] = r + Rf + Rs
+ * (wrapped into the box when periodic), Rs from the next Gaussians of the device stream.
+ * The system's positions stay where they were (the caller evaluates forces at coord_out and
+ * accepts or rejects, comp_step_ermak, diff_comp.c:152-179); bdh_get_last_displacement
+ * returns the move's Rf and Rs. */
+int bdh_ermak_move(bdh_system* sys, double dt, double* coord_out);
+
+/* The two phases of an IG-T step of length dt for a system created with algorithm
+ * igt_const / igt_var.  phase 0 stands in for compute_hydro_IGT (diff_equation.c:317-374):
+ * the predictor from the current positions and forces, its forces; if they report an
+ * overlap, *result = 1 and the positions are back at the start (the forces are those of the
+ * predictor); otherwise the mean deterministic move and the factor of the mean tensor, then
+ * phase 1.  phase 1 stands in for compute_hydro_IGT_second_phase (:288-315): the corrector
+ * from the current positions (the step's start) with the next Gaussians and the mean
+ * tensor's factor, its forces, *result = 2 on an overlap, else 0.  coord_out[4N] receives
+ * the positions reached. */
+int bdh_igt_phase(bdh_system* sys, int phase, double dt, double* coord_out, int* result);
+
+/* Replace the forces F[3N] of the current positions, load_forces (diff_decl.c:234-237). */
+int bdh_set_forces(bdh_system* sys, const double* F);
+
 int bdh_get_coords(bdh_system* sys, double* coord);          /* 4N, for save_output       */
 int bdh_set_coords(bdh_system* sys, const double* coord);    /* load_coord (diff_decl.c:229) */
 int bdh_get_forces(bdh_system* sys, double* F);              /* F[0] of the last step     */
