@@ -1244,6 +1244,17 @@ int bdh_igt_phase(bdh_system* s, int phase, double dt, double* coord_out, int* r
     return 0;
 }
 
+int bdh_tea_coefficients(bdh_system* s, double* beta, double* eps)
+{
+    BDH_CHECK(s && beta && eps && s->tea, "bdh_tea_coefficients: a TEA system");
+    double h[2];
+    BDH_HIP(hipMemcpyAsync(h, s->bt, sizeof(h), hipMemcpyDeviceToHost, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    *beta = h[0];
+    *eps = h[1];
+    return 0;
+}
+
 int bdh_set_forces(bdh_system* s, const double* F)
 {
     BDH_CHECK(s && F, "bdh_set_forces: bad arguments");
@@ -1703,6 +1714,7 @@ struct bdh_batch {
     bool have_L = false;
     EventRecorder* rec = nullptr;
     hipGraphExec_t factor_exec = nullptr;   // assembly + factorization of every chain, captured once
+    hipGraphExec_t cycle_exec = nullptr;    // a whole refactor cycle of device-stream steps
 };
 
 namespace {
@@ -1801,6 +1813,10 @@ int batch_set_schedule(bdh_batch* b, int left_max)
     BDH_HIP(hipStreamSynchronize(b->st));
     BDH_HIP(hipMemcpy(b->d_tab, b->tab.data(), sizeof(int) * b->tab.size(), hipMemcpyHostToDevice));
     BDH_HIP(hipMemcpy(b->d_ch, b->hc.data(), sizeof(BatchChain) * B, hipMemcpyHostToDevice));
+    if (b->cycle_exec) {
+        BDH_HIP(hipGraphExecDestroy(b->cycle_exec));
+        b->cycle_exec = nullptr;
+    }
     if (b->factor_exec) {
         BDH_HIP(hipGraphExecDestroy(b->factor_exec));
         b->factor_exec = nullptr;
@@ -1889,47 +1905,79 @@ int batch_factor_launches(bdh_batch* b)
 
 // The single-system schedule of run_steps (no rejections): chunks of up to TRMV_MAX_RHS
 // steps that never cross a refactorization; X == nullptr draws the Gaussians of every chain.
+// m steps of one refactor cycle (refactor: the cycle starts at its factorization).  Noise:
+// the caller's X (x_stride = Gaussians per step), or the chains' device streams (X null).
+// direct: the factorization as launches rather than its captured graph.
+int batch_cycle(bdh_batch* b, int m, bool refactor, const double* X, long long x_stride, bool direct)
+{
+    const double dt = b->cfg.dt;
+    const double rf_scale = dt / (kB * b->cfg.T);
+    if (!X) {
+        BREC_BEGIN(PH_NOISE);
+        if (launch_gauss_streams(b->rng, b->xi, b->rng_off, b->rng_cnt + (size_t)m * b->B, b->B,
+                                 b->flags + 1, b->st))
+            return -1;
+        BREC_END(PH_NOISE);
+        X = b->xi;
+        x_stride = 0;
+    }
+    if (refactor && (direct ? batch_factor_launches(b) : batch_factor(b))) return -1;
+    BREC_BEGIN(PH_TRMV);
+    if (b->mixed ? launch_trmv_batch<float>((const float*)b->A, X, x_stride, m, b->plan, b->Pt, b->st)
+                 : launch_trmv_batch<double>((const double*)b->A, X, x_stride, m, b->plan, b->Pt, b->st))
+        return -1;
+    if (launch_reduce_batch(b->Pt, 2, m, b->RsB, b->ntot, std::sqrt(2.0 * dt), b->plan, b->st)) return -1;
+    BREC_END(PH_TRMV);
+    for (int j = 0; j < m; ++j) {
+        BREC_BEGIN(PH_MATVEC);
+        if (b->mixed ? launch_pair_sums_batch<float>(b->pos, b->F, b->preDii, b->preDij, b->plan, b->Pp, b->st)
+                     : launch_pair_sums_batch<double>(b->pos, b->F, b->preDii, b->preDij, b->plan, b->Pp, b->st))
+            return -1;
+        BREC_END(PH_MATVEC);
+        BREC_BEGIN(PH_MOVE);
+        if (launch_reduce_batch(b->Pp, 0, 1, b->Rf, 0, rf_scale, b->plan, b->st)) return -1;
+        if (launch_em_update(b->pos, b->Ntot, b->Rf, b->RsB + (size_t)j * b->ntot, 0, 0.0, b->st)) return -1;
+        BREC_END(PH_MOVE);
+        if (batch_forces(b)) return -1;
+    }
+    return 0;
+}
+
+// Steps of every chain.  A whole refactor cycle on the device streams is one graph launch:
+// its ~(50 + 5 R) launches (the chains' Gaussians, assembly and factorization, L.xi of the
+// cycle, then per step D.F, move and forces) are captured on first use and replayed; every
+// argument (pointers, grid sizes, the stream counters kept in device memory) is the same from
+// one cycle to the next.  Partial cycles, given Gaussians and phase timing run the launches.
 int batch_run(bdh_batch* b, int steps, const double* X_user)
 {
     const int R = b->cfg.refactor_every;
-    const double dt = b->cfg.dt;
-    const double rf_scale = dt / (kB * b->cfg.T);
     for (int k = 0; k < steps;) {
         const int ph = (int)(b->steps_done % R);
         const bool refactor = !b->have_L || ph == 0;
         const int m = std::min(std::min(steps - k, TRMV_MAX_RHS), refactor ? R : R - ph);
-        const double* X = b->xi;
-        long long x_stride = 0;
-        if (X_user) {
-            X = X_user + (size_t)k * b->ntot;
-            x_stride = b->ntot;
+        if (!X_user && BDH_FACTOR_GRAPH && !b->rec && refactor && m == R) {
+            if (!b->cycle_exec) {
+                BDH_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+                const int rc = batch_cycle(b, R, true, nullptr, 0, true);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(b->st, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                BDH_HIP(e);
+                const hipError_t ei = hipGraphInstantiate(&b->cycle_exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) b->cycle_exec = nullptr;
+                BDH_HIP(ei);
+            }
+            BDH_HIP(hipGraphLaunch(b->cycle_exec, b->st));
+            b->have_L = true;
         } else {
-            BREC_BEGIN(PH_NOISE);
-            if (launch_gauss_streams(b->rng, b->xi, b->rng_off, b->rng_cnt + (size_t)m * b->B, b->B,
-                                     b->flags + 1, b->st))
-                return -1;
-            BREC_END(PH_NOISE);
+            const double* X = X_user ? X_user + (size_t)k * b->ntot : nullptr;
+            if (batch_cycle(b, m, refactor, X, X_user ? b->ntot : 0, false)) return -1;
         }
-        if (refactor && batch_factor(b)) return -1;
-        BREC_BEGIN(PH_TRMV);
-        if (b->mixed ? launch_trmv_batch<float>((const float*)b->A, X, x_stride, m, b->plan, b->Pt, b->st)
-                     : launch_trmv_batch<double>((const double*)b->A, X, x_stride, m, b->plan, b->Pt, b->st))
-            return -1;
-        if (launch_reduce_batch(b->Pt, 2, m, b->RsB, b->ntot, std::sqrt(2.0 * dt), b->plan, b->st)) return -1;
-        BREC_END(PH_TRMV);
-        for (int j = 0; j < m; ++j) {
-            BREC_BEGIN(PH_MATVEC);
-            if (b->mixed ? launch_pair_sums_batch<float>(b->pos, b->F, b->preDii, b->preDij, b->plan, b->Pp, b->st)
-                         : launch_pair_sums_batch<double>(b->pos, b->F, b->preDii, b->preDij, b->plan, b->Pp, b->st))
-                return -1;
-            BREC_END(PH_MATVEC);
-            BREC_BEGIN(PH_MOVE);
-            if (launch_reduce_batch(b->Pp, 0, 1, b->Rf, 0, rf_scale, b->plan, b->st)) return -1;
-            if (launch_em_update(b->pos, b->Ntot, b->Rf, b->RsB + (size_t)j * b->ntot, 0, 0.0, b->st)) return -1;
-            BREC_END(PH_MOVE);
-            if (batch_forces(b)) return -1;
-            b->steps_done++;
-        }
+        b->steps_done += m;
         k += m;
     }
     return 0;
@@ -2186,6 +2234,7 @@ void bdh_batch_destroy(bdh_batch* b)
     dfree(b->rng); dfree(b->rng_off); dfree(b->rng_cnt); dfree(b->flags); dfree(b->status);
     delete b->rec;
     if (b->factor_exec) (void)hipGraphExecDestroy(b->factor_exec);
+    if (b->cycle_exec) (void)hipGraphExecDestroy(b->cycle_exec);
     if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
 }

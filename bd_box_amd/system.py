@@ -20,6 +20,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
+from ._lib import BackendError  # noqa: F401  (raised by every call below)
 from .workloads import Params, System
 
 FP64 = 0
@@ -168,6 +169,12 @@ class BDSystem:
         _lib.check(_lib.load().bdh_igt_phase(self._h, int(phase), float(self.cfg.dt if dt is None else dt),
                                              _lib.dptr(out), C.byref(res)), "igt_phase")
         return out.reshape(-1, 4), res.value
+
+    def tea_coefficients(self):
+        """(beta, eps) of the last TEA displacement (bdh_tea_coefficients)."""
+        b, e = C.c_double(), C.c_double()
+        _lib.check(_lib.load().bdh_tea_coefficients(self._h, C.byref(b), C.byref(e)), "tea_coefficients")
+        return b.value, e.value
 
     def set_forces(self, F):
         f = np.ascontiguousarray(F, float).reshape(-1)

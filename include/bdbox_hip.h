@@ -162,6 +162,11 @@ int bdh_ermak_move(bdh_system* sys, double dt, double* coord_out);
  * the positions reached. */
 int bdh_igt_phase(bdh_system* sys, int phase, double dt, double* coord_out, int* result);
 
+/* beta and eps of the last Geyer-Winter TEA displacement (hydro geyer): the reference's
+ * eps (the epsilon accumulator of diff_tensor.h:136, summed, normalised and clamped in compute_r,
+ * src/diff_alg/diff_rand.c:197-206) and the beta it derives from it (:207-216). */
+int bdh_tea_coefficients(bdh_system* sys, double* beta, double* eps);
+
 /* Replace the forces F[3N] of the current positions, load_forces (diff_decl.c:234-237). */
 int bdh_set_forces(bdh_system* sys, const double* F);
 

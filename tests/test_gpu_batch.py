@@ -174,3 +174,30 @@ def test_batch_autotune(api):
         b.autotune()
     b.close()
     r.close()
+
+
+@pytest.mark.parametrize("prec", ["FP64", "MIXED"])
+def test_batch_cycle_graph_replays_match_launches(api, prec):
+    """A whole refactor cycle of device-stream steps is one captured graph: three replays,
+    with the host reading the state between them, give bitwise the positions and forces of
+    the same steps run as direct launches (phase timing on), cycle after cycle; a partial
+    cycle after them takes the launch path."""
+    S, _ = api
+    systems = wl.chains(SIZES, seed=510)
+    seeds = [2000 + 13 * k for k in range(len(SIZES))]
+    P = _params()
+    R = 4
+    print(f"batch cycle graph {prec}", flush=True)
+    g = S.BDBatch(systems, P, seeds, precision=getattr(S, prec), refactor_every=R)
+    d = S.BDBatch(systems, P, seeds, precision=getattr(S, prec), refactor_every=R)
+    d.phase_timing(True)
+    for cycle in range(3):
+        g.compute_hydro(R)
+        d.compute_hydro(R)
+        assert np.array_equal(g.coords(), d.coords()), (prec, cycle)
+        assert np.array_equal(g.forces(), d.forces()), (prec, cycle)
+    g.compute_hydro(2)
+    d.compute_hydro(2)
+    assert np.array_equal(g.coords(), d.coords())
+    g.close()
+    d.close()

@@ -1,0 +1,89 @@
+"""The periodic suspension (Beenakker-Ewald RPY, Geyer-Winter TEA noise, WCA wall over the
+device cell list) on the device.
+
+  N = 384 spheres of the N = 8192 recipe (a = 25 A, volume fraction 0.2, random sequential
+  addition, sigma 50 A, eps 1, cutoff 2^(1/6) sigma, lattice sphere R = 2, dt = 1 ps),
+  3 steps with given Gaussians: FP64 <= 1e-10, mixed <= 1e-5 relative displacement
+  against the FP64 oracle (the tensor's own TEA sums, the reference's stored-D path
+  diff_rand.c:136-164; Beenakker blocks diff_tensor.c:265-422).
+  N = 8192 itself: one step of the mixed path against the FP64 path from the same state with
+  the same Gaussians <= 1e-5 relative displacement; beta of the TEA is the closed form of
+  its eps (diff_rand.c:207-216) and both precisions agree on eps to 1e-6.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from bd_box_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _suspension(N, seed):
+    sysm, L = wl.spheres(N, seed=seed, radius=25.0, phi=0.2)
+    prm = wl.chain_params(bc="periodic", xbox=L, ybox=L, zbox=L, hydro="geyer",
+                          ewald_method="beenakker", ewald_real=2, ewald_recip=2, elec="no",
+                          alpha_lj=4.0, cutoff_lj=-1, lj_6_term="yes", dt=1.0,
+                          nb_list="bucket_list", rand_seed=20240 + seed)
+    return sysm, wl.parse_params(prm)
+
+
+def _beta(eps, N):
+    a = (3 * N - 1) * eps * eps - (3 * N - 2) * eps
+    return 0.5 if abs(a) < 1e-7 else (1 - math.sqrt(1 - a)) / a
+
+
+def test_suspension_384_vs_oracle():
+    from bd_box_amd import system as S
+    from oracle import pyoracle as po
+    sysm, P = _suspension(384, seed=3)
+    n = 3 * sysm.N
+    xi = np.random.default_rng(384).normal(size=(3, n))
+    ref = po.Sim(sysm, po.params_from(P), seed=P.rand_seed)
+    print("suspension N=384 oracle", flush=True)
+    traj = []
+    for k in range(3):
+        ref.run(1, xi[k])
+        traj.append(ref.coords())
+    x0 = sysm.coord[:, :3]
+    L = P.box
+    for prec, tol in ((S.FP64, 1e-10), (S.MIXED, 1e-5)):
+        print(f"suspension N=384 device prec={prec}", flush=True)
+        s = S.BDSystem(sysm, P, precision=prec)
+        for k in range(3):
+            s.compute_hydro(1, xi[k])
+            d_dev = s.coords()[:, :3] - x0
+            d_ref = traj[k][:, :3] - x0
+            # positions are wrapped into the box: compare displacements by minimum image
+            dd = d_dev - d_ref
+            dd -= L * np.floor(dd / L + 0.5)
+            dr = d_ref - L * np.floor(d_ref / L + 0.5)
+            err = np.linalg.norm(dd) / np.linalg.norm(dr)
+            assert err < tol, (prec, k, err)
+        beta, eps = s.tea_coefficients()
+        assert 0.0 < eps <= 1.0
+        assert abs(beta - _beta(eps, sysm.N)) <= 1e-12 * abs(beta)
+        s.close()
+
+
+def test_suspension_8192_mixed_vs_fp64():
+    from bd_box_amd import system as S
+    sysm, P = _suspension(8192, seed=1)
+    n = 3 * sysm.N
+    xi = np.random.default_rng(8192).normal(size=n)
+    out = {}
+    for name, prec in (("fp64", S.FP64), ("mixed", S.MIXED)):
+        print(f"suspension N=8192 {name}", flush=True)
+        s = S.BDSystem(sysm, P, precision=prec)
+        s.compute_hydro(1, xi)
+        Rf, Rs = s.last_displacement()
+        out[name] = (Rf + Rs, s.tea_coefficients())
+        s.close()
+    d64, (b64, e64) = out["fp64"]
+    d32, (b32, e32) = out["mixed"]
+    assert np.linalg.norm(d32 - d64) / np.linalg.norm(d64) < 1e-5
+    assert abs(e32 - e64) <= 1e-6 * abs(e64)
+    for beta, eps in ((b64, e64), (b32, e32)):
+        assert 0.0 < eps <= 1.0
+        assert abs(beta - _beta(eps, sysm.N)) <= 1e-12 * abs(beta)
