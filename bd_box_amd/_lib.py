@@ -29,7 +29,8 @@ EXPORTS = [
     "bdh_step_vjp_cached", "bdh_batch_set_schedule", "bdh_batch_get_schedule",
     "bdh_batch_autotune", "bdh_step_vjp_params", "bdh_set_params", "bdh_compute_Dij",
     "bdh_compute_r", "bdh_ermak_move", "bdh_igt_phase", "bdh_set_forces",
-    "bdh_tea_coefficients",
+    "bdh_tea_coefficients", "bdh_batch_set_tiles", "bdh_batch_get_tiles",
+    "bdh_batch_autotune_tiles",
 ]
 
 
@@ -119,6 +120,9 @@ def load():
     L.bdh_batch_set_schedule.argtypes = [C.c_void_p, C.c_int]
     L.bdh_batch_get_schedule.argtypes = [C.c_void_p, _ip]
     L.bdh_batch_autotune.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _ip, _ip]
+    L.bdh_batch_set_tiles.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.bdh_batch_get_tiles.argtypes = [C.c_void_p, _ip, _ip]
+    L.bdh_batch_autotune_tiles.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip]
     _lib = L
     return L
 

@@ -838,7 +838,7 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     // work buffers
     s->cells = cfg->nb_list != 0;
     s->fsplits = force_splits(N);
-    s->fblocks = s->cells ? cell_force_blocks(N) : ceil_div(N, 64) * s->fsplits;
+    s->fblocks = s->cells ? cell_force_blocks(N) : force_blocks(N);
     if (s->cells) {
         if (alloc_cells(s)) return -1;
     }
@@ -1691,7 +1691,8 @@ struct bdh_batch {
     std::vector<BatchChain> hc;
     std::vector<int> chol_blocks;   // workgroups of update launch k (host)
     std::vector<int> tab;           // host copy of every prefix table
-    int left_max = -1;              // update schedule (batch_set_schedule)
+    int left_max = -1;              // update schedule (batch_set_tiles)
+    int outer = 0;                  // outer panel width of every chain (batch_set_tiles)
     BatchPlan plan;
     BatchChain* d_ch = nullptr;
     int* d_tab = nullptr;           // every prefix table
@@ -1790,21 +1791,25 @@ int batch_factor(bdh_batch* b)
     return 0;
 }
 
-// Update schedule of every chain's factorization: chains with n_pad <= left_max update
-// left-looking inside each outer panel, the others right-looking (chol_step); left_max < 0
-// gives each chain the schedule a single system of its size uses.  The update launches'
-// prefix tables are rewritten, and the captured refactor graph, whose grids they size, is
-// dropped.
-int batch_set_schedule(bdh_batch* b, int left_max)
+// Update schedule and outer panel width of every chain's factorization: chains with n_pad <=
+// left_max update left-looking inside each outer panel of `outer` columns, the others
+// right-looking (chol_step); left_max < 0 gives each chain the schedule a single system of
+// its size uses.  The update launches' prefix tables are rewritten, and the captured graphs,
+// whose grids they size, are dropped.
+int batch_set_tiles(bdh_batch* b, int left_max, int outer)
 {
+    BDH_CHECK(outer == 256 || outer == 512 || outer == 1024, "bdh_batch_set_tiles: outer width 256, 512 or 1024");
     const int B = b->B, W = B + 1;
-    for (auto& h : b->hc) h.left = left_max < 0 ? chol_left_looking(h.n_pad) : h.n_pad <= left_max;
+    for (auto& h : b->hc) {
+        h.left = left_max < 0 ? chol_left_looking(h.n_pad) : h.n_pad <= left_max;
+        h.outer = outer;
+    }
     for (int k = 0; k < b->plan.chol_steps; ++k) {
         int* t = b->tab.data() + (size_t)(6 + k) * W;
         long long acc = 0;
         for (int c = 0; c < B; ++c) {
             t[c] = (int)acc;
-            acc += chol_update_blocks(b->hc[c].n_pad, b->hc[c].left != 0, k);
+            acc += chol_update_blocks(b->hc[c].n_pad, b->hc[c].left != 0, b->hc[c].outer, k);
         }
         BDH_CHECK(acc <= (long long)INT32_MAX, "bdh_batch_set_schedule: batch too large");
         t[B] = (int)acc;
@@ -1822,8 +1827,11 @@ int batch_set_schedule(bdh_batch* b, int left_max)
         b->factor_exec = nullptr;
     }
     b->left_max = left_max;
+    b->outer = outer;
     return 0;
 }
+
+int batch_set_schedule(bdh_batch* b, int left_max) { return batch_set_tiles(b, left_max, b->outer); }
 
 // Candidate schedules: all right-looking (0), then left-looking for the chains up to each
 // of at most BDH_TUNE_MAX - 1 of the batch's distinct padded sizes (evenly spaced in sorted
@@ -1840,19 +1848,20 @@ std::vector<int> batch_tune_candidates(const bdh_batch* b)
     return cand;
 }
 
-// Each candidate: one refactor to capture its graph, then `reps` timed replays (HIP events
-// on the batch's stream); its time is the fastest replay.  Every replay recomputes the same
-// matrices from the unchanged positions, so the candidates see identical work.
-int batch_autotune(bdh_batch* b, int reps, int* cand_out, double* ms_out, int* ncand, int* best)
+// Each candidate (left_max, outer): one refactor to capture its graph, then `reps` timed
+// replays (HIP events on the batch's stream); its time is the fastest replay.  Every replay
+// recomputes the same matrices from the unchanged positions, so the candidates see identical
+// work.  The fastest candidate is kept.
+int batch_tune(bdh_batch* b, int reps, const std::vector<std::pair<int, int>>& cand, int* ib_out,
+               std::vector<double>& t)
 {
-    const std::vector<int> cand = batch_tune_candidates(b);
     hipEvent_t e0, e1;
     BDH_HIP(hipEventCreate(&e0));
     BDH_HIP(hipEventCreate(&e1));
     int rc = 0, ib = 0;
-    std::vector<double> t(cand.size(), 0.0);
+    t.assign(cand.size(), 0.0);
     for (size_t i = 0; i < cand.size() && !rc; ++i) {
-        rc = batch_set_schedule(b, cand[i]) || batch_factor(b) || batch_check(b) ? -1 : 0;
+        rc = batch_set_tiles(b, cand[i].first, cand[i].second) || batch_factor(b) || batch_check(b) ? -1 : 0;
         double tmin = 1e30;
         for (int r = 0; r < reps && !rc; ++r) {
             float ms = 0.f;
@@ -1872,12 +1881,49 @@ int batch_autotune(bdh_batch* b, int reps, int* cand_out, double* ms_out, int* n
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    if (batch_set_schedule(b, cand[ib])) return -1;
+    if (batch_set_tiles(b, cand[ib].first, cand[ib].second)) return -1;
     b->have_L = false;
+    *ib_out = ib;
+    return 0;
+}
+
+int batch_autotune(bdh_batch* b, int reps, int* cand_out, double* ms_out, int* ncand, int* best)
+{
+    std::vector<std::pair<int, int>> cand;
+    for (int l : batch_tune_candidates(b)) cand.emplace_back(l, b->outer);
+    std::vector<double> t;
+    int ib = 0;
+    if (batch_tune(b, reps, cand, &ib, t)) return -1;
     *ncand = (int)cand.size();
-    *best = cand[ib];
+    *best = cand[ib].first;
     for (size_t i = 0; i < cand.size(); ++i) {
-        if (cand_out) cand_out[i] = cand[i];
+        if (cand_out) cand_out[i] = cand[i].first;
+        if (ms_out) ms_out[i] = t[i];
+    }
+    return 0;
+}
+
+// The schedules of batch_tune_candidates under each outer panel width 256, 512 and 1024
+// (widths no wider than the largest chain's padded order).
+int batch_autotune_tiles(bdh_batch* b, int reps, int* cand_left, int* cand_outer, double* ms_out,
+                         int* ncand, int* best)
+{
+    int nmax = 0;
+    for (const auto& h : b->hc) nmax = std::max(nmax, h.n_pad);
+    std::vector<std::pair<int, int>> cand;
+    for (int outer : {256, 512, 1024}) {
+        if (outer > 256 && outer > nmax) break;
+        for (int l : batch_tune_candidates(b)) cand.emplace_back(l, outer);
+    }
+    BDH_CHECK((int)cand.size() <= BDH_TUNE_TILES_MAX, "bdh_batch_autotune_tiles: too many candidates");
+    std::vector<double> t;
+    int ib = 0;
+    if (batch_tune(b, reps, cand, &ib, t)) return -1;
+    *ncand = (int)cand.size();
+    *best = ib;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        if (cand_left) cand_left[i] = cand[i].first;
+        if (cand_outer) cand_outer[i] = cand[i].second;
         if (ms_out) ms_out[i] = t[i];
     }
     return 0;
@@ -2012,6 +2058,7 @@ int batch_create_impl(const bdh_config* cfg, int B, const int* N, const double* 
     b->hc.resize(B);
     long long bead = 0, mat = 0, lb = 0, xo = 0, pp = 0, pf = 0, pt = 0, bonds_tot = 0;
     int max_steps = 0;
+    b->outer = chol_outer_default();
     for (int c = 0; c < B; ++c) {
         BDH_CHECK(N[c] > 0 && N[c] <= (1 << 20), "bdh_batch_create: chain size");
         BDH_CHECK(nbonds[c] >= 0 && (nbonds[c] == 0 || (bonds && bond_parms)), "bdh_batch_create: bonds");
@@ -2025,6 +2072,7 @@ int batch_create_impl(const bdh_config* cfg, int B, const int* N, const double* 
         h.tcols = trmv_col_blocks(h.n);
         h.ldx = h.n + (h.n & 1);
         h.left = chol_left_looking(h.n_pad);
+        h.outer = chol_outer_default();
         h.mat = mat;
         h.lb = lb;
         h.xi_off = xo;
@@ -2063,13 +2111,13 @@ int batch_create_impl(const bdh_config* cfg, int B, const int* N, const double* 
     };
     b->plan.asm_blocks = fill(0, [](const BatchChain& h) { return rpy_blocks(h.N); });
     b->plan.pair_blocks = fill(1, [](const BatchChain& h) { return pair_eparts(h.N); });
-    b->plan.force_blocks = fill(2, [](const BatchChain& h) { return ceil_div(h.N, 64) * h.fsplits; });
+    b->plan.force_blocks = fill(2, [](const BatchChain& h) { return force_blocks(h.N); });
     b->plan.trmv_blocks = b->mixed ? fill(3, [](const BatchChain& h) { return trmv_blocks<float>(h.n); })
                                    : fill(3, [](const BatchChain& h) { return trmv_blocks<double>(h.n); });
     b->plan.rows_blocks = fill(4, [](const BatchChain& h) { return ceil_div(h.n, 64); });
     b->plan.trsm_blocks = fill(5, [](const BatchChain& h) { return chol_trsm0_blocks(h.n_pad); });
     for (int k = 0; k < max_steps; ++k)
-        b->chol_blocks[k] = fill(6 + k, [k](const BatchChain& h) { return chol_update_blocks(h.n_pad, h.left, k); });
+        b->chol_blocks[k] = fill(6 + k, [k](const BatchChain& h) { return chol_update_blocks(h.n_pad, h.left, h.outer, k); });
     BDH_CHECK(b->plan.asm_blocks >= 0 && b->plan.pair_blocks >= 0 && b->plan.force_blocks >= 0 &&
               b->plan.trmv_blocks >= 0 && b->plan.rows_blocks >= 0 && b->plan.trsm_blocks >= 0,
               "bdh_batch_create: batch too large");
@@ -2280,6 +2328,29 @@ int bdh_batch_set_schedule(bdh_batch* b, int left_max)
 {
     BDH_CHECK(b, "bdh_batch_set_schedule: null batch");
     return batch_set_schedule(b, left_max);
+}
+
+int bdh_batch_set_tiles(bdh_batch* b, int left_max, int outer)
+{
+    BDH_CHECK(b, "bdh_batch_set_tiles: null batch");
+    return batch_set_tiles(b, left_max, outer);
+}
+
+int bdh_batch_get_tiles(bdh_batch* b, int* left_max, int* outer)
+{
+    BDH_CHECK(b && left_max && outer, "bdh_batch_get_tiles: bad arguments");
+    *left_max = b->left_max;
+    *outer = b->outer;
+    return 0;
+}
+
+int bdh_batch_autotune_tiles(bdh_batch* b, int reps, int* cand_left, int* cand_outer, double* ms,
+                             int* ncand, int* best)
+{
+    BDH_CHECK(b && reps > 0 && ncand && best, "bdh_batch_autotune_tiles: bad arguments");
+    BDH_CHECK(b->steps_done == 0, "bdh_batch_autotune_tiles: call before the first step (the tuning "
+                                  "refactors at the current positions)");
+    return batch_autotune_tiles(b, reps, cand_left, cand_outer, ms, ncand, best);
 }
 
 int bdh_batch_get_schedule(bdh_batch* b, int* left_max)

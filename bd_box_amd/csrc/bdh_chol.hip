@@ -881,22 +881,23 @@ constexpr int CHOL_OUTER = CHOL_OUTER_COLS;
 static_assert(CHOL_OUTER % CHOL_NB == 0 && CHOL_OUTER >= CHOL_NB, "outer panel width");
 
 // Update launch k (k = 0, 1, ...) of an n_pad matrix, after panel k = columns 64 k ..:
-// inside an outer panel, the update of the outer panel's later columns; at the end of an
-// outer panel, the whole outer panel's update of the trailing matrix.  Inside the outer
-// panel the schedule is either right-looking (panel k updates all of the outer panel's later
-// columns, K = 64) or left-looking (all of the outer panel's panels so far update only the
-// next panel's 64 columns, K = 64, 128, 192).  Both leave the same factor up to rounding.
-// False once no columns are left.
+// inside an outer panel of `outer` columns, the update of the outer panel's later columns; at
+// the end of an outer panel, the whole outer panel's update of the trailing matrix.  Inside
+// the outer panel the schedule is either right-looking (panel k updates all of the outer
+// panel's later columns, K = 64) or left-looking (all of the outer panel's panels so far
+// update only the next panel's 64 columns, K = 64, 128, ...).  Every schedule and width
+// leaves the same factor up to rounding.  False once no columns are left.
 #ifndef CHOL_LEFT_MAX
 #define CHOL_LEFT_MAX 4096
 #endif
-__host__ __device__ inline bool chol_step(int n_pad, bool left, int k, int& kb, int& K, int& t0, int& t1)
+__host__ __device__ inline bool chol_step(int n_pad, bool left, int outer, int k, int& kb, int& K, int& t0,
+                                          int& t1)
 {
     const int c0 = k * CHOL_NB;
     t0 = c0 + CHOL_NB;
     if (t0 >= n_pad) return false;
-    const int P0 = c0 / CHOL_OUTER * CHOL_OUTER;
-    const int P1 = n_pad < P0 + CHOL_OUTER ? n_pad : P0 + CHOL_OUTER;
+    const int P0 = c0 / outer * outer;
+    const int P1 = n_pad < P0 + outer ? n_pad : P0 + outer;
     if (t0 < P1 && left) {
         kb = P0;
         K = c0 + CHOL_NB - P0;
@@ -918,6 +919,7 @@ __host__ __device__ inline bool chol_step(int n_pad, bool left, int k, int& kb, 
 // diagonal blocks bounds them and the longer left-looking update of the first tile would
 // lengthen it.
 bool chol_left_looking(int n_pad) { return n_pad <= CHOL_LEFT_MAX; }
+int chol_outer_default() { return CHOL_OUTER; }
 
 // Workgroups of an update launch: the lower-trapezoid tiles of rows t0..n_pad, columns
 // t0..t1.
@@ -975,7 +977,7 @@ int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, 
     if (rec) rec->end(PH_PANEL, st);
     int kb, K, t0, t1;
     const bool left = chol_left_looking(n_pad);
-    for (int k = 0; chol_step(n_pad, left, k, kb, K, t0, t1); ++k) {
+    for (int k = 0; chol_step(n_pad, left, CHOL_OUTER, k, kb, K, t0, t1); ++k) {
         if (rec) rec->begin(PH_SYRK, st);
         const int rc = launch_update<T>(d_A, ld, kb, K, t0, t1, n_pad, d_Lbuf + (k + 1) * LB,
                                         d_flags + k + 1, d_status, syrk_mode, st);
@@ -1136,14 +1138,14 @@ int trmv_blk_cols() { return TRMV_COLS; }
 int chol_update_steps(int n_pad)
 {
     int kb, K, t0, t1, k = 0;
-    while (chol_step(n_pad, true, k, kb, K, t0, t1)) ++k;
+    while (chol_step(n_pad, true, CHOL_OUTER, k, kb, K, t0, t1)) ++k;
     return k;
 }
 
-int chol_update_blocks(int n_pad, bool left, int k)
+int chol_update_blocks(int n_pad, bool left, int outer, int k)
 {
     int kb, K, t0, t1;
-    return chol_step(n_pad, left, k, kb, K, t0, t1) ? update_tiles(n_pad, t0, t1) : 0;
+    return chol_step(n_pad, left, outer, k, kb, K, t0, t1) ? update_tiles(n_pad, t0, t1) : 0;
 }
 
 int chol_trsm0_blocks(int n_pad) { return n_pad > CHOL_NB ? ceil_div(n_pad - CHOL_NB, TRSM_ROWS) : 0; }
@@ -1184,7 +1186,7 @@ syrk_f32_batch_kernel(float* __restrict__ A, double* __restrict__ Lbuf, const Ba
     const int c = batch_find(prefix, nch, blockIdx.x);
     const BatchChain b = ch[c];
     int kb, K, t0, t1;
-    chol_step(b.n_pad, b.left != 0, k, kb, K, t0, t1);
+    chol_step(b.n_pad, b.left != 0, b.outer, k, kb, K, t0, t1);
     double* Lb = Lbuf + b.lb;
     syrk_f32_body<SPLIT>(A + b.mat, b.n_pad, kb, K, t0, t1, b.n_pad, Lb + (long long)(k + 1) * LB_STRIDE,
                          chol_flags(Lb, b.n_pad) + k + 1, status + c, blockIdx.x - prefix[c]);
@@ -1197,7 +1199,7 @@ syrk_f64_batch_kernel(double* __restrict__ A, double* __restrict__ Lbuf, const B
     const int c = batch_find(prefix, nch, blockIdx.x);
     const BatchChain b = ch[c];
     int kb, K, t0, t1;
-    chol_step(b.n_pad, b.left != 0, k, kb, K, t0, t1);
+    chol_step(b.n_pad, b.left != 0, b.outer, k, kb, K, t0, t1);
     double* Lb = Lbuf + b.lb;
     syrk_f64_body(A + b.mat, b.n_pad, kb, K, t0, t1, b.n_pad, Lb + (long long)(k + 1) * LB_STRIDE,
                   chol_flags(Lb, b.n_pad) + k + 1, status + c, blockIdx.x - prefix[c]);

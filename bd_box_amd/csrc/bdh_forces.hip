@@ -5,10 +5,15 @@
 // in bdh_pairterm.hpp.  Bonded partners use bond_lj_scale / bond_c_scale (0 removes the
 // term) whenever a scale differs from 1, exactly when the reference allocates its partner
 // table (bond.c:113-130); otherwise bonded pairs interact like any other pair.
-// Each workgroup owns 64 row beads and one split of the column beads and accumulates the
-// full row sums F_i = sum_j f_ij (every pair is evaluated from both sides; no atomics, so
-// the sums are deterministic).  Pair arithmetic runs in T (float in the mixed-precision
-// step), separations and accumulation in FP64.
+// Up to FT_MAX_N beads (forces_tiles_body): tiles of 64 beads, one workgroup per tile pair
+// (I <= J) of the upper triangle, each unordered pair evaluated once, as the reference's
+// loop over j > i does (calc_func.c:188-228): the pair's force goes to the row bead i and its
+// negative to the column bead j.  Row sums of tile I land in split J, column sums of tile J
+// in split I, so every split of every bead is written exactly once (no clearing, no
+// atomics, deterministic sums).  Beyond FT_MAX_N (forces_body): 64 row beads and one split
+// of the column beads per workgroup, full row sums with every pair evaluated from both
+// sides.  Pair arithmetic runs in T (float in the mixed-precision step), separations and
+// accumulation in FP64.
 #include "bdh_common.hpp"
 #include "bdh_kernels.hpp"
 #include "bdh_pairterm.hpp"
@@ -88,6 +93,118 @@ __device__ __forceinline__ void forces_body(const double4* __restrict__ pos, con
     if (a.rep & 2) atomicOr(flags, 2);
 }
 
+constexpr int FT_TILE = 64;
+constexpr int FT_MAX_N = 16384;   // tile-pair partials: 3N x ceil(N / 64) doubles
+
+// Tile pair b = J (J + 1) / 2 + I of the upper triangle of 64-bead tiles.  Lane t owns row
+// bead I*64 + t; wave w takes the shifts s = w, w + 4, ... and pairs row t with column bead
+// (t + s) mod 64 of tile J, so the column beads of one shift are distinct within a wave and
+// its column sums go to the wave's own LDS rows.  Diagonal tiles (I = J) visit every ordered
+// pair and keep row sums only; their workgroup also adds every row bead's bond terms.
+template <typename T>
+__device__ __forceinline__ void forces_tiles_body(const double4* __restrict__ pos, const double* __restrict__ Q,
+                                                  const double2* __restrict__ LJ, int N, const Topology& topo,
+                                                  const ForceParams& fp, double* __restrict__ PF,
+                                                  double* __restrict__ PE, int* __restrict__ flags, int b,
+                                                  int pe_idx)
+{
+    __shared__ double4 s_pos[FT_TILE];
+    __shared__ double s_q[FT_TILE];
+    __shared__ double2 s_lj[FT_TILE];
+    __shared__ double s_col[4][3][FT_TILE];
+    __shared__ double s_row[4][6][FT_TILE];
+    const int t = threadIdx.x, w = threadIdx.y, tid = w * FT_TILE + t;
+    int J = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while (J * (J + 1) / 2 > b) --J;
+    while ((J + 1) * (J + 2) / 2 <= b) ++J;
+    const int I = b - J * (J + 1) / 2;
+    const bool diag = I == J;
+    const int i = I * FT_TILE + t, j0 = J * FT_TILE;
+    if (tid < FT_TILE) {
+        const int j = j0 + tid;
+        const bool ok = j < N;
+        s_pos[tid] = ok ? pos[j] : make_double4(0, 0, 0, 1);
+        s_q[tid] = ok ? Q[j] : 0.0;
+        s_lj[tid] = ok ? LJ[j] : make_double2(0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s_col[w][k][t] = 0.0;
+    const bool iv = i < N;
+    const double4 pi = iv ? pos[i] : make_double4(0, 0, 0, 1);
+    const double qi = iv ? Q[i] : 0.0;
+    const double2 lji = iv ? LJ[i] : make_double2(0, 0);
+    const int nb0 = (iv && fp.use_conn) ? topo.conn_off[i] : 0;
+    const int nb1 = (iv && fp.use_conn) ? topo.conn_off[i + 1] : 0;
+    __syncthreads();
+    PairAcc a{0, 0, 0, 0, 0, 0};
+    for (int sh = w; sh < FT_TILE; sh += 4) {
+        const int jl = (t + sh) & (FT_TILE - 1);
+        const int j = j0 + jl;
+        if (!iv || j >= N || (diag && sh == 0)) continue;
+        double dx = s_pos[jl].x - pi.x, dy = s_pos[jl].y - pi.y, dz = s_pos[jl].z - pi.z;
+        const double r2 = pair_r2(dx, dy, dz, fp.pbc, fp.box);
+        if (r2 > fp.far_r2) continue;
+        PairAcc p{0, 0, 0, 0, 0, 0};
+        pair_term<T>(dx, dy, dz, r2, qi, s_q[jl], lji, s_lj[jl], bonded(topo, nb0, nb1, j), fp, p);
+        a.fx += p.fx;
+        a.fy += p.fy;
+        a.fz += p.fz;
+        a.e_lj += p.e_lj;
+        a.e_c += p.e_c;
+        a.rep |= p.rep;
+        if (!diag) {
+            s_col[w][0][jl] -= p.fx;
+            s_col[w][1][jl] -= p.fy;
+            s_col[w][2][jl] -= p.fz;
+        }
+    }
+    double e_b = 0.0;
+    if (iv && diag && w == 0) e_b = bond_terms(pos, i, pi, topo, fp, a);
+    s_row[w][0][t] = a.fx;
+    s_row[w][1][t] = a.fy;
+    s_row[w][2][t] = a.fz;
+    s_row[w][3][t] = a.e_lj;
+    s_row[w][4][t] = a.e_c;
+    s_row[w][5][t] = e_b;
+    __syncthreads();
+#define FT_SUM4(arr, v) ((((arr)[0][v][t] + (arr)[1][v][t]) + (arr)[2][v][t]) + (arr)[3][v][t])
+    if (w == 0 && iv) {
+        double* out = PF + (long long)J * 3 * N + 3 * i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = FT_SUM4(s_row, k);
+    }
+    if (w == 1 && !diag && j0 + t < N) {
+        double* out = PF + (long long)I * 3 * N + 3 * (j0 + t);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = FT_SUM4(s_col, k);
+    }
+    if (PE && w == 2) {
+        // pairs of a diagonal tile are seen from both ends; bonds from both of theirs
+        double e[3] = {FT_SUM4(s_row, 3), FT_SUM4(s_row, 4), FT_SUM4(s_row, 5)};
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) e[k] += __shfl_xor(e[k], off);
+        if (t == 0) {
+            PE[3 * pe_idx] = diag ? 0.5 * e[0] : e[0];
+            PE[3 * pe_idx + 1] = diag ? 0.5 * e[1] : e[1];
+            PE[3 * pe_idx + 2] = 0.5 * e[2];
+        }
+    }
+#undef FT_SUM4
+    if (a.rep && fp.check_overlap) atomicOr(flags, 1);
+    if (a.rep & 2) atomicOr(flags, 2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+forces_tiles_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
+                    const double2* __restrict__ LJ, int N, Topology topo, ForceParams fp,
+                    double* __restrict__ PF, double* __restrict__ PE, int* __restrict__ flags)
+{
+    forces_tiles_body<T>(pos, Q, LJ, N, topo, fp, PF, PE, flags, blockIdx.x, blockIdx.x);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 forces_kernel(const double4* __restrict__ pos, const double* __restrict__ Q,
@@ -112,12 +229,17 @@ forces_batch_kernel(const double4* __restrict__ pos, const double* __restrict__ 
     const int c = batch_find(prefix, nch, blockIdx.x);
     const BatchChain b = ch[c];
     const int x = blockIdx.x - prefix[c];
-    const int gx = (b.N + FR_ROWS - 1) / FR_ROWS;
     Topology t = topo;
     t.conn_off += b.bead0 + c;
     t.bond_off += b.bead0 + c;
-    forces_body<T>(pos + b.bead0, Q + b.bead0, LJ + b.bead0, b.N, t, fp, (b.N + b.fsplits - 1) / b.fsplits,
-                   Pf + b.pf, nullptr, flags, x % gx, x / gx, 0);
+    if (b.N <= FT_MAX_N) {
+        forces_tiles_body<T>(pos + b.bead0, Q + b.bead0, LJ + b.bead0, b.N, t, fp, Pf + b.pf, nullptr, flags,
+                             x, 0);
+    } else {
+        const int gx = (b.N + FR_ROWS - 1) / FR_ROWS;
+        forces_body<T>(pos + b.bead0, Q + b.bead0, LJ + b.bead0, b.N, t, fp, (b.N + b.fsplits - 1) / b.fsplits,
+                       Pf + b.pf, nullptr, flags, x % gx, x / gx, 0);
+    }
 }
 
 template <typename T>
@@ -133,6 +255,7 @@ int launch_forces_batch(const double4* d_pos, const double* d_Q, const double2* 
 
 int force_splits(int N)
 {
+    if (N <= FT_MAX_N) return ceil_div(N, FT_TILE);
     int rows = ceil_div(N, FR_ROWS);
     int s = ceil_div(1024, rows);
     int max_s = ceil_div(N, 64);
@@ -140,11 +263,27 @@ int force_splits(int N)
     return s < 1 ? 1 : s;
 }
 
+int force_blocks(int N)
+{
+    if (N <= FT_MAX_N) {
+        const int nt = ceil_div(N, FT_TILE);
+        return nt * (nt + 1) / 2;
+    }
+    return ceil_div(N, FR_ROWS) * force_splits(N);
+}
+
 template <typename T>
 int launch_forces(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
                   const Topology& topo, const ForceParams& fp, double* d_PF, double* d_PE,
                   int* d_flags, int splits, hipStream_t st)
 {
+    if (N <= FT_MAX_N) {
+        BDH_CHECK(splits == ceil_div(N, FT_TILE), "forces: tile-pair partials expect one split per tile");
+        hipLaunchKernelGGL(forces_tiles_kernel<T>, dim3(force_blocks(N)), dim3(FT_TILE, 4), 0, st, d_pos, d_Q,
+                           d_LJ, N, topo, fp, d_PF, d_PE, d_flags);
+        BDH_LAUNCH_CHECK();
+        return 0;
+    }
     dim3 grid(ceil_div(N, FR_ROWS), splits);
     hipLaunchKernelGGL(forces_kernel<T>, grid, dim3(FR_ROWS, 4), 0, st, d_pos, d_Q, d_LJ, N,
                        topo, fp, ceil_div(N, splits), d_PF, d_PE, d_flags);

@@ -24,12 +24,12 @@ struct PhaseRecorder {
 // (rows of ldx), and partial-sum rows at pp (pair sums: psplits rows), pf (forces: fsplits
 // rows) and pt (L.x: tcols x 16 rows).  Every per-chain launch shape is the one the
 // single-system launchers use, so a chain's arithmetic is the same as on its own, as long as
-// `left` (the update schedule of its factorization, chol_left_looking) is the one the single
-// system picks for its n_pad.
+// `left` (the update schedule of its factorization, chol_left_looking) and `outer` (its outer
+// panel width) are the ones the single system picks for its n_pad.
 struct BatchChain {
     int N, n, n_pad, bead0;
     int psplits, fsplits, tcols, ldx;
-    int left;
+    int left, outer;
     long long mat, lb, xi_off, pp, pf, pt;
 };
 
@@ -114,7 +114,8 @@ int trmv_blk_cols();   // columns per partial of the L.x kernel
 // Per-chain workgroup counts of the batched launches (host).
 int chol_update_steps(int n_pad);                 // update launches of one factorization
 bool chol_left_looking(int n_pad);                // the single-system update schedule
-int chol_update_blocks(int n_pad, bool left, int k);   // workgroups of launch k (0 when none)
+int chol_outer_default();                         // the single-system outer panel width
+int chol_update_blocks(int n_pad, bool left, int outer, int k);   // workgroups of launch k (0 when none)
 int chol_trsm0_blocks(int n_pad);                 // panel-0 solve workgroups
 template <typename T>
 int trmv_blocks(int n);                           // row x column blocks of the L.x partials
@@ -152,7 +153,8 @@ struct Topology {
     const int* bond_p;
     const double* bond_parm;
 };
-int force_splits(int N);
+int force_splits(int N);   // partial rows of the forces (one per 64-bead tile up to 16384 beads)
+int force_blocks(int N);   // workgroups (and energy partials) of one force launch
 template <typename T>
 int launch_forces(const double4* d_pos, const double* d_Q, const double2* d_LJ, int N,
                   const Topology& topo, const ForceParams& fp, double* d_PF, double* d_PE,

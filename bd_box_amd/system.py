@@ -396,6 +396,27 @@ class BDBatch:
                                                   C.byref(n), C.byref(best)), "autotune")
         return best.value, {int(cand[i]): float(ms[i]) for i in range(n.value)}
 
+    def set_tiles(self, left_max: int, outer: int):
+        _lib.check(_lib.load().bdh_batch_set_tiles(self._h, int(left_max), int(outer)), "set_tiles")
+
+    def tiles(self):
+        lm, ou = C.c_int(), C.c_int()
+        _lib.check(_lib.load().bdh_batch_get_tiles(self._h, C.byref(lm), C.byref(ou)), "get_tiles")
+        return lm.value, ou.value
+
+    def autotune_tiles(self, reps: int = 3):
+        """Times the refactor under every (schedule, outer panel width) candidate and keeps
+        the fastest (before the first step).  Returns ((left_max, outer), {(left_max, outer):
+        ms})."""
+        M = 24
+        cl, co = (C.c_int * M)(), (C.c_int * M)()
+        ms = np.zeros(M)
+        n, best = C.c_int(), C.c_int()
+        _lib.check(_lib.load().bdh_batch_autotune_tiles(self._h, int(reps), cl, co, _lib.dptr(ms),
+                                                        C.byref(n), C.byref(best)), "autotune_tiles")
+        times = {(int(cl[i]), int(co[i])): float(ms[i]) for i in range(n.value)}
+        return (int(cl[best.value]), int(co[best.value])), times
+
 
 def gauss(state: int, count: int):
     """`count` Gaussians of the drand48 polar Box-Muller stream from 48-bit `state`."""

@@ -129,12 +129,12 @@ def update_steps(n_pad, nb=64, outer=256, left_max=4096):
         k += 1
 
 
-def update_bytes_per_factor(n_pad, word=4, left_max=4096):
+def update_bytes_per_factor(n_pad, word=4, left_max=4096, outer=256):
     """Algorithmic HBM bytes of one factorization's update launches: every updated C entry
     (rows t0..n_pad, columns t0..t1, lower part) read and written once, and the operand rows
     t0..n_pad x K read once per launch."""
     b = 0
-    for kb, K, t0, t1 in update_steps(n_pad, left_max=left_max):
+    for kb, K, t0, t1 in update_steps(n_pad, outer=outer, left_max=left_max):
         rows = n_pad - t0
         w = t1 - t0
         c_entries = w * rows - w * (w - 1) // 2
@@ -522,10 +522,11 @@ def bench_periodic(args):
 
 def bench_batch(args):
     """Steps/s of a batch of independent chains (every chain advances one step per step),
-    mixed (split-bf16 trailing updates) and FP64, with each phase one launch over all
-    chains."""
+    mixed (FP32 MFMA trailing updates, FP64 diagonal blocks) beside FP64 and split-bf16,
+    each phase one launch over all chains and every full refactor cycle one graph launch;
+    tile shapes (update schedule, outer panel width) chosen by the autotuner per precision."""
     import torch
-    from bd_box_amd.system import BDBatch, FP64, MIXED_SPLIT
+    from bd_box_amd.system import BDBatch, FP64, MIXED, MIXED_SPLIT
     systems, prm, P, seeds = batch_workload(args.chains, args.seed)
     sizes = [s.N for s in systems]
     Ntot = int(sum(sizes))
@@ -535,12 +536,13 @@ def bench_batch(args):
     log(f"batch workload: {len(sizes)} chains, {Ntot} beads (N {min(sizes)}..{max(sizes)}), "
         f"{steps} timed steps")
     out, sched = {}, {}
-    for name, prec in (("mixed", MIXED_SPLIT), ("fp64", FP64)):
+    for name, prec in (("mixed", MIXED), ("fp64", FP64), ("split", MIXED_SPLIT)):
         b = BDBatch(systems, P, seeds, precision=prec, refactor_every=R)
         if args.autotune:
-            best, times = b.autotune(reps=3)
-            sched[name] = {"left_max": best, "refactor_ms": {str(k): v for k, v in times.items()}}
-            log(f"{name}: autotuned update schedule left_max={best} (refactor ms {times})")
+            best, times = b.autotune_tiles(reps=3)
+            sched[name] = {"left_max": best[0], "outer": best[1],
+                           "refactor_ms": {f"{k[0]}/{k[1]}": v for k, v in times.items()}}
+            log(f"{name}: autotuned tiles left_max={best[0]} outer={best[1]} (refactor ms {times})")
         b.compute_hydro(warm)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -556,10 +558,10 @@ def bench_batch(args):
         b.close()
     # accuracy: R steps of both precisions from the same state with the same streams
     disp = {}
-    for name, prec in (("mixed", MIXED_SPLIT), ("fp64", FP64)):
+    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
         b = BDBatch(systems, P, seeds, precision=prec, refactor_every=R)
         if name in sched:
-            b.set_schedule(sched[name]["left_max"])
+            b.set_tiles(sched[name]["left_max"], sched[name]["outer"])
         b.compute_hydro(R)
         disp[name] = b.coords()[:, :3]
         b.close()
@@ -575,12 +577,14 @@ def bench_batch(args):
     ms, cnt = ph["syrk"]
     fl = sum(syrk_flops_per_factor(3 * N, -(-3 * N // 128) * 128) for N in sizes) * 2
     lm = sched["mixed"]["left_max"] if "mixed" in sched else 4096
-    by = sum(update_bytes_per_factor(-(-3 * N // 128) * 128, left_max=lm) for N in sizes) * 2
+    ow = sched["mixed"]["outer"] if "mixed" in sched else 256
+    by = sum(update_bytes_per_factor(-(-3 * N // 128) * 128, left_max=lm, outer=ow) for N in sizes) * 2
     ach = by / (ms * 1e-3) / 1e9
-    tr, _ = measured_traffic("syrk_f32_batch_kernel<true>", TRAFFIC_FILE_BATCH)
+    tr, _ = measured_traffic("syrk_f32_batch_kernel<false>", TRAFFIC_FILE_BATCH)
     roof = {"kernel": "syrk", "launches": cnt, "avg_ms": ms / max(cnt, 1), "bound": "hbm",
             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": tr, "mfma_tflops": fl / (ms * 1e-3) / 1e12,
+            "mfma_frac": fl / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["f32_mfma"],
             "basis": "sum over chains and update launches of the updated lower-trapezoid C "
                      "entries read and written (FP32) plus the K-column operand rows read once, "
                      "2 factorizations; traffic = measured HBM bytes per launch"}
@@ -592,7 +596,8 @@ def bench_batch(args):
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"{len(sizes)} bead-spring chains, lengths log-uniform in "
                                f"[64, 1024] (sum {Ntot} beads), free-space RPY + Cholesky "
-                               f"refactored every {R} steps, split-bf16 trailing updates",
+                               f"refactored every {R} steps, FP32 MFMA trailing updates, FP64 "
+                               f"diagonal blocks, each refactor cycle one graph launch",
                    "chains": len(sizes), "beads": Ntot, "refactor_every": R,
                    "update_schedule": sched or "single-system default (left_max 4096)",
                    "parallelism": "1 GPU"},
@@ -600,6 +605,7 @@ def bench_batch(args):
         "bead_steps_per_s": value * Ntot,
         "fp64_steps_per_s": out["fp64"][0],
         "speedup_vs_fp64": value / out["fp64"][0],
+        "split_bf16_steps_per_s": out["split"][0],
         "rel_disp_err_max": max(errs),
         "phases_ms_per_step": {k: v[0] / cyc for k, v in ph.items()},
         "fp64_phases_ms_per_step": {k: v[0] / cyc for k, v in out["fp64"][1].items()},

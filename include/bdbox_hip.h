@@ -285,6 +285,20 @@ int bdh_batch_get_schedule(bdh_batch* batch, int* left_max);
 #define BDH_TUNE_MAX 8
 int bdh_batch_autotune(bdh_batch* batch, int reps, int* cand, double* ms, int* ncand, int* best);
 
+/* Tile shape of the batch's factorizations (no counterpart in the reference): the update
+ * schedule as in bdh_batch_set_schedule, and the width of the outer panels (256, 512 or
+ * 1024 columns) within which panels of 64 columns update, before the trailing matrix is
+ * updated with the whole outer panel.  Widths and schedules differ in rounding only. */
+int bdh_batch_set_tiles(bdh_batch* batch, int left_max, int outer);
+int bdh_batch_get_tiles(bdh_batch* batch, int* left_max, int* outer);
+/* bdh_batch_autotune over tile shapes: every schedule candidate of bdh_batch_autotune under
+ * each outer panel width (256, then 512 and 1024 where the largest chain reaches them), at
+ * most BDH_TUNE_TILES_MAX candidates.  cand_left / cand_outer / ms (may be NULL) receive
+ * the candidates and their refactor times, *ncand their count, *best the index kept. */
+#define BDH_TUNE_TILES_MAX 24
+int bdh_batch_autotune_tiles(bdh_batch* batch, int reps, int* cand_left, int* cand_outer, double* ms,
+                             int* ncand, int* best);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,11 +107,13 @@ def test_batch_config_checks(api):
         S.BDBatch(systems, _params(check_overlap="yes"), [1, 2], precision=S.FP64)
 
 
-@pytest.mark.parametrize("left_max", [0, 512, 8192])
-def test_batch_update_schedules(api, left_max):
+@pytest.mark.parametrize("left_max,outer", [(0, 256), (512, 256), (8192, 256), (-1, 512), (0, 1024),
+                                            (8192, 1024)])
+def test_batch_update_schedules(api, left_max, outer):
     """Every update schedule leaves the factor up to rounding: all right-looking (0), mixed
     (512: n_pad 128 and 512 chains left-looking, the rest right-looking) and all left-looking
-    (8192, including a chain with n_pad 4224 that a single system factors right-looking).
+    (8192, including a chain with n_pad 4224 that a single system factors right-looking),
+    inside outer panels of 256, 512 or 1024 columns.
     FP64 vs the oracle (small chains) and vs the chain's own FP64 system (N = 1400):
     <= 1e-10 relative displacement; split-bf16 vs the oracle: <= 1e-5."""
     S, po = api
@@ -121,12 +123,12 @@ def test_batch_update_schedules(api, left_max):
     steps, refactor = 3, 2
     xi = np.random.default_rng(6).normal(size=(steps, 3 * sum(sizes)))
     off = np.concatenate([[0], np.cumsum(sizes)])
-    print(f"batch update schedule left_max={left_max}", flush=True)
+    print(f"batch update schedule left_max={left_max} outer={outer}", flush=True)
     bb = {}
     for prec in ("FP64", "MIXED_SPLIT"):
         b = S.BDBatch(systems, P, [1, 2, 3, 4], precision=getattr(S, prec), refactor_every=refactor)
-        b.set_schedule(left_max)
-        assert b.schedule() == left_max
+        b.set_tiles(left_max, outer)
+        assert b.tiles() == (left_max, outer)
         b.compute_hydro(steps, xi)
         bb[prec] = b
     for c, sysm in enumerate(systems):
@@ -201,3 +203,25 @@ def test_batch_cycle_graph_replays_match_launches(api, prec):
     assert np.array_equal(g.coords(), d.coords())
     g.close()
     d.close()
+
+
+def test_batch_autotune_tiles(api):
+    """The tile tuner times every (schedule, outer width) candidate, keeps the fastest, and
+    the batch then steps exactly as one with those tiles set by hand."""
+    S, _ = api
+    systems = wl.chains([300, 700, 90], seed=520)   # n_pad 1024 and 2176 reach widths 512, 1024
+    P = _params()
+    xi = np.random.default_rng(8).normal(size=(4, 3 * 1090))
+    print("batch autotune tiles", flush=True)
+    b = S.BDBatch(systems, P, [1, 2, 3], precision=S.MIXED, refactor_every=2)
+    best, times = b.autotune_tiles(reps=2)
+    assert {o for _, o in times} == {256, 512, 1024}
+    assert best in times and times[best] == min(times.values())
+    assert b.tiles() == best
+    b.compute_hydro(4, xi)
+    r = S.BDBatch(systems, P, [1, 2, 3], precision=S.MIXED, refactor_every=2)
+    r.set_tiles(*best)
+    r.compute_hydro(4, xi)
+    assert np.array_equal(b.coords(), r.coords())
+    b.close()
+    r.close()
