@@ -159,6 +159,9 @@ struct bdh_system {
     int* bond_id = nullptr;         // bond index of each per-bead bond entry
     std::vector<int> csr_bond;      // the same on the host
     double* pgrad = nullptr;        // parameter cotangents: eps, R, Q (N each), H (nbonds)
+    double ew_scale = 0.0;          // Beenakker split (bdh_set_ewald_split): alpha / (sqrt(pi) / L),
+                                    // 0 for the reference's split
+    int ew_Rr = -1, ew_Rk = -1;     // real / reciprocal sphere radii (-1: ewald_real)
 };
 
 namespace {
@@ -249,8 +252,8 @@ int compute_forces_dev(bdh_system* s, bool energies = false)
 
 // The Ewald tables live in constant memory of the module: reload them when the system
 // asks for a different box or lattice radius than the last one loaded.
-double g_ew_box = -1.0, g_ew_alpha = -1.0;
-int g_ew_R = -1, g_ew_method = -1;
+double g_ew_box = -1.0, g_ew_alpha = -1.0, g_ew_scale = -1.0;
+int g_ew_R = -1, g_ew_method = -1, g_ew_Rr = -2, g_ew_Rk = -2;
 int g_ew_dev = -1;
 
 int ensure_ewald(bdh_system* s)
@@ -258,13 +261,18 @@ int ensure_ewald(bdh_system* s)
     if (!s->ewald) return 0;
     const bdh_config& c = s->cfg;
     if (g_ew_box == c.box && g_ew_R == c.ewald_real && g_ew_method == c.ewald_method &&
-        (c.ewald_method != EW_SMITH || g_ew_alpha == c.ewald_alpha) && g_ew_dev == s->device)
+        (c.ewald_method != EW_SMITH || g_ew_alpha == c.ewald_alpha) && g_ew_dev == s->device &&
+        g_ew_scale == s->ew_scale && g_ew_Rr == s->ew_Rr && g_ew_Rk == s->ew_Rk)
         return 0;
-    if (ewald_upload(c.box, c.ewald_real, c.ewald_method, c.ewald_alpha) < 0) return -1;
+    if (ewald_upload(c.box, c.ewald_real, c.ewald_method, c.ewald_alpha, s->ew_scale, s->ew_Rr, s->ew_Rk) < 0)
+        return -1;
     g_ew_box = c.box;
     g_ew_R = c.ewald_real;
     g_ew_method = c.ewald_method;
     g_ew_alpha = c.ewald_alpha;
+    g_ew_scale = s->ew_scale;
+    g_ew_Rr = s->ew_Rr;
+    g_ew_Rk = s->ew_Rk;
     g_ew_dev = s->device;
     return 0;
 }
@@ -1253,6 +1261,25 @@ int bdh_tea_coefficients(bdh_system* s, double* beta, double* eps)
     *beta = h[0];
     *eps = h[1];
     return 0;
+}
+
+int bdh_set_ewald_split(bdh_system* s, double alpha_scale, int R_real, int R_recip)
+{
+    BDH_CHECK(s && s->ewald && s->cfg.ewald_method == EW_BEENAKKER,
+              "bdh_set_ewald_split: a periodic system with the Beenakker tensor");
+    BDH_CHECK(alpha_scale >= 0.0 && (alpha_scale == 0.0 || (R_real >= 0 && R_real <= 5 && R_recip >= 0 &&
+                                                             R_recip <= 5)),
+              "bdh_set_ewald_split: alpha scale >= 0, radii 0..5");
+    BDH_HIP(hipSetDevice(s->device));
+    s->ew_scale = alpha_scale;
+    s->ew_Rr = alpha_scale > 0.0 ? R_real : -1;
+    s->ew_Rk = alpha_scale > 0.0 ? R_recip : -1;
+    // the captured refactor reads the tables of the last upload: rebuild it with the new ones
+    if (s->factor_exec) {
+        BDH_HIP(hipGraphExecDestroy(s->factor_exec));
+        s->factor_exec = nullptr;
+    }
+    return ensure_ewald(s);
 }
 
 int bdh_set_forces(bdh_system* s, const double* F)

@@ -3,7 +3,14 @@
 //
 // alpha = sqrt(pi)/L (forced by init_tensor, diff_tensor.c:551-554); real and reciprocal
 // lattices share the sphere h^2 + f^2 + g^2 <= R^2 (ewald_real == ewald_recip,
-// src/input.c:402-409).  The lattice vectors and every pair-independent factor of the
+// src/input.c:402-409), around the pair's separation as the wrapped positions give it (no
+// minimum image).  An extension beyond the reference (bdh_set_ewald_split) scales alpha,
+// gives the real and reciprocal sums spheres of their own radii -- the loops run over the
+// larger sphere, real images beyond R_real are skipped and reciprocal vectors beyond
+// R_recip carry a zero factor in the table -- and centres the image sum on the minimum-image
+// separation, so the omitted images of every pair lie at least (sqrt(R_real^2 + 1) -
+// sqrt(3)/2) L away.  The infinite sums do not depend on alpha or on the image chosen as the
+// centre (the reciprocal phases are periodic in the separation); the truncated ones do.  The lattice vectors and every pair-independent factor of the
 // reciprocal term are tabulated once on the host in FP64 (with the reference's float pi):
 //   G_k = 6 pi (1 + (k^2/4a^2)(1 + k^2/2a^2)) exp(-k^2/4a^2) / k^2,  Rk = k^2 / 3,
 //   k = 2 pi (h, f, g) / L  (in units where the box edge is folded into preQ = 1/L^3)
@@ -24,7 +31,9 @@ enum EwaldMethod { EW_SMITH = 0, EW_BEENAKKER = 1 };
 
 struct EwaldTables {
     int nv;                  // vectors in the sphere (including 0)
-    int R;                   // lattice sphere radius
+    int R;                   // lattice sphere radius of the loops
+    int RR_real;             // real-space images with h^2 + f^2 + g^2 <= RR_real
+    int min_image;           // image sum around the minimum-image separation (extension)
     int method;              // EwaldMethod
     double box, inv_L, alpha, alpha2, alpha3, inv_alpha2, preQ;
     double self_a, self_b;   // Beenakker self term: -6 alpha/sqrt(pi), 40 alpha^3/(3 sqrt(pi))
@@ -40,8 +49,11 @@ extern __constant__ double c_recip_z[EW_MAXV];   // e_z
 extern __constant__ double c_recip_w[EW_MAXV];
 
 // Upload the tables of one method for box edge L, lattice radius R and (Smith) the
-// splitting parameter alpha in units of 1/L; returns the vector count.
-int ewald_upload(double box, int R, int method, double alpha);
+// splitting parameter alpha in units of 1/L; returns the vector count.  Beenakker with
+// alpha_scale > 0 (the extension): alpha = alpha_scale sqrt(pi) / L, spheres of radius
+// R_real (images) and R_recip (reciprocal vectors) instead of R, minimum-image centres.
+int ewald_upload(double box, int R, int method, double alpha, double alpha_scale = 0.0, int R_real = -1,
+                 int R_recip = -1);
 
 struct Blk6 {
     double xx, yy, zz, xy, xz, yz;
@@ -134,6 +146,11 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
                                                 double preDij)
 {
     const EwaldTables& E = c_ew;
+    if (E.min_image) {
+        rx -= E.box * rint(rx * E.inv_L);
+        ry -= E.box * rint(ry * E.inv_L);
+        rz -= E.box * rint(rz * E.inv_L);
+    }
     const double sigma_sq = 0.5 * (ai * ai + aj * aj);
     double O[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz
     double Qr[6] = {0, 0, 0, 0, 0, 0};
@@ -168,7 +185,7 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
                 // real space image r + L (h, f, g)
                 const T x = (T)(rx + E.box * h), y = (T)(ry + E.box * f), z = (T)(rz + E.box * g);
                 const T rr2 = x * x + y * y + z * z;
-                if (rr2 != (T)0) {
+                if (rr2 != (T)0 && h * h + f * f + g * g <= E.RR_real) {
                     const InvPowers<T> ip = inv_powers(rr2);
                     const T rn = ip.r, inv_r = ip.inv_r, inv_r2 = ip.inv_r2, inv_r3 = ip.inv_r3;
                     const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;

@@ -76,7 +76,8 @@ int launch_rpy_assemble_batch(const double4* d_pos, double preDii, double preDij
                               const BatchPlan& plan, hipStream_t st);
 
 // ---- pair sums, Ewald, TEA (bdh_pairs.hip)
-int ewald_upload(double box, int R, int method, double alpha_smith);
+int ewald_upload(double box, int R, int method, double alpha_smith, double alpha_scale, int R_real,
+                 int R_recip);
 int pair_splits(int N, bool ewald);   // row partials per bead: one per 64-bead tile
 int pair_eparts(int N);                // eps partials of the TEA sums: one per tile pair
 template <typename T>

@@ -32,10 +32,16 @@ __constant__ double c_recip_w[EW_MAXV];
 // launch is preceded by an upload check for the calling system).
 static int g_uploaded_method = EW_BEENAKKER;
 
-int ewald_upload(double box, int R, int method, double alpha_smith)
+int ewald_upload(double box, int R, int method, double alpha_smith, double alpha_scale, int R_real,
+                 int R_recip)
 {
     BDH_CHECK(R >= 0 && R <= 5, "Ewald lattice radius must be 0..5");
     BDH_CHECK(method == EW_SMITH || method == EW_BEENAKKER, "Ewald method");
+    const bool ext = alpha_scale > 0.0;
+    if (!ext) R_real = R_recip = R;
+    BDH_CHECK(method == EW_BEENAKKER || !ext, "Ewald: a split of its own only for the Beenakker sums");
+    BDH_CHECK(R_real >= 0 && R_real <= 5 && R_recip >= 0 && R_recip <= 5, "Ewald split: radii 0..5");
+    R = R_real > R_recip ? R_real : R_recip;
     EwaldTables t{};
     int nv = 0;
     std::vector<double4> rec;
@@ -45,6 +51,7 @@ int ewald_upload(double box, int R, int method, double alpha_smith)
     t.inv_L = 1.0 / box;
     if (method == EW_BEENAKKER) {
         t.alpha = SQRTPI * t.inv_L;            // init_tensor, diff_tensor.c:551-554
+        if (ext) t.alpha *= alpha_scale;
     } else {
         t.alpha = alpha_smith;                  // reduced units (the .prm ewald_alpha)
     }
@@ -77,8 +84,9 @@ int ewald_upload(double box, int R, int method, double alpha_smith)
                 const double nrm = std::sqrt(s);
                 if (method == EW_BEENAKKER) {
                     const double k2 = s * 4.0 * PI_F * PI_F;
-                    const double G = (1.0 + (t.inv_alpha2 * k2 / 4.0) * (1.0 + t.inv_alpha2 * k2 / 2.0)) *
-                                     (6.0 * PI_F * std::exp(-t.inv_alpha2 * k2 / 4.0) / k2);
+                    double G = (1.0 + (t.inv_alpha2 * k2 / 4.0) * (1.0 + t.inv_alpha2 * k2 / 2.0)) *
+                               (6.0 * PI_F * std::exp(-t.inv_alpha2 * k2 / 4.0) / k2);
+                    if (s > (double)(R_recip * R_recip)) G = 0.0;
                     rec.push_back(make_double4(G, k2 / 3.0, h / nrm, f / nrm));
                     recz.push_back(g / nrm);
                     recw.push_back(0.0);
@@ -96,6 +104,8 @@ int ewald_upload(double box, int R, int method, double alpha_smith)
     }
     t.nv = nv;
     t.R = R;
+    t.RR_real = R_real * R_real;
+    t.min_image = ext ? 1 : 0;
     BDH_CHECK(t.nv <= EW_MAXV, "Ewald table overflow");
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ew), &t, sizeof(t)));
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip), rec.data(), sizeof(double4) * rec.size()));

@@ -176,6 +176,15 @@ class BDSystem:
         _lib.check(_lib.load().bdh_tea_coefficients(self._h, C.byref(b), C.byref(e)), "tea_coefficients")
         return b.value, e.value
 
+    def set_ewald_split(self, alpha_scale: float = 1.0, R_real: Optional[int] = None,
+                        R_recip: Optional[int] = None):
+        """Beenakker split alpha = alpha_scale sqrt(pi)/L with real / reciprocal sphere radii
+        around minimum-image separations (bdh_set_ewald_split); None keeps ewald_real;
+        alpha_scale 0 returns to the reference's split."""
+        Rr = self.cfg.ewald_real if R_real is None else int(R_real)
+        Rk = self.cfg.ewald_real if R_recip is None else int(R_recip)
+        _lib.check(_lib.load().bdh_set_ewald_split(self._h, float(alpha_scale), Rr, Rk), "set_ewald_split")
+
     def set_forces(self, F):
         f = np.ascontiguousarray(F, float).reshape(-1)
         _lib.check(_lib.load().bdh_set_forces(self._h, _lib.dptr(f)), "set_forces")

@@ -167,6 +167,16 @@ int bdh_igt_phase(bdh_system* sys, int phase, double dt, double* coord_out, int*
  * src/diff_alg/diff_rand.c:197-206) and the beta it derives from it (:207-216). */
 int bdh_tea_coefficients(bdh_system* sys, double* beta, double* eps);
 
+/* Extension of the Beenakker-Ewald tensor (no counterpart in the reference, which forces
+ * alpha = sqrt(pi)/L in init_tensor, src/diff_alg/diff_tensor.c:551-554, sums the real and
+ * reciprocal lattices over one sphere, src/input.c:402-409, and centres the image sum on the
+ * separation of the wrapped positions): for alpha_scale > 0, splitting parameter
+ * alpha = alpha_scale sqrt(pi)/L, real-space images with h^2 + f^2 + g^2 <= R_real^2 around
+ * the minimum-image separation and reciprocal vectors with h^2 + f^2 + g^2 <= R_recip^2
+ * (radii 0..5), so the split can be chosen for a target truncation error.  alpha_scale 0
+ * returns to the reference's tensor.  Periodic systems with ewald_method beenakker. */
+int bdh_set_ewald_split(bdh_system* sys, double alpha_scale, int R_real, int R_recip);
+
 /* Replace the forces F[3N] of the current positions, load_forces (diff_decl.c:234-237). */
 int bdh_set_forces(bdh_system* sys, const double* F);
 

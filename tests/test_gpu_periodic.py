@@ -87,3 +87,49 @@ def test_suspension_8192_mixed_vs_fp64():
     for beta, eps in ((b64, e64), (b32, e32)):
         assert 0.0 < eps <= 1.0
         assert abs(beta - _beta(eps, sysm.N)) <= 1e-12 * abs(beta)
+
+
+def ewald_split_for(scale: float, tol: float = 1e-6):
+    """Real and reciprocal sphere radii for the Beenakker split alpha = scale sqrt(pi)/L at a
+    truncation error of about tol, image sums centred on minimum-image separations
+    (|r| <= sqrt(3)/2 L): an omitted image n (|n|^2 > R_real^2) lies at least
+    (sqrt(R_real^2 + 1) - sqrt(3)/2) L from the pair, where the image terms fall as
+    exp(-alpha^2 r^2); an omitted reciprocal vector carries exp(-k^2 / 4 alpha^2) =
+    exp(-pi |n|^2 / scale^2) with |n|^2 >= R_recip^2 + 1.  Both exponents are taken 3 above
+    ln(1/tol) for the polynomial prefactors."""
+    x2 = (math.log(1.0 / tol) + 3.0) / math.pi
+    d = math.sqrt(x2) / scale + math.sqrt(3.0) / 2.0
+    Rr = math.ceil(math.sqrt(max(0.0, d * d - 1.0)))
+    Rk = math.ceil(math.sqrt(max(0.0, x2 * scale * scale - 1.0)))
+    return Rr, Rk
+
+
+def test_beenakker_alpha_invariance():
+    """The Beenakker tensor does not depend on the splitting parameter once both sums have
+    converged: D at alpha = (0.8, 1, 1.25) sqrt(pi)/L, with radii chosen for 1e-6 and image
+    sums around minimum-image separations, agrees with a fully converged tensor (radius 5)
+    to 1e-6 of its largest entry; a split with too few images does not.  The reference's
+    own split (alpha = sqrt(pi)/L, one sphere of radius 2 around the wrapped separation)
+    approximates the same tensor, less closely for pairs across the box."""
+    from bd_box_amd import system as S
+    sysm, P = _suspension(96, seed=5)
+    s = S.BDSystem(sysm, P, precision=S.FP64)
+    print("beenakker alpha invariance", flush=True)
+    s.set_ewald_split(1.0, 5, 5)
+    ref = s.compute_D()
+    scale = np.max(np.abs(ref))
+    for a in (0.8, 1.0, 1.25):
+        Rr, Rk = ewald_split_for(a)
+        assert Rr <= 5 and Rk <= 5
+        s.set_ewald_split(a, Rr, Rk)
+        D = s.compute_D()
+        err = np.max(np.abs(D - ref)) / scale
+        print(f"alpha {a} radii {Rr}/{Rk}: {err:.2e}", flush=True)
+        assert err < 1e-6, (a, Rr, Rk, err)
+    s.set_ewald_split(0.8, 1, 1)
+    assert np.max(np.abs(s.compute_D() - ref)) / scale > 1e-4
+    s.set_ewald_split(0.0)
+    err = np.max(np.abs(s.compute_D() - ref)) / scale
+    print(f"reference split, radius {P.ewald_real}: {err:.2e}", flush=True)
+    assert err < 1e-2
+    s.close()
