@@ -685,7 +685,7 @@ int bdh_init(int device)
     BDH_HIP(hipGetDeviceCount(&n));
     BDH_CHECK(device >= 0 && device < n, "bdh_init: no such device");
     BDH_HIP(hipSetDevice(device));
-    return 0;
+    return gauss_prepare();   // a synchronous upload, kept out of any later stream capture
 }
 
 void bdh_destroy(bdh_system* s)
@@ -2030,6 +2030,7 @@ int batch_run(bdh_batch* b, int steps, const double* X_user)
         const int m = std::min(std::min(steps - k, TRMV_MAX_RHS), refactor ? R : R - ph);
         if (!X_user && BDH_FACTOR_GRAPH && !b->rec && refactor && m == R) {
             if (!b->cycle_exec) {
+                if (gauss_prepare()) return -1;   // its table upload cannot be captured
                 BDH_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
                 const int rc = batch_cycle(b, R, true, nullptr, 0, true);
                 hipGraph_t g = nullptr;

@@ -63,6 +63,9 @@ struct BatchPlan {
 uint64_t lcg_seed(uint32_t seed);
 int launch_gauss_streams(uint64_t* d_states, double* d_out, const long long* d_offsets,
                          const int* d_counts, int nstreams, int* d_status, hipStream_t st);
+// Uploads the LCG jump tables to the current device (once); a synchronous copy, so it must
+// run before any stream capture that records a Gaussian launch.
+int gauss_prepare();
 
 // ---- RPY tensor (bdh_rpy.hip)
 template <typename T>

@@ -162,6 +162,8 @@ int ensure_jumps()
 
 uint64_t lcg_seed(uint32_t seed) { return (((uint64_t)seed) << 16) | 0x330Eu; }
 
+int gauss_prepare() { return ensure_jumps(); }
+
 int launch_gauss_streams(uint64_t* d_states, double* d_out, const long long* d_offsets,
                          const int* d_counts, int nstreams, int* d_status, hipStream_t st)
 {
