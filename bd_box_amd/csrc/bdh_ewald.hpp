@@ -9,11 +9,15 @@
 // larger sphere, real images beyond R_real are skipped and reciprocal vectors beyond
 // R_recip carry a zero factor in the table -- and centres the image sum on the minimum-image
 // separation, so the omitted images of every pair lie at least (sqrt(R_real^2 + 1) -
-// sqrt(3)/2) L away.  The infinite sums do not depend on alpha or on the image chosen as the
-// centre (the reciprocal phases are periodic in the separation); the truncated ones do.  The lattice vectors and every pair-independent factor of the
-// reciprocal term are tabulated once on the host in FP64 (with the reference's float pi):
+// sqrt(3)/2) L away, and forms the wave vectors k = 2 pi (h, f, g) / L of the box (the
+// reference's k = 2 pi (h, f, g) is in box units against alpha and sigma in A, so its
+// reciprocal terms underflow to zero).  The infinite sums do not depend on alpha or on the
+// image chosen as the centre (the reciprocal phases are periodic in the separation); the
+// truncated ones do.  The lattice vectors and every pair-independent factor of the
+// reciprocal term are tabulated once on the host in FP64 (with the reference's float pi;
+// the extension's with the double pi):
 //   G_k = 6 pi (1 + (k^2/4a^2)(1 + k^2/2a^2)) exp(-k^2/4a^2) / k^2,  Rk = k^2 / 3,
-//   k = 2 pi (h, f, g) / L  (in units where the box edge is folded into preQ = 1/L^3)
+//   k = 2 pi (h, f, g)  (reference),  k = 2 pi (h, f, g) / L  (extension)
 // so a pair's reciprocal sum is sum_k (1 - sigma^2 Rk) G_k cos(2 pi v.hfg) (I - e_k e_k^T),
 // with v = r / L.  The reciprocal and self sums run in FP64 in both precisions; the
 // real-space image sum runs in T.

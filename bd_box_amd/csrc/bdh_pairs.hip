@@ -83,9 +83,14 @@ int ewald_upload(double box, int R, int method, double alpha_smith, double alpha
                 }
                 const double nrm = std::sqrt(s);
                 if (method == EW_BEENAKKER) {
-                    const double k2 = s * 4.0 * PI_F * PI_F;
+                    // The reference forms k = 2 pi (h, f, g) in box units (diff_tensor.c:372-376)
+                    // against alpha and sigma in A, so its reciprocal terms underflow to zero for
+                    // any box of more than a few A; the extension takes k = 2 pi (h, f, g) / L,
+                    // the wave vectors of the box, as Beenakker's sum has them.
+                    const double k2 = ext ? s * 4.0 * M_PI * M_PI * t.inv_L * t.inv_L : s * 4.0 * PI_F * PI_F;
+                    const double six_pi = ext ? 6.0 * M_PI : 6.0 * PI_F;
                     double G = (1.0 + (t.inv_alpha2 * k2 / 4.0) * (1.0 + t.inv_alpha2 * k2 / 2.0)) *
-                               (6.0 * PI_F * std::exp(-t.inv_alpha2 * k2 / 4.0) / k2);
+                               (six_pi * std::exp(-t.inv_alpha2 * k2 / 4.0) / k2);
                     if (s > (double)(R_recip * R_recip)) G = 0.0;
                     rec.push_back(make_double4(G, k2 / 3.0, h / nrm, f / nrm));
                     recz.push_back(g / nrm);

@@ -8,7 +8,12 @@ device cell list) on the device.
   diff_rand.c:136-164; Beenakker blocks diff_tensor.c:265-422).
   N = 8192 itself: one step of the mixed path against the FP64 path from the same state with
   the same Gaussians <= 1e-5 relative displacement; beta of the TEA is the closed form of
-  its eps (diff_rand.c:207-216) and both precisions agree on eps to 1e-6.
+  its eps (diff_rand.c:207-216) and both precisions agree on eps to 1e-6 of a self
+  mobility D_ii.
+The TEA eps is the mean off-diagonal entry of the raw tensor (diff_rand.c:197-206): it is
+clamped at 1 but not at 0, and the Ewald tensor's off-diagonal entries sum to about zero, so
+eps is small and of either sign; at N = 384 it is checked against the mean off-diagonal entry
+of the oracle's tensor at the same positions.
 """
 import math
 
@@ -27,6 +32,20 @@ def _suspension(N, seed):
                           alpha_lj=4.0, cutoff_lj=-1, lj_6_term="yes", dt=1.0,
                           nb_list="bucket_list", rand_seed=20240 + seed)
     return sysm, wl.parse_params(prm)
+
+
+KB = 0.001986248532617   # kB in kcal/(mol K), src/data.h:224
+
+
+def _self_mobility(P, a):
+    """D_ii = kB T / (6 pi eta a) with the reference's viscosity units (diff_decl.c:83-175)."""
+    return KB * P.T / (6.0 * math.pi * P.visc * P.vfactor * a)
+
+
+def _eps_of(D):
+    """2 sum_{i<j} D_ij / (n (n - 1)) of a full n x n tensor, clamped at 1 (diff_rand.c:197-206)."""
+    n = D.shape[0]
+    return min(1.0, 2.0 * np.tril(D, -1).sum() / (n * (n - 1)))
 
 
 def _beta(eps, N):
@@ -61,8 +80,11 @@ def test_suspension_384_vs_oracle():
             dr = d_ref - L * np.floor(d_ref / L + 0.5)
             err = np.linalg.norm(dd) / np.linalg.norm(dr)
             assert err < tol, (prec, k, err)
+        # eps of the last step: the tensor at the positions where that step started
         beta, eps = s.tea_coefficients()
-        assert 0.0 < eps <= 1.0
+        eps_ref = _eps_of(po.compute_D(traj[1], po.params_from(P)))
+        scale = _self_mobility(P, 25.0)
+        assert abs(eps - eps_ref) <= (1e-12 if prec == S.FP64 else 1e-6) * scale, (prec, eps, eps_ref)
         assert abs(beta - _beta(eps, sysm.N)) <= 1e-12 * abs(beta)
         s.close()
 
@@ -83,9 +105,9 @@ def test_suspension_8192_mixed_vs_fp64():
     d64, (b64, e64) = out["fp64"]
     d32, (b32, e32) = out["mixed"]
     assert np.linalg.norm(d32 - d64) / np.linalg.norm(d64) < 1e-5
-    assert abs(e32 - e64) <= 1e-6 * abs(e64)
+    assert abs(e32 - e64) <= 1e-6 * _self_mobility(P, 25.0), (e32, e64)
     for beta, eps in ((b64, e64), (b32, e32)):
-        assert 0.0 < eps <= 1.0
+        assert eps <= 1.0
         assert abs(beta - _beta(eps, sysm.N)) <= 1e-12 * abs(beta)
 
 
@@ -110,7 +132,9 @@ def test_beenakker_alpha_invariance():
     sums around minimum-image separations, agrees with a fully converged tensor (radius 5)
     to 1e-6 of its largest entry; a split with too few images does not.  The reference's
     own split (alpha = sqrt(pi)/L, one sphere of radius 2 around the wrapped separation)
-    approximates the same tensor, less closely for pairs across the box."""
+    is not that tensor: it forms the reciprocal wave vectors in box units against alpha and
+    the radii in A (diff_tensor.c:372-376), so its reciprocal terms underflow and only the
+    truncated real-space sum is left."""
     from bd_box_amd import system as S
     sysm, P = _suspension(96, seed=5)
     s = S.BDSystem(sysm, P, precision=S.FP64)
@@ -131,5 +155,5 @@ def test_beenakker_alpha_invariance():
     s.set_ewald_split(0.0)
     err = np.max(np.abs(s.compute_D() - ref)) / scale
     print(f"reference split, radius {P.ewald_real}: {err:.2e}", flush=True)
-    assert err < 1e-2
+    assert err > 1e-3
     s.close()

@@ -39,7 +39,7 @@ def test_schedule_chunks_cross_cycles(R):
     for m in chunks:
         s.compute_hydro(m, xi[k:k + m])
         k += m
-        err = _rel(s.coords()[:, :3] - traj[k - 1][:, :3], traj[k - 1][:, :3] - x0)
+        err = _rel(s.coords()[:, :3] - x0, traj[k - 1][:, :3] - x0)
         assert err < 1e-10, (k, err)
     # device stream: the oracle's Sim is pinned to the composition (test_oracle.py)
     ref = po.Sim(sysm, po.params_from(P), seed=P.rand_seed, refactor_every=R)
@@ -50,7 +50,7 @@ def test_schedule_chunks_cross_cycles(R):
         d.compute_hydro(m)
         k += m
         assert d.rng_state() == ref.rng_state(), k
-        err = _rel(d.coords()[:, :3] - ref.coords()[:, :3], ref.coords()[:, :3] - x0)
+        err = _rel(d.coords()[:, :3] - x0, ref.coords()[:, :3] - x0)
         assert err < 1e-10, (k, err)
     s.close()
     d.close()
@@ -81,7 +81,7 @@ def test_chain3000_cycle(chain3000, prec_name, tol):
     x0 = sysm.coord[:, :3]
     s = S.BDSystem(sysm, P, precision=prec, refactor_every=10)
     s.compute_hydro(10, xi)
-    err = _rel(s.coords()[:, :3] - traj[-1][:, :3], traj[-1][:, :3] - x0)
+    err = _rel(s.coords()[:, :3] - x0, traj[-1][:, :3] - x0)
     assert err < tol, ("whole cycle", err)
     F_dev = s.forces()
     F_ref, _, _ = po.forces(sysm, po.params_from(P), s.coords())
