@@ -30,7 +30,7 @@ EXPORTS = [
     "bdh_batch_autotune", "bdh_step_vjp_params", "bdh_set_params", "bdh_compute_Dij",
     "bdh_compute_r", "bdh_ermak_move", "bdh_igt_phase", "bdh_set_forces",
     "bdh_tea_coefficients", "bdh_batch_set_tiles", "bdh_batch_get_tiles",
-    "bdh_batch_autotune_tiles", "bdh_set_ewald_split",
+    "bdh_batch_autotune_tiles", "bdh_set_ewald_split", "bdh_gauss_noalloc",
 ]
 
 
@@ -87,6 +87,7 @@ def load():
     L.bdh_dpotrf.argtypes = [C.c_char, _dp, C.c_int, C.c_int, C.c_int]
     L.bdh_dtrmv.argtypes = [C.c_char, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
     L.bdh_gauss.argtypes = [_u64p, _dp, C.c_int]
+    L.bdh_gauss_noalloc.argtypes = [_u64p, _dp, C.c_int]
     L.bdh_seed_state.argtypes = [C.c_uint32]
     L.bdh_seed_state.restype = C.c_uint64
     L.bdh_probe_mfma.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]

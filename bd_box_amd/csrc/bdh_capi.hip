@@ -13,7 +13,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -673,6 +675,37 @@ int dpotrf_impl(char uplo, double* A, int n, int ld, int* info, int syrk_mode)
     return rc;
 }
 
+// ------------------------------------------------------------------ allocation-free Gaussians
+// bdh_gauss_noalloc stages through this file-scope device array instead of buffers from
+// hipMalloc: a header (stream state at 0, output offset at 8, count at 16, status at 20) and
+// room for BDH_GAUSS_NOALLOC_MAX doubles after it.  Every device the library runs on holds its
+// own copy until the process ends.  In host code `scratch` is only a stand-in for the symbol:
+// its device address is looked up once per device (hipGetSymbolAddress, with that device
+// current) and only that address is handed to copies and kernels.
+constexpr size_t GAUSS_SCRATCH_HDR = 256;
+__device__ __align__(256) unsigned char scratch[GAUSS_SCRATCH_HDR + sizeof(double) * BDH_GAUSS_NOALLOC_MAX];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_addr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up on the first call per device).
+int scratch_address(unsigned char** addr)
+{
+    int dev = 0;
+    BDH_HIP(hipGetDevice(&dev));
+    BDH_CHECK(dev >= 0 && dev < SCRATCH_MAX_DEVICES, "scratch: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        BDH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        BDH_CHECK(p != nullptr, "scratch: no device address");
+        g_scratch_addr[dev] = static_cast<unsigned char*>(p);
+    }
+    *addr = g_scratch_addr[dev];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -685,7 +718,9 @@ int bdh_init(int device)
     BDH_HIP(hipGetDeviceCount(&n));
     BDH_CHECK(device >= 0 && device < n, "bdh_init: no such device");
     BDH_HIP(hipSetDevice(device));
-    return gauss_prepare();   // a synchronous upload, kept out of any later stream capture
+    if (gauss_prepare()) return -1;
+    unsigned char* scratch_base = nullptr;   // the allocation-free entry points' array
+    return scratch_address(&scratch_base);
 }
 
 void bdh_destroy(bdh_system* s)
@@ -1543,6 +1578,42 @@ int bdh_gauss(uint64_t* state, double* out, int count)
     }
     dfree(d_state); dfree(d_out); dfree(d_off); dfree(d_cnt); dfree(d_status);
     (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int bdh_gauss_noalloc(uint64_t* state, double* out, int count)
+{
+    BDH_CHECK(state && (out || count == 0) && count >= 0, "bdh_gauss_noalloc: bad arguments");
+    BDH_CHECK(count <= BDH_GAUSS_NOALLOC_MAX,
+              "bdh_gauss_noalloc: count exceeds BDH_GAUSS_NOALLOC_MAX (1048576); use bdh_gauss");
+    if (count == 0) return 0;
+    unsigned char* base = nullptr;
+    if (scratch_address(&base)) return -1;
+    // the header is written whole on every call: the array keeps the previous call's contents
+    struct Hdr {
+        uint64_t state;
+        long long off;
+        int cnt;
+        int status;
+    };
+    static_assert(sizeof(Hdr) == 24 && sizeof(Hdr) <= GAUSS_SCRATCH_HDR, "scratch header layout");
+    const Hdr h{*state, 0, count, 0};
+    Hdr back{};
+    uint64_t* d_state = reinterpret_cast<uint64_t*>(base);
+    long long* d_off = reinterpret_cast<long long*>(base + offsetof(Hdr, off));
+    int* d_cnt = reinterpret_cast<int*>(base + offsetof(Hdr, cnt));
+    int* d_status = reinterpret_cast<int*>(base + offsetof(Hdr, status));
+    double* d_out = reinterpret_cast<double*>(base + GAUSS_SCRATCH_HDR);
+    const hipStream_t st = hipStreamPerThread;
+    BDH_HIP(hipMemcpyAsync(base, &h, sizeof(Hdr), hipMemcpyHostToDevice, st));
+    int rc = launch_gauss_streams(d_state, d_out, d_off, d_cnt, 1, d_status, st);
+    if (!rc) {
+        BDH_HIP(hipMemcpyAsync(out, d_out, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+        BDH_HIP(hipMemcpyAsync(&back, base, sizeof(Hdr), hipMemcpyDeviceToHost, st));
+        BDH_HIP(hipStreamSynchronize(st));
+        BDH_CHECK(back.status == 0, "bdh_gauss_noalloc: stream did not complete");
+        *state = back.state;
+    }
     return rc;
 }
 

@@ -202,6 +202,15 @@ int bdh_gauss(uint64_t* state, double* out, int count);
 /* 48-bit drand48 state for srand48_r(seed) (src/rand_move.c:53-80). */
 uint64_t bdh_seed_state(uint32_t seed);
 
+/* bdh_gauss without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  The stream state and the output are staged through a fixed array that each
+ * device holds until the process ends, so count <= BDH_GAUSS_NOALLOC_MAX (a larger count
+ * is refused with an error).  Calls on one device share that array: calls that overlap on
+ * one device race.  Results are identical to bdh_gauss for the same state and count. */
+#define BDH_GAUSS_NOALLOC_MAX (1 << 20)
+int bdh_gauss_noalloc(uint64_t* state, double* out, int count);
+
 /* Test hook: one MFMA on a single wavefront with the operand layouts this backend assumes
  * (kind 0 f32 32x32x2, 1 f32 16x16x4, 2 f64 16x16x4, 3 bf16 32x32x16, 4 bf16 16x16x32).
  * A (MxK) and B (KxN) row-major host arrays (float, or double for kind 2); C = A B. */
