@@ -6,10 +6,10 @@
 // multiple of PAD_TO with an identity block so every tile is whole.
 //
 // Per panel of CHOL_NB = 64 columns c0..c0+63:
-//   diagonal block  L_kk of the 64x64 block in FP64 (factor_diag64): 16-column sub-panels
-//                   factored by one wavefront in registers (pivot and column entries
-//                   broadcast with v_readlane), the rest of the block updated by all four
-//                   wavefronts from LDS.  Stored in T into A and in FP64 into Lbuf.
+//   diagonal block  L_kk of the 64x64 block in FP64 (factor_diag64): the four wavefronts
+//                   own 16 columns each and run as a pipeline, each applying the earlier
+//                   columns' updates as they are published in LDS and then factoring its
+//                   own.  Stored in T into A and in FP64 into Lbuf.
 //   TRSM            L_ik = A_ik L_kk^{-T} for every row below, FP64 forward substitution,
 //                   four lanes per row (a quad exchanges each finished entry through DPP).
 //   trailing SYRK   A22 -= L21 L21^T over 128x128 lower tiles on MFMA:
@@ -51,165 +51,156 @@ __device__ __forceinline__ double rsqrt_f64(double x)
     return y;
 }
 
-// Sub-panel B0..B0+15 of the diagonal block, one wavefront, lane = row: pivot and column
-// entries broadcast with v_readlane (lane indices are compile-time constants), the rank-1
-// updates of the sub-panel's later columns in registers.  The chain from one pivot to the
-// next is kept short: pivot t+1 is formed from its two pre-update entries,
-//   d_{t+1} = a_{t+1,t+1} - (a_{t+1,t} / sqrt(d_t))^2,
-// with the same FMA the owning lane applies to its own entry, so both agree bit for bit.
-// A non-positive (or NaN) pivot propagates NaN; the first such row is noted in `bad`.
-template <int B0>
-__device__ __forceinline__ void diag_subpanel(double* __restrict__ sD, int lane, int& bad)
-{
-    double p[16];
-#pragma unroll
-    for (int t = 0; t < 16; ++t) p[t] = (lane >= B0) ? sD[lane * DG_LD + B0 + t] : 0.0;
-    double piv = readlane_f64(p[0], B0);
-    double mydiag = 1.0;   // this lane's pivot, if it owns one in the sub-panel
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        // column t's entries of the later sub-panel rows, unscaled: final since column t - 1,
-        // so these broadcasts do not wait for the pivot's reciprocal square root
-        double a[16];
-#pragma unroll
-        for (int u = t + 1; u < 16; ++u) a[u] = readlane_f64(p[t], B0 + u);
-        mydiag = (lane == B0 + t) ? piv : mydiag;
-        const double rs = rsqrt_f64(piv);
-        double nxt = 0.0;
-        if (t < 15) {
-            const double le = a[t + 1] * rs;
-            nxt = __builtin_fma(-le, le, readlane_f64(p[t + 1], B0 + t + 1));
-        }
-        const double l = (lane == B0 + t) ? piv * rs : p[t] * rs;
-        p[t] = l;
-        // p[u] -= l (a_u rs)
-        const double lr2 = l * rs;
-#pragma unroll
-        for (int u = t + 1; u < 16; ++u) p[u] = __builtin_fma(-lr2, a[u], p[u]);
-        piv = nxt;
-    }
-    const unsigned long long fail = __ballot(lane >= B0 && lane < B0 + 16 && !(mydiag > 0.0));
-    if (bad < 0 && fail) bad = __builtin_ctzll(fail);
-    if (lane >= B0) {
-#pragma unroll
-        for (int t = 0; t < 16; ++t) sD[lane * DG_LD + B0 + t] = (B0 + t <= lane) ? p[t] : 0.0;
-    }
-}
-
-// Update of the block's rows/columns >= B0 + 16 with sub-panel B0 (16 columns) on
-// v_mfma_f64_16x16x4_f64: the lower 16x16 tiles are shared out over the four wavefronts.
-template <int B0>
-__device__ __forceinline__ void diag_update(double* __restrict__ sD, int lane, int wave)
-{
-    constexpr int r0 = B0 + 16, NT = (CHOL_NB - r0) / 16;
-    constexpr int NTILES = NT * (NT + 1) / 2;
-    for (int tt = wave; tt < NTILES; tt += 4) {
-        int ti = 0, x = tt;
-        while (x > ti) {
-            x -= ti + 1;
-            ++ti;
-        }
-        const int rb = r0 + 16 * ti, cb = r0 + 16 * x;
-        doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int k = 0; k < 16; k += 4) {
-            const double a = sD[(rb + (lane & 15)) * DG_LD + B0 + k + (lane >> 4)];
-            const double b = sD[(cb + (lane & 15)) * DG_LD + B0 + k + (lane >> 4)];
-            acc = mfma_f64_16x16x4(a, b, acc);
-        }
-        // accumulator r holds row 4 r + lane / 16, column lane % 16 of the tile
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = rb + 4 * r + (lane >> 4), col = cb + (lane & 15);
-            if (row >= col) sD[row * DG_LD + col] -= acc[r];
-        }
-    }
-}
-
 // Per diagonal block, the workspace Lb holds (FP64):
 //   Lb[0 .. 4095]      L_kk column-major, zeros above the diagonal
 //   Lb[4096 + 256 J]   inverse of its J-th 16x16 diagonal sub-block, column-major, J = 0..3
 constexpr int LB_STRIDE = CHOL_NB * CHOL_NB + 4 * 256;
 
-// Columns 16 J .. 16 J + 15 of the factored block into Lb, by the 128 threads q of two
-// wavefronts (8 entries each).
-__device__ __forceinline__ void publish_cols(const double* __restrict__ sD, double* __restrict__ Lb,
-                                             int J, int q)
+// Bounded wait (inside one workgroup) until the LDS counter reaches `need`; returns the
+// counter's value then, or -1 if it never did.  Every wavefront that publishes to the counter
+// runs concurrently with the waiting one, so the wait ends unless a wavefront faulted; the
+// bound keeps a defect from hanging the GPU.
+constexpr int LDS_SPIN = 1 << 22;
+__device__ __forceinline__ int wait_lds_count(int* cnt, int need)
 {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int e = q + 128 * i, r = e % CHOL_NB, c = 16 * J + e / CHOL_NB;
-        Lb[c * CHOL_NB + r] = (r >= c) ? sD[r * DG_LD + c] : 0.0;
+    for (int it = 0; it < LDS_SPIN; ++it) {
+        const int v = __hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (v >= need) return v;
+        __builtin_amdgcn_s_sleep(1);
     }
+    return -1;
 }
 
-// Inverse of the J-th 16x16 diagonal sub-block into Lb (lane c < 16 of one wavefront solves
-// L_JJ y = e_c by forward substitution; column-major result).
-__device__ __forceinline__ void publish_inv(const double* __restrict__ sD, double* __restrict__ Lb,
-                                            int J, int lane)
-{
-    if (lane >= 16) return;
-    const int b = 16 * J, c = lane;
-    double rd[16], y[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) rd[r] = 1.0 / sD[(b + r) * DG_LD + b + r];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        double acc = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-        for (int t = 0; t < r; ++t) acc -= sD[(b + r) * DG_LD + b + t] * y[t];
-        y[r] = acc * rd[r];
-    }
-    double* inv = Lb + CHOL_NB * CHOL_NB + 256 * J;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) inv[c * 16 + r] = y[r];
-}
-
-// In-place FP64 Cholesky of the 64x64 block sD[r * DG_LD + c] (lower part read; on return
-// the lower part holds L and the strict upper part zeros).  Called by all 256 threads.
-// A non-positive pivot records row c0 + j + 1 in *status (first one wins) and the
+// In-place FP64 Cholesky of the 64x64 block held in LDS, called by all 256 threads.
+//   in : sD[r * DG_LD + c], lower part (r >= c) read
+//   out: the same memory as L[c * 64 + r] (column-major, pitch 64, zeros above the diagonal)
+//        and Lb (L_kk and the inverses of its four 16x16 diagonal sub-blocks)
+// The four wavefronts form a pipeline over column groups: wavefront w owns columns
+// 16w .. 16w+15 (lane = row, one register per column).  It applies the rank-1 updates of the
+// 16w earlier columns as the wavefronts before it publish them (column in LDS, then an LDS
+// counter), then factors its own 16 columns one by one, publishing each.  So the pivot chain
+// runs through the wavefronts back to back and every update off it runs beside it.
+// Within a wavefront's own columns, pivot t+1 = a_{t+1,t+1} - (a_{t+1,t} / sqrt(d_t))^2 is
+// formed from broadcasts (v_readlane, into scalar registers) of the pre-update entries, which
+// are issued before the pivot's reciprocal square root is known; the column's rank-1 update
+// of the later own columns takes the same unscaled broadcasts.
+// A non-positive (or NaN) pivot records row c0 + j + 1 in *status (the first one wins) and the
 // factorization continues with NaN, as the reference's dpotrf reports INFO.
-// The factor and its 16x16 diagonal-block inverses are also written to Lb: while wavefront
-// 0 factors sub-panel J + 1, wavefront 1 inverts block J and wavefronts 2-3 copy the columns
-// of sub-panel J (final once it is factored), so only sub-panel 3 is left at the end.
 __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, int* __restrict__ status,
                                               double* __restrict__ Lb)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int bad = -1;
-    if (wave == 0) diag_subpanel<0>(sD, lane, bad);
-    __syncthreads();
-    diag_update<0>(sD, lane, wave);
-    __syncthreads();
-    if (wave == 0) diag_subpanel<16>(sD, lane, bad);
-    else if (wave == 1) publish_inv(sD, Lb, 0, lane);
-    else publish_cols(sD, Lb, 0, tid - 128);
-    __syncthreads();
-    diag_update<16>(sD, lane, wave);
-    __syncthreads();
-    if (wave == 0) diag_subpanel<32>(sD, lane, bad);
-    else if (wave == 1) publish_inv(sD, Lb, 1, lane);
-    else publish_cols(sD, Lb, 1, tid - 128);
-    __syncthreads();
-    diag_update<32>(sD, lane, wave);
-    __syncthreads();
-    if (wave == 0) diag_subpanel<48>(sD, lane, bad);
-    else if (wave == 1) publish_inv(sD, Lb, 2, lane);
-    else publish_cols(sD, Lb, 2, tid - 128);
-    __syncthreads();
-    if (wave == 0) publish_inv(sD, Lb, 3, lane);
-    else if (wave >= 2) publish_cols(sD, Lb, 3, tid - 128);
-    if (wave == 0 && lane == 0 && bad >= 0) atomicCAS(status, 0, c0 + bad + 1);
+    __shared__ int s_done;   // columns published so far
+    __shared__ int s_bad;    // first non-positive pivot (64: none); -1: a wait timed out
+    __shared__ double s_rs[CHOL_NB];   // 1 / l_cc of every column
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int cw = 16 * w;   // first own column
+    __syncthreads();   // the block in sD may have been written by one wavefront alone
+    double p[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) p[u] = (lane >= cw + u) ? sD[lane * DG_LD + cw + u] : 0.0;
+    if (tid == 0) {
+        s_done = 0;
+        s_bad = CHOL_NB;
+    }
+    __syncthreads();   // every block entry is in registers: sD is reused for L
+    double* L = sD;
+    bool ok = true;
+    // updates by the earlier column groups, two columns at a time as they are published
+    // (cw is a multiple of 16, so a pair never straddles two groups)
+    int avail = 0;
+    for (int c = 0; c < cw; c += 2) {
+        if (avail < c + 2) {
+            avail = wait_lds_count(&s_done, c + 2);
+            if (avail < 0) {
+                ok = false;
+                break;
+            }
+        }
+        const double x0 = L[c * CHOL_NB + lane], x1 = L[(c + 1) * CHOL_NB + lane];
+        const double2* b0 = reinterpret_cast<const double2*>(L + c * CHOL_NB + cw);
+        const double2* b1 = reinterpret_cast<const double2*>(L + (c + 1) * CHOL_NB + cw);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double2 v0 = b0[u], v1 = b1[u];
+            p[2 * u] = __builtin_fma(-x1, v1.x, __builtin_fma(-x0, v0.x, p[2 * u]));
+            p[2 * u + 1] = __builtin_fma(-x1, v1.y, __builtin_fma(-x0, v0.y, p[2 * u + 1]));
+        }
+    }
+    int bad = CHOL_NB;
+    if (ok) {
+        double piv = readlane_f64(p[0], cw);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int c = cw + t;
+            if (!(piv > 0.0) && bad == CHOL_NB) bad = c;
+            // unscaled entries a_{cw+u, c} of the later own rows, final since column c - 1:
+            // these broadcasts do not wait for the pivot's reciprocal square root
+            double a[16];
+#pragma unroll
+            for (int u = t + 1; u < 16; ++u) a[u] = readlane_f64(p[t], cw + u);
+            const double rs = rsqrt_f64(piv);
+            double nxt = 0.0;
+            if (t < 15) {
+                const double le = a[t + 1] * rs;   // l_{c+1, c}
+                nxt = __builtin_fma(-le, le, readlane_f64(p[t + 1], c + 1));
+            }
+            const double lp = p[t] * rs, ld = piv * rs;
+            double l = (lane == c) ? ld : lp;
+            l = (lane < c) ? 0.0 : l;
+            p[t] = l;
+            L[c * CHOL_NB + lane] = l;
+            if (lane == 0) s_rs[c] = rs;
+            // p[u] -= l (a_u rs)
+            const double lr2 = l * rs;
+#pragma unroll
+            for (int u = t + 1; u < 16; ++u) p[u] = __builtin_fma(-lr2, a[u], p[u]);
+            // columns are published in pairs (the consumers take them in pairs)
+            if (t & 1) __hip_atomic_store(&s_done, c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            piv = nxt;
+        }
+        // own columns into Lb (zeros above the diagonal)
+#pragma unroll
+        for (int u = 0; u < 16; ++u) Lb[(cw + u) * CHOL_NB + lane] = (lane >= cw + u) ? p[u] : 0.0;
+        __builtin_amdgcn_sched_barrier(0);
+        // inverse of the own 16x16 diagonal sub-block: lane q < 16 solves L_ww y = e_q column by
+        // column (right-looking: the updates of one column are independent), with 1 / l_cc the
+        // pivot's reciprocal square root
+        if (lane < 16) {
+            double y[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) y[r] = (r == lane) ? 1.0 : 0.0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                y[t] *= s_rs[cw + t];
+#pragma unroll
+                for (int r = t + 1; r < 16; ++r) y[r] = __builtin_fma(-L[(cw + t) * CHOL_NB + cw + r], y[t], y[r]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            double* inv = Lb + CHOL_NB * CHOL_NB + 256 * w;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) inv[lane * 16 + r] = y[r];
+        }
+    }
+    if (lane == 0) {
+        if (!ok) atomicExch(&s_bad, -1);
+        else if (bad < CHOL_NB) atomicMin(&s_bad, bad);
+    }
+    __syncthreads();   // L complete in LDS, every wavefront's Lb stores issued
+    if (tid == 0) {
+        if (s_bad < 0) atomicCAS(status, 0, -1);
+        else if (s_bad < CHOL_NB) atomicCAS(status, 0, c0 + s_bad + 1);
+    }
 }
 
-// L (lower part) of the factored block into A, in T.
+// L (lower part) of the factored block, L[c * 64 + r] in LDS, into A, in T.
 template <typename T>
-__device__ __forceinline__ void store_diag64(const double* __restrict__ sD, T* __restrict__ A,
+__device__ __forceinline__ void store_diag64(const double* __restrict__ L, T* __restrict__ A,
                                              long long ld, int c0)
 {
     for (int e = threadIdx.x; e < CHOL_NB * CHOL_NB; e += 256) {
         const int r = e % CHOL_NB, c = e / CHOL_NB;
-        if (r >= c) A[(long long)(c0 + c) * ld + c0 + r] = (T)sD[r * DG_LD + c];
+        if (r >= c) A[(long long)(c0 + c) * ld + c0 + r] = (T)L[c * CHOL_NB + r];
     }
 }
 
@@ -232,8 +223,7 @@ __device__ __forceinline__ void chol_diag_body(T* __restrict__ A, long long ld, 
         const int e = threadIdx.x + 256 * i, r = e % CHOL_NB, c = e / CHOL_NB;
         sD[r * DG_LD + c] = v[i];
     }
-    __syncthreads();
-    factor_diag64(sD, c0, status, Lb);
+    factor_diag64(sD, c0, status, Lb);   // begins with a barrier
     store_diag64<T>(sD, A, ld, c0);
 }
 
@@ -572,39 +562,22 @@ __device__ __forceinline__ void split_bf16(float x, unsigned& hi, unsigned& lo)
     lo = bf16_bits(x - __builtin_bit_cast(float, hi << 16));
 }
 
-// Three waves per SIMD: the accumulators live in VGPRs next to the fused TRSM's registers.
+// acc += A[ri0 .. ri0+127, kb .. kb+K) A[rj0 .. rj0+127, kb .. kb+K)^T for one 128x128 tile on
+// MFMA (rows at or beyond n_pad read as zero).  Workgroup 256 = 4 waves in 2x2, each 64x64 =
+// 2x2 blocks of 32x32; acc[a][b] is the 32x32 block (a, b) of the wave in the MFMA result
+// layout.  The operand stages use smem (SY_SMEM bytes).
 template <bool SPLIT>
-__device__ __forceinline__ void syrk_f32_body(float* __restrict__ A, long long ld, int kb, int K, int t0,
-                                              int t1, int n_pad, double* __restrict__ Lb_next,
-                                              int* __restrict__ flag_next, int* __restrict__ status,
-                                              int bid)
+__device__ __forceinline__ void tile_acc_f32(const float* __restrict__ A, long long ld, int kb, int K, int ri0,
+                                             int rj0, int n_pad, unsigned char* smem, floatx16 (&acc)[2][2])
 {
-    __shared__ __align__(16) unsigned char smem[SY_SMEM];
     float (*sA)[SY_TILE + 4] = reinterpret_cast<float (*)[SY_TILE + 4]>(smem);
     float (*sB)[SY_TILE + 4] = reinterpret_cast<float (*)[SY_TILE + 4]>(smem + SY_KC32 * (SY_TILE + 4) * 4);
     // split stages: [0] A hi, [1] A lo, [2] B hi, [3] B lo; row-major (row, k pairs)
     unsigned (*sH)[SY_TILE][SY_HP] = reinterpret_cast<unsigned (*)[SY_TILE][SY_HP]>(smem);
-    double* sD = reinterpret_cast<double*>(smem);
-    // workgroup 0 holds tile (0, 0), whose top-left 64x64 block (wave 0) is the next
-    // panel's diagonal block
-    const bool diag = bid == 0;
-    int ti, tj;
-    trap_tile(bid, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
-    const int ri0 = t0 + ti * SY_TILE;
-    const int rj0 = t0 + tj * SY_TILE;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const int l32 = lane & 31, lh = lane >> 5;
-
-    floatx16 acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
-
     if constexpr (SPLIT) {
         // staging: thread t loads row t of the A rows (t < 128) or of the B rows, SY_KCH
         // consecutive k (one coalesced load per k across the wave), splits, and writes
@@ -696,11 +669,22 @@ __device__ __forceinline__ void syrk_f32_body(float* __restrict__ A, long long l
             }
         }
     }
-    if (diag) __syncthreads();   // operand stages are reused for the diagonal block
+}
+
+// C -= acc for the tile (ri0, rj0): columns at or beyond t1 and rows at or beyond n_pad are
+// left alone.  With to_lds (wavefront 0 of the workgroup whose tile holds the next diagonal
+// block) the updated block goes to sD (row pitch DG_LD, FP64) instead of A.
+__device__ __forceinline__ void tile_epilogue_f32(float* __restrict__ A, long long ld, int ri0, int rj0, int t1,
+                                                  int n_pad, const floatx16 (&acc)[2][2], bool to_lds,
+                                                  double* __restrict__ sD)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int l32 = lane & 31, lh = lane >> 5;
     // C -= acc ; lane owns column l32 and rows 8*(r/4) + 4*lh + (0..3) of each 32x32 block.
     // All 16 loads of C are issued before any store (the stores could alias them as far as
     // the compiler knows, which would serialise every load behind the previous store).
-    const bool to_lds = diag && wave == 0;
     float4 cv[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -742,6 +726,34 @@ __device__ __forceinline__ void syrk_f32_body(float* __restrict__ A, long long l
                 }
             }
         }
+}
+
+// Three waves per SIMD: the accumulators live in VGPRs next to the fused TRSM's registers.
+template <bool SPLIT>
+__device__ __forceinline__ void syrk_f32_body(float* __restrict__ A, long long ld, int kb, int K, int t0,
+                                              int t1, int n_pad, double* __restrict__ Lb_next,
+                                              int* __restrict__ flag_next, int* __restrict__ status,
+                                              int bid)
+{
+    __shared__ __align__(16) unsigned char smem[SY_SMEM];
+    double* sD = reinterpret_cast<double*>(smem);
+    // workgroup 0 holds tile (0, 0), whose top-left 64x64 block (wave 0) is the next
+    // panel's diagonal block
+    const bool diag = bid == 0;
+    int ti, tj;
+    trap_tile(bid, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
+    const int ri0 = t0 + ti * SY_TILE;
+    const int rj0 = t0 + tj * SY_TILE;
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+    tile_acc_f32<SPLIT>(A, ld, kb, K, ri0, rj0, n_pad, smem, acc);
+    if (diag) __syncthreads();   // operand stages are reused for the diagonal block
+    tile_epilogue_f32(A, ld, ri0, rj0, t1, n_pad, acc, diag && (threadIdx.x >> 6) == 0, sD);
     // wavefront 0 wrote the whole diagonal block to LDS itself and starts factoring it while
     // the other wavefronts finish their stores (the factor's first barrier joins them)
     if (diag) factor_diag64(sD, t0, status, Lb_next);
@@ -763,35 +775,19 @@ syrk_f32_kernel(float* __restrict__ A, long long ld, int kb, int K, int t0, int 
 // lane owns 4 consecutive rows of one column (two 16-byte stores).
 constexpr int SY_KC64 = 16;
 
-__device__ __forceinline__ void syrk_f64_body(double* __restrict__ A, long long ld, int kb, int K, int t0,
-                                              int t1, int n_pad, double* __restrict__ Lb_next,
-                                              int* __restrict__ flag_next, int* __restrict__ status,
-                                              int bid)
+// acc += A[ri0.., kb .. kb+K) A[rj0.., kb .. kb+K)^T for one 128x128 tile on FP64 MFMA:
+// each wave 64x64 = 4x4 blocks of 16x16, A-operand rows fed in the order perm(m) (above).
+__device__ __forceinline__ void tile_acc_f64(const double* __restrict__ A, long long ld, int kb, int K, int ri0,
+                                             int rj0, int n_pad, unsigned char* smem, doublex4 (&acc)[4][4])
 {
     static_assert(2 * SY_KC64 * (SY_TILE + 2) * 8 <= SY_SMEM, "SYRK LDS");
-    __shared__ __align__(16) unsigned char smem[SY_SMEM];
     double (*sA)[SY_TILE + 2] = reinterpret_cast<double (*)[SY_TILE + 2]>(smem);
     double (*sB)[SY_TILE + 2] = reinterpret_cast<double (*)[SY_TILE + 2]>(smem + SY_KC64 * (SY_TILE + 2) * 8);
-    double* sD = reinterpret_cast<double*>(smem);
-    const bool diag = bid == 0;
-    int ti, tj;
-    trap_tile(bid, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
-    const int ri0 = t0 + ti * SY_TILE;
-    const int rj0 = t0 + tj * SY_TILE;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const int l16 = lane & 15, lq = lane >> 4;
     const int prow = 4 * (l16 & 3) + (l16 >> 2);
-
-    doublex4 acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.0;
-
     for (int k0 = 0; k0 < K; k0 += SY_KC64) {
         __syncthreads();
         for (int idx = tid; idx < SY_KC64 * SY_TILE / 2; idx += 256) {
@@ -818,10 +814,19 @@ __device__ __forceinline__ void syrk_f64_body(double* __restrict__ A, long long 
                 for (int b = 0; b < 4; ++b) acc[a][b] = mfma_f64_16x16x4(av[a], bv[b], acc[a][b]);
         }
     }
-    if (diag) __syncthreads();   // operand stages are reused for the diagonal block
+}
+
+// C -= acc for the tile (ri0, rj0) (columns < t1, rows < n_pad), or into sD (to_lds).
+__device__ __forceinline__ void tile_epilogue_f64(double* __restrict__ A, long long ld, int ri0, int rj0, int t1,
+                                                  int n_pad, const doublex4 (&acc)[4][4], bool to_lds,
+                                                  double* __restrict__ sD)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int l16 = lane & 15, lq = lane >> 4;
     // lane owns column l16 and rows 4*lq + (0..3) of each 16x16 block; every load of C is
     // issued before the stores
-    const bool to_lds = diag && wave == 0;
     double2 cv[4][4][2];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -857,6 +862,30 @@ __device__ __forceinline__ void syrk_f64_body(double* __restrict__ A, long long 
                 *reinterpret_cast<double2*>(cp + 2) = c1v;
             }
         }
+}
+
+__device__ __forceinline__ void syrk_f64_body(double* __restrict__ A, long long ld, int kb, int K, int t0,
+                                              int t1, int n_pad, double* __restrict__ Lb_next,
+                                              int* __restrict__ flag_next, int* __restrict__ status,
+                                              int bid)
+{
+    __shared__ __align__(16) unsigned char smem[SY_SMEM];
+    double* sD = reinterpret_cast<double*>(smem);
+    const bool diag = bid == 0;
+    int ti, tj;
+    trap_tile(bid, ceil_div_dev(n_pad - t0, SY_TILE), ti, tj);
+    const int ri0 = t0 + ti * SY_TILE;
+    const int rj0 = t0 + tj * SY_TILE;
+    doublex4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.0;
+    tile_acc_f64(A, ld, kb, K, ri0, rj0, n_pad, smem, acc);
+    if (diag) __syncthreads();   // operand stages are reused for the diagonal block
+    tile_epilogue_f64(A, ld, ri0, rj0, t1, n_pad, acc, diag && (threadIdx.x >> 6) == 0, sD);
     // wavefront 0 wrote the whole diagonal block to LDS itself and starts factoring it while
     // the other wavefronts finish their stores (the factor's first barrier joins them)
     if (diag) factor_diag64(sD, t0, status, Lb_next);
@@ -953,8 +982,293 @@ int launch_update(T* d_A, long long ld, int kb, int K, int t0, int t1, int n_pad
     return 0;
 }
 
-// One launch per panel: the update kernel of panel k also factors diagonal block k + 1
-// (workgroup 0) and solves panel k + 1 (its tile column 0, after workgroup 0's flag).
+// ------------------------------------------------------------------ factorization by epochs
+// One launch per outer panel P (columns P0 .. P1, CHOL_OUTER wide) for the large single-system
+// matrices.  Each launch runs three things side by side, which the per-panel launches above
+// run one after another:
+//   the panel factorization of P (its 64-column panels k: diagonal block, triangular solve of
+//   the rows below, right-looking update of P's later columns), on the row strips of P;
+//   the lookahead: the next outer panel's columns (P1 .. Q1) receive the previous outer
+//   panel's update and then each panel k's as soon as its rows are solved, so they are final
+//   when the launch ends and the next launch starts its diagonal block at once;
+//   the previous outer panel's update of every column beyond Q1.
+// Invariant at the start of epoch P: columns P0 .. P1 carry the updates of every column below
+// P0; columns beyond P1 those of every column below P0 - CHOL_OUTER.
+// Roles by workgroup index (the chain's workgroups have the lowest indices, so they are
+// dispatched first and all resident; every wait is for a lower index):
+//   b < R            row strip s = b (rows P0 + 128 s ..); for each panel k of P, in order:
+//                    the diagonal block if it lies in the strip (flag D_k) or a wait for D_k;
+//                    the solve of the strip's rows below block k (flag T_k,strip); the strip's
+//                    rows of P's later columns updated with panel k, after the flags T_k of the
+//                    strips that hold those columns' rows.
+//   R <= b < R + NL  a 128 x 128 tile of the lookahead columns: K = CHOL_OUTER from the previous
+//                    outer panel, then K = 64 from each panel k after T_k of its two strips,
+//                    accumulated in registers and applied to C once.
+//   later b          a tile beyond Q1: the previous outer panel's update (K = CHOL_OUTER).
+struct EpochArgs {
+    int P0, P1, Q1;   // this outer panel, the end of the lookahead columns
+    int prevP0;       // the previous outer panel's first column (< 0 in the first epoch)
+    int R, NL;        // row strips, lookahead tiles
+};
+
+// Flags of the epoch schedule, after the diagonal-block flags in the workspace: T_k,g for
+// panel k and global row strip g (rows 128 g ..), n_pad / 64 x n_pad / 128 of them.
+__device__ __forceinline__ int* epoch_tflags(double* Lbuf, int n_pad)
+{
+    return chol_flags(Lbuf, n_pad) + n_pad / CHOL_NB;
+}
+
+// The workgroup's stores so far made visible at agent scope, then *flag set.
+__device__ __forceinline__ void publish_flag(int* flag)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// All threads: wait until *flag is set (thread 0 polls, bounded); false after a timeout, which
+// is reported through status.
+__device__ __forceinline__ bool wg_wait_flag(int* flag, int* status)
+{
+    __shared__ int s_flag_ok;
+    if (threadIdx.x == 0) {
+        s_flag_ok = wait_flag(flag) ? 1 : 0;
+        if (!s_flag_ok) atomicCAS(status, 0, -1);
+    }
+    __syncthreads();
+    return s_flag_ok != 0;
+}
+
+// C[ri0.., rj0..] -= A[ri0.., kb .. kb+K) A[rj0.., kb .. kb+K)^T for one 128 x 128 tile
+// (columns < t1, rows < n_pad).
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void epoch_tile_update(T* __restrict__ A, long long ld, int kb, int K, int ri0,
+                                                  int rj0, int t1, int n_pad, unsigned char* smem)
+{
+    if constexpr (sizeof(T) == 4) {
+        floatx16 acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+        tile_acc_f32<SPLIT>(A, ld, kb, K, ri0, rj0, n_pad, smem, acc);
+        tile_epilogue_f32(A, ld, ri0, rj0, t1, n_pad, acc, false, nullptr);
+    } else {
+        doublex4 acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.0;
+        tile_acc_f64(A, ld, kb, K, ri0, rj0, n_pad, smem, acc);
+        tile_epilogue_f64(A, ld, ri0, rj0, t1, n_pad, acc, false, nullptr);
+    }
+}
+
+// Row strip s of outer panel P (see above).
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int n_pad, double* __restrict__ Lbuf,
+                                            int* __restrict__ status, const EpochArgs& e, int s,
+                                            unsigned char* smem)
+{
+    int* dflags = chol_flags(Lbuf, n_pad);
+    int* tflags = epoch_tflags(Lbuf, n_pad);
+    const int gstrips = n_pad / SY_TILE;
+    const int r0 = e.P0 + SY_TILE * s, r1 = min(r0 + SY_TILE, n_pad);
+    const int g = r0 / SY_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double* sD = reinterpret_cast<double*>(smem);
+    double* sLT = sD;
+    double* sInv = sLT + CHOL_NB * CHOL_NB;
+    double* sc = sInv + 4 * 256 + wave * 16 * SC_PITCH;
+    for (int c0 = e.P0; c0 < e.P1 && c0 < r1; c0 += CHOL_NB) {
+        const int k = c0 / CHOL_NB;
+        double* Lb = Lbuf + (long long)k * LB_STRIDE;
+        int rs = r0;   // first row of the strip solved with panel k
+        __syncthreads();   // smem is free (the previous panel's solve and updates)
+        if (c0 >= r0) {
+            // diagonal block k lies in this strip and carries every update (this strip's own)
+            for (int e2 = tid; e2 < CHOL_NB * CHOL_NB; e2 += 256) {
+                const int r = e2 % CHOL_NB, c = e2 / CHOL_NB;
+                sD[r * DG_LD + c] = (r >= c) ? (double)A[(long long)(c0 + c) * ld + c0 + r] : 0.0;
+            }
+            factor_diag64(sD, c0, status, Lb);   // begins with a barrier
+            store_diag64<T>(sD, A, ld, c0);
+            publish_flag(dflags + k);
+            rs = c0 + CHOL_NB;
+        } else if (!wg_wait_flag(dflags + k, status)) {
+            return;
+        }
+        if (rs < r1) {
+            // rows rs .. r1 of panel k, two 16-row groups per wavefront; the rows are loaded
+            // before L_kk (the diagonal case still reads sD in store_diag64 before the barrier
+            // that precedes load_lkk)
+            const int g0 = rs + 16 * wave, g1 = g0 + 64;
+            Trsm16 x0, x1;
+            if (g0 < r1) trsm16_load<T>(A, ld, c0, g0, r1, lane, x0);
+            if (g1 < r1) trsm16_load<T>(A, ld, c0, g1, r1, lane, x1);
+            else x1 = Trsm16{};
+            __syncthreads();
+            load_lkk(Lb, sLT, sInv);   // ends with a barrier
+            if (g1 < r1) trsm16x2_solve_store<T>(A, ld, c0, g0, g1, r1, lane, x0, x1, sLT, sInv, sc);
+            else if (g0 < r1) trsm16_solve_store<T>(A, ld, c0, g0, r1, lane, x0, sLT, sInv, sc);
+        }
+        publish_flag(tflags + (long long)k * gstrips + g);
+        // the strip's rows of P's later columns cb .. ce, updated with panel k (columns at or
+        // beyond r1 lie above the strip's rows)
+        const int cb = c0 + CHOL_NB, ce = min(e.P1, r1);
+        if (cb < ce) {
+            for (int gg = cb / SY_TILE; gg * SY_TILE < ce; ++gg)
+                if (gg != g && !wg_wait_flag(tflags + (long long)k * gstrips + gg, status)) return;
+            for (int rj0 = cb; rj0 < ce; rj0 += SY_TILE) {
+                __syncthreads();   // smem is free (the solve's operands, the previous tile's stages)
+                epoch_tile_update<T, SPLIT>(A, ld, c0, CHOL_NB, r0, rj0, ce, n_pad, smem);
+            }
+        }
+    }
+}
+
+// Lookahead tile x (rows from P1, columns P1 .. Q1).
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void epoch_lookahead(T* __restrict__ A, long long ld, int n_pad,
+                                                double* __restrict__ Lbuf, int* __restrict__ status,
+                                                const EpochArgs& e, int x, unsigned char* smem)
+{
+    int* tflags = epoch_tflags(Lbuf, n_pad);
+    const int gstrips = n_pad / SY_TILE;
+    int ti, tj;
+    trap_tile(x, ceil_div_dev(n_pad - e.P1, SY_TILE), ti, tj);
+    const int ri0 = e.P1 + SY_TILE * ti, rj0 = e.P1 + SY_TILE * tj;
+    const int gi = ri0 / SY_TILE, gj = rj0 / SY_TILE;
+    if constexpr (sizeof(T) == 4) {
+        floatx16 acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+        if (e.prevP0 >= 0) tile_acc_f32<SPLIT>(A, ld, e.prevP0, e.P0 - e.prevP0, ri0, rj0, n_pad, smem, acc);
+        for (int c0 = e.P0; c0 < e.P1; c0 += CHOL_NB) {
+            const int k = c0 / CHOL_NB;
+            if (!wg_wait_flag(tflags + (long long)k * gstrips + gi, status)) return;
+            if (gj != gi && !wg_wait_flag(tflags + (long long)k * gstrips + gj, status)) return;
+            tile_acc_f32<SPLIT>(A, ld, c0, CHOL_NB, ri0, rj0, n_pad, smem, acc);
+        }
+        tile_epilogue_f32(A, ld, ri0, rj0, e.Q1, n_pad, acc, false, nullptr);
+    } else {
+        doublex4 acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.0;
+        if (e.prevP0 >= 0) tile_acc_f64(A, ld, e.prevP0, e.P0 - e.prevP0, ri0, rj0, n_pad, smem, acc);
+        for (int c0 = e.P0; c0 < e.P1; c0 += CHOL_NB) {
+            const int k = c0 / CHOL_NB;
+            if (!wg_wait_flag(tflags + (long long)k * gstrips + gi, status)) return;
+            if (gj != gi && !wg_wait_flag(tflags + (long long)k * gstrips + gj, status)) return;
+            tile_acc_f64(A, ld, c0, CHOL_NB, ri0, rj0, n_pad, smem, acc);
+        }
+        tile_epilogue_f64(A, ld, ri0, rj0, e.Q1, n_pad, acc, false, nullptr);
+    }
+}
+
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void chol_epoch_body(T* __restrict__ A, long long ld, int n_pad,
+                                                double* __restrict__ Lbuf, int* __restrict__ status,
+                                                EpochArgs e)
+{
+    __shared__ __align__(16) unsigned char smem[SY_SMEM];
+    const int b = blockIdx.x;
+    if (b < e.R) {
+        epoch_strip<T, SPLIT>(A, ld, n_pad, Lbuf, status, e, b, smem);
+    } else if (b < e.R + e.NL) {
+        epoch_lookahead<T, SPLIT>(A, ld, n_pad, Lbuf, status, e, b - e.R, smem);
+    } else {
+        int ti, tj;
+        trap_tile(b - e.R - e.NL, ceil_div_dev(n_pad - e.Q1, SY_TILE), ti, tj);
+        epoch_tile_update<T, SPLIT>(A, ld, e.prevP0, e.P0 - e.prevP0, e.Q1 + SY_TILE * ti, e.Q1 + SY_TILE * tj,
+                                    n_pad, n_pad, smem);
+    }
+}
+
+// Two waves per SIMD: the row strips hold the diagonal factor's, the solve's and the tile
+// update's registers in one kernel.
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+chol_epoch_f32_kernel(float* __restrict__ A, long long ld, int n_pad, double* __restrict__ Lbuf,
+                      int* __restrict__ status, EpochArgs e)
+{
+    chol_epoch_body<float, SPLIT>(A, ld, n_pad, Lbuf, status, e);
+}
+
+__global__ void __launch_bounds__(256)
+chol_epoch_f64_kernel(double* __restrict__ A, long long ld, int n_pad, double* __restrict__ Lbuf,
+                      int* __restrict__ status, EpochArgs e)
+{
+    chol_epoch_body<double, false>(A, ld, n_pad, Lbuf, status, e);
+}
+
+// Clears the diagonal-block and solve flags of one factorization (a kernel rather than a
+// memset node, so a captured factorization replays it correctly).
+__global__ void chol_epoch_clear_kernel(int* __restrict__ flags, int count)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) flags[i] = 0;
+}
+
+// Matrices at least this large factor by epochs; smaller ones (one or two outer panels, where
+// nothing is left to overlap) by the per-panel launches.
+#ifndef CHOL_EPOCH_MIN
+#define CHOL_EPOCH_MIN 1024
+#endif
+
+template <typename T>
+int chol_factor_epochs(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, int syrk_mode,
+                       hipStream_t st, PhaseRecorder* rec)
+{
+    const int nblk = n_pad / CHOL_NB, gstrips = n_pad / SY_TILE;
+    int* d_flags = reinterpret_cast<int*>(d_Lbuf + (long long)nblk * LB_STRIDE);
+    const int nflags = nblk + nblk * gstrips;
+    if (rec) rec->begin(PH_PANEL, st);
+    hipLaunchKernelGGL(chol_epoch_clear_kernel, dim3(ceil_div(nflags, 256)), dim3(256), 0, st, d_flags, nflags);
+    BDH_LAUNCH_CHECK();
+    if (rec) rec->end(PH_PANEL, st);
+    for (int P0 = 0; P0 < n_pad; P0 += CHOL_OUTER) {
+        EpochArgs e;
+        e.P0 = P0;
+        e.P1 = std::min(P0 + CHOL_OUTER, n_pad);
+        e.Q1 = std::min(e.P1 + CHOL_OUTER, n_pad);
+        e.prevP0 = P0 - CHOL_OUTER;
+        e.R = (n_pad - P0) / SY_TILE;
+        e.NL = e.Q1 > e.P1 ? update_tiles(n_pad, e.P1, e.Q1) : 0;
+        const int nbulk = (e.prevP0 >= 0 && e.Q1 < n_pad) ? update_tiles(n_pad, e.Q1, n_pad) : 0;
+        const dim3 grid(e.R + e.NL + nbulk);
+        if (rec) rec->begin(PH_SYRK, st);
+        if constexpr (sizeof(T) == 4) {
+            if (syrk_mode == SYRK_BF16X3)
+                hipLaunchKernelGGL(chol_epoch_f32_kernel<true>, grid, dim3(256), 0, st, reinterpret_cast<float*>(d_A),
+                                   ld, n_pad, d_Lbuf, d_status, e);
+            else
+                hipLaunchKernelGGL(chol_epoch_f32_kernel<false>, grid, dim3(256), 0, st, reinterpret_cast<float*>(d_A),
+                                   ld, n_pad, d_Lbuf, d_status, e);
+        } else {
+            hipLaunchKernelGGL(chol_epoch_f64_kernel, grid, dim3(256), 0, st, reinterpret_cast<double*>(d_A), ld,
+                               n_pad, d_Lbuf, d_status, e);
+        }
+        BDH_LAUNCH_CHECK();
+        if (rec) rec->end(PH_SYRK, st);
+    }
+    return 0;
+}
+
+// Matrices of CHOL_EPOCH_MIN and more factor by epochs (above).  Smaller ones: one launch per
+// panel, the update kernel of panel k also factoring diagonal block k + 1 (workgroup 0) and
+// solving panel k + 1 (its tile column 0, after workgroup 0's flag).
 template <typename T>
 int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, int syrk_mode,
                 hipStream_t st, PhaseRecorder* rec)
@@ -962,6 +1276,7 @@ int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, 
     BDH_CHECK(n_pad % PAD_TO == 0, "chol_factor: n_pad must be a multiple of 128");
     BDH_CHECK(ld >= n_pad && ld % 4 == 0, "chol_factor: bad leading dimension");
     BDH_CHECK(syrk_mode != SYRK_BF16X3 || sizeof(T) == 4, "chol_factor: split SYRK needs FP32 storage");
+    if (n_pad >= CHOL_EPOCH_MIN) return chol_factor_epochs<T>(d_A, ld, n_pad, d_Lbuf, d_status, syrk_mode, st, rec);
     constexpr long long LB = LB_STRIDE;
     const int nblk = n_pad / CHOL_NB;
     int* d_flags = reinterpret_cast<int*>(d_Lbuf + nblk * LB);
@@ -989,8 +1304,10 @@ int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, 
 
 size_t chol_lbuf_doubles(int n_pad)
 {
-    const size_t nblk = n_pad / CHOL_NB;
-    return nblk * LB_STRIDE + (nblk + 1) / 2;   // FP64 blocks with inverses, then int flags
+    const size_t nblk = n_pad / CHOL_NB, gstrips = n_pad / SY_TILE;
+    // FP64 blocks with inverses, then the int flags: one per diagonal block, then the epoch
+    // schedule's solve flags (one per panel and row strip)
+    return nblk * LB_STRIDE + (nblk + nblk * gstrips + 1) / 2;
 }
 
 // ------------------------------------------------------------------ Y = L X (partials)
