@@ -73,8 +73,10 @@ __device__ __forceinline__ int wait_lds_count(int* cnt, int need)
 
 // In-place FP64 Cholesky of the 64x64 block held in LDS, called by all 256 threads.
 //   in : sD[r * DG_LD + c], lower part (r >= c) read
-//   out: the same memory as L[c * 64 + r] (column-major, pitch 64, zeros above the diagonal)
-//        and Lb (L_kk and the inverses of its four 16x16 diagonal sub-blocks)
+//   out: the same memory as L[c * 64 + r] (column-major, pitch 64, zeros above the diagonal),
+//        the inverses of its four 16x16 diagonal sub-blocks after it (sD + 4096 + 256 J, the
+//        layout load_lkk leaves: sD is then sLT / sInv of the solve), and the same in Lb.
+//        sD must hold CHOL_NB * CHOL_NB + 4 * 256 doubles.
 // The four wavefronts form a pipeline over column groups: wavefront w owns columns
 // 16w .. 16w+15 (lane = row, one register per column).  It applies the rank-1 updates of the
 // 16w earlier columns as the wavefronts before it publish them (column in LDS, then an LDS
@@ -178,8 +180,12 @@ __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, i
                 __builtin_amdgcn_sched_barrier(0);
             }
             double* inv = Lb + CHOL_NB * CHOL_NB + 256 * w;
+            double* sinv = sD + CHOL_NB * CHOL_NB + 256 * w;   // the solve's sInv layout
 #pragma unroll
-            for (int r = 0; r < 16; ++r) inv[lane * 16 + r] = y[r];
+            for (int r = 0; r < 16; ++r) {
+                inv[lane * 16 + r] = y[r];
+                sinv[lane * 16 + r] = y[r];
+            }
         }
     }
     if (lane == 0) {
@@ -210,7 +216,8 @@ __device__ __forceinline__ void chol_diag_body(T* __restrict__ A, long long ld, 
                                                double* __restrict__ Lb, int* __restrict__ flags,
                                                int nflags, int* __restrict__ status)
 {
-    __shared__ double sD[CHOL_NB * DG_LD];
+    __shared__ double sD[CHOL_NB * CHOL_NB + 4 * 256];
+    static_assert(CHOL_NB * CHOL_NB + 4 * 256 >= CHOL_NB * DG_LD, "diagonal block LDS");
     for (int k = threadIdx.x; k < nflags; k += 256) flags[k] = 0;
     double v[CHOL_NB * CHOL_NB / 256];
 #pragma unroll
@@ -641,20 +648,34 @@ __device__ __forceinline__ void tile_acc_f32(const float* __restrict__ A, long l
             }
         }
     } else {
-        for (int k0 = 0; k0 < K; k0 += SY_KC32) {
-            __syncthreads();
-            // 32 columns x 128 rows, float4 per thread-iteration
-            for (int idx = tid; idx < SY_KC32 * SY_TILE / 4; idx += 256) {
+        // register double buffering: the next chunk's global loads are issued before this
+        // chunk's MFMAs (each thread: 4 float4 of the A rows and 4 of the B rows per chunk)
+        constexpr int NV = SY_KC32 * SY_TILE / 4 / 256;
+        float4 va[NV], vb[NV];
+        auto load_chunk = [&](int k0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int idx = tid + 256 * i;
                 const int k = idx / (SY_TILE / 4);
                 const int r4 = (idx % (SY_TILE / 4)) * 4;
                 const float* col = A + (long long)(kb + k0 + k) * ld;
-                float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;
-                if (ri0 + r4 < n_pad) va = *reinterpret_cast<const float4*>(col + ri0 + r4);
-                if (rj0 + r4 < n_pad) vb = *reinterpret_cast<const float4*>(col + rj0 + r4);
-                sA[k][r4 + 0] = va.x; sA[k][r4 + 1] = va.y; sA[k][r4 + 2] = va.z; sA[k][r4 + 3] = va.w;
-                sB[k][r4 + 0] = vb.x; sB[k][r4 + 1] = vb.y; sB[k][r4 + 2] = vb.z; sB[k][r4 + 3] = vb.w;
+                va[i] = (ri0 + r4 < n_pad) ? *reinterpret_cast<const float4*>(col + ri0 + r4) : make_float4(0.f, 0.f, 0.f, 0.f);
+                vb[i] = (rj0 + r4 < n_pad) ? *reinterpret_cast<const float4*>(col + rj0 + r4) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        };
+        load_chunk(0);
+        for (int k0 = 0; k0 < K; k0 += SY_KC32) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int idx = tid + 256 * i;
+                const int k = idx / (SY_TILE / 4);
+                const int r4 = (idx % (SY_TILE / 4)) * 4;
+                *reinterpret_cast<float4*>(&sA[k][r4]) = va[i];
+                *reinterpret_cast<float4*>(&sB[k][r4]) = vb[i];
             }
             __syncthreads();
+            if (k0 + SY_KC32 < K) load_chunk(k0 + SY_KC32);
 #pragma unroll 4
             for (int k = 0; k < SY_KC32; k += 2) {
                 float av[2], bv[2];
@@ -999,8 +1020,9 @@ int launch_update(T* d_A, long long ld, int kb, int K, int t0, int t1, int n_pad
 //   b < R            row strip s = b (rows P0 + 128 s ..); for each panel k of P, in order:
 //                    the diagonal block if it lies in the strip (flag D_k) or a wait for D_k;
 //                    the solve of the strip's rows below block k (flag T_k,strip); the strip's
-//                    rows of P's later columns updated with panel k, after the flags T_k of the
-//                    strips that hold those columns' rows.
+//                    rows of panel k + 1's columns updated with every panel of P so far
+//                    (left-looking inside P), after the flag T_k of the strip that holds those
+//                    columns' rows.
 //   R <= b < R + NL  a 128 x 128 tile of the lookahead columns: K = CHOL_OUTER from the previous
 //                    outer panel, then K = 64 from each panel k after T_k of its two strips,
 //                    accumulated in registers and applied to C once.
@@ -1068,7 +1090,92 @@ __device__ __forceinline__ void epoch_tile_update(T* __restrict__ A, long long l
     }
 }
 
-// Row strip s of outer panel P (see above).
+// sD[r * DG_LD + c] = A[c1 + r][c1 + c] - sum_k A[c1 + r][kb + k] A[c1 + c][kb + k] for the 64 x 64
+// diagonal block at c1 (the strict upper 32 x 32 quadrant is not written; the factor reads only
+// r >= c), on MFMA in T: three wavefronts take the quadrants (0, 0), (1, 0), (1, 1).
+template <typename T>
+__device__ __forceinline__ void diag_block_update(const T* __restrict__ A, long long ld, int c1, int kb, int K,
+                                                  double* __restrict__ sD)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (wave == 3) return;
+    const int qr = wave == 0 ? 0 : 1, qc = wave == 2 ? 1 : 0;
+    const int r0 = c1 + 32 * qr, q0 = c1 + 32 * qc;
+    if constexpr (sizeof(T) == 4) {
+        // v_mfma_f32_32x32x2_f32: lane l feeds row l % 32, k = l / 32; acc[r] holds row
+        // 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+        const int l32 = lane & 31, lh = lane >> 5;
+        floatx16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+        // 16 k per chunk; the next chunk's loads are issued before this chunk's MFMAs
+        float av[8], bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const T* col = A + (long long)(kb + 2 * j + lh) * ld;
+            av[j] = col[r0 + l32];
+            bv[j] = col[q0 + l32];
+        }
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            float an[8], bn[8];
+            if (k0 + 16 < K) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const T* col = A + (long long)(kb + k0 + 16 + 2 * j + lh) * ld;
+                    an[j] = col[r0 + l32];
+                    bn[j] = col[q0 + l32];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc = mfma_f32_32x32x2(av[j], bv[j], acc);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                av[j] = an[j];
+                bv[j] = bn[j];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = 8 * (r >> 2) + 4 * lh + (r & 3), col = l32;
+            sD[(32 * qr + row) * DG_LD + 32 * qc + col] =
+                (double)(A[(long long)(q0 + col) * ld + r0 + row] - acc[r]);
+        }
+    } else {
+        // v_mfma_f64_16x16x4_f64 on the 2 x 2 tiles of the quadrant: lane l feeds row l % 16,
+        // k = l / 16; acc[r] holds row 4 r + l / 16, column l % 16
+        const int l16 = lane & 15, lq = lane >> 4;
+        doublex4 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < K; k0 += 4) {
+            const T* col = A + (long long)(kb + k0 + lq) * ld;
+            const double a0 = col[r0 + l16], a1 = col[r0 + 16 + l16];
+            const double b0 = col[q0 + l16], b1 = col[q0 + 16 + l16];
+            acc[0][0] = mfma_f64_16x16x4(a0, b0, acc[0][0]);
+            acc[0][1] = mfma_f64_16x16x4(a0, b1, acc[0][1]);
+            acc[1][0] = mfma_f64_16x16x4(a1, b0, acc[1][0]);
+            acc[1][1] = mfma_f64_16x16x4(a1, b1, acc[1][1]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * i + 4 * r + lq, col = 16 * j + l16;
+                    sD[(32 * qr + row) * DG_LD + 32 * qc + col] =
+                        A[(long long)(q0 + col) * ld + r0 + row] - acc[i][j][r];
+                }
+    }
+}
+
+// Row strip s of outer panel P (see above).  The diagonal path stays in LDS: the strip that
+// factors block k solves the rows below it from the factor in LDS, and a strip that factors
+// block k + 1 next builds that block's update straight into LDS (a 64 x 64 update when block k
+// was its own, wavefront 0's quadrant of its 128 x 64 update after a hand-off), so only the
+// first block of an epoch is read from A.
 template <typename T, bool SPLIT>
 __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int n_pad, double* __restrict__ Lbuf,
                                             int* __restrict__ status, const EpochArgs& e, int s,
@@ -1084,48 +1191,106 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
     double* sLT = sD;
     double* sInv = sLT + CHOL_NB * CHOL_NB;
     double* sc = sInv + 4 * 256 + wave * 16 * SC_PITCH;
+    bool block_in_lds = false;   // sD holds the next diagonal block, fully updated
     for (int c0 = e.P0; c0 < e.P1 && c0 < r1; c0 += CHOL_NB) {
         const int k = c0 / CHOL_NB;
         double* Lb = Lbuf + (long long)k * LB_STRIDE;
-        int rs = r0;   // first row of the strip solved with panel k
-        __syncthreads();   // smem is free (the previous panel's solve and updates)
-        if (c0 >= r0) {
-            // diagonal block k lies in this strip and carries every update (this strip's own)
-            for (int e2 = tid; e2 < CHOL_NB * CHOL_NB; e2 += 256) {
-                const int r = e2 % CHOL_NB, c = e2 / CHOL_NB;
-                sD[r * DG_LD + c] = (r >= c) ? (double)A[(long long)(c0 + c) * ld + c0 + r] : 0.0;
+        const bool diag = c0 >= r0;   // block k lies in this strip
+        int rs = r0;                  // first row of the strip solved with panel k
+        if (diag) {
+            if (!block_in_lds) {
+                // the epoch's first block: it carries every update (earlier launches')
+                __syncthreads();
+                for (int e2 = tid; e2 < CHOL_NB * CHOL_NB; e2 += 256) {
+                    const int r = e2 % CHOL_NB, c = e2 / CHOL_NB;
+                    sD[r * DG_LD + c] = (r >= c) ? (double)A[(long long)(c0 + c) * ld + c0 + r] : 0.0;
+                }
             }
-            factor_diag64(sD, c0, status, Lb);   // begins with a barrier
+            factor_diag64(sD, c0, status, Lb);   // begins with a barrier; leaves sLT / sInv
+            publish_flag(dflags + k);            // Lb (the stores of store_diag64 are not read here)
             store_diag64<T>(sD, A, ld, c0);
-            publish_flag(dflags + k);
             rs = c0 + CHOL_NB;
-        } else if (!wg_wait_flag(dflags + k, status)) {
-            return;
         }
+        block_in_lds = false;
         if (rs < r1) {
-            // rows rs .. r1 of panel k, two 16-row groups per wavefront; the rows are loaded
-            // before L_kk (the diagonal case still reads sD in store_diag64 before the barrier
-            // that precedes load_lkk)
+            // rows rs .. r1 of panel k, two 16-row groups per wavefront; the rows carry every
+            // update once the previous panel's update (this workgroup's stores, other threads'
+            // rows) is done, so they are loaded after a barrier but before the wait for D_k
+            if (!diag) __syncthreads();
             const int g0 = rs + 16 * wave, g1 = g0 + 64;
             Trsm16 x0, x1;
             if (g0 < r1) trsm16_load<T>(A, ld, c0, g0, r1, lane, x0);
             if (g1 < r1) trsm16_load<T>(A, ld, c0, g1, r1, lane, x1);
             else x1 = Trsm16{};
-            __syncthreads();
-            load_lkk(Lb, sLT, sInv);   // ends with a barrier
+            if (!diag) {
+                if (!wg_wait_flag(dflags + k, status)) return;   // a barrier: smem is free
+                load_lkk(Lb, sLT, sInv);    // ends with a barrier
+            }
             if (g1 < r1) trsm16x2_solve_store<T>(A, ld, c0, g0, g1, r1, lane, x0, x1, sLT, sInv, sc);
             else if (g0 < r1) trsm16_solve_store<T>(A, ld, c0, g0, r1, lane, x0, sLT, sInv, sc);
         }
         publish_flag(tflags + (long long)k * gstrips + g);
-        // the strip's rows of P's later columns cb .. ce, updated with panel k (columns at or
-        // beyond r1 lie above the strip's rows)
-        const int cb = c0 + CHOL_NB, ce = min(e.P1, r1);
-        if (cb < ce) {
-            for (int gg = cb / SY_TILE; gg * SY_TILE < ce; ++gg)
+        // The strip's rows of the later columns of P take panel k's contribution (and, left-
+        // looking, every earlier panel's) before the strip moves to panel k + 1.
+        if constexpr (sizeof(T) == 4) {
+            // left-looking: the next panel's columns (cb .. ce) take every panel of P solved so
+            // far (K = cb - P0), after the flag T_k of the strip holding those columns' rows
+            // (T_k implies T of every earlier panel)
+            const int cb = c0 + CHOL_NB, ce = min(min(cb + CHOL_NB, e.P1), r1);
+            if (cb < ce) {
+                const int gg = cb / SY_TILE;
                 if (gg != g && !wg_wait_flag(tflags + (long long)k * gstrips + gg, status)) return;
-            for (int rj0 = cb; rj0 < ce; rj0 += SY_TILE) {
-                __syncthreads();   // smem is free (the solve's operands, the previous tile's stages)
-                epoch_tile_update<T, SPLIT>(A, ld, c0, CHOL_NB, r0, rj0, ce, n_pad, smem);
+                __syncthreads();   // smem is free (the solve's operands); the solve's stores done
+                if (diag && cb == r0 + CHOL_NB) {
+                    // block k was the strip's top block: below the strip's diagonal the next
+                    // panel's columns hold block k + 1 alone, which goes to LDS for the next factor
+                    diag_block_update<T>(A, ld, cb, e.P0, cb - e.P0, sD);
+                } else {
+                    // after a hand-off the next block is the strip's top block: wavefront 0's
+                    // quadrant of the update is that block, kept in LDS for the next factor
+                    const bool to_lds = cb == r0;
+                    floatx16 acc[2][2];
+#pragma unroll
+                    for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+                        for (int b2 = 0; b2 < 2; ++b2)
+#pragma unroll
+                            for (int r = 0; r < 16; ++r) acc[a2][b2][r] = 0.f;
+                    tile_acc_f32<SPLIT>(A, ld, e.P0, cb - e.P0, r0, cb, n_pad, smem, acc);
+                    if (to_lds) __syncthreads();   // the stages become sD
+                    tile_epilogue_f32(A, ld, r0, cb, ce, n_pad, acc, to_lds && wave == 0, sD);
+                }
+                block_in_lds = diag || cb == r0;
+            }
+        } else {
+            // right-looking (faster for the FP64 tiles): panel k's contribution (K = 64) to the
+            // remaining columns of P (cb .. ce), after the flags T_k of the strips holding those
+            // columns' rows
+            const int cb = c0 + CHOL_NB, ce = min(e.P1, r1);
+            if (cb < ce) {
+                for (int gg = cb / SY_TILE; gg <= (ce - 1) / SY_TILE; ++gg)
+                    if (gg != g && !wg_wait_flag(tflags + (long long)k * gstrips + gg, status)) return;
+                __syncthreads();   // smem is free (the solve's operands); the solve's stores done
+                if (diag && cb == r0 + CHOL_NB) {
+                    diag_block_update<T>(A, ld, cb, c0, CHOL_NB, sD);
+                } else {
+                    for (int rj0 = cb; rj0 < ce; rj0 += SY_TILE) {
+                        // cb == r0 leaves one tile (ce <= r1 = r0 + 128), whose wavefront-0
+                        // quadrant is the strip's top block, kept in LDS
+                        const bool to_lds = cb == r0;
+                        doublex4 acc[4][4];
+#pragma unroll
+                        for (int a2 = 0; a2 < 4; ++a2)
+#pragma unroll
+                            for (int b2 = 0; b2 < 4; ++b2)
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) acc[a2][b2][r] = 0.0;
+                        tile_acc_f64(A, ld, c0, CHOL_NB, r0, rj0, n_pad, smem, acc);
+                        __syncthreads();   // the stages are free (the next tile, or sD)
+                        tile_epilogue_f64(A, ld, r0, rj0, ce, n_pad, acc, to_lds && wave == 0, sD);
+                    }
+                }
+                block_in_lds = diag || cb == r0;
             }
         }
     }

@@ -157,6 +157,24 @@ def test_factor_large_right_looking(api):
     assert res < 5e-6, res
 
 
+@pytest.mark.parametrize("n", [2000, 9000])
+def test_dpotrf_decaying_vs_lapack(api, n):
+    """A dense SPD matrix with entries 0.9 / (1 + 0.05 d^2) off the diagonal (d = |i - j|) and
+    2 on it (its symbol is at least 1.1): every entry of the device factor against LAPACK's.
+    Off-diagonal entries far below the diagonal's size expose a missed or stale block update
+    that a norm-wise residual can hide."""
+    S, _ = api
+    d = np.abs(np.subtract.outer(np.arange(n), np.arange(n))).astype(float)
+    M = np.where(d == 0, 2.0, 0.9 / (1.0 + 0.05 * d * d))
+    Lref = np.linalg.cholesky(M)
+    print(f"dpotrf decaying n={n}", flush=True)
+    for prec, tol in ((S.FP64, 1e-13), (S.MIXED, 2e-6), (S.MIXED_SPLIT, 1e-5)):
+        A, info = S.dpotrf(M, "L", precision=prec)
+        assert info == 0
+        err = np.max(np.abs(np.tril(A) - Lref))
+        assert err < tol, (prec, err)
+
+
 def test_factor_vs_reference_golden(api, golden):
     S, _ = api
     for case in ("chain16", "overlap10"):
