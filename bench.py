@@ -106,6 +106,81 @@ def measured_traffic(kernel, path=TRAFFIC_FILE):
     return None, None
 
 
+def _counter_rows(d):
+    """(kernel name, counter, value) of every dispatch of the rocprofv3 --pmc output under d:
+    its SQLite database (the default output) or its CSV files (--output-format csv)."""
+    import csv
+    import glob
+    import sqlite3
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                rows.append((r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"])))
+    for f in glob.glob(os.path.join(d, "**", "*.db"), recursive=True):
+        with sqlite3.connect(f) as db:
+            rows += [(n, c, float(v)) for n, c, v in
+                     db.execute("select name, counter_name, counter_value from pmc_events")]
+    return rows
+
+
+def _short_kernel(name):
+    """'void bdh::chol_epoch_f32_kernel<false>(float*, ...)' -> 'chol_epoch_f32_kernel<false>'."""
+    name = name.strip().replace("(anonymous namespace)::", "")
+    if name.startswith("void "):
+        name = name[5:]
+    depth, cut = 0, len(name)
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    name = name[:cut]
+    return name[5:] if name.startswith("bdh::") else name
+
+
+def summarize_traffic(fetch_dir, write_dir, out_path, command):
+    """HBM bytes per launch of every kernel of a profile run from two rocprofv3 --pmc runs
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
+    reads for the run's plain device copy of known size (--profile-only ends with three
+    copies of 256 MiB each way)."""
+    copy_bytes = (64 << 20) * 4   # bytes each way per copy
+    per = {}
+    for d, ctr in ((fetch_dir, "FETCH_SIZE"), (write_dir, "WRITE_SIZE")):
+        for name, c, v in _counter_rows(d):
+            if c != ctr:
+                continue
+            per.setdefault(name, {}).setdefault(ctr, []).append(v)
+    copies = [k for k in per if "copy" in k.lower()]
+    if not copies:
+        sys.exit("summarize_traffic: no copy kernel in the counter runs")
+    # the copy kernel also moves small staging buffers: the three 256 MiB copies are its three
+    # largest dispatches
+    cp = {c: sorted(v)[-3:] for c, v in per[copies[0]].items()}
+    fetch_factor = copy_bytes / (np.mean(cp["FETCH_SIZE"]) * 1024.0)
+    write_factor = copy_bytes / (np.mean(cp["WRITE_SIZE"]) * 1024.0)
+    kernels = {}
+    for name, c in per.items():
+        if name in copies or "FETCH_SIZE" not in c or "WRITE_SIZE" not in c:
+            continue
+        fk, wk = float(np.mean(c["FETCH_SIZE"])), float(np.mean(c["WRITE_SIZE"]))
+        short = _short_kernel(name)
+        kernels[short] = {"kernel": short, "launches": len(c["FETCH_SIZE"]),
+                          "fetch_kib_avg": fk, "write_kib_avg": wk,
+                          "hbm_bytes_per_launch": fk * 1024.0 * fetch_factor
+                          + wk * 1024.0 * write_factor}
+    out = {"command": command,
+           "calibration": {"copy_bytes_each_way": copy_bytes, "fetch_factor": fetch_factor,
+                           "write_factor": write_factor},
+           "kernels": kernels}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps({k: round(v["hbm_bytes_per_launch"]) for k, v in kernels.items()}))
+
+
 def update_steps(n_pad, nb=64, outer=256, left_max=4096):
     """(kb, K, t0, t1) of each update launch of one factorization: inside an outer panel
     the next panel's columns (n_pad <= left_max: left-looking, all of the outer panel's
@@ -144,6 +219,14 @@ def update_bytes_per_factor(n_pad, word=4, left_max=4096, outer=256):
 
 SYRK_KERNEL = {"fp64": "syrk_f64_kernel", "f32": "syrk_f32_kernel<false>",
                "split": "syrk_f32_kernel<true>"}
+EPOCH_KERNEL = {"fp64": "chol_epoch_f64_kernel", "f32": "chol_epoch_f32_kernel<false>",
+                "split": "chol_epoch_f32_kernel<true>"}
+CHOL_EPOCH_MIN = 1024   # n_pad from which the factorization runs as one launch per outer panel
+
+
+def chol_flops(n):
+    """Flops of a Cholesky factorization of order n (LAPACK's dpotrf count)."""
+    return n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0
 
 
 def roofline(phases, n, mode, refactors):
@@ -154,18 +237,28 @@ def roofline(phases, n, mode, refactors):
     ms, cnt = phases[dom]
     out = {"kernel": dom, "launches": cnt, "avg_ms": ms / max(cnt, 1)}
     if dom == "syrk":
-        fl = syrk_flops_per_factor(n, n_pad) * refactors
-        achieved = fl / (ms * 1e-3) / 1e12
         # the pipe the kernel issues on: FP32 MFMA (f32), FP64 MFMA (fp64), or bf16 MFMA at
         # three products per FP32 product (split: the ceiling is a third of the bf16 peak)
         peak = {"fp64": PEAK_TFLOPS["f64_mfma"], "f32": PEAK_TFLOPS["f32_mfma"],
                 "split": PEAK_TFLOPS["bf16_mfma"] / 3.0}[mode]
-        basis = (f"n={n}: sum over 64-wide panels of 64 m (m+1) flops (lower-triangle "
-                 f"trailing updates), {refactors} factorizations / {cnt} launches of "
-                 f"{SYRK_KERNEL[mode]}")
+        if n_pad >= CHOL_EPOCH_MIN:
+            # one launch per 256-column outer panel does the whole factorization (diagonal
+            # blocks, panel solves, trailing updates)
+            kern = EPOCH_KERNEL[mode]
+            fl = chol_flops(n) * refactors
+            basis = (f"n={n}: n^3/3 + n^2/2 + n/6 flops per factorization (every flop of the "
+                     f"factorization runs in these launches), {refactors} factorizations / "
+                     f"{cnt} launches of {kern}")
+        else:
+            kern = SYRK_KERNEL[mode]
+            fl = syrk_flops_per_factor(n, n_pad) * refactors
+            basis = (f"n={n}: sum over 64-wide panels of 64 m (m+1) flops (lower-triangle "
+                     f"trailing updates), {refactors} factorizations / {cnt} launches of {kern}")
+        achieved = fl / (ms * 1e-3) / 1e12
+        out["kernel_symbol"] = kern
         if mode == "split":
             basis += "; peak = bf16 dense MFMA peak / 3 (three bf16 products per FP32 product)"
-        tr, cmd = measured_traffic(SYRK_KERNEL[mode])
+        tr, cmd = measured_traffic(kern)
         out.update(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s",
                    frac=achieved / peak, traffic=tr, basis=basis,
                    traffic_source=cmd and f"profiles/traffic_N3000.json: {cmd}")
@@ -341,10 +434,20 @@ def main():
                          "of 64-1024 beads stepped together; adjoint: parameter gradient of an "
                          "N=1024 FP64 trajectory (each its own JSON line)")
     ap.add_argument("--chains", type=int, default=256, help="chains of the batch workload")
+    ap.add_argument("--ewald", choices=("converged", "reference"), default="converged",
+                    help="periodic workload: the Ewald split for a 1e-6 truncation error, or the "
+                         "reference's own split")
+    ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT"),
+                    help="write the HBM bytes per launch of every kernel from two rocprofv3 "
+                         "--pmc runs of --profile-only (FETCH_SIZE, WRITE_SIZE) to OUT and exit")
+    ap.add_argument("--traffic-command", default="",
+                    help="with --summarize-traffic: the commands that made the counter runs")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="batch workload: keep the single-system update schedule instead of "
                          "timing the candidates before the run")
     args = ap.parse_args()
+    if args.summarize_traffic:
+        return summarize_traffic(*args.summarize_traffic, args.traffic_command)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step of a "
               "single chain runs on one MI355X, and this build has no multi-GPU path)",
@@ -477,19 +580,93 @@ def main():
     print(json.dumps(line))
 
 
+def ewald_split_for(scale: float, tol: float = 1e-6):
+    """Real and reciprocal sphere radii of the Beenakker split alpha = scale sqrt(pi)/L for a
+    truncation error of about tol (image sums around minimum-image separations): an omitted
+    image lies at least (sqrt(R_real^2 + 1) - sqrt(3)/2) L from the pair, where the terms fall
+    as exp(-alpha^2 r^2); an omitted reciprocal vector carries exp(-pi |n|^2 / scale^2).  Both
+    exponents are taken 3 above ln(1/tol) for the polynomial prefactors."""
+    x2 = (np.log(1.0 / tol) + 3.0) / np.pi
+    d = np.sqrt(x2) / scale + np.sqrt(3.0) / 2.0
+    Rr = int(np.ceil(np.sqrt(max(0.0, d * d - 1.0))))
+    Rk = int(np.ceil(np.sqrt(max(0.0, x2 * scale * scale - 1.0))))
+    return Rr, Rk
+
+
+def lattice_points(R):
+    return sum(1 for h in range(-R, R + 1) for f in range(-R, R + 1) for g in range(-R, R + 1)
+               if h * h + f * f + g * g <= R * R)
+
+
+# FP32-equivalent VALU operations of one pair block (bdh_ewald.hpp, beenakker_block): a
+# real-space image (shifted separation, inverse powers from one reciprocal square root,
+# exp, erfc from that exp, the two scalar coefficients, six FP64 accumulations) and a
+# reciprocal vector (FP64: the tabulated factor, the phase from stepped per-axis phasors, six
+# accumulations), FP64 operations counted twice (the FP64 VALU rate is half the FP32 one).
+OPS_REAL_IMAGE = 80
+OPS_RECIP_VECTOR = 40
+FP32_VALU_PEAK_TFLOPS = 157.3
+
+
+def cpu_baseline_periodic(sysm, prm, threads, timeout=600):
+    """One step of the reference binary on the periodic suspension (its own Ewald split,
+    alpha = sqrt(pi)/L, lattice sphere R = 2, on-the-fly TEA): wall(bdsteps=2) -
+    wall(bdsteps=1)."""
+    from bd_box_amd import workloads as wl
+    exe = os.path.join(ROOT, "oracle", "_ref", "bd_box_lapack")
+    if not os.path.exists(exe):
+        exe = os.path.join(ROOT, "oracle", "_ref", "bd_box")
+        if not os.path.exists(exe):
+            return None
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads))
+    times = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        sp = os.path.join(tmp, "spheres.str")
+        wl.write_str(sp, sysm)
+        for b in (1, 2):
+            pp = os.path.join(tmp, f"s{b}.prm")
+            wl.write_prm(pp, dict(prm, bdsteps=b), sp, tmp)
+            t0 = time.perf_counter()
+            try:
+                r = subprocess.run([exe, pp], cwd=tmp, env=env, capture_output=True, text=True,
+                                   timeout=timeout)
+            except subprocess.TimeoutExpired:
+                log(f"reference periodic run exceeded {timeout} s")
+                return None
+            times[b] = time.perf_counter() - t0
+            if r.returncode != 0:
+                log(f"reference run failed: {r.stdout[-500:]} {r.stderr[-500:]}")
+                return None
+    dt = times[2] - times[1]
+    return {"value": 1.0 / dt, "unit": "steps/s", "cores": threads, "kind": "reference",
+            "sample": f"1 step of the N={sysm.N} suspension, reference semantics (its Beenakker "
+                      f"split alpha = sqrt(pi)/L, R = 2; on-the-fly TEA), "
+                      f"{os.path.basename(exe)}: wall(bdsteps=2) - wall(bdsteps=1) = {dt:.2f} s"}
+
+
 def bench_periodic(args):
     """Steps/s of the periodic suspension with Beenakker-Ewald TEA noise, mixed (FP32 pair
-    blocks, FP64 Ewald reciprocal/self sums and accumulation) and FP64."""
+    blocks, FP64 Ewald reciprocal/self sums and accumulation) and FP64.  The Ewald split is
+    chosen for a 1e-6 truncation error (alpha = 1.25 sqrt(pi)/L, real and reciprocal spheres
+    of radius 3 around minimum-image separations, the box's wave vectors); --ewald reference
+    runs the reference's own split (alpha = sqrt(pi)/L, one sphere of radius 2 around the
+    wrapped separation), whose reciprocal terms underflow."""
     import torch
     from bd_box_amd.system import BDSystem, FP64, MIXED
     N = 8192 if args.N == 3000 else args.N
     steps = min(args.steps, 50)
     warm = min(args.warmup, 2)
     sysm, prm, P, L = periodic_workload(N, args.seed)
-    log(f"periodic workload: N={N} box={L:.1f} A, {steps} timed steps")
+    converged = args.ewald == "converged"
+    scale = 1.25
+    Rr, Rk = ewald_split_for(scale) if converged else (P.ewald_real, P.ewald_real)
+    log(f"periodic workload: N={N} box={L:.1f} A, {steps} timed steps, Ewald split "
+        f"{'alpha = %.2f sqrt(pi)/L, radii %d/%d' % (scale, Rr, Rk) if converged else 'reference'}")
     out = {}
     for name, prec in (("mixed", MIXED), ("fp64", FP64)):
         s = BDSystem(sysm, P, precision=prec)
+        if converged:
+            s.set_ewald_split(scale, Rr, Rk)
         s.compute_hydro(warm)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -503,19 +680,43 @@ def bench_periodic(args):
         out[name] = (steps / t, {k: v[0] / 2 for k, v in ph.items()})
         log(f"{name}: {steps / t:.2f} steps/s phases {out[name][1]}")
         s.close()
+    # roofline of the pair sums (HIP events on the launch stream): FP32-equivalent VALU
+    # operations of every unordered pair's block (and the self blocks) per step
+    n_real = lattice_points(Rr) - 1 if converged else lattice_points(Rr)
+    n_recip = lattice_points(Rk) - 1
+    pairs = N * (N + 1) // 2
+    ops = pairs * (n_real * OPS_REAL_IMAGE + n_recip * OPS_RECIP_VECTOR)
+    t_pairs = out["mixed"][1].get("matvec", 0.0) * 1e-3
+    roof = None
+    if t_pairs > 0:
+        ach = ops / t_pairs / 1e12
+        roof = {"kernel": "pair_sums_kernel<float, beenakker, tea>", "bound": "valu",
+                "achieved": ach, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP32_VALU_PEAK_TFLOPS, "traffic": None,
+                "basis": f"{pairs} pair blocks per step x ({n_real} real-space images x "
+                         f"{OPS_REAL_IMAGE} + {n_recip} reciprocal vectors x {OPS_RECIP_VECTOR}) "
+                         f"FP32-equivalent VALU operations (FP64 counted twice) / the pair-sum "
+                         f"phase's time per step"}
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = os.cpu_count() or 1
+        cpu = cpu_baseline_periodic(sysm, prm, threads)
     line = {
         "metric": "BD steps/s, N=8192 periodic Beenakker-Ewald RPY + TEA",
         "value": out["mixed"][0], "unit": "steps/s", "n_gpus": 1, "steps": steps,
         "warmup": warm, "ms_per_step": 1e3 / out["mixed"][0], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"N={N} spheres a=25 A, phi=0.2, periodic cube L={L:.1f} A, "
-                               "WCA wall sigma 50 A eps 1, Beenakker-Ewald R=2, TEA, dt 1 ps",
+                               "WCA wall sigma 50 A eps 1, Beenakker-Ewald, TEA, dt 1 ps",
+                   "ewald": (f"alpha = {scale} sqrt(pi)/L, real/reciprocal radii {Rr}/{Rk} "
+                             "(1e-6 truncation)") if converged else
+                            "reference split: alpha = sqrt(pi)/L, radius 2",
                    "N": N, "box": L, "parallelism": "1 GPU"},
         "fp64_steps_per_s": out["fp64"][0],
         "phases_ms_per_step": out["mixed"][1],
         "fp64_phases_ms_per_step": out["fp64"][1],
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line))
 
