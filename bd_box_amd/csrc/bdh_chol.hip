@@ -83,9 +83,10 @@ __device__ __forceinline__ int wait_lds_count(int* cnt, int need)
 // counter), then factors its own 16 columns one by one, publishing each.  So the pivot chain
 // runs through the wavefronts back to back and every update off it runs beside it.
 // Within a wavefront's own columns, pivot t+1 = a_{t+1,t+1} - (a_{t+1,t} / sqrt(d_t))^2 is
-// formed from broadcasts (v_readlane, into scalar registers) of the pre-update entries, which
-// are issued before the pivot's reciprocal square root is known; the column's rank-1 update
-// of the later own columns takes the same unscaled broadcasts.
+// formed from two broadcasts (v_readlane) of the pre-update entries, issued before the
+// pivot's reciprocal square root is known; the rank-1 update of the later own columns reads
+// the scaled column back from LDS as broadcasts, and the same entries build the inverse of
+// the own 16x16 sub-block step by step, so it is complete with the last own column.
 // A non-positive (or NaN) pivot records row c0 + j + 1 in *status (the first one wins) and the
 // factorization continues with NaN, as the reference's dpotrf reports INFO.
 __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, int* __restrict__ status,
@@ -93,7 +94,6 @@ __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, i
 {
     __shared__ int s_done;   // columns published so far
     __shared__ int s_bad;    // first non-positive pivot (64: none); -1: a wait timed out
-    __shared__ double s_rs[CHOL_NB];   // 1 / l_cc of every column
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int cw = 16 * w;   // first own column
@@ -119,44 +119,72 @@ __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, i
                 break;
             }
         }
+        // every LDS read of the pair is issued before its first use
         const double x0 = L[c * CHOL_NB + lane], x1 = L[(c + 1) * CHOL_NB + lane];
         const double2* b0 = reinterpret_cast<const double2*>(L + c * CHOL_NB + cw);
         const double2* b1 = reinterpret_cast<const double2*>(L + (c + 1) * CHOL_NB + cw);
+        double2 v0[8], v1[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const double2 v0 = b0[u], v1 = b1[u];
-            p[2 * u] = __builtin_fma(-x1, v1.x, __builtin_fma(-x0, v0.x, p[2 * u]));
-            p[2 * u + 1] = __builtin_fma(-x1, v1.y, __builtin_fma(-x0, v0.y, p[2 * u + 1]));
+            v0[u] = b0[u];
+            v1[u] = b1[u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            p[2 * u] = __builtin_fma(-x1, v1[u].x, __builtin_fma(-x0, v0[u].x, p[2 * u]));
+            p[2 * u + 1] = __builtin_fma(-x1, v1[u].y, __builtin_fma(-x0, v0[u].y, p[2 * u + 1]));
         }
     }
     int bad = CHOL_NB;
     if (ok) {
+        // inverse of the own 16x16 diagonal sub-block, built beside the factorization: lane
+        // q < 16 solves L_ww y = e_q right-looking, step t taking column t's 1 / l_tt and the
+        // entries l_{cw+u, cw+t} the column update broadcasts anyway
+        double y[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) y[r] = (r == lane) ? 1.0 : 0.0;
         double piv = readlane_f64(p[0], cw);
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             const int c = cw + t;
             if (!(piv > 0.0) && bad == CHOL_NB) bad = c;
-            // unscaled entries a_{cw+u, c} of the later own rows, final since column c - 1:
-            // these broadcasts do not wait for the pivot's reciprocal square root
-            double a[16];
-#pragma unroll
-            for (int u = t + 1; u < 16; ++u) a[u] = readlane_f64(p[t], cw + u);
+            // the next pivot a_{c+1,c+1} - l_{c+1,c}^2 from the entries before this column's
+            // update (two broadcasts issued before the reciprocal square root is known)
+            double a1 = 0.0, a2 = 0.0;
+            if (t < 15) {
+                a1 = readlane_f64(p[t], c + 1);
+                a2 = readlane_f64(p[t + 1], c + 1);
+            }
             const double rs = rsqrt_f64(piv);
             double nxt = 0.0;
             if (t < 15) {
-                const double le = a[t + 1] * rs;   // l_{c+1, c}
-                nxt = __builtin_fma(-le, le, readlane_f64(p[t + 1], c + 1));
+                const double le = a1 * rs;   // l_{c+1, c}
+                nxt = __builtin_fma(-le, le, a2);
             }
             const double lp = p[t] * rs, ld = piv * rs;
             double l = (lane == c) ? ld : lp;
             l = (lane < c) ? 0.0 : l;
             p[t] = l;
             L[c * CHOL_NB + lane] = l;
-            if (lane == 0) s_rs[c] = rs;
-            // p[u] -= l (a_u rs)
-            const double lr2 = l * rs;
+            y[t] *= rs;
+            if (t < 15) {
+                // l_{cw+u, c} of the later own columns: LDS broadcasts of the column this
+                // wavefront has just stored (its own stores, in order before these loads)
+                __builtin_amdgcn_wave_barrier();
+                double lc[16];
 #pragma unroll
-            for (int u = t + 1; u < 16; ++u) p[u] = __builtin_fma(-lr2, a[u], p[u]);
+                for (int u = t + 1; u < 16; ++u) lc[u] = L[c * CHOL_NB + cw + u];
+#pragma unroll
+                for (int u = t + 1; u < 16; ++u) p[u] = __builtin_fma(-l, lc[u], p[u]);
+#pragma unroll
+                for (int u = t + 1; u < 16; ++u) {
+                    y[u] = __builtin_fma(-lc[u], y[t], y[u]);
+                    // keeps the step here: left to the scheduler, the steps sink to the stores
+                    // at the end and every broadcast stays live until then
+                    asm volatile("" : "+v"(y[u]));
+                }
+            }
             // columns are published in pairs (the consumers take them in pairs)
             if (t & 1) __hip_atomic_store(&s_done, c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             piv = nxt;
@@ -164,21 +192,7 @@ __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, i
         // own columns into Lb (zeros above the diagonal)
 #pragma unroll
         for (int u = 0; u < 16; ++u) Lb[(cw + u) * CHOL_NB + lane] = (lane >= cw + u) ? p[u] : 0.0;
-        __builtin_amdgcn_sched_barrier(0);
-        // inverse of the own 16x16 diagonal sub-block: lane q < 16 solves L_ww y = e_q column by
-        // column (right-looking: the updates of one column are independent), with 1 / l_cc the
-        // pivot's reciprocal square root
         if (lane < 16) {
-            double y[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) y[r] = (r == lane) ? 1.0 : 0.0;
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                y[t] *= s_rs[cw + t];
-#pragma unroll
-                for (int r = t + 1; r < 16; ++r) y[r] = __builtin_fma(-L[(cw + t) * CHOL_NB + cw + r], y[t], y[r]);
-                __builtin_amdgcn_sched_barrier(0);
-            }
             double* inv = Lb + CHOL_NB * CHOL_NB + 256 * w;
             double* sinv = sD + CHOL_NB * CHOL_NB + 256 * w;   // the solve's sInv layout
 #pragma unroll
@@ -1090,92 +1104,200 @@ __device__ __forceinline__ void epoch_tile_update(T* __restrict__ A, long long l
     }
 }
 
+// acc[q] = sum_k A[ra + i][kb + k] A[ca + 32 q + j][kb + k] over NQ 32 x 32 quadrants side by side
+// (rows ra .., columns ca + 32 q ..), K a multiple of 32, on one wavefront's MFMA with operands
+// loaded straight from A (no LDS), KC = 32 / NQ k at a time with the next KC in flight.
+// v_mfma_f32_32x32x2_f32: lane l feeds row l % 32, k = l / 32; acc[q][r] holds row
+// 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32 of quadrant q.
+template <int NQ>
+__device__ __forceinline__ void quad_acc(const float* __restrict__ A, long long ld, int ra, int ca, int kb, int K,
+                                         floatx16 (&acc)[NQ])
+{
+    constexpr int KC = 32 / NQ, NJ = KC / 2;
+    const int lane = threadIdx.x & 63, l32 = lane & 31, lh = lane >> 5;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
+    float av[NJ], bv[NQ][NJ];
+    auto load = [&](int k0, float (&a)[NJ], float (&bq)[NQ][NJ]) {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const float* col = A + (long long)(kb + k0 + 2 * j + lh) * ld;
+            a[j] = col[ra + l32];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) bq[q][j] = col[ca + 32 * q + l32];
+        }
+    };
+    load(0, av, bv);
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        float an[NJ], bn[NQ][NJ];
+        if (k0 + KC < K) load(k0 + KC, an, bn);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc[q] = mfma_f32_32x32x2(av[j], bv[q][j], acc[q]);
+        if (k0 + KC < K) {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                av[j] = an[j];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) bv[q][j] = bn[q][j];
+            }
+        }
+    }
+}
+
+// The same in FP64, K a multiple of 16: v_mfma_f64_16x16x4_f64 on the 2 x 2 tiles of each
+// quadrant, lane l feeds row l % 16, k = l / 16; acc[q][i][j][r] holds row 16 i + 4 r + l / 16,
+// column 16 j + l % 16 of quadrant q.  16 k at a time with the next 16 in flight.
+template <int NQ>
+__device__ __forceinline__ void quad_acc(const double* __restrict__ A, long long ld, int ra, int ca, int kb, int K,
+                                         doublex4 (&acc)[NQ][2][2])
+{
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[q][i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+    double av[4][2], bv[NQ][4][2];
+    auto load = [&](int k0, double (&a)[4][2], double (&bq)[NQ][4][2]) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const double* col = A + (long long)(kb + k0 + 4 * s4 + lq) * ld;
+            a[s4][0] = col[ra + l16];
+            a[s4][1] = col[ra + 16 + l16];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                bq[q][s4][0] = col[ca + 32 * q + l16];
+                bq[q][s4][1] = col[ca + 32 * q + 16 + l16];
+            }
+        }
+    };
+    load(0, av, bv);
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        double an[4][2], bn[NQ][4][2];
+        if (k0 + 16 < K) load(k0 + 16, an, bn);
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[q][i][j] = mfma_f64_16x16x4(av[s4][i], bv[q][s4][j], acc[q][i][j]);
+        if (k0 + 16 < K) {
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    av[s4][i] = an[s4][i];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) bv[q][s4][i] = bn[q][s4][i];
+                }
+        }
+    }
+}
+
+// A[ra + i][ca + j] - acc of a quadrant, into dst[i * pitch + j] (LDS, FP64) or back into A
+// (dst null).
+__device__ __forceinline__ void quad_sub(float* __restrict__ A, long long ld, int ra, int ca, const floatx16& acc,
+                                         double* __restrict__ dst, int pitch)
+{
+    const int lane = threadIdx.x & 63, l32 = lane & 31, lh = lane >> 5;
+    float v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * lh + (r & 3);
+        v[r] = A[(long long)(ca + l32) * ld + ra + row] - acc[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * lh + (r & 3);
+        if (dst) dst[row * pitch + l32] = (double)v[r];
+        else A[(long long)(ca + l32) * ld + ra + row] = v[r];
+    }
+}
+
+__device__ __forceinline__ void quad_sub(double* __restrict__ A, long long ld, int ra, int ca,
+                                         const doublex4 (&acc)[2][2], double* __restrict__ dst, int pitch)
+{
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lq = lane >> 4;
+    double v[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                v[i][j][r] = A[(long long)(ca + 16 * j + l16) * ld + ra + 16 * i + 4 * r + lq] - acc[i][j][r];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * i + 4 * r + lq, col = 16 * j + l16;
+                if (dst) dst[row * pitch + col] = v[i][j][r];
+                else A[(long long)(ca + col) * ld + ra + row] = v[i][j][r];
+            }
+}
+
+template <typename T, int NQ>
+struct QuadAcc;
+template <int NQ>
+struct QuadAcc<float, NQ> {
+    floatx16 v[NQ];
+};
+template <int NQ>
+struct QuadAcc<double, NQ> {
+    doublex4 v[NQ][2][2];
+};
+
 // sD[r * DG_LD + c] = A[c1 + r][c1 + c] - sum_k A[c1 + r][kb + k] A[c1 + c][kb + k] for the 64 x 64
 // diagonal block at c1 (the strict upper 32 x 32 quadrant is not written; the factor reads only
 // r >= c), on MFMA in T: three wavefronts take the quadrants (0, 0), (1, 0), (1, 1).
 template <typename T>
-__device__ __forceinline__ void diag_block_update(const T* __restrict__ A, long long ld, int c1, int kb, int K,
+__device__ __forceinline__ void diag_block_update(T* __restrict__ A, long long ld, int c1, int kb, int K,
                                                   double* __restrict__ sD)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wave = threadIdx.x >> 6;
     if (wave == 3) return;
     const int qr = wave == 0 ? 0 : 1, qc = wave == 2 ? 1 : 0;
-    const int r0 = c1 + 32 * qr, q0 = c1 + 32 * qc;
-    if constexpr (sizeof(T) == 4) {
-        // v_mfma_f32_32x32x2_f32: lane l feeds row l % 32, k = l / 32; acc[r] holds row
-        // 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
-        const int l32 = lane & 31, lh = lane >> 5;
-        floatx16 acc;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-        // 16 k per chunk; the next chunk's loads are issued before this chunk's MFMAs
-        float av[8], bv[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const T* col = A + (long long)(kb + 2 * j + lh) * ld;
-            av[j] = col[r0 + l32];
-            bv[j] = col[q0 + l32];
-        }
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            float an[8], bn[8];
-            if (k0 + 16 < K) {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const T* col = A + (long long)(kb + k0 + 16 + 2 * j + lh) * ld;
-                    an[j] = col[r0 + l32];
-                    bn[j] = col[q0 + l32];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) acc = mfma_f32_32x32x2(av[j], bv[j], acc);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                av[j] = an[j];
-                bv[j] = bn[j];
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = 8 * (r >> 2) + 4 * lh + (r & 3), col = l32;
-            sD[(32 * qr + row) * DG_LD + 32 * qc + col] =
-                (double)(A[(long long)(q0 + col) * ld + r0 + row] - acc[r]);
-        }
-    } else {
-        // v_mfma_f64_16x16x4_f64 on the 2 x 2 tiles of the quadrant: lane l feeds row l % 16,
-        // k = l / 16; acc[r] holds row 4 r + l / 16, column l % 16
-        const int l16 = lane & 15, lq = lane >> 4;
-        doublex4 acc[2][2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < K; k0 += 4) {
-            const T* col = A + (long long)(kb + k0 + lq) * ld;
-            const double a0 = col[r0 + l16], a1 = col[r0 + 16 + l16];
-            const double b0 = col[q0 + l16], b1 = col[q0 + 16 + l16];
-            acc[0][0] = mfma_f64_16x16x4(a0, b0, acc[0][0]);
-            acc[0][1] = mfma_f64_16x16x4(a0, b1, acc[0][1]);
-            acc[1][0] = mfma_f64_16x16x4(a1, b0, acc[1][0]);
-            acc[1][1] = mfma_f64_16x16x4(a1, b1, acc[1][1]);
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * i + 4 * r + lq, col = 16 * j + l16;
-                    sD[(32 * qr + row) * DG_LD + 32 * qc + col] =
-                        A[(long long)(q0 + col) * ld + r0 + row] - acc[i][j][r];
-                }
-    }
+    QuadAcc<T, 1> acc;
+    quad_acc<1>(A, ld, c1 + 32 * qr, c1 + 32 * qc, kb, K, acc.v);
+    quad_sub(A, ld, c1 + 32 * qr, c1 + 32 * qc, acc.v[0], sD + 32 * qr * DG_LD + 32 * qc, DG_LD);
 }
 
-// Row strip s of outer panel P (see above).  The diagonal path stays in LDS: the strip that
-// factors block k solves the rows below it from the factor in LDS, and a strip that factors
-// block k + 1 next builds that block's update straight into LDS (a 64 x 64 update when block k
-// was its own, wavefront 0's quadrant of its 128 x 64 update after a hand-off), so only the
-// first block of an epoch is read from A.
+// A[rb + i][cb + j] -= sum_k A[rb + i][kb + k] A[cb + j][kb + k] for the rows rb .. rb + 32 W
+// of the 64 columns cb .. (the block lies below the diagonal): wavefront w < W takes rows
+// rb + 32 w .. with both column quadrants.
+template <typename T, int W>
+__device__ __forceinline__ void block_update64(T* __restrict__ A, long long ld, int rb, int cb, int kb, int K)
+{
+    const int wave = threadIdx.x >> 6;
+    if (wave >= W) return;
+    const int ra = rb + 32 * wave;
+    QuadAcc<T, 2> acc;
+    quad_acc<2>(A, ld, ra, cb, kb, K, acc.v);
+    quad_sub(A, ld, ra, cb, acc.v[0], nullptr, 0);
+    quad_sub(A, ld, ra, cb + 32, acc.v[1], nullptr, 0);
+}
+
+// Row strip s of outer panel P (see above), right-looking: after solving its rows of panel k
+// the strip applies panel k to its rows of the later columns of P.  The diagonal chain comes
+// first and stays in LDS.  A strip holding the next two diagonal blocks kA (rows r0 ..) and
+// kB = kA + 1:
+//   panel kA - 1: only block kA's update (64 x 64, K = 64) is made, into LDS; the strip's lower
+//                 block row (rows r0 + 64 ..) keeps panel kA - 1 pending;
+//   block kA:     factor, publish D_kA, then the pending panel's update of block (kB, kA)
+//                 (64 x 64, into A), the solve of rows r0 + 64 .. with L_kA, and block kB's
+//                 update with panels kA - 1 and kA (K = 128) into LDS;
+//   block kB:     factor, publish D_kB.
+// Only the epoch's first block is read from A.
 template <typename T, bool SPLIT>
 __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int n_pad, double* __restrict__ Lbuf,
                                             int* __restrict__ status, const EpochArgs& e, int s,
@@ -1192,6 +1314,7 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
     double* sInv = sLT + CHOL_NB * CHOL_NB;
     double* sc = sInv + 4 * 256 + wave * 16 * SC_PITCH;
     bool block_in_lds = false;   // sD holds the next diagonal block, fully updated
+    int pend = -1;   // first column of panel kA - 1 while it is pending on the lower block row
     for (int c0 = e.P0; c0 < e.P1 && c0 < r1; c0 += CHOL_NB) {
         const int k = c0 / CHOL_NB;
         double* Lb = Lbuf + (long long)k * LB_STRIDE;
@@ -1199,24 +1322,37 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
         int rs = r0;                  // first row of the strip solved with panel k
         if (diag) {
             if (!block_in_lds) {
-                // the epoch's first block: it carries every update (earlier launches')
+                // the epoch's first block: it carries every update (earlier launches'); all 16
+                // loads of a thread are issued before its LDS stores
                 __syncthreads();
-                for (int e2 = tid; e2 < CHOL_NB * CHOL_NB; e2 += 256) {
-                    const int r = e2 % CHOL_NB, c = e2 / CHOL_NB;
-                    sD[r * DG_LD + c] = (r >= c) ? (double)A[(long long)(c0 + c) * ld + c0 + r] : 0.0;
+                T v[CHOL_NB * CHOL_NB / 256];
+#pragma unroll
+                for (int i = 0; i < CHOL_NB * CHOL_NB / 256; ++i) {
+                    const int e2 = tid + 256 * i, r = e2 % CHOL_NB, c = e2 / CHOL_NB;
+                    v[i] = A[(long long)(c0 + c) * ld + c0 + r];
+                }
+#pragma unroll
+                for (int i = 0; i < CHOL_NB * CHOL_NB / 256; ++i) {
+                    const int e2 = tid + 256 * i, r = e2 % CHOL_NB, c = e2 / CHOL_NB;
+                    sD[r * DG_LD + c] = (r >= c) ? (double)v[i] : 0.0;
                 }
             }
             factor_diag64(sD, c0, status, Lb);   // begins with a barrier; leaves sLT / sInv
             publish_flag(dflags + k);            // Lb (the stores of store_diag64 are not read here)
             store_diag64<T>(sD, A, ld, c0);
             rs = c0 + CHOL_NB;
+            if (pend >= 0 && rs < r1) {
+                // the pending panel on block (kB, kA), which the solve below reads (the block's
+                // rows of that panel were solved by this workgroup, behind the factor's barriers)
+                block_update64<T, 2>(A, ld, rs, c0, pend, CHOL_NB);
+            }
         }
         block_in_lds = false;
         if (rs < r1) {
             // rows rs .. r1 of panel k, two 16-row groups per wavefront; the rows carry every
-            // update once the previous panel's update (this workgroup's stores, other threads'
-            // rows) is done, so they are loaded after a barrier but before the wait for D_k
-            if (!diag) __syncthreads();
+            // update once this workgroup's stores to them are done (a barrier), and are loaded
+            // before the wait for D_k
+            __syncthreads();
             const int g0 = rs + 16 * wave, g1 = g0 + 64;
             Trsm16 x0, x1;
             if (g0 < r1) trsm16_load<T>(A, ld, c0, g0, r1, lane, x0);
@@ -1230,25 +1366,32 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
             else if (g0 < r1) trsm16_solve_store<T>(A, ld, c0, g0, r1, lane, x0, sLT, sInv, sc);
         }
         publish_flag(tflags + (long long)k * gstrips + g);
-        // The strip's rows of the later columns of P take panel k's contribution (and, left-
-        // looking, every earlier panel's) before the strip moves to panel k + 1.
-        if constexpr (sizeof(T) == 4) {
-            // left-looking: the next panel's columns (cb .. ce) take every panel of P solved so
-            // far (K = cb - P0), after the flag T_k of the strip holding those columns' rows
-            // (T_k implies T of every earlier panel)
-            const int cb = c0 + CHOL_NB, ce = min(min(cb + CHOL_NB, e.P1), r1);
-            if (cb < ce) {
-                const int gg = cb / SY_TILE;
-                if (gg != g && !wg_wait_flag(tflags + (long long)k * gstrips + gg, status)) return;
-                __syncthreads();   // smem is free (the solve's operands); the solve's stores done
-                if (diag && cb == r0 + CHOL_NB) {
-                    // block k was the strip's top block: below the strip's diagonal the next
-                    // panel's columns hold block k + 1 alone, which goes to LDS for the next factor
-                    diag_block_update<T>(A, ld, cb, e.P0, cb - e.P0, sD);
-                } else {
-                    // after a hand-off the next block is the strip's top block: wavefront 0's
-                    // quadrant of the update is that block, kept in LDS for the next factor
-                    const bool to_lds = cb == r0;
+        // panel k on the strip's rows of columns cb .. ce of P
+        const int cb = c0 + CHOL_NB, ce = min(e.P1, r1);
+        if (cb >= ce) continue;
+        if (diag) {
+            // block kA was factored: below the strip's diagonal the later columns hold block kB
+            // alone, which takes the pending panel and panel kA (contiguous) into LDS
+            __syncthreads();   // the solve's stores done; smem is free
+            const int kb = pend >= 0 ? pend : c0;
+            diag_block_update<T>(A, ld, cb, kb, cb - kb, sD);
+            block_in_lds = true;
+            pend = -1;
+        } else if (cb == r0) {
+            // the next panel is block kA: its update (the strip's own rows of panel k) into LDS,
+            // the lower block row's left pending
+            __syncthreads();
+            diag_block_update<T>(A, ld, cb, c0, CHOL_NB, sD);
+            block_in_lds = true;
+            pend = c0;
+        } else {
+            if constexpr (sizeof(T) == 4) {
+                // tiles of up to 128 columns (LDS-staged, faster in FP32 than the pieces below),
+                // after the flags T_k of the strips holding rows cb .. ce
+                for (int gg = cb / SY_TILE; gg <= (ce - 1) / SY_TILE; ++gg)
+                    if (gg != g && !wg_wait_flag(tflags + (long long)k * gstrips + gg, status)) return;
+                for (int rj0 = cb; rj0 < ce; rj0 += SY_TILE) {
+                    __syncthreads();   // smem is free (the solve's operands, the previous tile's stages)
                     floatx16 acc[2][2];
 #pragma unroll
                     for (int a2 = 0; a2 < 2; ++a2)
@@ -1256,41 +1399,21 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
                         for (int b2 = 0; b2 < 2; ++b2)
 #pragma unroll
                             for (int r = 0; r < 16; ++r) acc[a2][b2][r] = 0.f;
-                    tile_acc_f32<SPLIT>(A, ld, e.P0, cb - e.P0, r0, cb, n_pad, smem, acc);
-                    if (to_lds) __syncthreads();   // the stages become sD
-                    tile_epilogue_f32(A, ld, r0, cb, ce, n_pad, acc, to_lds && wave == 0, sD);
+                    tile_acc_f32<SPLIT>(A, ld, c0, CHOL_NB, r0, rj0, n_pad, smem, acc);
+                    tile_epilogue_f32(A, ld, r0, rj0, ce, n_pad, acc, false, nullptr);
                 }
-                block_in_lds = diag || cb == r0;
-            }
-        } else {
-            // right-looking (faster for the FP64 tiles): panel k's contribution (K = 64) to the
-            // remaining columns of P (cb .. ce), after the flags T_k of the strips holding those
-            // columns' rows
-            const int cb = c0 + CHOL_NB, ce = min(e.P1, r1);
-            if (cb < ce) {
-                for (int gg = cb / SY_TILE; gg <= (ce - 1) / SY_TILE; ++gg)
+            } else {
+                // 64-column pieces in column order (the next panel's first), each after the
+                // flag T_k of the strip holding its rows; a piece on the strip's own diagonal
+                // blocks takes only the rows from its diagonal down
+                __syncthreads();   // the solve's stores done
+                for (int j = cb; j < ce; j += CHOL_NB) {
+                    const int gg = j / SY_TILE;
                     if (gg != g && !wg_wait_flag(tflags + (long long)k * gstrips + gg, status)) return;
-                __syncthreads();   // smem is free (the solve's operands); the solve's stores done
-                if (diag && cb == r0 + CHOL_NB) {
-                    diag_block_update<T>(A, ld, cb, c0, CHOL_NB, sD);
-                } else {
-                    for (int rj0 = cb; rj0 < ce; rj0 += SY_TILE) {
-                        // cb == r0 leaves one tile (ce <= r1 = r0 + 128), whose wavefront-0
-                        // quadrant is the strip's top block, kept in LDS
-                        const bool to_lds = cb == r0;
-                        doublex4 acc[4][4];
-#pragma unroll
-                        for (int a2 = 0; a2 < 4; ++a2)
-#pragma unroll
-                            for (int b2 = 0; b2 < 4; ++b2)
-#pragma unroll
-                                for (int r = 0; r < 4; ++r) acc[a2][b2][r] = 0.0;
-                        tile_acc_f64(A, ld, c0, CHOL_NB, r0, rj0, n_pad, smem, acc);
-                        __syncthreads();   // the stages are free (the next tile, or sD)
-                        tile_epilogue_f64(A, ld, r0, rj0, ce, n_pad, acc, to_lds && wave == 0, sD);
-                    }
+                    if (j < r0) block_update64<T, 4>(A, ld, r0, j, c0, CHOL_NB);
+                    else if (j == r0) block_update64<T, 4>(A, ld, j, j, c0, CHOL_NB);
+                    else block_update64<T, 2>(A, ld, j, j, c0, CHOL_NB);
                 }
-                block_in_lds = diag || cb == r0;
             }
         }
     }
