@@ -477,14 +477,16 @@ __device__ __forceinline__ int* chol_flags(double* Lbuf, int n_pad)
 // polls are relaxed; one agent-scope acquire follows the flag (an acquiring poll would
 // invalidate this XCD's L2 at every iteration, under the feet of the other workgroups).
 constexpr int CHOL_SPIN = 1 << 20;
+// SLEEP: s_sleep units (64 cycles each) between polls; the bound on the polls scales with it.
+template <int SLEEP = 2>
 __device__ __forceinline__ bool wait_flag(int* flag)
 {
-    for (int it = 0; it < CHOL_SPIN; ++it) {
+    for (int it = 0; it < CHOL_SPIN * 2 / SLEEP; ++it) {
         if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             return true;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(SLEEP);
     }
     return false;
 }
@@ -1033,10 +1035,10 @@ int launch_update(T* d_A, long long ld, int kb, int K, int t0, int t1, int n_pad
 // dispatched first and all resident; every wait is for a lower index):
 //   b < R            row strip s = b (rows P0 + 128 s ..); for each panel k of P, in order:
 //                    the diagonal block if it lies in the strip (flag D_k) or a wait for D_k;
-//                    the solve of the strip's rows below block k (flag T_k,strip); the strip's
-//                    rows of panel k + 1's columns updated with every panel of P so far
-//                    (left-looking inside P), after the flag T_k of the strip that holds those
-//                    columns' rows.
+//                    the solve of the strip's rows below block k (flag T_k,strip); panel k's
+//                    update of the strip's rows of P's later columns (right-looking, the next
+//                    diagonal block first; see epoch_strip), after the flags T_k of the strips
+//                    that hold those columns' rows.
 //   R <= b < R + NL  a 128 x 128 tile of the lookahead columns: K = CHOL_OUTER from the previous
 //                    outer panel, then K = 64 from each panel k after T_k of its two strips,
 //                    accumulated in registers and applied to C once.
@@ -1044,14 +1046,23 @@ int launch_update(T* d_A, long long ld, int kb, int K, int t0, int t1, int n_pad
 struct EpochArgs {
     int P0, P1, Q1;   // this outer panel, the end of the lookahead columns
     int prevP0;       // the previous outer panel's first column (< 0 in the first epoch)
-    int R, NL;        // row strips, lookahead tiles
+    int R, NL, NB;    // row strips, lookahead tiles, bulk tiles
 };
+
 
 // Flags of the epoch schedule, after the diagonal-block flags in the workspace: T_k,g for
 // panel k and global row strip g (rows 128 g ..), n_pad / 64 x n_pad / 128 of them.
 __device__ __forceinline__ int* epoch_tflags(double* Lbuf, int n_pad)
 {
     return chol_flags(Lbuf, n_pad) + n_pad / CHOL_NB;
+}
+
+// The epoch schedule's solve flags sit one per 128-byte line, so the tiles that poll one of
+// them do not share a line with the flags the diagonal chain sets and reads.
+constexpr int FLAG_STRIDE = 32;
+__device__ __forceinline__ int* tflag(int* tflags, int k, int g, int gstrips)
+{
+    return tflags + ((long long)k * gstrips + g) * FLAG_STRIDE;
 }
 
 // The workgroup's stores so far made visible at agent scope, then *flag set.
@@ -1064,15 +1075,40 @@ __device__ __forceinline__ void publish_flag(int* flag)
 
 // All threads: wait until *flag is set (thread 0 polls, bounded); false after a timeout, which
 // is reported through status.
+template <int SLEEP = 2>
 __device__ __forceinline__ bool wg_wait_flag(int* flag, int* status)
 {
     __shared__ int s_flag_ok;
     if (threadIdx.x == 0) {
-        s_flag_ok = wait_flag(flag) ? 1 : 0;
+        s_flag_ok = wait_flag<SLEEP>(flag) ? 1 : 0;
         if (!s_flag_ok) atomicCAS(status, 0, -1);
     }
     __syncthreads();
     return s_flag_ok != 0;
+}
+
+// All threads: wait until *f1 and *f2 are set (thread 0 polls both relaxed, then acquires once:
+// each agent-scope acquire invalidates this XCD's L2, which the tiles around share).
+template <int SLEEP = 2>
+__device__ __forceinline__ bool wg_wait_flags2(int* f1, int* f2, int* status)
+{
+    __shared__ int s_flags_ok;
+    if (threadIdx.x == 0) {
+        int ok = 0;
+        for (int it = 0; it < CHOL_SPIN * 2 / SLEEP; ++it) {
+            if (__hip_atomic_load(f1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 &&
+                __hip_atomic_load(f2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                ok = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(SLEEP);
+        }
+        s_flags_ok = ok;
+        if (!ok) atomicCAS(status, 0, -1);
+    }
+    __syncthreads();
+    return s_flags_ok != 0;
 }
 
 // C[ri0.., rj0..] -= A[ri0.., kb .. kb+K) A[rj0.., kb .. kb+K)^T for one 128 x 128 tile
@@ -1365,7 +1401,7 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
             if (g1 < r1) trsm16x2_solve_store<T>(A, ld, c0, g0, g1, r1, lane, x0, x1, sLT, sInv, sc);
             else if (g0 < r1) trsm16_solve_store<T>(A, ld, c0, g0, r1, lane, x0, sLT, sInv, sc);
         }
-        publish_flag(tflags + (long long)k * gstrips + g);
+        publish_flag(tflag(tflags, k, g, gstrips));
         // panel k on the strip's rows of columns cb .. ce of P
         const int cb = c0 + CHOL_NB, ce = min(e.P1, r1);
         if (cb >= ce) continue;
@@ -1389,7 +1425,7 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
                 // tiles of up to 128 columns (LDS-staged, faster in FP32 than the pieces below),
                 // after the flags T_k of the strips holding rows cb .. ce
                 for (int gg = cb / SY_TILE; gg <= (ce - 1) / SY_TILE; ++gg)
-                    if (gg != g && !wg_wait_flag(tflags + (long long)k * gstrips + gg, status)) return;
+                    if (gg != g && !wg_wait_flag(tflag(tflags, k, gg, gstrips), status)) return;
                 for (int rj0 = cb; rj0 < ce; rj0 += SY_TILE) {
                     __syncthreads();   // smem is free (the solve's operands, the previous tile's stages)
                     floatx16 acc[2][2];
@@ -1409,7 +1445,7 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
                 __syncthreads();   // the solve's stores done
                 for (int j = cb; j < ce; j += CHOL_NB) {
                     const int gg = j / SY_TILE;
-                    if (gg != g && !wg_wait_flag(tflags + (long long)k * gstrips + gg, status)) return;
+                    if (gg != g && !wg_wait_flag(tflag(tflags, k, gg, gstrips), status)) return;
                     if (j < r0) block_update64<T, 4>(A, ld, r0, j, c0, CHOL_NB);
                     else if (j == r0) block_update64<T, 4>(A, ld, j, j, c0, CHOL_NB);
                     else block_update64<T, 2>(A, ld, j, j, c0, CHOL_NB);
@@ -1442,8 +1478,7 @@ __device__ __forceinline__ void epoch_lookahead(T* __restrict__ A, long long ld,
         if (e.prevP0 >= 0) tile_acc_f32<SPLIT>(A, ld, e.prevP0, e.P0 - e.prevP0, ri0, rj0, n_pad, smem, acc);
         for (int c0 = e.P0; c0 < e.P1; c0 += CHOL_NB) {
             const int k = c0 / CHOL_NB;
-            if (!wg_wait_flag(tflags + (long long)k * gstrips + gi, status)) return;
-            if (gj != gi && !wg_wait_flag(tflags + (long long)k * gstrips + gj, status)) return;
+            if (!wg_wait_flags2<16>(tflag(tflags, k, gi, gstrips), tflag(tflags, k, gj, gstrips), status)) return;
             tile_acc_f32<SPLIT>(A, ld, c0, CHOL_NB, ri0, rj0, n_pad, smem, acc);
         }
         tile_epilogue_f32(A, ld, ri0, rj0, e.Q1, n_pad, acc, false, nullptr);
@@ -1458,8 +1493,7 @@ __device__ __forceinline__ void epoch_lookahead(T* __restrict__ A, long long ld,
         if (e.prevP0 >= 0) tile_acc_f64(A, ld, e.prevP0, e.P0 - e.prevP0, ri0, rj0, n_pad, smem, acc);
         for (int c0 = e.P0; c0 < e.P1; c0 += CHOL_NB) {
             const int k = c0 / CHOL_NB;
-            if (!wg_wait_flag(tflags + (long long)k * gstrips + gi, status)) return;
-            if (gj != gi && !wg_wait_flag(tflags + (long long)k * gstrips + gj, status)) return;
+            if (!wg_wait_flags2<16>(tflag(tflags, k, gi, gstrips), tflag(tflags, k, gj, gstrips), status)) return;
             tile_acc_f64(A, ld, c0, CHOL_NB, ri0, rj0, n_pad, smem, acc);
         }
         tile_epilogue_f64(A, ld, ri0, rj0, e.Q1, n_pad, acc, false, nullptr);
@@ -1521,7 +1555,7 @@ int chol_factor_epochs(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_s
 {
     const int nblk = n_pad / CHOL_NB, gstrips = n_pad / SY_TILE;
     int* d_flags = reinterpret_cast<int*>(d_Lbuf + (long long)nblk * LB_STRIDE);
-    const int nflags = nblk + nblk * gstrips;
+    const int nflags = nblk + nblk * gstrips * FLAG_STRIDE;
     if (rec) rec->begin(PH_PANEL, st);
     hipLaunchKernelGGL(chol_epoch_clear_kernel, dim3(ceil_div(nflags, 256)), dim3(256), 0, st, d_flags, nflags);
     BDH_LAUNCH_CHECK();
@@ -1534,8 +1568,8 @@ int chol_factor_epochs(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_s
         e.prevP0 = P0 - CHOL_OUTER;
         e.R = (n_pad - P0) / SY_TILE;
         e.NL = e.Q1 > e.P1 ? update_tiles(n_pad, e.P1, e.Q1) : 0;
-        const int nbulk = (e.prevP0 >= 0 && e.Q1 < n_pad) ? update_tiles(n_pad, e.Q1, n_pad) : 0;
-        const dim3 grid(e.R + e.NL + nbulk);
+        e.NB = (e.prevP0 >= 0 && e.Q1 < n_pad) ? update_tiles(n_pad, e.Q1, n_pad) : 0;
+        const dim3 grid(e.R + e.NL + e.NB);
         if (rec) rec->begin(PH_SYRK, st);
         if constexpr (sizeof(T) == 4) {
             if (syrk_mode == SYRK_BF16X3)
@@ -1593,9 +1627,10 @@ int chol_factor(T* d_A, long long ld, int n_pad, double* d_Lbuf, int* d_status, 
 size_t chol_lbuf_doubles(int n_pad)
 {
     const size_t nblk = n_pad / CHOL_NB, gstrips = n_pad / SY_TILE;
-    // FP64 blocks with inverses, then the int flags: one per diagonal block, then the epoch
-    // schedule's solve flags (one per panel and row strip)
-    return nblk * LB_STRIDE + (nblk + nblk * gstrips + 1) / 2;
+    // FP64 blocks with inverses, then the int flags: one per diagonal block, then (epochs
+    // only) the solve flags, one per panel and row strip, each on its own 128-byte line
+    if (n_pad < CHOL_EPOCH_MIN) return nblk * LB_STRIDE + (nblk + 1) / 2;
+    return nblk * LB_STRIDE + (nblk + nblk * gstrips * FLAG_STRIDE + 1) / 2;
 }
 
 // ------------------------------------------------------------------ Y = L X (partials)
