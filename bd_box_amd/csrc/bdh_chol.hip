@@ -89,8 +89,13 @@ __device__ __forceinline__ int wait_lds_count(int* cnt, int need)
 // the own 16x16 sub-block step by step, so it is complete with the last own column.
 // A non-positive (or NaN) pivot records row c0 + j + 1 in *status (the first one wins) and the
 // factorization continues with NaN, as the reference's dpotrf reports INFO.
+// head(), tail(): work of the caller's for the wavefronts' idle stretches (wavefront w's own
+// columns begin only after the 16 w columns before them, and wavefront 0 waits from its last
+// column to the end): each wavefront runs head() once after its block entries are in
+// registers, before it takes the earlier columns, and tail() once after its own columns.
+template <typename Head, typename Tail>
 __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, int* __restrict__ status,
-                                              double* __restrict__ Lb)
+                                              double* __restrict__ Lb, Head&& head, Tail&& tail)
 {
     __shared__ int s_done;   // columns published so far
     __shared__ int s_bad;    // first non-positive pivot (64: none); -1: a wait timed out
@@ -106,6 +111,7 @@ __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, i
         s_bad = CHOL_NB;
     }
     __syncthreads();   // every block entry is in registers: sD is reused for L
+    head();
     double* L = sD;
     bool ok = true;
     // updates by the earlier column groups, two columns at a time as they are published
@@ -202,6 +208,7 @@ __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, i
             }
         }
     }
+    tail();
     if (lane == 0) {
         if (!ok) atomicExch(&s_bad, -1);
         else if (bad < CHOL_NB) atomicMin(&s_bad, bad);
@@ -211,6 +218,12 @@ __device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, i
         if (s_bad < 0) atomicCAS(status, 0, -1);
         else if (s_bad < CHOL_NB) atomicCAS(status, 0, c0 + s_bad + 1);
     }
+}
+
+__device__ __forceinline__ void factor_diag64(double* __restrict__ sD, int c0, int* __restrict__ status,
+                                              double* __restrict__ Lb)
+{
+    factor_diag64(sD, c0, status, Lb, [] {}, [] {});
 }
 
 // L (lower part) of the factored block, L[c * 64 + r] in LDS, into A, in T.
@@ -1308,14 +1321,25 @@ __device__ __forceinline__ void diag_block_update(T* __restrict__ A, long long l
     quad_sub(A, ld, c1 + 32 * qr, c1 + 32 * qc, acc.v[0], sD + 32 * qr * DG_LD + 32 * qc, DG_LD);
 }
 
+// One wavefront: A[ra + i][ca + j] -= sum_k A[ra + i][kb + k] A[ca + j][kb + k] over the
+// 32 x 32 NQ quadrants at rows ra .., columns ca, ca + 32, ...
+template <typename T, int NQ>
+__device__ __forceinline__ void wave_quads_update(T* __restrict__ A, long long ld, int ra, int ca, int kb, int K)
+{
+    QuadAcc<T, NQ> acc;
+    quad_acc<NQ>(A, ld, ra, ca, kb, K, acc.v);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) quad_sub(A, ld, ra, ca + 32 * q, acc.v[q], nullptr, 0);
+}
+
 // A[rb + i][cb + j] -= sum_k A[rb + i][kb + k] A[cb + j][kb + k] for the rows rb .. rb + 32 W
-// of the 64 columns cb .. (the block lies below the diagonal): wavefront w < W takes rows
-// rb + 32 w .. with both column quadrants.
-template <typename T, int W>
+// of the 64 columns cb .. (the block lies below the diagonal): wavefront W0 + w, w < W, takes
+// rows rb + 32 w .. with both column quadrants.
+template <typename T, int W, int W0 = 0>
 __device__ __forceinline__ void block_update64(T* __restrict__ A, long long ld, int rb, int cb, int kb, int K)
 {
-    const int wave = threadIdx.x >> 6;
-    if (wave >= W) return;
+    const int wave = (threadIdx.x >> 6) - W0;
+    if (wave < 0 || wave >= W) return;
     const int ra = rb + 32 * wave;
     QuadAcc<T, 2> acc;
     quad_acc<2>(A, ld, ra, cb, kb, K, acc.v);
@@ -1351,6 +1375,7 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
     double* sc = sInv + 4 * 256 + wave * 16 * SC_PITCH;
     bool block_in_lds = false;   // sD holds the next diagonal block, fully updated
     int pend = -1;   // first column of panel kA - 1 while it is pending on the lower block row
+    int kdef = -1;   // first column of panel kA - 2 while it is deferred on block kB
     for (int c0 = e.P0; c0 < e.P1 && c0 < r1; c0 += CHOL_NB) {
         const int k = c0 / CHOL_NB;
         double* Lb = Lbuf + (long long)k * LB_STRIDE;
@@ -1373,15 +1398,36 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
                     sD[r * DG_LD + c] = (r >= c) ? (double)v[i] : 0.0;
                 }
             }
-            factor_diag64(sD, c0, status, Lb);   // begins with a barrier; leaves sLT / sInv
+            rs = c0 + CHOL_NB;
+            // Made in the factor's idle stretches (the operand rows were solved by this
+            // workgroup, behind a barrier): the pending panel on block (kB, kA), which the solve
+            // below reads (wavefront 3, before its columns), and the deferred and pending panels
+            // on block kB, so that its update after the solve takes panel kA alone (quadrants
+            // (1, 0), (1, 1) by wavefront 2 before its columns, (0, 0) by wavefront 0 after).
+            // In FP64 (slower MFMA, shorter idle stretches to fill) only block (kB, kA) is made
+            // there, by wavefronts 2 and 3, and block kB takes its panels after the solve.
+            const bool lower = pend >= 0 && rs < r1;
+            const int klo = kdef >= 0 ? kdef : pend, kK = lower ? pend + CHOL_NB - klo : 0;
+            factor_diag64(sD, c0, status, Lb,
+                [&] {
+                    if (!lower) return;
+                    if constexpr (sizeof(T) == 4) {
+                        if (wave == 3) {
+                            wave_quads_update<T, 2>(A, ld, rs, c0, pend, CHOL_NB);
+                            wave_quads_update<T, 2>(A, ld, rs + 32, c0, pend, CHOL_NB);
+                        } else if (wave == 2) {
+                            wave_quads_update<T, 2>(A, ld, rs + 32, rs, klo, kK);
+                        }
+                    } else {
+                        block_update64<T, 2, 2>(A, ld, rs, c0, pend, CHOL_NB);
+                    }
+                },
+                [&] {
+                    if constexpr (sizeof(T) == 4)
+                        if (lower && wave == 0) wave_quads_update<T, 1>(A, ld, rs, rs, klo, kK);
+                });                              // begins with a barrier; leaves sLT / sInv
             publish_flag(dflags + k);            // Lb (the stores of store_diag64 are not read here)
             store_diag64<T>(sD, A, ld, c0);
-            rs = c0 + CHOL_NB;
-            if (pend >= 0 && rs < r1) {
-                // the pending panel on block (kB, kA), which the solve below reads (the block's
-                // rows of that panel were solved by this workgroup, behind the factor's barriers)
-                block_update64<T, 2>(A, ld, rs, c0, pend, CHOL_NB);
-            }
         }
         block_in_lds = false;
         if (rs < r1) {
@@ -1407,12 +1453,14 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
         if (cb >= ce) continue;
         if (diag) {
             // block kA was factored: below the strip's diagonal the later columns hold block kB
-            // alone, which takes the pending panel and panel kA (contiguous) into LDS
+            // alone, which takes panel kA into LDS (FP32: the earlier panels came in the
+            // factor; FP64: with the pending panel, contiguous)
             __syncthreads();   // the solve's stores done; smem is free
-            const int kb = pend >= 0 ? pend : c0;
+            const int kb = sizeof(T) == 4 || pend < 0 ? c0 : pend;
             diag_block_update<T>(A, ld, cb, kb, cb - kb, sD);
             block_in_lds = true;
             pend = -1;
+            kdef = -1;
         } else if (cb == r0) {
             // the next panel is block kA: its update (the strip's own rows of panel k) into LDS,
             // the lower block row's left pending
@@ -1421,12 +1469,16 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
             block_in_lds = true;
             pend = c0;
         } else {
+            // on a strip holding diagonal blocks, FP32 leaves block kB's columns to its
+            // factor's idle stretches (kdef)
+            const int cu = sizeof(T) == 4 && r0 < e.P1 ? min(ce, r0 + CHOL_NB) : ce;
+            if (cu < ce) kdef = c0;
             if constexpr (sizeof(T) == 4) {
                 // tiles of up to 128 columns (LDS-staged, faster in FP32 than the pieces below),
-                // after the flags T_k of the strips holding rows cb .. ce
-                for (int gg = cb / SY_TILE; gg <= (ce - 1) / SY_TILE; ++gg)
+                // after the flags T_k of the strips holding rows cb .. cu
+                for (int gg = cb / SY_TILE; gg <= (cu - 1) / SY_TILE; ++gg)
                     if (gg != g && !wg_wait_flag(tflag(tflags, k, gg, gstrips), status)) return;
-                for (int rj0 = cb; rj0 < ce; rj0 += SY_TILE) {
+                for (int rj0 = cb; rj0 < cu; rj0 += SY_TILE) {
                     __syncthreads();   // smem is free (the solve's operands, the previous tile's stages)
                     floatx16 acc[2][2];
 #pragma unroll
@@ -1436,14 +1488,14 @@ __device__ __forceinline__ void epoch_strip(T* __restrict__ A, long long ld, int
 #pragma unroll
                             for (int r = 0; r < 16; ++r) acc[a2][b2][r] = 0.f;
                     tile_acc_f32<SPLIT>(A, ld, c0, CHOL_NB, r0, rj0, n_pad, smem, acc);
-                    tile_epilogue_f32(A, ld, r0, rj0, ce, n_pad, acc, false, nullptr);
+                    tile_epilogue_f32(A, ld, r0, rj0, cu, n_pad, acc, false, nullptr);
                 }
             } else {
                 // 64-column pieces in column order (the next panel's first), each after the
                 // flag T_k of the strip holding its rows; a piece on the strip's own diagonal
                 // blocks takes only the rows from its diagonal down
                 __syncthreads();   // the solve's stores done
-                for (int j = cb; j < ce; j += CHOL_NB) {
+                for (int j = cb; j < cu; j += CHOL_NB) {
                     const int gg = j / SY_TILE;
                     if (gg != g && !wg_wait_flag(tflag(tflags, k, gg, gstrips), status)) return;
                     if (j < r0) block_update64<T, 4>(A, ld, r0, j, c0, CHOL_NB);
