@@ -132,7 +132,6 @@ void init_diff_tensor(void)
         UNERR("HIP backend: IG-T with hydro geyer is not supported");
     if (hydro != DIFF_NONE && hydro != DIFF_CHOLESKY && hydro != DIFF_GEYER)
         UNERR("HIP backend: hydro none / cholesky / geyer");
-    if (vel_grad_tensor) UNERR("HIP backend: no flow field");
     if (bc != BC_NONE && bc != BC_BOX) UNERR("HIP backend: bc none or periodic");
     if (bc == BC_BOX && (box[0] != box[1] || box[0] != box[2]))
         UNERR("HIP backend: periodic box must be a cube");
@@ -168,6 +167,8 @@ void init_diff_tensor(void)
     if (rc) fail("create");
     /* continue the host stream where the driver left it (init_rand in main.c:89) */
     if (bdh_set_rng_state(g_sys, host_rng_state())) fail("rng state");
+    /* a flow field, vel_grad_tensor (diff_equation.c:265-285) */
+    if (vel_grad_tensor && bdh_set_flow(g_sys, vel_grad_tensor)) fail("vel_grad_tensor");
     /* diag_D as init_diff_tensor builds it (diff_decl.c:93-98) */
     preDii = kB_1_6PI * T / (visc * vfactor);
     preDij = kB_1_8PI * T / (visc * vfactor);
@@ -374,11 +375,11 @@ void compute_r(DOUBLE* _D)
     if (bdh_compute_r(g_sys, Rs[0])) fail("compute_r");
 }
 
-/* compute_hydro_ermak (diff_equation.c:216-264): coord_out = coord + Rf + Rs, Rf[0] and Rs[0]
-   the move's parts; the caller evaluates forces at coord_out */
+/* compute_hydro_ermak (diff_equation.c:216-285): coord_out = coord + Rf + Rs (+ dt G coord
+   in a flow field), Rf[0] and Rs[0] the move's parts; the caller evaluates forces at
+   coord_out */
 void compute_hydro_ermak(DOUBLE* coord_out)
 {
-    if (vel_grad_tensor) UNERR("HIP backend: no flow field");
     if (bdh_ermak_move(g_sys, curr_dt > 0 ? curr_dt : dt, coord_out)) fail("compute_hydro_ermak");
     if (bdh_get_last_displacement(g_sys, Rf[0], Rs[0])) fail("displacement");
 }

@@ -164,6 +164,9 @@ struct bdh_system {
     double ew_scale = 0.0;          // Beenakker split (bdh_set_ewald_split): alpha / (sqrt(pi) / L),
                                     // 0 for the reference's split
     int ew_Rr = -1, ew_Rk = -1;     // real / reciprocal sphere radii (-1: ewald_real)
+    bool flow = false;              // a flow field (bdh_set_flow): velocity gradient G
+    double G[9] = {};
+    double* FS = nullptr;           // the flow terms of the last predictor (IG-T corrector)
 };
 
 namespace {
@@ -372,19 +375,39 @@ int cycle_noise(bdh_system* s, const double* X, long long ldx, int m, double dt,
     return 0;
 }
 
+// The flow-field term of a move of length dt (FLOW_NONE without a flow field).
+FlowTerm flow_term(const bdh_system* s, int mode, double dt, double* FS = nullptr)
+{
+    FlowTerm f;
+    if (!s->flow) return f;
+    for (int k = 0; k < 9; ++k) f.G[k] = s->G[k];
+    f.dt = dt;
+    f.FS = FS;
+    f.mode = mode;
+    return f;
+}
+
 // One E-M step.  Cholesky: Rs_dev = this step's precomputed sqrt(2 dt) L xi.
 int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev, double dt)
 {
     const bdh_config& c = s->cfg;
     const double rf_scale = dt / (kB * c.T);
+    const FlowTerm fl = flow_term(s, FLOW_MOVE, dt);
     if (c.hydro == BDH_HYDRO_CHOLESKY) {
         REC_BEGIN(PH_MATVEC);
         if (matvec_dev(s)) return -1;
         REC_END(PH_MATVEC);
         REC_BEGIN(PH_MOVE);
-        if (launch_reduce_move(s->Pf, s->msplits, s->n, rf_scale, s->Rf, s->pos, Rs_dev, c.bc == 1, c.box,
-                               s->st))
+        if (s->flow) {
+            // the flow term needs all three components of a bead: Rf first, then the move
+            ReduceJobs jobs{};
+            jobs.job[0] = ReduceJob{s->Pf, s->Rf, s->n, s->msplits, 0, rf_scale};
+            if (launch_reduce_partials(jobs, 1, s->n, s->st)) return -1;
+            if (launch_em_update(s->pos, s->N, s->Rf, Rs_dev, c.bc == 1, c.box, s->st, &fl)) return -1;
+        } else if (launch_reduce_move(s->Pf, s->msplits, s->n, rf_scale, s->Rf, s->pos, Rs_dev, c.bc == 1,
+                                      c.box, s->st)) {
             return -1;
+        }
         s->last_Rs = Rs_dev;
         REC_END(PH_MOVE);
     } else if (c.hydro == BDH_HYDRO_GEYER) {
@@ -401,14 +424,14 @@ int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev, double d
         if (launch_tea_close(s->pos, xi_dev, s->Feff, s->Cs, s->Pe, pair_eparts(s->N), s->N, s->preDii, dt,
                              s->bt, s->Rs, s->st))
             return -1;
-        if (launch_em_update(s->pos, s->N, s->Rf, s->Rs, c.bc == 1, c.box, s->st)) return -1;
+        if (launch_em_update(s->pos, s->N, s->Rf, s->Rs, c.bc == 1, c.box, s->st, &fl)) return -1;
         REC_END(PH_MOVE);
     } else {
         REC_BEGIN(PH_MOVE);
         hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st,
                            s->pos, s->F, xi_dev, s->N, s->preDii, dt, rf_scale, s->Rf, s->Rs);
         BDH_LAUNCH_CHECK();
-        if (launch_em_update(s->pos, s->N, s->Rf, s->Rs, c.bc == 1, c.box, s->st)) return -1;
+        if (launch_em_update(s->pos, s->N, s->Rf, s->Rs, c.bc == 1, c.box, s->st, &fl)) return -1;
         REC_END(PH_MOVE);
     }
     if (compute_forces_dev(s)) return -1;
@@ -527,7 +550,8 @@ int step_igt(bdh_system* s)
         // predictor
         if (rf_dev(s, s->Rf, dt)) return -1;
         if (!(Rs = rs_dev(s, dt, s->A))) return -1;
-        if (launch_em_update(s->pos, s->N, s->Rf, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+        const FlowTerm pf = flow_term(s, FLOW_MOVE, dt, s->FS);
+        if (launch_em_update(s->pos, s->N, s->Rf, Rs, s->cfg.bc == 1, s->cfg.box, s->st, &pf)) return -1;
         if (forces_flags(s, fl)) return -1;
         if (fl & 1) {
             BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, pb, hipMemcpyDeviceToDevice, s->st));
@@ -547,7 +571,8 @@ int step_igt(bdh_system* s)
         bool ok = false;
         for (int fails = 0;; ++fails) {
             if (!(Rs = rs_dev(s, dt, s->A2))) return -1;
-            if (launch_em_update(s->pos, s->N, s->Rf0, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+            const FlowTerm cf = flow_term(s, FLOW_CORRECTOR, dt, s->FS);
+            if (launch_em_update(s->pos, s->N, s->Rf0, Rs, s->cfg.bc == 1, s->cfg.box, s->st, &cf)) return -1;
             if (forces_flags(s, fl)) return -1;
             if (!(fl & 1)) {
                 ok = true;
@@ -734,7 +759,7 @@ void bdh_destroy(bdh_system* s)
     dfree(s->conn_off); dfree(s->conn); dfree(s->bond_off); dfree(s->bond_p); dfree(s->bond_parm);
     dfree(s->F); dfree(s->E); dfree(s->PF); dfree(s->PE);
     dfree(s->A); dfree(s->Lbuf); dfree(s->xi); dfree(s->Pf); dfree(s->Pr); dfree(s->Rf); dfree(s->Rs);
-    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->A2); dfree(s->Rf0); dfree(s->Rfs); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
+    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->A2); dfree(s->Rf0); dfree(s->Rfs); dfree(s->FS); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
     dfree(s->Px); dfree(s->Pc); dfree(s->Pe); dfree(s->Feff); dfree(s->Cs); dfree(s->bt);
     dfree(s->cl.grid); dfree(s->cl.key); dfree(s->cl.count); dfree(s->cl.fill);
     dfree(s->cl.start); dfree(s->cl.sorted);
@@ -1227,7 +1252,8 @@ int bdh_ermak_move(bdh_system* s, double dt, double* coord_out)
     if (rf_any(s, s->Rf, dt)) return -1;
     const double* Rs = noise_at(s, dt, refactor);
     if (!Rs) return -1;
-    if (launch_em_update(s->pos, s->N, s->Rf, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+    const FlowTerm fl = flow_term(s, FLOW_MOVE, dt, s->FS);
+    if (launch_em_update(s->pos, s->N, s->Rf, Rs, s->cfg.bc == 1, s->cfg.box, s->st, &fl)) return -1;
     BDH_HIP(hipMemcpyAsync(coord_out, s->pos, sizeof(double4) * s->N, hipMemcpyDeviceToHost, s->st));
     BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, sizeof(double4) * s->N, hipMemcpyDeviceToDevice, s->st));
     if (Rs != s->Rs) BDH_HIP(hipMemcpyAsync(s->Rs, Rs, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->st));
@@ -1256,7 +1282,8 @@ int bdh_igt_phase(bdh_system* s, int phase, double dt, double* coord_out, int* r
         if (rf_dev(s, s->Rf, dt)) return -1;
         const double* Rs = rs_dev(s, dt, s->A);
         if (!Rs) return -1;
-        if (launch_em_update(s->pos, s->N, s->Rf, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+        const FlowTerm pf = flow_term(s, FLOW_MOVE, dt, s->FS);
+        if (launch_em_update(s->pos, s->N, s->Rf, Rs, s->cfg.bc == 1, s->cfg.box, s->st, &pf)) return -1;
         if (forces_flags(s, fl)) return -1;
         if (fl & 1) {
             // positions back at r0; the forces stay those of r* (the caller restores them)
@@ -1277,7 +1304,8 @@ int bdh_igt_phase(bdh_system* s, int phase, double dt, double* coord_out, int* r
     // corrector r = r0 + Rf0 + sqrt(2 dt) L_avg xi2 and forces at r
     const double* Rs = rs_dev(s, dt, s->A2);
     if (!Rs) return -1;
-    if (launch_em_update(s->pos, s->N, s->Rf0, Rs, s->cfg.bc == 1, s->cfg.box, s->st)) return -1;
+    const FlowTerm cf = flow_term(s, FLOW_CORRECTOR, dt, s->FS);
+    if (launch_em_update(s->pos, s->N, s->Rf0, Rs, s->cfg.bc == 1, s->cfg.box, s->st, &cf)) return -1;
     if (forces_flags(s, fl)) return -1;
     BDH_HIP(hipMemcpyAsync(coord_out, s->pos, pb, hipMemcpyDeviceToHost, s->st));
     BDH_HIP(hipStreamSynchronize(s->st));
@@ -1315,6 +1343,22 @@ int bdh_set_ewald_split(bdh_system* s, double alpha_scale, int R_real, int R_rec
         s->factor_exec = nullptr;
     }
     return ensure_ewald(s);
+}
+
+int bdh_set_flow(bdh_system* s, const double* G)
+{
+    BDH_CHECK(s, "bdh_set_flow: null system");
+    BDH_HIP(hipSetDevice(s->device));
+    if (!G) {
+        s->flow = false;
+        return 0;
+    }
+    for (int k = 0; k < 9; ++k) BDH_CHECK(std::isfinite(G[k]), "bdh_set_flow: G must be finite");
+    if (!s->FS && dmalloc(&s->FS, sizeof(double) * s->n)) return -1;
+    BDH_HIP(hipMemsetAsync(s->FS, 0, sizeof(double) * s->n, s->st));
+    for (int k = 0; k < 9; ++k) s->G[k] = G[k];
+    s->flow = true;
+    return 0;
 }
 
 int bdh_set_forces(bdh_system* s, const double* F)
@@ -1431,6 +1475,7 @@ static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const dou
     // which the given-xi step does not follow
     BDH_CHECK(!s->cfg.check_overlap && s->cfg.algorithm == BDH_ERMAK_CONST,
               "bdh_step_vjp: ermak_const steps without check_overlap only");
+    BDH_CHECK(!s->flow, "bdh_step_vjp: no flow field");
     BDH_HIP(hipSetDevice(s->device));
     const int n = s->n, np = s->n_pad;
     const size_t nn = (size_t)np * np;

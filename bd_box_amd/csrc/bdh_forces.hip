@@ -410,16 +410,38 @@ int launch_reduce_partials(const ReduceJobs& jobs, int njobs, int max_n, hipStre
 
 // ------------------------------------------------------------------ E-M position update
 // r_i <- r_i + Rf_i + Rs_i, then the periodic wrap of trans_to_box (diff_equation.c:254-264,
-// trans.c:101-107).
+// trans.c:101-107).  MODE adds a flow field's term (FlowTerm); its products are rounded one
+// by one, as the reference's host code forms them.
+template <int MODE>
 __global__ void em_update_kernel(double4* __restrict__ pos, int N, const double* __restrict__ Rf,
-                                 const double* __restrict__ Rs, int pbc, double box)
+                                 const double* __restrict__ Rs, int pbc, double box, FlowTerm fl)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double4 p = pos[i];
-    p.x = (Rf[3 * i] + Rs[3 * i]) + p.x;
-    p.y = (Rf[3 * i + 1] + Rs[3 * i + 1]) + p.y;
-    p.z = (Rf[3 * i + 2] + Rs[3 * i + 2]) + p.z;
+    if constexpr (MODE == FLOW_NONE) {
+        p.x = (Rf[3 * i] + Rs[3 * i]) + p.x;
+        p.y = (Rf[3 * i + 1] + Rs[3 * i + 1]) + p.y;
+        p.z = (Rf[3 * i + 2] + Rs[3 * i + 2]) + p.z;
+    } else if constexpr (MODE == FLOW_MOVE) {
+#pragma clang fp contract(off)
+        const double* G = fl.G;
+        const double t0 = fl.dt * (G[0] * p.x + G[1] * p.y + G[2] * p.z);
+        const double t1 = fl.dt * (G[3] * p.x + G[4] * p.y + G[5] * p.z);
+        const double t2 = fl.dt * (G[6] * p.x + G[7] * p.y + G[8] * p.z);
+        p.x = Rf[3 * i] + Rs[3 * i] + p.x + t0;
+        p.y = Rf[3 * i + 1] + Rs[3 * i + 1] + p.y + t1;
+        p.z = Rf[3 * i + 2] + Rs[3 * i + 2] + p.z + t2;
+        if (fl.FS) {
+            fl.FS[3 * i] = t0 + fl.dt * (G[0] * p.x + G[1] * p.y + G[2] * p.z);
+            fl.FS[3 * i + 1] = t1 + fl.dt * (G[3] * p.x + G[4] * p.y + G[5] * p.z);
+            fl.FS[3 * i + 2] = t2 + fl.dt * (G[6] * p.x + G[7] * p.y + G[8] * p.z);
+        }
+    } else {
+        p.x = p.x + (Rf[3 * i] + Rs[3 * i] + 0.5 * fl.FS[3 * i]);
+        p.y = p.y + (Rf[3 * i + 1] + Rs[3 * i + 1] + 0.5 * fl.FS[3 * i + 1]);
+        p.z = p.z + (Rf[3 * i + 2] + Rs[3 * i + 2] + 0.5 * fl.FS[3 * i + 2]);
+    }
     if (pbc) {
         const double inv = 1.0 / box;
         p.x -= box * floor(p.x * inv + 0.5);
@@ -485,10 +507,19 @@ int launch_reduce_move(const double* d_P, int S, int n, double scale, double* d_
 }
 
 int launch_em_update(double4* d_pos, int N, const double* d_Rf, const double* d_Rs, int pbc,
-                     double box, hipStream_t st)
+                     double box, hipStream_t st, const FlowTerm* flow)
 {
-    hipLaunchKernelGGL(em_update_kernel, dim3(ceil_div(N, 128)), dim3(128), 0, st, d_pos, N, d_Rf,
-                       d_Rs, pbc, box);
+    const FlowTerm fl = flow ? *flow : FlowTerm{};
+    const dim3 grid(ceil_div(N, 128)), block(128);
+    if (fl.mode == FLOW_MOVE) {
+        hipLaunchKernelGGL(em_update_kernel<FLOW_MOVE>, grid, block, 0, st, d_pos, N, d_Rf, d_Rs, pbc, box, fl);
+    } else if (fl.mode == FLOW_CORRECTOR) {
+        BDH_CHECK(fl.FS != nullptr, "em_update: the corrector's flow term needs FS");
+        hipLaunchKernelGGL(em_update_kernel<FLOW_CORRECTOR>, grid, block, 0, st, d_pos, N, d_Rf, d_Rs, pbc, box,
+                           fl);
+    } else {
+        hipLaunchKernelGGL(em_update_kernel<FLOW_NONE>, grid, block, 0, st, d_pos, N, d_Rf, d_Rs, pbc, box, fl);
+    }
     BDH_LAUNCH_CHECK();
     return 0;
 }

@@ -215,8 +215,19 @@ int launch_forces_batch(const double4* d_pos, const double* d_Q, const double2* 
 // launch_em_update with that Rf; one launch, bitwise the same as the two.
 int launch_reduce_move(const double* d_P, int S, int n, double scale, double* d_Rf, double4* d_pos,
                        const double* d_Rs, int pbc, double box, hipStream_t st);
+// A flow field's share of the move, vel_grad_tensor (diff_equation.c:265-285, :304-312):
+//   FLOW_MOVE       r' = ((Rf + Rs) + r) + t with t = dt G r, and, when FS is set,
+//                   FS = t + dt G r' (r' before the box wrap);
+//   FLOW_CORRECTOR  r' = r + ((Rf + Rs) + FS / 2), the IG-T corrector.
+enum { FLOW_NONE = 0, FLOW_MOVE = 1, FLOW_CORRECTOR = 2 };
+struct FlowTerm {
+    double G[9] = {};        // velocity gradient, row-major (row, column)
+    double dt = 0.0;
+    double* FS = nullptr;    // 3N: written by FLOW_MOVE (if set), read by FLOW_CORRECTOR
+    int mode = FLOW_NONE;
+};
 int launch_em_update(double4* d_pos, int N, const double* d_Rf, const double* d_Rs, int pbc,
-                     double box, hipStream_t st);
+                     double box, hipStream_t st, const FlowTerm* flow = nullptr);
 
 // ---- reverse-mode derivative of an FP64 step (bdh_adjoint.hip)
 int launch_adj_lt_g(const double* d_L, long long ld, int n, const double* d_g, double* d_a, hipStream_t st);

@@ -74,6 +74,8 @@ class BDSystem:
                                 bonds.ctypes.data_as(C.POINTER(C.c_int)), _lib.dptr(parms),
                                 C.byref(h)), "bdh_create")
         self._h = h
+        if P.vel_grad_tensor is not None:
+            self.set_flow(P.vel_grad_tensor)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -184,6 +186,15 @@ class BDSystem:
         Rr = self.cfg.ewald_real if R_real is None else int(R_real)
         Rk = self.cfg.ewald_real if R_recip is None else int(R_recip)
         _lib.check(_lib.load().bdh_set_ewald_split(self._h, float(alpha_scale), Rr, Rk), "set_ewald_split")
+
+    def set_flow(self, G=None):
+        """A flow field with velocity gradient G (3x3, row, column; vel_grad_tensor,
+        diff_equation.c:265-285), or none for G None (bdh_set_flow)."""
+        if G is None:
+            _lib.check(_lib.load().bdh_set_flow(self._h, None), "set_flow")
+            return
+        self._G = np.ascontiguousarray(G, float).reshape(9)
+        _lib.check(_lib.load().bdh_set_flow(self._h, _lib.dptr(self._G)), "set_flow")
 
     def set_forces(self, F):
         f = np.ascontiguousarray(F, float).reshape(-1)

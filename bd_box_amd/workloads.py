@@ -302,6 +302,7 @@ class Params:
     nb_list: int
     geyer_on_the_fly: int
     move_attempts: int = 10000
+    vel_grad_tensor: Optional[Tuple[float, ...]] = None   # G (row, column) of a flow field
 
 
 _DEFAULTS = dict(  # code defaults of the reference registry (MAKE_STR_IN lines)
@@ -334,6 +335,14 @@ def parse_params(params: Dict[str, object]) -> Params:
     g = yn("geyer_on_the_fly")
     if hydro != 2 or algorithm >= 3:  # src/input.c:497-501
         g = 0
+    # vel_grad_tensor: nine doubles, row by row (input.c:1110-1125, scanned as %lg)
+    G = raw.get("vel_grad_tensor")
+    if G is not None and not isinstance(G, (tuple, list, np.ndarray)):
+        G = str(G).split()
+    if G is not None:
+        G = tuple(float(x) for x in G)
+        if len(G) != 9:
+            raise ValueError("vel_grad_tensor: nine values")
     return Params(
         dt=dbl("dt"), T=dbl("T"), visc=dbl("visc"), vfactor=dbl("vfactor"),
         alpha_lj=dbl("alpha_lj"), lj_6_term=yn("lj_6_term"), cutoff_lj=dbl("cutoff_lj"),
@@ -344,6 +353,7 @@ def parse_params(params: Dict[str, object]) -> Params:
         ewald_real=int(raw["ewald_real"]), ewald_recip=int(raw["ewald_recip"]),
         ewald_alpha=dbl("ewald_alpha"), box=dbl("xbox"), rand_seed=int(raw["rand_seed"]),
         nb_list=enum("nb_list"), geyer_on_the_fly=g, move_attempts=int(raw["move_attempts"]),
+        vel_grad_tensor=G,
     )
 
 

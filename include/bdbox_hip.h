@@ -177,6 +177,14 @@ int bdh_tea_coefficients(bdh_system* sys, double* beta, double* eps);
  * returns to the reference's tensor.  Periodic systems with ewald_method beenakker. */
 int bdh_set_ewald_split(bdh_system* sys, double alpha_scale, int R_real, int R_recip);
 
+/* A flow field: the velocity gradient tensor G[9] (row, column), the .prm keyword
+ * vel_grad_tensor (src/diff_alg/diff_decl.c:72, parsed at src/input.c:1110-1125); NULL
+ * removes it.  Every move then adds dt G r as compute_hydro_ermak does
+ * (src/diff_alg/diff_equation.c:265-285), and the IG-T corrector adds half of the
+ * predictor's FS = dt G r0 + dt G r* (:304-312).  bdh_step_vjp refuses systems with a flow
+ * field. */
+int bdh_set_flow(bdh_system* sys, const double* G);
+
 /* Replace the forces F[3N] of the current positions, load_forces (diff_decl.c:234-237). */
 int bdh_set_forces(bdh_system* sys, const double* F);
 

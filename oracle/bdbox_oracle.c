@@ -764,6 +764,9 @@ struct orc_sim {
     int algorithm, move_attempts;   /* 0 ermak_const, 1 ermak_var; rejections allowed */
     double last_dt;
     long long rejections;
+    int flow;                       /* vel_grad_tensor set: G (row, column) and FS */
+    double G[9];
+    double* FS;
 };
 
 orc_sim* orc_sim_create(int N, const double* coord, const double* Q, const double* LJ,
@@ -803,6 +806,49 @@ orc_sim* orc_sim_create(int N, const double* coord, const double* Q, const doubl
     return s;
 }
 
+/* The move of compute_hydro_ermak in a flow field (diff_equation.c:265-285): with
+   t = dt G r at the step's start, r' = ((a + b) + r) + t, and FS = t + dt G r' (r' before
+   the box wrap), which the IG-T corrector adds half of (sim_move_fs). */
+static void sim_move_flow(orc_sim* s, const double* a, const double* b, double dt)
+{
+    const double* G = s->G;
+    int i, k;
+    for (i = 0; i < s->N; ++i)
+    {
+        double* c = s->coord + 4 * i;
+        double t[3], o[3];
+        for (k = 0; k < 3; ++k) t[k] = dt * (G[3 * k] * c[0] + G[3 * k + 1] * c[1] + G[3 * k + 2] * c[2]);
+        for (k = 0; k < 3; ++k) o[k] = a[3 * i + k] + b[3 * i + k] + c[k] + t[k];
+        for (k = 0; k < 3; ++k)
+            s->FS[3 * i + k] = t[k] + dt * (G[3 * k] * o[0] + G[3 * k + 1] * o[1] + G[3 * k + 2] * o[2]);
+        for (k = 0; k < 3; ++k)
+        {
+            c[k] = o[k];
+            if (s->p.bc == 1)
+            {
+                double inv = 1 / s->p.box;
+                c[k] -= s->p.box * floor(c[k] * inv + 0.5f);
+            }
+        }
+    }
+}
+
+/* The IG-T corrector in a flow field (diff_equation.c:304-312): r += (a + b) + FS / 2. */
+static void sim_move_fs(orc_sim* s, const double* a, const double* b)
+{
+    int i, k;
+    for (i = 0; i < s->N; ++i)
+        for (k = 0; k < 3; ++k)
+        {
+            s->coord[4 * i + k] += a[3 * i + k] + b[3 * i + k] + 0.5 * s->FS[3 * i + k];
+            if (s->p.bc == 1)
+            {
+                double inv = 1 / s->p.box;
+                s->coord[4 * i + k] -= s->p.box * floor(s->coord[4 * i + k] * inv + 0.5f);
+            }
+        }
+}
+
 /* one attempt of a step of length dt from the current state: Rf, Rs, move, forces;
    returns orc_forces' status (> 0 repeat, < 0 overstretched bond) */
 static int sim_attempt(orc_sim* s, double dt, const double* X_in, double preDii)
@@ -833,19 +879,22 @@ static int sim_attempt(orc_sim* s, double dt, const double* X_in, double preDii)
         orc_geyer_rs(N, s->coord, &s->p, dt, s->X, s->Rs);
     else
         for (i = 0; i < n; ++i) s->Rs[i] = s->X[i] * sqrt((preDii / s->coord[4 * (i / 3) + 3]) * 2 * dt);
-    /* move (diff_equation.c:254-264) */
-    for (i = 0; i < N; ++i)
-    {
-        int k;
-        for (k = 0; k < 3; ++k)
-            s->coord[4 * i + k] = s->Rf[3 * i + k] + s->Rs[3 * i + k] + s->coord[4 * i + k];
-        if (s->p.bc == 1)
+    /* move (diff_equation.c:254-285) */
+    if (s->flow)
+        sim_move_flow(s, s->Rf, s->Rs, dt);
+    else
+        for (i = 0; i < N; ++i)
         {
-            double inv = 1 / s->p.box;
+            int k;
             for (k = 0; k < 3; ++k)
-                s->coord[4 * i + k] -= s->p.box * floor(s->coord[4 * i + k] * inv + 0.5f);
+                s->coord[4 * i + k] = s->Rf[3 * i + k] + s->Rs[3 * i + k] + s->coord[4 * i + k];
+            if (s->p.bc == 1)
+            {
+                double inv = 1 / s->p.box;
+                for (k = 0; k < 3; ++k)
+                    s->coord[4 * i + k] -= s->p.box * floor(s->coord[4 * i + k] * inv + 0.5f);
+            }
         }
-    }
     return orc_forces(N, s->coord, s->Q, s->LJ, s->nb, s->bonds, s->bond_parms, &s->p, s->F, NULL);
 }
 
@@ -929,7 +978,8 @@ static int sim_step_igt(orc_sim* s, double preDii)
         /* predictor */
         sim_rf(s, s->D, pre, preDii, s->Rf);
         sim_rs(s, s->L, dt, preDii);
-        sim_move(s, s->Rf, s->Rs);
+        if (s->flow) sim_move_flow(s, s->Rf, s->Rs, dt);
+        else sim_move(s, s->Rf, s->Rs);
         rep = orc_forces(N, s->coord, s->Q, s->LJ, s->nb, s->bonds, s->bond_parms, &s->p, s->F, NULL);
         if (rep < 0) return 2;
         if (rep && s->p.check_overlap)
@@ -955,7 +1005,8 @@ static int sim_step_igt(orc_sim* s, double preDii)
         for (;;)
         {
             sim_rs(s, s->L2, dt, preDii);
-            sim_move(s, s->Rf0, s->Rs);
+            if (s->flow) sim_move_fs(s, s->Rf0, s->Rs);
+            else sim_move(s, s->Rf0, s->Rs);
             rep = orc_forces(N, s->coord, s->Q, s->LJ, s->nb, s->bonds, s->bond_parms, &s->p, s->F, NULL);
             if (rep < 0) return 2;
             if (!(rep && s->p.check_overlap)) break;
@@ -1043,6 +1094,14 @@ void orc_sim_set_algorithm(orc_sim* s, int algorithm, int move_attempts)
     }
 }
 
+void orc_sim_set_flow(orc_sim* s, const double* G)
+{
+    s->flow = G != NULL;
+    if (!G) return;
+    memcpy(s->G, G, sizeof(s->G));
+    if (!s->FS) s->FS = (double*)calloc(3 * (size_t)s->N, sizeof(double));
+}
+
 double orc_sim_last_dt(const orc_sim* s) { return s->last_dt; }
 long long orc_sim_rejections(const orc_sim* s) { return s->rejections; }
 
@@ -1061,6 +1120,6 @@ void orc_sim_destroy(orc_sim* s)
     free(s->coord); free(s->Q); free(s->LJ); free(s->bonds); free(s->bond_parms);
     free(s->F); free(s->D); free(s->L); free(s->Rf); free(s->Rs); free(s->X);
     free(s->coord_prev); free(s->F_prev);
-    free(s->D1); free(s->L2); free(s->Rf0);
+    free(s->D1); free(s->L2); free(s->Rf0); free(s->FS);
     free(s);
 }

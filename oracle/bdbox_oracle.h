@@ -108,6 +108,9 @@ int  orc_sim_run(orc_sim* s, int steps, const double* xi);
    hydro none or cholesky and stream noise) and the rejection limit (default 10000,
    diff_decl.c:71) */
 void orc_sim_set_algorithm(orc_sim* s, int algorithm, int move_attempts);
+/* the velocity gradient tensor G[9] (row, column) of a flow field, vel_grad_tensor
+   (diff_decl.c:72, diff_equation.c:265-285, :304-312); NULL for none */
+void orc_sim_set_flow(orc_sim* s, const double* G);
 double orc_sim_last_dt(const orc_sim* s);          /* dt of the last step taken */
 long long orc_sim_rejections(const orc_sim* s);    /* rejected attempts so far */
 void orc_sim_coords(const orc_sim* s, double* coord_out);

@@ -72,6 +72,7 @@ def lib():
         L.orc_sim_rng_state.restype = C.c_uint64
         L.orc_sim_destroy.argtypes = [C.c_void_p]
         L.orc_sim_set_algorithm.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_sim_set_flow.argtypes = [C.c_void_p, _dp]
         L.orc_sim_last_dt.argtypes = [C.c_void_p]
         L.orc_sim_last_dt.restype = C.c_double
         L.orc_sim_rejections.argtypes = [C.c_void_p]
@@ -178,7 +179,7 @@ class Sim:
     """Stateful FP64 Ermak-McCammon integrator (orc_sim_*)."""
 
     def __init__(self, sysm, p: OrcParams, seed: int, refactor_every: int = 1, coord=None,
-                 algorithm: int = 0, move_attempts: int = 10000):
+                 algorithm: int = 0, move_attempts: int = 10000, flow=None):
         self.N = sysm.N
         c = np.ascontiguousarray(sysm.coord if coord is None else coord, float).reshape(-1)
         self._keep = [c, np.ascontiguousarray(sysm.Q, float),
@@ -191,6 +192,10 @@ class Sim:
                                       self._keep[3].ctypes.data_as(_ip), _d(self._keep[4]),
                                       C.byref(self.p), seed, refactor_every)
         lib().orc_sim_set_algorithm(self.h, algorithm, move_attempts)
+        if flow is not None:
+            # velocity gradient tensor (row, column), vel_grad_tensor (diff_decl.c:72)
+            self._G = np.ascontiguousarray(flow, float).reshape(9)
+            lib().orc_sim_set_flow(self.h, _d(self._G))
 
     def run(self, steps: int, xi=None):
         x = None if xi is None else np.ascontiguousarray(xi, float).reshape(-1)

@@ -171,6 +171,21 @@ def main():
                                           check_overlap="yes", move_attempts=200,
                                           rand_seed=11),
              [["traj", "8"]])
+    # 8. a flow field (vel_grad_tensor, diff_equation.c:265-285, :304-312): every entry of G
+    #    nonzero and distinct, so a transposed or misplaced entry changes the trajectory;
+    #    E-M with Cholesky and TEA noise, IG-T (the FS half-term), and rejected steps
+    G = "0.1 2.0 -0.3 0.4 -0.2 0.5 -0.6 0.7 0.15"
+    case("flow_ermak", s16i, wl.chain_params(vel_grad_tensor=G, rand_seed=4242), [["traj", "6"]])
+    case("flow_geyer", s24, wl.chain_params(hydro="geyer", vel_grad_tensor=G, rand_seed=99),
+         [["traj", "4"]])
+    case("flow_igt", s16i, wl.chain_params(algorithm="igt_const", vel_grad_tensor=G, rand_seed=4242),
+         [["traj", "6"]])
+    Gs = "0.002 0.04 -0.006 0.008 -0.004 0.01 -0.012 0.014 0.003"
+    for name, alg, hydro in (("flow_igt_retry", "igt_const", "cholesky"),
+                             ("flow_var_retry_none", "ermak_var", "none")):
+        case(name, s24r, wl.chain_params(hydro=hydro, algorithm=alg, dt=0.5, check_overlap="yes",
+                                          move_attempts=200, rand_seed=11, vel_grad_tensor=Gs),
+             [["traj", "8"]])
 
 
 if __name__ == "__main__":

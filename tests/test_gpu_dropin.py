@@ -83,6 +83,10 @@ DROPIN_CASES = {
     "bucket_space_periodic": ({}, dict(hydro="none", bc="periodic", xbox=60.0, ybox=60.0,
                                        zbox=60.0, nb_list="bucket_space")),
     "bucket_space": ({}, dict(hydro="none", nb_list="bucket_space")),
+    # a flow field (vel_grad_tensor, diff_equation.c:265-285, :304-312)
+    "flow": ({}, dict(hydro="cholesky", vel_grad_tensor="0.1 2.0 -0.3 0.4 -0.2 0.5 -0.6 0.7 0.15")),
+    "flow_igt": ({}, dict(hydro="cholesky", algorithm="igt_const",
+                          vel_grad_tensor="0.1 2.0 -0.3 0.4 -0.2 0.5 -0.6 0.7 0.15")),
 }
 
 

@@ -357,3 +357,60 @@ def test_smith_differs_from_beenakker_method(golden):
     p.ewald_method = 1
     Db = po.compute_D(coord, p)
     assert np.max(np.abs(Ds - Db)) > 1e-3 * np.max(np.abs(Db))
+
+
+FLOW_CASES = [("flow_ermak", 6), ("flow_geyer", 4), ("flow_igt", 6), ("flow_igt_retry", 8),
+              ("flow_var_retry_none", 8)]
+
+
+@pytest.mark.parametrize("case,steps", FLOW_CASES)
+def test_flow_field_vs_reference(golden, case, steps):
+    """A flow field, vel_grad_tensor (compute_hydro_ermak's move r + Rf + Rs + dt G r and its
+    FS = dt G r + dt G r', diff_equation.c:265-285; the IG-T corrector's half FS, :304-312),
+    against the reference's own trajectories and per-step curr_dt, rejections included."""
+    g, sysm, P, p = golden(case)
+    assert P.vel_grad_tensor is not None
+    sim = po.Sim(sysm, p, seed=P.rand_seed, algorithm=P.algorithm,
+                 move_attempts=P.move_attempts, flow=P.vel_grad_tensor)
+    traj = g["traj.coord"].reshape(steps, -1, 4)
+    for k in range(steps):
+        sim.run(1)
+        assert sim.last_dt() == g["traj.dt"][k], k
+        err = np.max(np.abs(sim.coords() - traj[k]))
+        assert err < 1e-11, (k, err)
+    if "retry" in case:
+        assert sim.rejections() > 0
+
+
+@pytest.mark.parametrize("case", ["flow_ermak", "flow_igt"])
+def test_flow_field_orientation_matters(golden, case):
+    """The goldens discriminate: G transposed, or no flow at all, moves the first step by far
+    more than the tolerance above (G's entries are all distinct)."""
+    g, sysm, P, p = golden(case)
+    ref = g["traj.coord"].reshape(6, -1, 4)[0]
+    G = np.array(P.vel_grad_tensor).reshape(3, 3)
+    for flow in (G.T.copy(), None):
+        sim = po.Sim(sysm, p, seed=P.rand_seed, algorithm=P.algorithm,
+                     move_attempts=P.move_attempts, flow=flow)
+        sim.run(1)
+        assert np.max(np.abs(sim.coords() - ref)) > 1e-3
+
+
+def test_flow_field_closed_form_without_noise():
+    """With no forces and no noise (T small, no interactions), one E-M step in a flow field is
+    r' = r + dt G r exactly: a plain shear moves each bead by dt * rate * y along x."""
+    c = wl.chain(6, seed=2)
+    sysm = wl.System(c.coord, np.zeros(6), c.LJ, np.zeros((0, 2), np.int32), np.zeros((0, 3)))
+    prm = wl.chain_params(hydro="none", elec="no", alpha_lj=0.0, T=1e-30)
+    P = wl.parse_params(prm)
+    p = po.params_from(P)
+    rate = 0.7
+    G = np.zeros(9)
+    G[1] = rate
+    sim = po.Sim(sysm, p, seed=1, flow=G)
+    c0 = sysm.coord.copy()
+    sim.run(1)
+    c1 = sim.coords()
+    dt = float(P.dt)
+    np.testing.assert_allclose(c1[:, 0], c0[:, 0] + dt * rate * c0[:, 1], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(c1[:, 1:3], c0[:, 1:3], rtol=0, atol=1e-12)
