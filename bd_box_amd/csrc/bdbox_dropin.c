@@ -128,8 +128,6 @@ void init_diff_tensor(void)
     if (algorithm != DIFF_ALG_ERMAK_CONST && algorithm != DIFF_ALG_ERMAK_VAR &&
         algorithm != DIFF_ALG_IGT_CONST && algorithm != DIFF_ALG_IGT_VAR)
         UNERR("HIP backend: algorithm ermak_const / ermak_var / igt_const / igt_var");
-    if ((algorithm == DIFF_ALG_IGT_CONST || algorithm == DIFF_ALG_IGT_VAR) && hydro == DIFF_GEYER)
-        UNERR("HIP backend: IG-T with hydro geyer is not supported");
     if (hydro != DIFF_NONE && hydro != DIFF_CHOLESKY && hydro != DIFF_GEYER)
         UNERR("HIP backend: hydro none / cholesky / geyer");
     if (bc != BC_NONE && bc != BC_BOX) UNERR("HIP backend: bc none or periodic");

@@ -167,6 +167,7 @@ struct bdh_system {
     bool flow = false;              // a flow field (bdh_set_flow): velocity gradient G
     double G[9] = {};
     double* FS = nullptr;           // the flow terms of the last predictor (IG-T corrector)
+    double4* pos_star = nullptr;    // IG-T with TEA noise: the predictor's positions r*
 };
 
 namespace {
@@ -341,16 +342,34 @@ int assemble_and_factor_launches(bdh_system* s, void* Abuf, const double4* pos2)
     return 0;
 }
 
-// Pair sums over blocks evaluated on the fly: D.F, and for TEA also D.X, C and eps.
-int matvec_dev(bdh_system* s, const double* xi = nullptr, bool tea = false, double dt = 0.0)
+// Pair sums over blocks evaluated on the fly: D.F, and for TEA also D.X, C and eps (of the
+// mean tensor of the positions and pos2 when pos2 is set).
+int matvec_dev(bdh_system* s, const double* xi = nullptr, bool tea = false, double dt = 0.0,
+               const double4* pos2 = nullptr)
 {
     if (ensure_ewald(s)) return -1;
     if (dt == 0.0) dt = s->cfg.dt;
     if (s->mixed)
         return launch_pair_sums<float>(s->pos, s->F, xi, s->N, s->preDii, s->preDij, dt, s->ewald,
-                                       tea, s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st);
+                                       tea, s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st, pos2);
     return launch_pair_sums<double>(s->pos, s->F, xi, s->N, s->preDii, s->preDij, dt, s->ewald,
-                                    tea, s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st);
+                                    tea, s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st, pos2);
+}
+
+// The Geyer-Winter TEA displacement Rs (into s->Rs) from the Gaussians X, one pass over the
+// pairs (diff_rand.c:86-230), and with Rf_out also Rf = (dt / kT) D F from the same pass.
+// pos2: the sums of the mean tensor of the positions and pos2 (the IG-T corrector).
+int tea_parts(bdh_system* s, const double* X, double dt, double* Rf_out, const double4* pos2 = nullptr)
+{
+    if (matvec_dev(s, X, true, dt, pos2)) return -1;
+    ReduceJobs jobs{};
+    int nj = 0;
+    if (Rf_out) jobs.job[nj++] = ReduceJob{s->Pf, Rf_out, s->n, s->msplits, 0, dt / (kB * s->cfg.T)};
+    jobs.job[nj++] = ReduceJob{s->Px, s->Feff, s->n, s->msplits, 0, 1.0};
+    jobs.job[nj++] = ReduceJob{s->Pc, s->Cs, s->n, s->msplits, 0, 1.0};
+    if (launch_reduce_partials(jobs, nj, s->n, s->st)) return -1;
+    return launch_tea_close(s->pos, X, s->Feff, s->Cs, s->Pe, pair_eparts(s->N), s->N, s->preDii, dt, s->bt,
+                            s->Rs, s->st);
 }
 
 int trmv_dev(bdh_system* s, const double* X, long long ldx, int m, const void* L = nullptr)
@@ -413,17 +432,9 @@ int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev, double d
     } else if (c.hydro == BDH_HYDRO_GEYER) {
         // TEA-HI: one pass over the pairs gives D.F, D.X, C and eps (diff_rand.c:86-230)
         REC_BEGIN(PH_MATVEC);
-        if (matvec_dev(s, xi_dev, true, dt)) return -1;
+        if (tea_parts(s, xi_dev, dt, s->Rf)) return -1;
         REC_END(PH_MATVEC);
         REC_BEGIN(PH_MOVE);
-        ReduceJobs jobs{};
-        jobs.job[0] = ReduceJob{s->Pf, s->Rf, s->n, s->msplits, 0, rf_scale};
-        jobs.job[1] = ReduceJob{s->Px, s->Feff, s->n, s->msplits, 0, 1.0};
-        jobs.job[2] = ReduceJob{s->Pc, s->Cs, s->n, s->msplits, 0, 1.0};
-        if (launch_reduce_partials(jobs, 3, s->n, s->st)) return -1;
-        if (launch_tea_close(s->pos, xi_dev, s->Feff, s->Cs, s->Pe, pair_eparts(s->N), s->N, s->preDii, dt,
-                             s->bt, s->Rs, s->st))
-            return -1;
         if (launch_em_update(s->pos, s->N, s->Rf, s->Rs, c.bc == 1, c.box, s->st, &fl)) return -1;
         REC_END(PH_MOVE);
     } else {
@@ -487,12 +498,12 @@ int step_with_retries(bdh_system* s, const double* xi_given, bool refactor)
     return 0;
 }
 
-// Rf = (dt / kT) D F at the current positions and forces into out (D.F on the fly, or
-// diag_D without hydrodynamics).
+// Rf = (dt / kT) D F at the current positions and forces into out, for every hydro mode
+// (compute_Rf, diff_equation.c:119-214: D.F on the fly, or diag_D without hydrodynamics).
 int rf_dev(bdh_system* s, double* out, double dt)
 {
     const double scale = dt / (kB * s->cfg.T);
-    if (s->cfg.hydro == BDH_HYDRO_CHOLESKY) {
+    if (s->cfg.hydro != BDH_HYDRO_NONE) {
         if (matvec_dev(s)) return -1;
         ReduceJobs jobs{};
         jobs.job[0] = ReduceJob{s->Pf, out, s->n, s->msplits, 0, scale};
@@ -504,13 +515,15 @@ int rf_dev(bdh_system* s, double* out, double dt)
     return 0;
 }
 
-// Rs from the next Gaussians of the device stream: sqrt(2 dt) L xi (L the factor in Lm), or
-// xi sqrt(2 dt diag_D) without hydrodynamics.
-const double* rs_dev(bdh_system* s, double dt, const void* Lm)
+// Rs from the next Gaussians of the device stream (compute_r, diff_rand.c:47-236):
+// sqrt(2 dt) L xi (L the factor in Lm), the TEA displacement (of the mean tensor of the
+// positions and pos2 when pos2 is set), or xi sqrt(2 dt diag_D) without hydrodynamics.
+const double* rs_dev(bdh_system* s, double dt, const void* Lm, const double4* pos2 = nullptr)
 {
     if (draw_noise(s, 1)) return nullptr;
     if (s->cfg.hydro == BDH_HYDRO_CHOLESKY)
         return cycle_noise(s, s->xi, s->n, 1, dt, Lm) ? nullptr : s->RsB;
+    if (s->cfg.hydro == BDH_HYDRO_GEYER) return tea_parts(s, s->xi, dt, nullptr, pos2) ? nullptr : s->Rs;
     hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st, s->pos,
                        s->F, (const double*)s->xi, s->N, s->preDii, dt, 0.0, (double*)nullptr, s->Rs);
     return hipGetLastError() == hipSuccess ? s->Rs : nullptr;
@@ -532,6 +545,9 @@ int forces_flags(bdh_system* s, int& fl)
 //   predictor  r* = r0 + Rf(r0) + sqrt(2 dt) L(r0) xi1, forces at r*;
 //   corrector  r  = r0 + (Rf(r0) + Rf(r*)) / 2 + sqrt(2 dt) L_avg xi2, forces at r,
 //              L_avg the factor of (D(r0) + D(r*)) / 2.
+// With TEA noise the two random moves are the TEA displacements of D(r0) and of
+// (D(r0) + D(r*)) / 2, whose blocks are re-evaluated at r0 and r* (input.c:497-501 makes the
+// reference read them from its stored tensors).
 // A rejected predictor is redrawn with L(r0) kept (igt_var also halves dt).  A rejected
 // corrector is redrawn alone with L_avg kept (igt_const; its first rejection is not
 // counted), or undoes the whole step (igt_var: dt halved, L(r0) kept).
@@ -566,11 +582,12 @@ int step_igt(bdh_system* s)
                            s->n, s->Rf0);
         BDH_LAUNCH_CHECK();
         if (chol && assemble_and_factor(s, s->A2, s->pos_prev)) return -1;
+        if (s->tea) BDH_HIP(hipMemcpyAsync(s->pos_star, s->pos, pb, hipMemcpyDeviceToDevice, s->st));
         BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, pb, hipMemcpyDeviceToDevice, s->st));
         // corrector
         bool ok = false;
         for (int fails = 0;; ++fails) {
-            if (!(Rs = rs_dev(s, dt, s->A2))) return -1;
+            if (!(Rs = rs_dev(s, dt, s->A2, s->pos_star))) return -1;
             const FlowTerm cf = flow_term(s, FLOW_CORRECTOR, dt, s->FS);
             if (launch_em_update(s->pos, s->N, s->Rf0, Rs, s->cfg.bc == 1, s->cfg.box, s->st, &cf)) return -1;
             if (forces_flags(s, fl)) return -1;
@@ -759,7 +776,7 @@ void bdh_destroy(bdh_system* s)
     dfree(s->conn_off); dfree(s->conn); dfree(s->bond_off); dfree(s->bond_p); dfree(s->bond_parm);
     dfree(s->F); dfree(s->E); dfree(s->PF); dfree(s->PE);
     dfree(s->A); dfree(s->Lbuf); dfree(s->xi); dfree(s->Pf); dfree(s->Pr); dfree(s->Rf); dfree(s->Rs);
-    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->A2); dfree(s->Rf0); dfree(s->Rfs); dfree(s->FS); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
+    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->A2); dfree(s->Rf0); dfree(s->Rfs); dfree(s->FS); dfree(s->pos_star); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
     dfree(s->Px); dfree(s->Pc); dfree(s->Pe); dfree(s->Feff); dfree(s->Cs); dfree(s->bt);
     dfree(s->cl.grid); dfree(s->cl.key); dfree(s->cl.count); dfree(s->cl.fill);
     dfree(s->cl.start); dfree(s->cl.sorted);
@@ -819,7 +836,6 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     const bool igt = cfg->algorithm == BDH_IGT_CONST || cfg->algorithm == BDH_IGT_VAR;
     BDH_CHECK(cfg->algorithm == BDH_ERMAK_CONST || cfg->algorithm == BDH_ERMAK_VAR || igt,
               "bdh_create: algorithm (ermak_const, ermak_var, igt_const or igt_var)");
-    BDH_CHECK(!igt || cfg->hydro != BDH_HYDRO_GEYER, "bdh_create: IG-T with TEA noise is not supported");
     BDH_CHECK(!igt || cfg->refactor_every <= 1, "bdh_create: IG-T factors D at every step (refactor_every 1)");
     BDH_CHECK(cfg->move_attempts >= 0, "bdh_create: move_attempts");
     s->cfg = *cfg;
@@ -936,6 +952,7 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     if (igt) {
         if (dmalloc(&s->Rf0, sizeof(double) * s->n)) return -1;
         if (dmalloc(&s->Rfs, sizeof(double) * s->n)) return -1;
+        if (s->tea && dmalloc(&s->pos_star, sizeof(double4) * N)) return -1;
     }
     if (dmalloc(&s->Rf, sizeof(double) * s->n)) return -1;
     if (dmalloc(&s->Rs, sizeof(double) * s->n)) return -1;
@@ -1165,40 +1182,13 @@ int bdh_compute_Rf(bdh_system* s, double* Rf)
     return 0;
 }
 
-// Rf = (dt / kT) D F at the current positions and forces into out, for every hydro mode
-// (compute_Rf, diff_equation.c:119-214).
-static int rf_any(bdh_system* s, double* out, double dt)
-{
-    const double scale = dt / (kB * s->cfg.T);
-    if (s->cfg.hydro != BDH_HYDRO_NONE) {
-        if (matvec_dev(s)) return -1;
-        ReduceJobs jobs{};
-        jobs.job[0] = ReduceJob{s->Pf, out, s->n, s->msplits, 0, scale};
-        return launch_reduce_partials(jobs, 1, s->n, s->st);
-    }
-    hipLaunchKernelGGL(diag_mobility_kernel, dim3(ceil_div(s->n, 256)), dim3(256), 0, s->st, s->pos, s->F,
-                       (const double*)nullptr, s->N, s->preDii, dt, scale, out, (double*)nullptr);
-    BDH_LAUNCH_CHECK();
-    return 0;
-}
-
 // Rs of compute_r (diff_rand.c:47-236) at the current positions from the next 3N Gaussians
 // of the device stream: sqrt(2 dt) L xi (the factor refreshed when `refactor`), the TEA
 // displacement, or sqrt(2 dt diag_D) xi.  Returns the device vector, nullptr on error.
 static const double* noise_at(bdh_system* s, double dt, bool refactor)
 {
     if (s->cfg.hydro == BDH_HYDRO_CHOLESKY && (refactor || !s->have_L) && assemble_and_factor(s)) return nullptr;
-    if (s->cfg.hydro != BDH_HYDRO_GEYER) return rs_dev(s, dt, s->A);
-    if (draw_noise(s, 1)) return nullptr;
-    if (matvec_dev(s, s->xi, true, dt)) return nullptr;
-    ReduceJobs jobs{};
-    jobs.job[0] = ReduceJob{s->Px, s->Feff, s->n, s->msplits, 0, 1.0};
-    jobs.job[1] = ReduceJob{s->Pc, s->Cs, s->n, s->msplits, 0, 1.0};
-    if (launch_reduce_partials(jobs, 2, s->n, s->st)) return nullptr;
-    if (launch_tea_close(s->pos, s->xi, s->Feff, s->Cs, s->Pe, pair_eparts(s->N), s->N, s->preDii, dt, s->bt,
-                         s->Rs, s->st))
-        return nullptr;
-    return s->Rs;
+    return rs_dev(s, dt, s->A);
 }
 
 static int ensure_prev(bdh_system* s)
@@ -1249,7 +1239,7 @@ int bdh_ermak_move(bdh_system* s, double dt, double* coord_out)
     if (ensure_prev(s)) return -1;
     const bool refactor = !s->have_L || s->steps_done % s->cfg.refactor_every == 0;
     BDH_HIP(hipMemcpyAsync(s->pos_prev, s->pos, sizeof(double4) * s->N, hipMemcpyDeviceToDevice, s->st));
-    if (rf_any(s, s->Rf, dt)) return -1;
+    if (rf_dev(s, s->Rf, dt)) return -1;
     const double* Rs = noise_at(s, dt, refactor);
     if (!Rs) return -1;
     const FlowTerm fl = flow_term(s, FLOW_MOVE, dt, s->FS);
@@ -1299,10 +1289,12 @@ int bdh_igt_phase(bdh_system* s, int phase, double dt, double* coord_out, int* r
                            s->Rf0);
         BDH_LAUNCH_CHECK();
         if (chol && assemble_and_factor(s, s->A2, s->pos_prev)) return -1;
+        if (s->tea) BDH_HIP(hipMemcpyAsync(s->pos_star, s->pos, pb, hipMemcpyDeviceToDevice, s->st));
         BDH_HIP(hipMemcpyAsync(s->pos, s->pos_prev, pb, hipMemcpyDeviceToDevice, s->st));
     }
-    // corrector r = r0 + Rf0 + sqrt(2 dt) L_avg xi2 and forces at r
-    const double* Rs = rs_dev(s, dt, s->A2);
+    // corrector r = r0 + Rf0 + sqrt(2 dt) L_avg xi2 (TEA: the sums of the mean tensor) and
+    // forces at r
+    const double* Rs = rs_dev(s, dt, s->A2, s->pos_star);
     if (!Rs) return -1;
     const FlowTerm cf = flow_term(s, FLOW_CORRECTOR, dt, s->FS);
     if (launch_em_update(s->pos, s->N, s->Rf0, Rs, s->cfg.bc == 1, s->cfg.box, s->st, &cf)) return -1;

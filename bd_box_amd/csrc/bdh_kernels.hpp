@@ -83,10 +83,12 @@ int ewald_upload(double box, int R, int method, double alpha_smith, double alpha
                  int R_recip);
 int pair_splits(int N, bool ewald);   // row partials per bead: one per 64-bead tile
 int pair_eparts(int N);                // eps partials of the TEA sums: one per tile pair
+// d_pos2 (TEA only): the sums of the mean tensor (D(pos2) + D(pos)) / 2 (the IG-T corrector).
 template <typename T>
 int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi, int N,
                      double preDii, double preDij, double dt, bool ewald, bool tea, int splits,
-                     double* d_Pf, double* d_Px, double* d_Pc, double* d_Pe, hipStream_t st);
+                     double* d_Pf, double* d_Px, double* d_Pc, double* d_Pe, hipStream_t st,
+                     const double4* d_pos2 = nullptr);
 // Free-space D.F partials of every chain (pair_sums_kernel's arithmetic, no TEA).
 template <typename T>
 int launch_pair_sums_batch(const double4* d_pos, const double* d_F, double preDii, double preDij,

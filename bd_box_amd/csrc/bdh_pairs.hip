@@ -7,6 +7,8 @@
 //   c_i += sum_l D_ij[., l]^2 / D_jj,l                       (j != i)
 //   e_i += sum of the 9 entries of D_ij                      (j != i)
 // D_ij is the free-space RPY block (EW = false) or the Beenakker-Ewald block (EW = true);
+// with MEAN it is the IG-T mean (D_ij(r*) + D_ij(r0)) / 2 of the blocks at two position sets
+// (pos2 = r*, diff_equation.c:341-366), whose TEA sums the IG-T corrector draws from;
 // D_jj in the TEA sums is the free-space diagonal kT/(6 pi eta a_j) (diag_D, as the
 // reference uses it).  Each unordered pair is evaluated once and serves both of its rows
 // (no atomics); accumulation is FP64.
@@ -193,15 +195,17 @@ constexpr int PS_TILE = 64;
 int pair_splits(int N, bool) { return ceil_div(N, PS_TILE); }
 int pair_eparts(int N) { const int t = ceil_div(N, PS_TILE); return t * (t + 1) / 2; }
 
-template <typename T, int EW, bool TEA>
+template <typename T, int EW, bool TEA, bool MEAN = false>
 __device__ __forceinline__ void pair_sums_body(const double4* __restrict__ pos, const double* __restrict__ F,
                                                const double* __restrict__ xi, int N, double preDii,
                                                double preDij, double dt, double* __restrict__ Pf,
                                                double* __restrict__ Px, double* __restrict__ Pc,
-                                               double* __restrict__ Pe, int b)
+                                               double* __restrict__ Pe, int b,
+                                               const double4* __restrict__ pos2 = nullptr)
 {
     constexpr int NV = TEA ? 10 : 3;
     __shared__ double4 s_pos[PS_TILE];
+    __shared__ double4 s_pos2[MEAN ? PS_TILE : 1];
     __shared__ double s_f[3 * PS_TILE];
     __shared__ double s_x[TEA ? 3 * PS_TILE : 1];
     __shared__ double s_col[4][NV][PS_TILE];
@@ -218,6 +222,7 @@ __device__ __forceinline__ void pair_sums_body(const double4* __restrict__ pos, 
         const bool ok = j < N;
         const double4 p = ok ? pos[j] : make_double4(0, 0, 0, 1);
         s_pos[tid] = p;
+        if constexpr (MEAN) s_pos2[tid] = ok ? pos2[j] : make_double4(0, 0, 0, 1);
 #pragma unroll
         for (int k = 0; k < 3; ++k) s_f[3 * tid + k] = ok ? F[3 * j + k] : 0.0;
         if constexpr (TEA) {
@@ -230,6 +235,8 @@ __device__ __forceinline__ void pair_sums_body(const double4* __restrict__ pos, 
     for (int v = 0; v < NV; ++v) s_col[w][v][t] = 0.0;
     const bool iv = i < N;
     const double4 pi = iv ? pos[i] : make_double4(0, 0, 0, 1);
+    double4 pi2 = pi;
+    if constexpr (MEAN) pi2 = iv ? pos2[i] : make_double4(0, 0, 0, 1);
     double Fi[3] = {0, 0, 0}, Xi[3] = {0, 0, 0};
     if (iv && !diag) {
 #pragma unroll
@@ -250,7 +257,16 @@ __device__ __forceinline__ void pair_sums_body(const double4* __restrict__ pos, 
         if (!iv || j0 + jl >= N) continue;
         const bool self = diag && sh == 0;
         const double4 pj = s_pos[jl];
-        const Blk6 B = pair_block<T, EW>(pi, pj, self, preDii, preDij);
+        Blk6 B = pair_block<T, EW>(pi, pj, self, preDii, preDij);
+        if constexpr (MEAN) {
+            const Blk6 B2 = pair_block<T, EW>(pi2, s_pos2[jl], self, preDii, preDij);
+            B.xx = (B2.xx + B.xx) / 2;
+            B.yy = (B2.yy + B.yy) / 2;
+            B.zz = (B2.zz + B.zz) / 2;
+            B.xy = (B2.xy + B.xy) / 2;
+            B.xz = (B2.xz + B.xz) / 2;
+            B.yz = (B2.yz + B.yz) / 2;
+        }
         const double fx = s_f[3 * jl], fy = s_f[3 * jl + 1], fz = s_f[3 * jl + 2];
         acc[0] += B.xx * fx + B.xy * fy + B.xz * fz;
         acc[1] += B.xy * fx + B.yy * fy + B.yz * fz;
@@ -328,14 +344,14 @@ __device__ __forceinline__ void pair_sums_body(const double4* __restrict__ pos, 
 #undef sum4
 }
 
-template <typename T, int EW, bool TEA>
+template <typename T, int EW, bool TEA, bool MEAN = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
 pair_sums_kernel(const double4* __restrict__ pos, const double* __restrict__ F,
                  const double* __restrict__ xi, int N, double preDii, double preDij, double dt,
                  double* __restrict__ Pf, double* __restrict__ Px, double* __restrict__ Pc,
-                 double* __restrict__ Pe)
+                 double* __restrict__ Pe, const double4* __restrict__ pos2)
 {
-    pair_sums_body<T, EW, TEA>(pos, F, xi, N, preDii, preDij, dt, Pf, Px, Pc, Pe, blockIdx.x);
+    pair_sums_body<T, EW, TEA, MEAN>(pos, F, xi, N, preDii, preDij, dt, Pf, Px, Pc, Pe, blockIdx.x, pos2);
 }
 
 // Free-space D.F of every chain of a batch: workgroup x of chain c is the chain's tile pair x.
@@ -364,15 +380,24 @@ int launch_pair_sums_batch(const double4* d_pos, const double* d_F, double preDi
 template <typename T>
 int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi, int N,
                      double preDii, double preDij, double dt, bool ewald, bool tea, int splits,
-                     double* d_Pf, double* d_Px, double* d_Pc, double* d_Pe, hipStream_t st)
+                     double* d_Pf, double* d_Px, double* d_Pc, double* d_Pe, hipStream_t st,
+                     const double4* d_pos2)
 {
     BDH_CHECK(splits == pair_splits(N, ewald), "launch_pair_sums: split count");
+    BDH_CHECK(!d_pos2 || tea, "launch_pair_sums: the mean tensor only for TEA sums");
     const dim3 grid(pair_eparts(N)), blk(PS_TILE, 4);
     const int ew = ewald ? (g_uploaded_method == EW_SMITH ? 2 : 1) : 0;
 #define PS_LAUNCH(E, TE)                                                                          \
     hipLaunchKernelGGL((pair_sums_kernel<T, E, TE>), grid, blk, 0, st, d_pos, d_F, d_xi, N, preDii, \
-                       preDij, dt, d_Pf, d_Px, d_Pc, d_Pe)
-    if (ew == 1) {
+                       preDij, dt, d_Pf, d_Px, d_Pc, d_Pe, (const double4*)nullptr)
+#define PS_MEAN(E)                                                                                 \
+    hipLaunchKernelGGL((pair_sums_kernel<T, E, true, true>), grid, blk, 0, st, d_pos, d_F, d_xi, N, \
+                       preDii, preDij, dt, d_Pf, d_Px, d_Pc, d_Pe, d_pos2)
+    if (d_pos2) {
+        if (ew == 1) PS_MEAN(1);
+        else if (ew == 2) PS_MEAN(2);
+        else PS_MEAN(0);
+    } else if (ew == 1) {
         if (tea) PS_LAUNCH(1, true); else PS_LAUNCH(1, false);
     } else if (ew == 2) {
         if (tea) PS_LAUNCH(2, true); else PS_LAUNCH(2, false);
@@ -380,6 +405,7 @@ int launch_pair_sums(const double4* d_pos, const double* d_F, const double* d_xi
         if (tea) PS_LAUNCH(0, true); else PS_LAUNCH(0, false);
     }
 #undef PS_LAUNCH
+#undef PS_MEAN
     BDH_LAUNCH_CHECK();
     return 0;
 }
@@ -514,8 +540,8 @@ int launch_tea_close(const double4* d_pos, const double* d_xi, const double* d_F
     return 0;
 }
 
-template int launch_pair_sums<float>(const double4*, const double*, const double*, int, double, double, double, bool, bool, int, double*, double*, double*, double*, hipStream_t);
-template int launch_pair_sums<double>(const double4*, const double*, const double*, int, double, double, double, bool, bool, int, double*, double*, double*, double*, hipStream_t);
+template int launch_pair_sums<float>(const double4*, const double*, const double*, int, double, double, double, bool, bool, int, double*, double*, double*, double*, hipStream_t, const double4*);
+template int launch_pair_sums<double>(const double4*, const double*, const double*, int, double, double, double, bool, bool, int, double*, double*, double*, double*, hipStream_t, const double4*);
 template int launch_pair_sums_batch<float>(const double4*, const double*, double, double, const BatchPlan&, double*, hipStream_t);
 template int launch_pair_sums_batch<double>(const double4*, const double*, double, double, const BatchPlan&, double*, hipStream_t);
 template int launch_ewald_assemble<float>(const double4*, int, double, double, float*, long long, hipStream_t,

@@ -697,12 +697,40 @@ static double diag_of(int N, const double* coord, const orc_params* p, double pr
     return preDii / coord[4 * (dof / 3) + 3];
 }
 
+/* The close of compute_r's DIFF_GEYER branch (diff_rand.c:165-224): eps from the sum of the
+   off-diagonal entries, beta, then Rs = sqrt(1 / (1 + beta^2 C / D_rr)) (beta Feff + D_rr X);
+   Feff arrives in Rs. */
+static void geyer_close(int N, const double* coord, const orc_params* p, double eps,
+                        const double* C, const double* X, double* Rs)
+{
+    const int n = 3 * N;
+    double preDii, preDij, a, beta;
+    int i;
+    orc_prefactors(p, &preDii, &preDij);
+    eps /= n;
+    eps /= (n - 1);
+    eps *= 2;
+    if (eps > 1.0) eps = 1.0;
+    a = (3 * N - 1) * eps * eps - (3 * N - 2) * eps;
+    if (fabs(a) < 0.0000001f) beta = 0.5;
+    else beta = (1 - sqrt(1 - a)) / a;
+    for (i = 0; i < n; ++i)
+    {
+        double dii = diag_of(N, coord, p, preDii, preDij, i);
+        double c = C[i];
+        c *= beta * beta * (1 / dii);
+        c = 1 / (1 + c);
+        c = sqrt(c);
+        Rs[i] = c * (beta * Rs[i] + dii * X[i]);
+    }
+}
+
 void orc_geyer_rs(int N, const double* coord, const orc_params* p, double dt,
                   double* X, double* Rs)
 {
     /* compute_r, DIFF_GEYER branch, on the fly (diff_rand.c:86-135, 165-224) */
     const int n = 3 * N;
-    double preDii, preDij, eps = 0.0, a, beta;
+    double preDii, preDij, eps = 0.0;
     double* C = (double*)calloc((size_t)n, sizeof(double));
     double* Feff = Rs;
     int i, k, jj, l;
@@ -732,22 +760,43 @@ void orc_geyer_rs(int N, const double* coord, const orc_params* p, double dt,
         }
         eps += e_loc;
     }
-    eps /= n;
-    eps /= (n - 1);
-    eps *= 2;
-    if (eps > 1.0) eps = 1.0;
-    a = (3 * N - 1) * eps * eps - (3 * N - 2) * eps;
-    if (fabs(a) < 0.0000001f) beta = 0.5;
-    else beta = (1 - sqrt(1 - a)) / a;
-    for (i = 0; i < n; ++i)
+    geyer_close(N, coord, p, eps, C, X, Rs);
+    free(C);
+}
+
+/* compute_r, DIFF_GEYER branch over a stored tensor (geyer_on_the_fly no, diff_rand.c:136-164,
+   then :165-224): the sums run over the entries D[r][k], k < r, of the lower triangle of D
+   (ld, column-major lower = row-major upper), D symmetric. */
+void orc_geyer_rs_D(int N, const double* D, int ld, const double* coord, const orc_params* p,
+                    double dt, double* X, double* Rs)
+{
+    const int n = 3 * N;
+    double preDii, preDij, eps = 0.0;
+    double* C = (double*)calloc((size_t)n, sizeof(double));
+    double* Feff = Rs;
+    int r, k;
+    orc_prefactors(p, &preDii, &preDij);
+    memset(Rs, 0, sizeof(double) * n);
+    for (r = 0; r < n; ++r)
     {
-        double dii = diag_of(N, coord, p, preDii, preDij, i);
-        double c = C[i];
-        c *= beta * beta * (1 / dii);
-        c = 1 / (1 + c);
-        c = sqrt(c);
-        Rs[i] = c * (beta * Feff[i] + dii * X[i]);
+        double inv_d = 1 / diag_of(N, coord, p, preDii, preDij, r);
+        X[r] *= sqrt(2 * dt * inv_d);
     }
+    for (r = 0; r < n; ++r)
+    {
+        double e_loc = 0.0;
+        for (k = 0; k < r; ++k)
+        {
+            const double d = D[(size_t)k * ld + r];
+            C[r] += d * d * (1 / diag_of(N, coord, p, preDii, preDij, k));
+            C[k] += d * d * (1 / diag_of(N, coord, p, preDii, preDij, r));
+            Feff[r] += X[k] * d;
+            Feff[k] += X[r] * d;
+            e_loc += d;
+        }
+        eps += e_loc;
+    }
+    geyer_close(N, coord, p, eps, C, X, Rs);
     free(C);
 }
 
@@ -903,12 +952,12 @@ static int sim_attempt(orc_sim* s, double dt, const double* X_in, double preDii)
    (positions and forces restored, load_coord / load_forces) and redrawn with the next
    Gaussians of the stream, ermak_var halving dt for the first 16 rejections; more than
    move_attempts rejections fail. */
-/* Rf = pre D F (hydro cholesky: D stored in Dm; none: diag_D) */
+/* Rf = pre D F (hydro cholesky and geyer: D stored in Dm; none: diag_D) */
 static void sim_rf(orc_sim* s, const double* Dm, double pre, double preDii, double* out)
 {
     const int n = 3 * s->N;
     int i;
-    if (s->p.hydro == 1)
+    if (s->p.hydro == 1 || s->p.hydro == 2)
     {
         orc_symv_lower(n, Dm, n, s->F, out);
         for (i = 0; i < n; ++i) out[i] *= pre;
@@ -917,13 +966,16 @@ static void sim_rf(orc_sim* s, const double* Dm, double pre, double preDii, doub
         for (i = 0; i < n; ++i) out[i] = pre * (preDii / s->coord[4 * (i / 3) + 3]) * s->F[i];
 }
 
-/* Rs from the next Gaussians of the stream: L sqrt(2 dt) X, or X sqrt(2 dt diag_D) */
+/* Rs from the next Gaussians of the stream: L sqrt(2 dt) X, the TEA displacement over the
+   stored tensor in Lm (hydro geyer), or X sqrt(2 dt diag_D) */
 static void sim_rs(orc_sim* s, const double* Lm, double dt, double preDii)
 {
     const int n = 3 * s->N;
     int i;
     orc_gauss(&s->rng, s->X, n);
-    if (s->p.hydro == 1)
+    if (s->p.hydro == 2)
+        orc_geyer_rs_D(s->N, Lm, n, s->coord, &s->p, dt, s->X, s->Rs);
+    else if (s->p.hydro == 1)
     {
         double a = sqrt(2 * dt);
         for (i = 0; i < n; ++i) s->X[i] *= a;
@@ -971,6 +1023,13 @@ static int sim_step_igt(orc_sim* s, double preDii)
         memcpy(s->L, s->D, sizeof(double) * (size_t)n * n);
         if (orc_cholesky(n, s->L, n)) return 1;
     }
+    else if (s->p.hydro == 2)
+    {
+        /* TEA noise reads the stored tensor itself (input.c:497-501 turns the on-the-fly
+           sums off for IG-T) */
+        orc_compute_D(N, s->coord, &s->p, s->D, n);
+        memcpy(s->L, s->D, sizeof(double) * (size_t)n * n);
+    }
     for (;;)
     {
         const double pre = dt / (KB * s->p.T);
@@ -991,14 +1050,14 @@ static int sim_step_igt(orc_sim* s, double preDii)
             continue;
         }
         /* averaged deterministic move and tensor at (r0, r*) */
-        if (s->p.hydro == 1) orc_compute_D(N, s->coord, &s->p, s->D1, n);
+        if (s->p.hydro == 1 || s->p.hydro == 2) orc_compute_D(N, s->coord, &s->p, s->D1, n);
         sim_rf(s, s->D1, pre, preDii, s->Rs);
         for (i = 0; i < n; ++i) s->Rf0[i] = 0.5 * (s->Rf[i] + s->Rs[i]);
-        if (s->p.hydro == 1)
+        if (s->p.hydro == 1 || s->p.hydro == 2)
         {
             size_t e;
             for (e = 0; e < (size_t)n * n; ++e) s->L2[e] = (s->D[e] + s->D1[e]) / 2;
-            if (orc_cholesky(n, s->L2, n)) return 1;
+            if (s->p.hydro == 1 && orc_cholesky(n, s->L2, n)) return 1;
         }
         memcpy(s->coord, s->coord_prev, sizeof(double) * 4 * N);
         /* corrector */
@@ -1035,7 +1094,7 @@ int orc_sim_run(orc_sim* s, int steps, const double* xi)
     orc_prefactors(&s->p, &preDii, &preDij);
     if (s->algorithm >= 3)
     {
-        if (xi || s->p.hydro == 2) return 4;   /* IG-T: stream noise, hydro none / cholesky */
+        if (xi) return 4;   /* IG-T: stream noise */
         for (st = 0; st < steps; ++st)
         {
             int rc = sim_step_igt(s, preDii);
@@ -1086,7 +1145,12 @@ void orc_sim_set_algorithm(orc_sim* s, int algorithm, int move_attempts)
     if (algorithm >= 3 && !s->Rf0)
     {
         s->Rf0 = (double*)calloc(n, sizeof(double));
-        if (s->p.hydro == 1)
+        if (s->p.hydro == 2 && !s->D)
+        {
+            s->D = (double*)malloc(sizeof(double) * n * n);
+            s->L = (double*)malloc(sizeof(double) * n * n);
+        }
+        if (s->p.hydro == 1 || s->p.hydro == 2)
         {
             s->D1 = (double*)malloc(sizeof(double) * n * n);
             s->L2 = (double*)malloc(sizeof(double) * n * n);

@@ -85,6 +85,10 @@ void     orc_gauss(uint64_t* state, double* out, int count);
    scaled ones as in the reference).  Rs[3N] out. */
 void orc_geyer_rs(int N, const double* coord, const orc_params* p, double dt,
                   double* X, double* Rs);
+/* The same from a stored symmetric tensor D (ld; geyer_on_the_fly no, diff_rand.c:136-164):
+   the sums run over the entries below the diagonal, D[r][k], k < r. */
+void orc_geyer_rs_D(int N, const double* D, int ld, const double* coord, const orc_params* p,
+                    double dt, double* X, double* Rs);
 
 /* D.F for TEA runs: Rf = (dt/kT) D F with blocks recomputed on the fly (diff_equation.c:55-117) */
 void orc_rf_onfly(int N, const double* coord, const orc_params* p, double dt,

@@ -59,6 +59,8 @@ def lib():
         L.orc_drand48.restype = C.c_double
         L.orc_gauss.argtypes = [C.POINTER(C.c_uint64), _dp, C.c_int]
         L.orc_geyer_rs.argtypes = [C.c_int, _dp, C.POINTER(OrcParams), C.c_double, _dp, _dp]
+        L.orc_geyer_rs_D.argtypes = [C.c_int, _dp, C.c_int, _dp, C.POINTER(OrcParams), C.c_double,
+                                     _dp, _dp]
         L.orc_rf_onfly.argtypes = [C.c_int, _dp, C.POINTER(OrcParams), C.c_double, _dp, _dp]
         L.orc_sim_create.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, _dp,
                                      C.POINTER(OrcParams), C.c_uint32, C.c_int]

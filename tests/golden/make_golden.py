@@ -186,6 +186,17 @@ def main():
         case(name, s24r, wl.chain_params(hydro=hydro, algorithm=alg, dt=0.5, check_overlap="yes",
                                           move_attempts=200, rand_seed=11, vel_grad_tensor=Gs),
              [["traj", "8"]])
+    # 9. IG-T with TEA noise (input.c:497-501 turns geyer_on_the_fly off for IG-T, so the
+    #    predictor's sums come from the stored D(r0) and the corrector's from the stored mean
+    #    tensor, diff_rand.c:136-164 over diff_equation.c:317-374): free space, with
+    #    rejections, and the periodic Beenakker tensor
+    case("igt_geyer", s16i, wl.chain_params(hydro="geyer", algorithm="igt_const", rand_seed=4242),
+         [["traj", "6"]])
+    for name, alg in (("igt_geyer_retry", "igt_const"), ("igt_var_geyer_retry", "igt_var")):
+        case(name, s24r, wl.chain_params(hydro="geyer", algorithm=alg, dt=0.5, check_overlap="yes",
+                                          move_attempts=200, rand_seed=11),
+             [["traj", "8"]])
+    case("igt_geyer_beenakker", hs, dict(pe, algorithm="igt_const"), [["traj", "3"]])
 
 
 if __name__ == "__main__":

@@ -72,6 +72,7 @@ DROPIN_CASES = {
     "igt_var_retry": (dict(lj_diam=3.3), dict(hydro="cholesky", algorithm="igt_var", dt=2.0,
                                               check_overlap="yes", move_attempts=200)),
     "igt_const": ({}, dict(hydro="cholesky", algorithm="igt_const")),
+    "igt_geyer": ({}, dict(hydro="geyer", algorithm="igt_const")),
     # periodic chain with the Smith-Ewald tensor (diff_tensor.c:102-263)
     "smith_periodic": ({}, dict(hydro="cholesky", bc="periodic", xbox=100.0, ybox=100.0,
                                 zbox=100.0, ewald_method="smith", ewald_real=2, ewald_recip=2,

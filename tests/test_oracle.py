@@ -414,3 +414,49 @@ def test_flow_field_closed_form_without_noise():
     dt = float(P.dt)
     np.testing.assert_allclose(c1[:, 0], c0[:, 0] + dt * rate * c0[:, 1], rtol=0, atol=1e-12)
     np.testing.assert_allclose(c1[:, 1:3], c0[:, 1:3], rtol=0, atol=1e-12)
+
+
+IGT_TEA_CASES = [("igt_geyer", 6), ("igt_geyer_retry", 8), ("igt_var_geyer_retry", 8),
+                 ("igt_geyer_beenakker", 3)]
+
+
+def _start(golden, case):
+    # the periodic spheres start from the reference's centred table (as the Beenakker step)
+    return golden("beenakker12")[0]["params.coord"].reshape(-1, 4) if "beenakker" in case else None
+
+
+@pytest.mark.parametrize("case,steps", IGT_TEA_CASES)
+def test_igt_tea_vs_reference(golden, case, steps):
+    """IG-T with TEA noise: input.c:497-501 turns the on-the-fly sums off for IG-T, so the
+    predictor's TEA sums come from the stored D(r0) and the corrector's from the stored mean
+    tensor (D(r0) + D(r*)) / 2 (diff_rand.c:136-164 under diff_equation.c:317-374).  A
+    corrector that used D(r0) instead misses these trajectories by 6e-5 to 4e-2."""
+    g, sysm, P, p = golden(case)
+    assert P.algorithm in (3, 4) and P.hydro == 2 and P.geyer_on_the_fly == 0
+    sim = po.Sim(sysm, p, seed=P.rand_seed, algorithm=P.algorithm,
+                 move_attempts=P.move_attempts, coord=_start(golden, case))
+    traj = g["traj.coord"].reshape(steps, -1, 4)
+    for k in range(steps):
+        sim.run(1)
+        assert sim.last_dt() == g["traj.dt"][k], k
+        err = np.max(np.abs(sim.coords() - traj[k]))
+        assert err < 1e-11, (k, err)
+    if "retry" in case:
+        assert sim.rejections() > 0
+
+
+def test_stored_tea_equals_on_the_fly_in_free_space(golden):
+    """Free space: the stored-tensor TEA sums (diff_rand.c:136-164) and the on-the-fly ones
+    (:97-135) are the same sums in another order."""
+    g, sysm, P, p = golden("geyer24")
+    n = 3 * sysm.N
+    D = po.compute_D(sysm.coord, p)
+    X = np.random.default_rng(3).standard_normal(n)
+    a, b = np.zeros(n), np.zeros(n)
+    x1, x2 = X.copy(), X.copy()
+    po.lib().orc_geyer_rs_D(sysm.N, po._d(np.ascontiguousarray(D)), n,
+                            po._d(np.ascontiguousarray(sysm.coord, float).reshape(-1)),
+                            po.C.byref(p), 0.01, po._d(x1), po._d(a))
+    po.lib().orc_geyer_rs(sysm.N, po._d(np.ascontiguousarray(sysm.coord, float).reshape(-1)),
+                          po.C.byref(p), 0.01, po._d(x2), po._d(b))
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
