@@ -18,8 +18,10 @@
 // Kernels (FP64; the two solves are the O(n^3) part):
 //   adj_lt_g_kernel         a = L^T g
 //   adj_diag_inv_kernel     inverses of L's 64 x 64 diagonal blocks
-//   adj_rhs_kernel, adj_trsm_diag_kernel, adj_trsm_update_kernel
-//                           L^T X = B blocked by 64 rows, B = s Phi(a xi^T) or Y1^T
+//   adj_rhs_kernel, adj_trsm_super_kernel, adj_trsm_update_kernel
+//                           L^T X = B blocked by 64 rows, four block rows per step (their
+//                           triangle, then one K = 256 update of the rows above), B =
+//                           s Phi(a xi^T) or Y1^T
 //   adj_symmetrize_kernel   Z = Y2 + Y2^T
 //   adj_contract_kernel     grad_r <U, D(r)> over the RPY blocks (diff_tensor.c:45-92), column splits
 //   adj_hvp_pairs_kernel,   g + contraction - c H y  (pair and bond Hessians, the forward
@@ -56,6 +58,7 @@ adj_lt_g_kernel(const double* __restrict__ L, long long ld, int n, const double*
 // block up: X_J <- L_JJ^-T X_J, then X_I -= L_JI^T X_J for every I < J.  X is column-major
 // (ld np), solved in place after the right-hand sides are written into it.
 constexpr int ADJ_NB = 64;
+constexpr int ADJ_SUPER = 4;   // block rows of L^T X = B per triangular launch / update K
 
 // mode 0: X(i, q) = s a_i xi_q (i > q), s a_q xi_q / 2 (i = q), 0 (i < q or beyond n)
 // mode 1: X = Y^T (64 x 64 tiles through LDS)
@@ -76,24 +79,22 @@ adj_rhs_kernel(int mode, int np, int n, const double* __restrict__ a, const doub
         return;
     }
     // X(i, q) = Y(q, i): read the tile of Y with rows q0.., columns i0.. (column-major)
-    for (int ii = ty; ii < ADJ_NB; ii += 4) t[ii][tx] = Y[(long long)(i0 + ii) * np + q0 + tx];
+#pragma unroll
+    for (int u = 0; u < ADJ_NB / 4; ++u) t[ty + 4 * u][tx] = Y[(long long)(i0 + ty + 4 * u) * np + q0 + tx];
     __syncthreads();
-    for (int qq = ty; qq < ADJ_NB; qq += 4) X[(long long)(q0 + qq) * np + i0 + tx] = t[tx][qq];
+#pragma unroll
+    for (int u = 0; u < ADJ_NB / 4; ++u) X[(long long)(q0 + ty + 4 * u) * np + i0 + tx] = t[tx][ty + 4 * u];
 }
 
 // P = sA^T sB for 64 x 64 LDS tiles (sA[k][i], sB[k][q], k < 64) on v_mfma_f64_16x16x4_f64:
 // wavefront w owns the 32 x 32 quarter (rows 32 (w / 2), columns 32 (w % 2)) as 2 x 2 blocks;
 // accumulator r of lane l holds row 4 r + l / 16, column l % 16 of its block.
-__device__ __forceinline__ void tile_tn_mfma(const double (*sA)[ADJ_NB + 1], const double (*sB)[ADJ_NB + 1],
-                                             doublex4 acc[2][2])
+__device__ __forceinline__ void tile_tn_mfma_acc(const double (*sA)[ADJ_NB + 1],
+                                                 const double (*sB)[ADJ_NB + 1], doublex4 acc[2][2])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r0 = 32 * (wave >> 1), c0 = 32 * (wave & 1);
     const int lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
     for (int k = 0; k < ADJ_NB; k += 4) {
         double av[2], bv[2];
@@ -106,6 +107,49 @@ __device__ __forceinline__ void tile_tn_mfma(const double (*sA)[ADJ_NB + 1], con
 #pragma unroll
             for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f64_16x16x4(av[a], bv[b], acc[a][b]);
     }
+}
+
+__device__ __forceinline__ void tile_tn_mfma(const double (*sA)[ADJ_NB + 1], const double (*sB)[ADJ_NB + 1],
+                                             doublex4 acc[2][2])
+{
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+    tile_tn_mfma_acc(sA, sB, acc);
+}
+
+// A 64 x 64 tile of a column-major matrix (column c at P + c ld, rows contiguous) through
+// registers: every thread's 16 loads are issued before any of them is stored to LDS, as
+// sT[k][c] = P(k, c).
+struct TileRegs {
+    double v[16];
+};
+__device__ __forceinline__ void tile_load(const double* __restrict__ P, long long ld, TileRegs& t)
+{
+    const int k = threadIdx.x & 63, c = threadIdx.x >> 6;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) t.v[u] = P[(long long)(c + 4 * u) * ld + k];
+}
+__device__ __forceinline__ void tile_to_lds(const TileRegs& t, double (*sT)[ADJ_NB + 1])
+{
+    const int k = threadIdx.x & 63, c = threadIdx.x >> 6;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) sT[k][c + 4 * u] = t.v[u];
+}
+
+// sT[i][q] = P(i, q) from tile_tn_mfma's accumulators.
+__device__ __forceinline__ void tile_store_lds(double (*sT)[ADJ_NB + 1], const doublex4 acc[2][2])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = 32 * (wave >> 1), c0 = 32 * (wave & 1);
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sT[r0 + 16 * a + lk + 4 * r][c0 + 16 * b + lr] = acc[a][b][r];
 }
 
 // X(i0 + i, q0 + q) = P(i, q) (sub = false) or -= P(i, q) (sub = true) from tile_tn_mfma's
@@ -136,6 +180,7 @@ adj_diag_inv_kernel(const double* __restrict__ L, long long ld, double* __restri
 {
     __shared__ double sL[ADJ_NB][ADJ_NB + 1];   // sL[i][k] = L(64 J + i, 64 J + k)
     const int j0 = blockIdx.x * ADJ_NB, c = threadIdx.x;
+#pragma unroll 16
     for (int k = 0; k < ADJ_NB; ++k) sL[c][k] = L[(long long)(j0 + k) * ld + j0 + c];
     __syncthreads();
     double m[ADJ_NB];
@@ -151,41 +196,116 @@ adj_diag_inv_kernel(const double* __restrict__ L, long long ld, double* __restri
     for (int i = 0; i < ADJ_NB; ++i) out[i] = m[i];
 }
 
-// X_J <- L_JJ^-T X_J = M_J^T X_J for one 64-column tile of X (workgroup = column tile).
+// Block rows Jlo..Jhi (at most ADJ_SUPER) of L^T X = B for one ADJ_SC-column tile of X
+// (workgroup = column tile), the rows below Jhi already applied.  The tile's rows of these
+// block rows stay in LDS: for J = Jhi down to Jlo, X_J <- M_J^T X_J (M_J = L_JJ^-1), then
+// X_I -= L_JI^T X_J for Jlo <= I < J.  The A operands come in that fixed order (M_J, then the
+// L_JI), each loaded into registers while the previous one is multiplied.  Wavefront w owns
+// rows 16 w.. of every 64 x ADJ_SC product (two 16 x 16 blocks on v_mfma_f64_16x16x4_f64).
+constexpr int ADJ_SC = 32;
+
 __global__ void __launch_bounds__(256)
-adj_trsm_diag_kernel(const double* __restrict__ Minv, int np, int J, double* __restrict__ X)
+adj_trsm_super_kernel(const double* __restrict__ L, long long ld, const double* __restrict__ Minv, int np,
+                      int Jlo, int Jhi, double* __restrict__ X)
 {
-    __shared__ double sA[ADJ_NB][ADJ_NB + 1];   // sA[k][i] = M_J(k, i)
-    __shared__ double sB[ADJ_NB][ADJ_NB + 1];   // sB[k][q] = X(64 J + k, q0 + q)
-    const int q0 = blockIdx.x * ADJ_NB, j0 = J * ADJ_NB;
-    const double* M = Minv + (long long)J * ADJ_NB * ADJ_NB;
-    for (int e = threadIdx.x; e < ADJ_NB * ADJ_NB; e += 256) {
-        const int k = e % ADJ_NB, c = e / ADJ_NB;
-        sA[k][c] = M[c * ADJ_NB + k];
-        sB[k][c] = X[(long long)(q0 + c) * np + j0 + k];
+    __shared__ double sX[ADJ_SUPER][ADJ_NB][ADJ_SC + 1];   // sX[j][k][q] = X(64 (Jlo + j) + k, q0 + q)
+    __shared__ double sA[2][ADJ_NB][ADJ_NB + 1];          // sA[.][k][i] = A(k, i)
+    const int q0 = blockIdx.x * ADJ_SC;
+    const int nbr = Jhi - Jlo + 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int tk = threadIdx.x & 63, tc = threadIdx.x >> 6;   // loads: row tk, columns tc + 4 u
+    TileRegs ta;
+    tile_load(Minv + (long long)Jhi * ADJ_NB * ADJ_NB, ADJ_NB, ta);
+#pragma unroll
+    for (int j = 0; j < ADJ_SUPER; ++j) {
+        if (j < nbr) {
+            const double* src = X + (long long)q0 * np + (long long)(Jlo + j) * ADJ_NB + tk;
+            double v[ADJ_SC / 4];
+#pragma unroll
+            for (int u = 0; u < ADJ_SC / 4; ++u) v[u] = src[(long long)(tc + 4 * u) * np];
+#pragma unroll
+            for (int u = 0; u < ADJ_SC / 4; ++u) sX[j][tk][tc + 4 * u] = v[u];
+        }
+    }
+    int step = 0;
+    for (int J = Jhi; J >= Jlo; --J) {
+        const int j = J - Jlo;
+        for (int I = J; I >= Jlo; --I, ++step) {
+            const int buf = step & 1;
+            tile_to_lds(ta, sA[buf]);
+            __syncthreads();   // sA[buf] and the last writes of sX are visible
+            // the next A operand: L_J,I-1, else M_J-1
+            const int nJ = I > Jlo ? J : J - 1, nI = I > Jlo ? I - 1 : J - 1;
+            if (nJ >= Jlo) {
+                if (nI == nJ) tile_load(Minv + (long long)nJ * ADJ_NB * ADJ_NB, ADJ_NB, ta);
+                else tile_load(L + (long long)nI * ADJ_NB * ld + (long long)nJ * ADJ_NB, ld, ta);
+            }
+            doublex4 acc[2] = {doublex4{0.0, 0.0, 0.0, 0.0}, doublex4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll 4
+            for (int k = 0; k < ADJ_NB; k += 4) {
+                const double av = sA[buf][k + lk][16 * wave + lr];
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acc[b] = mfma_f64_16x16x4(av, sX[j][k + lk][16 * b + lr], acc[b]);
+            }
+            if (I == J) {
+                __syncthreads();   // every wavefront is done reading X_J
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) sX[j][16 * wave + lk + 4 * r][16 * b + lr] = acc[b][r];
+            } else {
+                const int i = I - Jlo;
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) sX[i][16 * wave + lk + 4 * r][16 * b + lr] -= acc[b][r];
+            }
+        }
     }
     __syncthreads();
-    doublex4 acc[2][2];
-    tile_tn_mfma(sA, sB, acc);
-    tile_store(X, np, j0, q0, acc, false);
+#pragma unroll
+    for (int j = 0; j < ADJ_SUPER; ++j) {
+        if (j < nbr) {
+            double* dst = X + (long long)q0 * np + (long long)(Jlo + j) * ADJ_NB + tk;
+#pragma unroll
+            for (int u = 0; u < ADJ_SC / 4; ++u) dst[(long long)(tc + 4 * u) * np] = sX[j][tk][tc + 4 * u];
+        }
+    }
 }
 
-// X_I -= L_JI^T X_J: workgroup (I, column tile), K = 64 on MFMA.
+// X_I -= sum_{kk < kb} L_{J+kk,I}^T X_{J+kk}: workgroup (I, column tile), K = 64 kb on MFMA,
+// the next 64-row slice of both operands loaded while the current one is multiplied.
 __global__ void __launch_bounds__(256)
-adj_trsm_update_kernel(const double* __restrict__ L, long long ld, int np, int J, double* __restrict__ X)
+adj_trsm_update_kernel(const double* __restrict__ L, long long ld, int np, int J, int kb,
+                       double* __restrict__ X)
 {
-    __shared__ double sA[ADJ_NB][ADJ_NB + 1];   // sA[k][i] = L(64 J + k, 64 I + i)
-    __shared__ double sB[ADJ_NB][ADJ_NB + 1];   // sB[k][q] = X(64 J + k, q0 + q)
+    __shared__ double sA[ADJ_NB][ADJ_NB + 1];   // sA[k][i] = L(64 (J + kk) + k, 64 I + i)
+    __shared__ double sB[ADJ_NB][ADJ_NB + 1];   // sB[k][q] = X(64 (J + kk) + k, q0 + q)
     const int I = blockIdx.x, q0 = blockIdx.y * ADJ_NB;
-    const int i0 = I * ADJ_NB, j0 = J * ADJ_NB;
-    for (int e = threadIdx.x; e < ADJ_NB * ADJ_NB; e += 256) {
-        const int k = e % ADJ_NB, c = e / ADJ_NB;
-        sA[k][c] = L[(long long)(i0 + c) * ld + j0 + k];
-        sB[k][c] = X[(long long)(q0 + c) * np + j0 + k];
-    }
-    __syncthreads();
+    const int i0 = I * ADJ_NB;
+    const double* La = L + (long long)i0 * ld;
+    const double* Xq = X + (long long)q0 * np;
     doublex4 acc[2][2];
-    tile_tn_mfma(sA, sB, acc);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+    TileRegs ta, tb;
+    tile_load(La + J * ADJ_NB, ld, ta);
+    tile_load(Xq + J * ADJ_NB, np, tb);
+    for (int kk = 0; kk < kb; ++kk) {
+        if (kk) __syncthreads();   // the previous slice's products are done with sA and sB
+        tile_to_lds(ta, sA);
+        tile_to_lds(tb, sB);
+        __syncthreads();
+        if (kk + 1 < kb) {
+            const int j1 = (J + kk + 1) * ADJ_NB;
+            tile_load(La + j1, ld, ta);
+            tile_load(Xq + j1, np, tb);
+        }
+        tile_tn_mfma_acc(sA, sB, acc);
+    }
     tile_store(X, np, i0, q0, acc, true);
 }
 
@@ -198,9 +318,12 @@ adj_symmetrize_kernel(int np, const double* __restrict__ Y, double* __restrict__
     const int i0 = blockIdx.x * ADJ_NB, q0 = blockIdx.y * ADJ_NB;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     // t[ii][tx] = Y(q0 + tx, i0 + ii) = Y^T(i0 + ii, q0 + tx)
-    for (int ii = ty; ii < ADJ_NB; ii += 4) t[ii][tx] = Y[(long long)(i0 + ii) * np + q0 + tx];
+#pragma unroll
+    for (int u = 0; u < ADJ_NB / 4; ++u) t[ty + 4 * u][tx] = Y[(long long)(i0 + ty + 4 * u) * np + q0 + tx];
     __syncthreads();
-    for (int qq = ty; qq < ADJ_NB; qq += 4) {
+#pragma unroll
+    for (int u = 0; u < ADJ_NB / 4; ++u) {
+        const int qq = ty + 4 * u;
         const long long e = (long long)(q0 + qq) * np + i0 + tx;
         Z[e] = Y[e] + t[tx][qq];
     }
@@ -431,11 +554,16 @@ int launch_adj_upper_solve(const double* d_L, long long ld, int np, int n, int m
     const int nb = np / ADJ_NB;
     hipLaunchKernelGGL(adj_rhs_kernel, dim3(nb, nb), dim3(256), 0, st, mode, np, n, d_a, d_xi, s, d_B, d_X);
     BDH_LAUNCH_CHECK();
-    for (int J = nb - 1; J >= 0; --J) {
-        hipLaunchKernelGGL(adj_trsm_diag_kernel, dim3(nb), dim3(256), 0, st, d_Minv, np, J, d_X);
+    // ADJ_SUPER block rows at a time: their own triangle in one launch (a workgroup per column
+    // tile), then every block row above them in one update launch with K = 64 ADJ_SUPER
+    for (int Jhi = nb - 1; Jhi >= 0; Jhi -= ADJ_SUPER) {
+        const int Jlo = std::max(0, Jhi - ADJ_SUPER + 1);
+        hipLaunchKernelGGL(adj_trsm_super_kernel, dim3(np / ADJ_SC), dim3(256), 0, st, d_L, ld, d_Minv, np, Jlo,
+                           Jhi, d_X);
         BDH_LAUNCH_CHECK();
-        if (J > 0) {
-            hipLaunchKernelGGL(adj_trsm_update_kernel, dim3(J, nb), dim3(256), 0, st, d_L, ld, np, J, d_X);
+        if (Jlo > 0) {
+            hipLaunchKernelGGL(adj_trsm_update_kernel, dim3(Jlo, nb), dim3(256), 0, st, d_L, ld, np, Jlo,
+                               Jhi - Jlo + 1, d_X);
             BDH_LAUNCH_CHECK();
         }
     }
