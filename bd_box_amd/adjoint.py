@@ -1,4 +1,4 @@
-"""Differentiable Brownian-dynamics trajectories: K FP64 Ermak-McCammon steps with given
+"""Differentiable Brownian-dynamics trajectories: K Ermak-McCammon steps with given
 Gaussians as torch.autograd.Functions, backed by the device's reverse-mode step
 (bdh_step_vjp, bdh_step_vjp_params).  The reference has no derivatives; the step
 differentiated is its comp_step_ermak with Cholesky noise (src/diff_alg/diff_comp.c:152-179),
@@ -14,6 +14,11 @@ rg_loss()           (mean squared radius of gyration over the frames - target)^2
 The forward pass keeps the positions of every `checkpoint`-th step only; the backward pass
 re-runs each segment from its checkpoint, saving the factor of each step on the device, and
 walks it back step by step with those factors.
+
+FP64 systems differentiate the FP64 step.  Mixed-precision systems (FP32 tensor, forces and
+trailing updates) differentiate the step they take: its FP32 factor, promoted to FP64, enters
+the reverse step, which runs in FP64; the factorizations of the forward pass and the
+re-runs, the bulk of the time, run in mixed precision.
 """
 from __future__ import annotations
 
@@ -67,13 +72,13 @@ class Trajectory(torch.autograd.Function):
 
 
 def check_differentiable(cfg) -> None:
-    """Raise ValueError unless the configuration's step is the one differentiated: FP64,
-    Cholesky noise refactored at every step, free space, ermak_const, no overlap check.
-    With check_overlap a rejected step is redrawn from the device stream (and ermak_var
-    halves dt), so the given-Gaussian step would not be the step taken."""
-    if cfg.precision != 0 or cfg.refactor_every != 1:
-        raise ValueError("trajectory: the system must be FP64 with refactor_every=1 (the "
-                         "derivative is that of a step refactored at its own positions)")
+    """Raise ValueError unless the configuration's step is the one differentiated: Cholesky
+    noise refactored at every step, free space, ermak_const, no overlap check (any
+    precision).  With check_overlap a rejected step is redrawn from the device stream (and
+    ermak_var halves dt), so the given-Gaussian step would not be the step taken."""
+    if cfg.refactor_every != 1:
+        raise ValueError("trajectory: the system must have refactor_every=1 (the derivative "
+                         "is that of a step refactored at its own positions)")
     if cfg.hydro != 1 or cfg.bc != 0:
         raise ValueError("trajectory: free-space Cholesky noise only (hydro cholesky, bc none)")
     if cfg.check_overlap or cfg.algorithm != 0:
@@ -84,8 +89,8 @@ def check_differentiable(cfg) -> None:
 
 def trajectory(system: BDSystem, xyz0: torch.Tensor, xi, radii, checkpoint: int = 10) -> torch.Tensor:
     """Positions (N, 3) after len(xi) steps from xyz0 (N, 3, float64) with Gaussians
-    xi (K, 3N); differentiable with respect to xyz0.  `system` must be an FP64 BDSystem
-    created with refactor_every=1 (its state is overwritten)."""
+    xi (K, 3N); differentiable with respect to xyz0.  `system` must be a BDSystem created
+    with refactor_every=1 (its state is overwritten)."""
     check_differentiable(system.cfg)
     if checkpoint < 1:
         raise ValueError("trajectory: checkpoint must be positive")
@@ -157,8 +162,8 @@ def param_trajectory(system: BDSystem, xyz0: torch.Tensor, eps: torch.Tensor, R:
     Gaussians xi (K, 3N), under per-bead LJ epsilon eps (N), LJ radius R (N; sigma_ij =
     R_i + R_j), charges Q (N) and per-bond force constants H (nbonds; bond_r0_rmax holds the
     bonds' (r0, r_max)); differentiable with respect to xyz0, eps, R, Q and H (float64
-    tensors).  `system` must be an FP64 BDSystem created with refactor_every=1 over the same
-    beads and bonds; its parameters and state are overwritten."""
+    tensors).  `system` must be a BDSystem created with refactor_every=1 over the same beads
+    and bonds; its parameters and state are overwritten."""
     check_differentiable(system.cfg)
     if checkpoint < 1:
         raise ValueError("param_trajectory: checkpoint must be positive")

@@ -531,6 +531,30 @@ adj_hvp_finish_kernel(const double4* __restrict__ pos, int N, Topology topo, con
     for (int u = 0; u < 3; ++u) out[3 * k + u] = g[3 * k + u] + contract[3 * k + u] - c * acc[u];
 }
 
+__global__ void __launch_bounds__(256)
+adj_promote_kernel(const float* __restrict__ A, long long count, double* __restrict__ Ld)
+{
+    const long long e = (long long)blockIdx.x * 1024 + 4 * threadIdx.x;
+    if (e + 3 < count) {
+        const float4 v = *reinterpret_cast<const float4*>(A + e);
+        Ld[e] = v.x;
+        Ld[e + 1] = v.y;
+        Ld[e + 2] = v.z;
+        Ld[e + 3] = v.w;
+    } else {
+        for (long long q = e; q < count; ++q) Ld[q] = A[q];
+    }
+}
+
+int launch_adj_promote(const float* d_A, long long count, double* d_Ld, hipStream_t st)
+{
+    BDH_CHECK(count > 0 && count % 4 == 0, "adjoint: promote count");
+    hipLaunchKernelGGL(adj_promote_kernel, dim3((unsigned)ceil_div_ll(count, 1024)), dim3(256), 0, st, d_A, count,
+                       d_Ld);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
+
 int launch_adj_lt_g(const double* d_L, long long ld, int n, const double* d_g, double* d_a, hipStream_t st)
 {
     hipLaunchKernelGGL(adj_lt_g_kernel, dim3(n), dim3(256), 0, st, d_L, ld, n, d_g, d_a);

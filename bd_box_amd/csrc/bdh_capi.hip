@@ -155,7 +155,7 @@ struct bdh_system {
     EventRecorder* rec = nullptr;   // non-null while phase timing is enabled
     hipGraphExec_t factor_exec = nullptr;   // assembly + factorization, captured once
     double* adj = nullptr;          // reverse-mode step: Y1, Y2 (n x n), then vectors
-    double* fcache = nullptr;       // saved factors (fcache_slots x n_pad^2, FP64)
+    unsigned char* fcache = nullptr;  // saved factors (fcache_slots x n_pad^2, the storage type of A)
     int fcache_slots = 0;
     int nbonds = 0;
     int* bond_id = nullptr;         // bond index of each per-bead bond entry
@@ -1379,16 +1379,21 @@ int bdh_set_coords(bdh_system* s, const double* coord)
     return 0;
 }
 
+// Bytes of the factor in A: n_pad^2 entries of FP64, or of FP32 for the mixed precisions.
+static size_t factor_bytes(const bdh_system* s)
+{
+    return (s->mixed ? sizeof(float) : sizeof(double)) * (size_t)s->n_pad * s->n_pad;
+}
+
 int bdh_factor_cache(bdh_system* s, int slots)
 {
     BDH_CHECK(s && slots >= 0, "bdh_factor_cache: bad arguments");
-    BDH_CHECK(!s->mixed, "bdh_factor_cache: FP64 systems only");
     BDH_HIP(hipSetDevice(s->device));
     BDH_HIP(hipStreamSynchronize(s->st));
     dfree(s->fcache);
     s->fcache_slots = 0;
     if (slots == 0) return 0;
-    if (dmalloc(&s->fcache, sizeof(double) * (size_t)s->n_pad * s->n_pad * slots)) return -1;
+    if (dmalloc(&s->fcache, factor_bytes(s) * slots)) return -1;
     s->fcache_slots = slots;
     return 0;
 }
@@ -1398,8 +1403,8 @@ int bdh_save_factor(bdh_system* s, int slot)
     BDH_CHECK(s && s->have_L, "bdh_save_factor: no factor");
     BDH_CHECK(slot >= 0 && slot < s->fcache_slots, "bdh_save_factor: slot out of range");
     BDH_HIP(hipSetDevice(s->device));
-    const size_t nn = (size_t)s->n_pad * s->n_pad;
-    BDH_HIP(hipMemcpyAsync(s->fcache + nn * slot, s->A, sizeof(double) * nn, hipMemcpyDeviceToDevice, s->st));
+    const size_t fb = factor_bytes(s);
+    BDH_HIP(hipMemcpyAsync(s->fcache + fb * slot, s->A, fb, hipMemcpyDeviceToDevice, s->st));
     return 0;
 }
 
@@ -1460,7 +1465,6 @@ static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const dou
                          const double* gbar, double* rbar, const ParamBars* pb)
 {
     BDH_CHECK(s && coord && xi && gbar && rbar, "bdh_step_vjp: bad arguments");
-    BDH_CHECK(!s->mixed, "bdh_step_vjp: FP64 systems only");
     BDH_CHECK(s->cfg.hydro == BDH_HYDRO_CHOLESKY && !s->ewald, "bdh_step_vjp: free-space Cholesky noise only");
     // the derivative is that of one accepted ermak_const step with the given Gaussians: a
     // rejected step (check_overlap) is redrawn from the stream with the same or a halved dt,
@@ -1472,7 +1476,9 @@ static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const dou
     const int n = s->n, np = s->n_pad;
     const size_t nn = (size_t)np * np;
     const size_t ninv = (size_t)(np / 64) * 4096;
-    if (!s->adj && dmalloc(&s->adj, sizeof(double) * (2 * nn + 7 * (size_t)n + ninv))) return -1;
+    // mixed precision: room for the FP32 factor promoted to FP64 after the vectors
+    const size_t nL = s->mixed ? nn : 0;
+    if (!s->adj && dmalloc(&s->adj, sizeof(double) * (2 * nn + 7 * (size_t)n + ninv + nL))) return -1;
     double* Y1 = s->adj;
     double* Y2 = Y1 + nn;
     double* xi_d = Y2 + nn;
@@ -1482,13 +1488,14 @@ static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const dou
     double* ctr = Dg + n;
     double* out = ctr + n;
     double* Minv = out + n;
+    double* Lp = Minv + ninv;   // mixed: the factor in FP64
     // the forward step's state at coord: forces and the factor of D
     BDH_HIP(hipMemcpyAsync(s->pos, coord, sizeof(double) * 4 * s->N, hipMemcpyHostToDevice, s->st));
     if (compute_forces_dev(s)) return -1;
     if (slot >= 0) {
         // the factor at coord, saved when the forward pass stepped from there
-        BDH_HIP(hipMemcpyAsync(s->A, s->fcache + nn * slot, sizeof(double) * nn, hipMemcpyDeviceToDevice,
-                               s->st));
+        BDH_HIP(hipMemcpyAsync(s->A, s->fcache + factor_bytes(s) * slot, factor_bytes(s),
+                               hipMemcpyDeviceToDevice, s->st));
         s->have_L = true;
     } else if (assemble_and_factor(s)) {
         return -1;
@@ -1498,7 +1505,13 @@ static int step_vjp_impl(bdh_system* s, int slot, const double* coord, const dou
     const double dt = s->cfg.dt;
     const double c = dt / (kB * s->cfg.T);
     const double sq = std::sqrt(2.0 * dt);
+    // the derivative of the step that was taken: with a mixed-precision factor, of the FP32
+    // factor the noise was drawn with (promoted; the solves below run in FP64)
     const double* L = (const double*)s->A;
+    if (s->mixed) {
+        if (launch_adj_promote((const float*)s->A, (long long)nn, Lp, s->st)) return -1;
+        L = Lp;
+    }
     // S^T = L^-T (L^-T s Phi(a xi^T))^T, a = L^T g
     if (launch_adj_lt_g(L, s->ld, n, g_d, a_d, s->st)) return -1;
     if (launch_adj_diag_inverses(L, s->ld, np, Minv, s->st)) return -1;

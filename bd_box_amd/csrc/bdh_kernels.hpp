@@ -237,6 +237,8 @@ int launch_adj_lt_g(const double* d_L, long long ld, int n, const double* d_g, d
 // (np = padded order, a multiple of 64; X and B are np x np column-major)
 // inverses of the 64 x 64 diagonal blocks (np / 64 x 4096 doubles)
 int launch_adj_diag_inverses(const double* d_L, long long ld, int np, double* d_Minv, hipStream_t st);
+// Ld[e] = A[e] for e < count (an FP32 factor promoted to FP64; count a multiple of 4)
+int launch_adj_promote(const float* d_A, long long count, double* d_Ld, hipStream_t st);
 int launch_adj_upper_solve(const double* d_L, long long ld, int np, int n, int mode, const double* d_a,
                            const double* d_xi, double s, const double* d_B, const double* d_Minv,
                            double* d_X, hipStream_t st);

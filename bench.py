@@ -825,7 +825,7 @@ def bench_adjoint(args):
     import torch
     from bd_box_amd import adjoint as adj
     from bd_box_amd import workloads as wl
-    from bd_box_amd.system import BDSystem, FP64
+    from bd_box_amd.system import BDSystem, FP64, MIXED
     N = 1024 if args.N == 3000 else args.N
     K = min(args.steps, 100)
     sysm = wl.chain(N, seed=args.seed)
@@ -834,26 +834,39 @@ def bench_adjoint(args):
     xi = rng.normal(size=(K, 3 * N))
     nb = len(sysm.bonds)
     log(f"adjoint workload: N={N}, {K} steps, checkpoints every 10")
-    s = BDSystem(sysm, P, precision=FP64, refactor_every=1)
     f64 = dict(dtype=torch.float64)
-    eps = torch.tensor(0.6, requires_grad=True, **f64)
-    sig = torch.tensor(4.0, requires_grad=True, **f64)
-    kb = torch.tensor(10.0, requires_grad=True, **f64)
-    Q = torch.tensor(np.array(sysm.Q, float), requires_grad=True, **f64)
     x0 = torch.tensor(sysm.coord[:, :3].copy(), **f64)
     r0rmax = np.array(sysm.bond_parms, float)[:, :2]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    frames = adj.param_trajectory(s, x0, eps * torch.ones(N, **f64), sig / 2 * torch.ones(N, **f64),
-                                  Q, kb * torch.ones(nb, **f64), xi, sysm.coord[:, 3], r0rmax,
-                                  checkpoint=10)
-    loss = adj.rg_loss(frames)
-    t1 = time.perf_counter()
-    loss.backward()
-    t2 = time.perf_counter()
-    grads = {"eps": eps.grad.item(), "sigma": sig.grad.item(), "k": kb.grad.item(),
-             "Q_norm": float(Q.grad.norm()), "Q_first": Q.grad[:4].tolist()}
-    log(f"forward {t1 - t0:.3f} s, backward {t2 - t1:.3f} s, loss {loss.item():.6e}, grads {grads}")
+
+    def run(precision):
+        s = BDSystem(sysm, P, precision=precision, refactor_every=1)
+        eps = torch.tensor(0.6, requires_grad=True, **f64)
+        sig = torch.tensor(4.0, requires_grad=True, **f64)
+        kb = torch.tensor(10.0, requires_grad=True, **f64)
+        Q = torch.tensor(np.array(sysm.Q, float), requires_grad=True, **f64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        frames = adj.param_trajectory(s, x0, eps * torch.ones(N, **f64), sig / 2 * torch.ones(N, **f64),
+                                      Q, kb * torch.ones(nb, **f64), xi, sysm.coord[:, 3], r0rmax,
+                                      checkpoint=10)
+        loss = adj.rg_loss(frames)
+        t1 = time.perf_counter()
+        loss.backward()
+        t2 = time.perf_counter()
+        s.close()
+        grads = {"eps": eps.grad.item(), "sigma": sig.grad.item(), "k": kb.grad.item(),
+                 "Q_norm": float(Q.grad.norm()), "Q_first": Q.grad[:4].tolist()}
+        qg = Q.grad.detach().numpy().copy()
+        log(f"precision {precision}: forward {t1 - t0:.3f} s, backward {t2 - t1:.3f} s, "
+            f"loss {loss.item():.6e}, grads {grads}")
+        return t0, t1, t2, loss.item(), grads, qg
+
+    t0, t1, t2, loss, grads, qg = run(FP64)
+    m0, m1, m2, mloss, mgrads, mqg = run(MIXED)
+    # the mixed-precision gradient against the FP64 one: each scalar parameter, and the
+    # charges' gradient as a vector
+    mixed_rel = max([abs(mgrads[k] - grads[k]) / abs(grads[k]) for k in ("eps", "sigma", "k")] +
+                    [float(np.linalg.norm(mqg - qg) / np.linalg.norm(qg))])
     line = {
         "metric": f"gradient steps/s, N={N} FP64 trajectory, parameter gradient of the Rg loss "
                   "(forward + reverse)",
@@ -865,7 +878,9 @@ def bench_adjoint(args):
                                "checkpoints every 10 steps, loss (mean Rg^2 - 400)^2, gradient "
                                "w.r.t. LJ eps and sigma, bond k and the per-bead charges",
                    "N": N, "parallelism": "1 GPU"},
-        "forward_s": t1 - t0, "backward_s": t2 - t1, "loss": loss.item(), "gradient": grads,
+        "forward_s": t1 - t0, "backward_s": t2 - t1, "loss": loss, "gradient": grads,
+        "mixed_gradient_steps_per_s": K / (m2 - m0), "mixed_forward_s": m1 - m0,
+        "mixed_backward_s": m2 - m1, "mixed_gradient_rel_diff_vs_fp64": mixed_rel,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line))

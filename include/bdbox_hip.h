@@ -241,12 +241,14 @@ int bdh_sync(bdh_system* sys);
  * derivatives): for the FP64 Ermak-McCammon step with Cholesky noise refactored at coord,
  * r' = r + (dt/kT) D(r) F(r) + sqrt(2 dt) L(r) xi with the given xi[3N], rbar[3N] receives
  * the cotangent of r for the cotangent gbar[3N] of r' (forces, RPY assembly and the
- * Cholesky factor all differentiated).  FP64 systems with hydro cholesky and bc none.  The
- * system is left at coord with its forces and factor there. */
+ * Cholesky factor all differentiated).  Systems with hydro cholesky and bc none; for the
+ * mixed precisions the factor differentiated is the FP32 factor the step drew its noise
+ * with, promoted to FP64, and the rest of the reverse step runs in FP64.  The system is left
+ * at coord with its forces and factor there. */
 int bdh_step_vjp(bdh_system* sys, const double* coord, const double* xi, const double* gbar,
                  double* rbar);
 /* Factor reuse for the reverse pass (no counterpart in the reference): bdh_factor_cache
- * allocates `slots` FP64 factors on the device; bdh_save_factor copies the factor of the
+ * allocates `slots` factors on the device (FP64, or FP32 for the mixed precisions); bdh_save_factor copies the factor of the
  * last step taken into a slot; bdh_step_vjp_cached is bdh_step_vjp with the factor at coord
  * taken from a slot instead of refactoring (the caller saved it when stepping from coord). */
 int bdh_factor_cache(bdh_system* sys, int slots);

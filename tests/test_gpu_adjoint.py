@@ -1,5 +1,5 @@
-"""Reverse-mode derivative of the FP64 step on the device (bdh_step_vjp,
-bdh_step_vjp_params) and the differentiable trajectories built on it (bd_box_amd.adjoint).
+"""Reverse-mode derivative of the step on the device (bdh_step_vjp, bdh_step_vjp_params)
+and the differentiable trajectories built on it (bd_box_amd.adjoint).
 
   one step vs the oracle adjoint (itself pinned by central differences): <= 1e-9 relative
   K = 25 steps, checkpoints every 10, vs the oracle's chain of step VJPs: <= 1e-8 relative
@@ -7,6 +7,8 @@ bdh_step_vjp_params) and the differentiable trajectories built on it (bd_box_amd
   parameter gradients of (mean Rg^2 - 400)^2 (LJ epsilon and sigma, bond k, charges) vs the
   oracle's FP64 gradient: <= 1e-8 relative; vs central differences of the device
   trajectory (h = 1e-4 relative, charges 1e-4 absolute): <= 1e-5 relative
+  mixed precision (FP32 factor promoted, FP64 reverse step) vs FP64: one step <= 1e-5
+  relative, a 25-step trajectory gradient <= 1e-4 relative
 Systems: the chain recipe (LJ, Debye-Hueckel, harmonic bonds) at its hydrodynamic radius
 1.9 A: for equal radii the RPY block is C1 at the contact distance 2a (f, f', g, g' agree on
 both branches), so bonded beads near 3.8 A have a derivative.
@@ -104,18 +106,65 @@ def test_trajectory_gradient(api):
 
 
 def test_trajectory_refuses_other_steps(api):
-    """Mixed precision or a factor reused over several steps is not the step differentiated."""
+    """A factor reused over several steps is not the step differentiated."""
     import torch
     from bd_box_amd import adjoint as adj
     S, ad, po = api
     sysm, P = _chain(10, seed=3)
     x0 = torch.tensor(sysm.coord[:, :3].copy(), dtype=torch.float64, requires_grad=True)
     xi = np.zeros((2, 30))
-    for prec, refactor in ((S.MIXED, 1), (S.FP64, 10)):
-        s = S.BDSystem(sysm, P, precision=prec, refactor_every=refactor)
-        with pytest.raises(ValueError):
+    for prec in (S.FP64, S.MIXED):
+        s = S.BDSystem(sysm, P, precision=prec, refactor_every=10)
+        with pytest.raises(ValueError, match="refactor_every"):
             adj.trajectory(s, x0, xi, sysm.coord[:, 3])
         s.close()
+
+
+@pytest.mark.parametrize("prec", ["MIXED", "MIXED_SPLIT"])
+def test_mixed_step_vjp_close_to_fp64(api, prec):
+    """The mixed-precision reverse step (the FP32 factor promoted, FP64 solves) against the
+    FP64 one at the same positions: the factors differ by the FP32 rounding (|LL^T - D| /
+    |D| ~ 1e-7), so the cotangents agree to <= 1e-5 relative."""
+    S, ad, po = api
+    sysm, P = _chain(90, seed=990)
+    rng = np.random.default_rng(90)
+    xi = rng.normal(size=3 * sysm.N)
+    g = rng.normal(size=3 * sysm.N)
+    print(f"mixed step vjp {prec}", flush=True)
+    a = S.BDSystem(sysm, P, precision=S.FP64, refactor_every=1)
+    b = S.BDSystem(sysm, P, precision=getattr(S, prec), refactor_every=1)
+    ra = a.step_vjp(sysm.coord, xi, g)
+    rb = b.step_vjp(sysm.coord, xi, g)
+    assert np.linalg.norm(rb - ra) <= 1e-5 * np.linalg.norm(ra)
+    # the cached FP32 factor gives the same cotangent as a refactor at coord
+    b.factor_cache(1)
+    b.set_coords(sysm.coord)
+    b.compute_hydro(1, xi)
+    b.save_factor(0)
+    assert np.array_equal(b.step_vjp(sysm.coord, xi, g, slot=0), rb)
+    b.factor_cache(0)
+
+
+def test_mixed_trajectory_gradient_close_to_fp64(api):
+    """K = 25 mixed-precision steps, checkpoints every 10: the gradient of a linear
+    functional of the final positions against the FP64 trajectory's, <= 1e-4 relative."""
+    import torch
+    from bd_box_amd import adjoint as adj
+    S, ad, po = api
+    N, K = 64, 25
+    sysm, P = _chain(N, seed=78)
+    rng = np.random.default_rng(9)
+    xi = rng.normal(size=(K, 3 * N))
+    w = torch.from_numpy(rng.normal(size=(N, 3)))
+    print("mixed trajectory gradient", flush=True)
+    grads = []
+    for prec in (S.FP64, S.MIXED):
+        s = S.BDSystem(sysm, P, precision=prec, refactor_every=1)
+        x0 = torch.tensor(sysm.coord[:, :3].copy(), dtype=torch.float64, requires_grad=True)
+        (adj.trajectory(s, x0, xi, sysm.coord[:, 3], checkpoint=10) * w).sum().backward()
+        grads.append(x0.grad.numpy().reshape(-1).copy())
+        s.close()
+    assert np.linalg.norm(grads[1] - grads[0]) <= 1e-4 * np.linalg.norm(grads[0])
 
 
 
