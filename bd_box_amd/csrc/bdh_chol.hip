@@ -1552,6 +1552,36 @@ __device__ __forceinline__ void epoch_lookahead(T* __restrict__ A, long long ld,
     }
 }
 
+// Tile (ti, tj), ti >= tj, of the lower triangle of an ni x ni tile matrix at position x of an
+// order that walks 8 x 8 super-tiles (column-major over the super-tiles, column-major inside
+// an off-diagonal one, trap_tile's order inside a diagonal one).
+__device__ __forceinline__ void super_tile(int x, int ni, int& ti, int& tj)
+{
+    constexpr int S = 8;
+    const int ns = (ni + S - 1) / S;
+    for (int SJ = 0; SJ < ns; ++SJ) {
+        const int c0 = S * SJ, mc = min(S, ni - c0);
+        const int dc = mc * (mc + 1) / 2;
+        if (x < dc) {
+            trap_tile(x, mc, ti, tj);
+            ti += c0;
+            tj += c0;
+            return;
+        }
+        x -= dc;
+        const int below = (ni - c0 - mc) * mc;   // the off-diagonal super-tiles of this column
+        if (x < below) {
+            const int blk = x / (S * mc), rem = x - blk * S * mc;
+            const int r0 = c0 + S * (blk + 1), mr = min(S, ni - r0);
+            ti = r0 + rem % mr;
+            tj = c0 + rem / mr;
+            return;
+        }
+        x -= below;
+    }
+    ti = tj = 0;
+}
+
 template <typename T, bool SPLIT>
 __device__ __forceinline__ void chol_epoch_body(T* __restrict__ A, long long ld, int n_pad,
                                                 double* __restrict__ Lbuf, int* __restrict__ status,
@@ -1564,8 +1594,13 @@ __device__ __forceinline__ void chol_epoch_body(T* __restrict__ A, long long ld,
     } else if (b < e.R + e.NL) {
         epoch_lookahead<T, SPLIT>(A, ld, n_pad, Lbuf, status, e, b - e.R, smem);
     } else {
+        // workgroups b, b + 8, ... run on one XCD: give each XCD a contiguous run of the
+        // super-tile order, so the tiles it runs at a time share their operand row blocks in
+        // its L2
+        const int xb = b - e.R - e.NL;
+        const int c = xb & 7, q = e.NB >> 3, r = e.NB & 7;
         int ti, tj;
-        trap_tile(b - e.R - e.NL, ceil_div_dev(n_pad - e.Q1, SY_TILE), ti, tj);
+        super_tile(c * q + min(c, r) + (xb >> 3), ceil_div_dev(n_pad - e.Q1, SY_TILE), ti, tj);
         epoch_tile_update<T, SPLIT>(A, ld, e.prevP0, e.P0 - e.prevP0, e.Q1 + SY_TILE * ti, e.Q1 + SY_TILE * tj,
                                     n_pad, n_pad, smem);
     }
