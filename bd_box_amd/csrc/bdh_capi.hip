@@ -149,6 +149,8 @@ struct bdh_system {
     uint64_t* rng = nullptr;
     long long* rng_off = nullptr;
     int* rng_cnt = nullptr;
+    void* gwork = nullptr;          // the many-workgroup Gaussian launches' plan and counts
+    size_t gwork_bytes = 0;
     int* flags = nullptr;   // [0] force flags, [1] Cholesky status, [2] rng status
     long long steps_done = 0;
     bool have_L = false;
@@ -455,7 +457,8 @@ int one_step(bdh_system* s, const double* xi_dev, const double* Rs_dev, double d
 int draw_noise(bdh_system* s, int m)
 {
     REC_BEGIN(PH_NOISE);
-    int rc = launch_gauss_streams(s->rng, s->xi, s->rng_off, s->rng_cnt + m, 1, s->flags + 2, s->st);
+    const int count = m == 1 ? s->n : (int)(m * s->ldx);   // as rng_cnt[m]
+    int rc = launch_gauss_wide(s->rng, s->xi, count, s->gwork, s->gwork_bytes, s->flags + 2, s->st);
     REC_END(PH_NOISE);
     return rc;
 }
@@ -776,7 +779,7 @@ void bdh_destroy(bdh_system* s)
     dfree(s->conn_off); dfree(s->conn); dfree(s->bond_off); dfree(s->bond_p); dfree(s->bond_parm);
     dfree(s->F); dfree(s->E); dfree(s->PF); dfree(s->PE);
     dfree(s->A); dfree(s->Lbuf); dfree(s->xi); dfree(s->Pf); dfree(s->Pr); dfree(s->Rf); dfree(s->Rs);
-    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->A2); dfree(s->Rf0); dfree(s->Rfs); dfree(s->FS); dfree(s->pos_star); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->flags);
+    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->A2); dfree(s->Rf0); dfree(s->Rfs); dfree(s->FS); dfree(s->pos_star); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->gwork); dfree(s->flags);
     dfree(s->Px); dfree(s->Pc); dfree(s->Pe); dfree(s->Feff); dfree(s->Cs); dfree(s->bt);
     dfree(s->cl.grid); dfree(s->cl.key); dfree(s->cl.count); dfree(s->cl.fill);
     dfree(s->cl.start); dfree(s->cl.sorted);
@@ -969,6 +972,8 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     if (dmalloc(&s->rng, sizeof(uint64_t))) return -1;
     if (dmalloc(&s->rng_off, sizeof(long long))) return -1;
     if (dmalloc(&s->rng_cnt, sizeof(int) * (TRMV_MAX_RHS + 1))) return -1;
+    s->gwork_bytes = gauss_wide_scratch_bytes((long long)s->xi_cap);
+    if (dmalloc(&s->gwork, s->gwork_bytes)) return -1;
     {
         uint64_t st0 = lcg_seed(cfg->rand_seed);
         long long o = 0;

@@ -63,6 +63,12 @@ struct BatchPlan {
 uint64_t lcg_seed(uint32_t seed);
 int launch_gauss_streams(uint64_t* d_states, double* d_out, const long long* d_offsets,
                          const int* d_counts, int nstreams, int* d_status, hipStream_t st);
+// `count` Gaussians of one stream (*d_state, updated) into d_out, the values and final state
+// of launch_gauss_streams for that stream, over many workgroups; d_scratch holds at least
+// gauss_wide_scratch_bytes(count) bytes.
+size_t gauss_wide_scratch_bytes(long long max_count);
+int launch_gauss_wide(uint64_t* d_state, double* d_out, int count, void* d_scratch, size_t scratch_bytes,
+                      int* d_status, hipStream_t st);
 // Uploads the LCG jump tables to the current device (once); a synchronous copy, so it must
 // run before any stream capture that records a Gaussian launch.
 int gauss_prepare();

@@ -7,8 +7,9 @@
 // (2p, 2p+1); for an odd table length the last v1*fac is dropped (the pair state is local
 // to the call).
 //
-// Here one workgroup owns one stream.  Each round, thread t jumps the LCG ahead to its
-// chunk of K attempts (jump constants A^(2^b), C_b for b < 48 live in constant memory),
+// gauss_streams_kernel: one workgroup owns one stream (the batch's chains, and the remainder
+// of a single stream's call below).  Each round, thread t jumps the LCG ahead to its chunk of
+// K attempts (jump constants A^(2^b), C_b for b < 48 live in constant memory),
 // evaluates them, and a workgroup scan of the accept counts gives every accepted attempt
 // its pair index p.  The stream state after the call is the state after the attempt that
 // produced the last needed pair -- so the integer stream is bit-identical to the host's,
@@ -138,6 +139,124 @@ gauss_streams_kernel(uint64_t* __restrict__ states, double* __restrict__ out,
     if (t == 0) states[s] = base;
 }
 
+// ------------------------------------------------------------------ one stream, many workgroups
+// A single long stream (the m steps' noise of one system) in four short launches instead of
+// one workgroup's rounds: (1) workgroup w evaluates attempts [w GW_ATT, (w + 1) GW_ATT) of the
+// call and counts its accepts; (2) one thread turns the counts into each workgroup's first
+// pair index; (3) every workgroup re-evaluates its attempts and writes the accepted pairs
+// below the needed count, the thread of the last needed pair setting the stream state after
+// its attempt; (4) gauss_streams_kernel finishes the call in the (never observed) case that
+// the attempts evaluated held too few accepts, and does nothing otherwise.  The values and the
+// final state are those of gauss_streams_kernel, attempt for attempt.
+constexpr int GW_THREADS = 256;
+constexpr int GW_K = 8;                        // attempts per thread
+constexpr int GW_ATT = GW_THREADS * GW_K;      // attempts per workgroup
+
+struct GaussPlan {
+    uint64_t base;      // stream state at the start of the call
+    long long out_off;  // continuation: index of its first value
+    int remaining;      // continuation: values left to draw (0: none)
+    int pad;
+};
+
+// The GW_K attempts of thread g of the call: uniforms, rsq, accept count.
+__device__ __forceinline__ int gw_attempts(uint64_t base, long long g, double (&v1)[GW_K], double (&v2)[GW_K],
+                                           double (&rsq)[GW_K])
+{
+#pragma clang fp contract(off)
+    uint64_t x = lcg_jump(base, 2ull * (uint64_t)g * GW_K);
+    int acc = 0;
+#pragma unroll
+    for (int q = 0; q < GW_K; ++q) {
+        x = lcg_step(x);
+        v1[q] = 2.0 * u48(x) - 1.0;
+        x = lcg_step(x);
+        v2[q] = 2.0 * u48(x) - 1.0;
+        rsq[q] = v1[q] * v1[q] + v2[q] * v2[q];
+        acc += (rsq[q] < 1.0 && rsq[q] != 0.0);
+    }
+    return acc;
+}
+
+__global__ void __launch_bounds__(GW_THREADS)
+gauss_wide_count_kernel(const uint64_t* __restrict__ state, int* __restrict__ wg_count)
+{
+    __shared__ int s_red[GW_THREADS / 64];
+    double v1[GW_K], v2[GW_K], rsq[GW_K];
+    int acc = gw_attempts(*state, (long long)blockIdx.x * GW_THREADS + threadIdx.x, v1, v2, rsq);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int sum = 0;
+#pragma unroll
+        for (int w = 0; w < GW_THREADS / 64; ++w) sum += s_red[w];
+        wg_count[blockIdx.x] = sum;
+    }
+}
+
+// wg_count[w] <- pairs before workgroup w; the continuation when all G workgroups' accepts
+// fall short of the call (the stream state then moves past every attempt evaluated).
+__global__ void gauss_wide_plan_kernel(uint64_t* __restrict__ state, int* __restrict__ wg_count, int G, int count,
+                                       GaussPlan* __restrict__ plan)
+{
+    if (threadIdx.x != 0) return;
+    const uint64_t base = *state;
+    const long long npairs = (count + 1) / 2;
+    long long run = 0;
+    for (int w = 0; w < G; ++w) {
+        const int c = wg_count[w];
+        wg_count[w] = (int)run;
+        run += c;
+    }
+    plan->base = base;
+    if (run >= npairs) {
+        plan->remaining = 0;
+        plan->out_off = 0;
+    } else {
+        plan->remaining = count - (int)(2 * run);
+        plan->out_off = 2 * run;
+        *state = lcg_jump(base, 2ull * (uint64_t)G * GW_ATT);
+    }
+}
+
+__global__ void __launch_bounds__(GW_THREADS)
+gauss_wide_write_kernel(uint64_t* __restrict__ state, double* __restrict__ out, const int* __restrict__ wg_base,
+                        int count, const GaussPlan* __restrict__ plan)
+{
+#pragma clang fp contract(off)
+    __shared__ int s_scan[GW_THREADS];
+    const int t = threadIdx.x;
+    const uint64_t base = plan->base;
+    const long long npairs = (count + 1) / 2;
+    const long long g = (long long)blockIdx.x * GW_THREADS + t;
+    double v1[GW_K], v2[GW_K], rsq[GW_K];
+    const int acc = gw_attempts(base, g, v1, v2, rsq);
+    s_scan[t] = acc;
+    __syncthreads();
+    for (int off = 1; off < GW_THREADS; off <<= 1) {
+        const int v = (t >= off) ? s_scan[t - off] : 0;
+        __syncthreads();
+        s_scan[t] += v;
+        __syncthreads();
+    }
+    long long p = wg_base[blockIdx.x] + (s_scan[t] - acc);
+#pragma unroll
+    for (int q = 0; q < GW_K; ++q) {
+        if (rsq[q] < 1.0 && rsq[q] != 0.0) {
+            if (p < npairs) {
+                const double lg = rsq[q] > 0.5 ? log_near_one(rsq[q]) : log(rsq[q]);
+                double fac = sqrt(-2.0 * lg / rsq[q]);
+                out[2 * p] = v2[q] * fac;
+                if (2 * p + 1 < count) out[2 * p + 1] = v1[q] * fac;
+                if (p == npairs - 1) *state = lcg_jump(base, 2ull * (uint64_t)(g * GW_K + q + 1));
+            }
+            ++p;
+        }
+    }
+}
+
 bool g_jumps_ready[64] = {false};
 
 int ensure_jumps()
@@ -163,6 +282,40 @@ int ensure_jumps()
 uint64_t lcg_seed(uint32_t seed) { return (((uint64_t)seed) << 16) | 0x330Eu; }
 
 int gauss_prepare() { return ensure_jumps(); }
+
+int gauss_wide_workgroups(long long count)
+{
+    // attempts: 1.34 per pair (acceptance pi / 4 = 0.785) plus a margin of 2048
+    const long long need = (long long)(1.34 * (double)((count + 1) / 2)) + 2048;
+    return (int)((need + GW_ATT - 1) / GW_ATT);
+}
+
+size_t gauss_wide_scratch_bytes(long long max_count)
+{
+    return sizeof(GaussPlan) + sizeof(int) * (size_t)gauss_wide_workgroups(max_count);
+}
+
+int launch_gauss_wide(uint64_t* d_state, double* d_out, int count, void* d_scratch, size_t scratch_bytes,
+                      int* d_status, hipStream_t st)
+{
+    if (count <= 0) return 0;
+    if (ensure_jumps()) return -1;
+    const int G = gauss_wide_workgroups(count);
+    BDH_CHECK(gauss_wide_scratch_bytes(count) <= scratch_bytes, "gauss: scratch too small for the count");
+    GaussPlan* plan = static_cast<GaussPlan*>(d_scratch);
+    int* wg = reinterpret_cast<int*>(plan + 1);
+    hipLaunchKernelGGL(gauss_wide_count_kernel, dim3(G), dim3(GW_THREADS), 0, st, (const uint64_t*)d_state, wg);
+    BDH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gauss_wide_plan_kernel, dim3(1), dim3(64), 0, st, d_state, wg, G, count, plan);
+    BDH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gauss_wide_write_kernel, dim3(G), dim3(GW_THREADS), 0, st, d_state, d_out, (const int*)wg,
+                       count, (const GaussPlan*)plan);
+    BDH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gauss_streams_kernel, dim3(1), dim3(RNG_THREADS), 0, st, d_state, d_out,
+                       (const long long*)&plan->out_off, (const int*)&plan->remaining, d_status);
+    BDH_LAUNCH_CHECK();
+    return 0;
+}
 
 int launch_gauss_streams(uint64_t* d_states, double* d_out, const long long* d_offsets,
                          const int* d_counts, int nstreams, int* d_status, hipStream_t st)
