@@ -31,6 +31,7 @@ EXPORTS = [
     "bdh_compute_r", "bdh_ermak_move", "bdh_igt_phase", "bdh_set_forces",
     "bdh_tea_coefficients", "bdh_batch_set_tiles", "bdh_batch_get_tiles",
     "bdh_batch_autotune_tiles", "bdh_set_ewald_split", "bdh_gauss_noalloc", "bdh_set_flow",
+    "bdh_set_ewald_cutoff",
 ]
 
 
@@ -110,6 +111,7 @@ def load():
     L.bdh_tea_coefficients.argtypes = [C.c_void_p, _dp, _dp]
     L.bdh_set_ewald_split.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
     L.bdh_set_flow.argtypes = [C.c_void_p, _dp]
+    L.bdh_set_ewald_cutoff.argtypes = [C.c_void_p, C.c_double]
     L.bdh_batch_create.argtypes = [C.POINTER(Config), C.c_int, _ip, _dp, _dp, _dp, _ip, _ip, _dp,
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]
     L.bdh_batch_destroy.argtypes = [C.c_void_p]

@@ -166,6 +166,7 @@ struct bdh_system {
     double ew_scale = 0.0;          // Beenakker split (bdh_set_ewald_split): alpha / (sqrt(pi) / L),
                                     // 0 for the reference's split
     int ew_Rr = -1, ew_Rk = -1;     // real / reciprocal sphere radii (-1: ewald_real)
+    double ew_rcut = 0.0;           // real-space image cutoff in A (bdh_set_ewald_cutoff; 0: none)
     bool flow = false;              // a flow field (bdh_set_flow): velocity gradient G
     double G[9] = {};
     double* FS = nullptr;           // the flow terms of the last predictor (IG-T corrector)
@@ -262,6 +263,7 @@ int compute_forces_dev(bdh_system* s, bool energies = false)
 // asks for a different box or lattice radius than the last one loaded.
 double g_ew_box = -1.0, g_ew_alpha = -1.0, g_ew_scale = -1.0;
 int g_ew_R = -1, g_ew_method = -1, g_ew_Rr = -2, g_ew_Rk = -2;
+double g_ew_rcut = -1.0;
 int g_ew_dev = -1;
 
 int ensure_ewald(bdh_system* s)
@@ -270,9 +272,10 @@ int ensure_ewald(bdh_system* s)
     const bdh_config& c = s->cfg;
     if (g_ew_box == c.box && g_ew_R == c.ewald_real && g_ew_method == c.ewald_method &&
         (c.ewald_method != EW_SMITH || g_ew_alpha == c.ewald_alpha) && g_ew_dev == s->device &&
-        g_ew_scale == s->ew_scale && g_ew_Rr == s->ew_Rr && g_ew_Rk == s->ew_Rk)
+        g_ew_scale == s->ew_scale && g_ew_Rr == s->ew_Rr && g_ew_Rk == s->ew_Rk && g_ew_rcut == s->ew_rcut)
         return 0;
-    if (ewald_upload(c.box, c.ewald_real, c.ewald_method, c.ewald_alpha, s->ew_scale, s->ew_Rr, s->ew_Rk) < 0)
+    if (ewald_upload(c.box, c.ewald_real, c.ewald_method, c.ewald_alpha, s->ew_scale, s->ew_Rr, s->ew_Rk,
+                     s->ew_rcut) < 0)
         return -1;
     g_ew_box = c.box;
     g_ew_R = c.ewald_real;
@@ -281,6 +284,7 @@ int ensure_ewald(bdh_system* s)
     g_ew_scale = s->ew_scale;
     g_ew_Rr = s->ew_Rr;
     g_ew_Rk = s->ew_Rk;
+    g_ew_rcut = s->ew_rcut;
     g_ew_dev = s->device;
     return 0;
 }
@@ -1334,7 +1338,22 @@ int bdh_set_ewald_split(bdh_system* s, double alpha_scale, int R_real, int R_rec
     s->ew_scale = alpha_scale;
     s->ew_Rr = alpha_scale > 0.0 ? R_real : -1;
     s->ew_Rk = alpha_scale > 0.0 ? R_recip : -1;
+    if (alpha_scale == 0.0) s->ew_rcut = 0.0;
     // the captured refactor reads the tables of the last upload: rebuild it with the new ones
+    if (s->factor_exec) {
+        BDH_HIP(hipGraphExecDestroy(s->factor_exec));
+        s->factor_exec = nullptr;
+    }
+    return ensure_ewald(s);
+}
+
+int bdh_set_ewald_cutoff(bdh_system* s, double r_cut)
+{
+    BDH_CHECK(s && s->ewald && s->cfg.ewald_method == EW_BEENAKKER && s->ew_scale > 0.0,
+              "bdh_set_ewald_cutoff: a periodic Beenakker system with its own split (bdh_set_ewald_split)");
+    BDH_CHECK(r_cut >= 0.0, "bdh_set_ewald_cutoff: r_cut >= 0");
+    BDH_HIP(hipSetDevice(s->device));
+    s->ew_rcut = r_cut;
     if (s->factor_exec) {
         BDH_HIP(hipGraphExecDestroy(s->factor_exec));
         s->factor_exec = nullptr;

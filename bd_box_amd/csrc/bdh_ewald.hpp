@@ -38,6 +38,7 @@ struct EwaldTables {
     int R;                   // lattice sphere radius of the loops
     int RR_real;             // real-space images with h^2 + f^2 + g^2 <= RR_real
     int min_image;           // image sum around the minimum-image separation (extension)
+    double rc2;              // extension: real-space images within this squared distance only
     int method;              // EwaldMethod
     double box, inv_L, alpha, alpha2, alpha3, inv_alpha2, preQ;
     double self_a, self_b;   // Beenakker self term: -6 alpha/sqrt(pi), 40 alpha^3/(3 sqrt(pi))
@@ -55,9 +56,10 @@ extern __constant__ double c_recip_w[EW_MAXV];
 // Upload the tables of one method for box edge L, lattice radius R and (Smith) the
 // splitting parameter alpha in units of 1/L; returns the vector count.  Beenakker with
 // alpha_scale > 0 (the extension): alpha = alpha_scale sqrt(pi) / L, spheres of radius
-// R_real (images) and R_recip (reciprocal vectors) instead of R, minimum-image centres.
+// R_real (images) and R_recip (reciprocal vectors) instead of R, minimum-image centres, and
+// with r_cut > 0 only the images within r_cut (A) of the pair.
 int ewald_upload(double box, int R, int method, double alpha, double alpha_scale = 0.0, int R_real = -1,
-                 int R_recip = -1);
+                 int R_recip = -1, double r_cut = 0.0);
 
 struct Blk6 {
     double xx, yy, zz, xy, xz, yz;
@@ -189,7 +191,7 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
                 // real space image r + L (h, f, g)
                 const T x = (T)(rx + E.box * h), y = (T)(ry + E.box * f), z = (T)(rz + E.box * g);
                 const T rr2 = x * x + y * y + z * z;
-                if (rr2 != (T)0 && h * h + f * f + g * g <= E.RR_real) {
+                if (rr2 != (T)0 && h * h + f * f + g * g <= E.RR_real && rr2 <= (T)E.rc2) {
                     const InvPowers<T> ip = inv_powers(rr2);
                     const T rn = ip.r, inv_r = ip.inv_r, inv_r2 = ip.inv_r2, inv_r3 = ip.inv_r3;
                     const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;

@@ -86,7 +86,7 @@ int launch_rpy_assemble_batch(const double4* d_pos, double preDii, double preDij
 
 // ---- pair sums, Ewald, TEA (bdh_pairs.hip)
 int ewald_upload(double box, int R, int method, double alpha_smith, double alpha_scale, int R_real,
-                 int R_recip);
+                 int R_recip, double r_cut);
 int pair_splits(int N, bool ewald);   // row partials per bead: one per 64-bead tile
 int pair_eparts(int N);                // eps partials of the TEA sums: one per tile pair
 // d_pos2 (TEA only): the sums of the mean tensor (D(pos2) + D(pos)) / 2 (the IG-T corrector).

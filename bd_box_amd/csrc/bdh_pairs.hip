@@ -35,7 +35,7 @@ __constant__ double c_recip_w[EW_MAXV];
 static int g_uploaded_method = EW_BEENAKKER;
 
 int ewald_upload(double box, int R, int method, double alpha_smith, double alpha_scale, int R_real,
-                 int R_recip)
+                 int R_recip, double r_cut)
 {
     BDH_CHECK(R >= 0 && R <= 5, "Ewald lattice radius must be 0..5");
     BDH_CHECK(method == EW_SMITH || method == EW_BEENAKKER, "Ewald method");
@@ -113,6 +113,8 @@ int ewald_upload(double box, int R, int method, double alpha_smith, double alpha
     t.R = R;
     t.RR_real = R_real * R_real;
     t.min_image = ext ? 1 : 0;
+    BDH_CHECK(r_cut <= 0.0 || ext, "Ewald: a real-space cutoff only for the extension split");
+    t.rc2 = r_cut > 0.0 ? r_cut * r_cut : 1e300;
     BDH_CHECK(t.nv <= EW_MAXV, "Ewald table overflow");
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ew), &t, sizeof(t)));
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip), rec.data(), sizeof(double4) * rec.size()));

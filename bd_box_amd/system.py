@@ -179,13 +179,17 @@ class BDSystem:
         return b.value, e.value
 
     def set_ewald_split(self, alpha_scale: float = 1.0, R_real: Optional[int] = None,
-                        R_recip: Optional[int] = None):
+                        R_recip: Optional[int] = None, r_cut: float = 0.0):
         """Beenakker split alpha = alpha_scale sqrt(pi)/L with real / reciprocal sphere radii
         around minimum-image separations (bdh_set_ewald_split); None keeps ewald_real;
-        alpha_scale 0 returns to the reference's split."""
+        r_cut > 0 (A) also drops the real-space images farther than r_cut from the pair
+        (bdh_set_ewald_cutoff); alpha_scale 0 returns to the reference's split."""
         Rr = self.cfg.ewald_real if R_real is None else int(R_real)
         Rk = self.cfg.ewald_real if R_recip is None else int(R_recip)
-        _lib.check(_lib.load().bdh_set_ewald_split(self._h, float(alpha_scale), Rr, Rk), "set_ewald_split")
+        L = _lib.load()
+        _lib.check(L.bdh_set_ewald_split(self._h, float(alpha_scale), Rr, Rk), "set_ewald_split")
+        if alpha_scale > 0.0:
+            _lib.check(L.bdh_set_ewald_cutoff(self._h, float(r_cut)), "set_ewald_cutoff")
 
     def set_flow(self, G=None):
         """A flow field with velocity gradient G (3x3, row, column; vel_grad_tensor,

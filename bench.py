@@ -582,15 +582,39 @@ def main():
 
 def ewald_split_for(scale: float, tol: float = 1e-6):
     """Real and reciprocal sphere radii of the Beenakker split alpha = scale sqrt(pi)/L for a
-    truncation error of about tol (image sums around minimum-image separations): an omitted
-    image lies at least (sqrt(R_real^2 + 1) - sqrt(3)/2) L from the pair, where the terms fall
-    as exp(-alpha^2 r^2); an omitted reciprocal vector carries exp(-pi |n|^2 / scale^2).  Both
-    exponents are taken 3 above ln(1/tol) for the polynomial prefactors."""
+    truncation error of about tol (image sums around minimum-image separations), and the
+    real-space cutoff r_c in units of L: an image farther than r_c = sqrt(x) / scale L from
+    the pair carries exp(-alpha^2 r^2) < exp(-pi x), and every image within r_c has
+    |n| <= r_c / L + sqrt(3)/2, which fixes R_real; an omitted reciprocal vector carries
+    exp(-pi |n|^2 / scale^2).  The reciprocal exponent pi x2 is taken 3 above ln(1/tol) for the
+    polynomial prefactors of one term, and the real-space one pi x 6 above, since a cutoff drops the many
+    images just beyond r_c at once (measured: 3 above leaves 1e-6 to 4e-6 of max |D| at
+    alpha = (0.8 .. 1.5) sqrt(pi)/L, 6 above ~1.5e-7)."""
     x2 = (np.log(1.0 / tol) + 3.0) / np.pi
-    d = np.sqrt(x2) / scale + np.sqrt(3.0) / 2.0
+    rc = float(np.sqrt((np.log(1.0 / tol) + 6.0) / np.pi) / scale)
+    d = rc + np.sqrt(3.0) / 2.0
     Rr = int(np.ceil(np.sqrt(max(0.0, d * d - 1.0))))
     Rk = int(np.ceil(np.sqrt(max(0.0, x2 * scale * scale - 1.0))))
-    return Rr, Rk
+    return Rr, Rk, rc
+
+
+def images_within(xyz, L, Rr, rc, samples=200000, seed=0):
+    """Mean number of real-space images r + n L (|n| <= Rr, r the minimum-image separation)
+    within rc L of a pair, over a sample of the pairs i < j of xyz."""
+    rng = np.random.default_rng(seed)
+    N = len(xyz)
+    i = rng.integers(0, N, size=samples)
+    j = rng.integers(0, N, size=samples)
+    keep = i != j
+    r = (xyz[j[keep]] - xyz[i[keep]]) / L
+    r -= np.rint(r)
+    ns = np.array([(h, f, g) for h in range(-Rr, Rr + 1) for f in range(-Rr, Rr + 1)
+                   for g in range(-Rr, Rr + 1) if h * h + f * f + g * g <= Rr * Rr], float)
+    cnt = 0.0
+    for c in range(0, len(r), 20000):
+        d2 = ((r[c:c + 20000, None, :] + ns[None, :, :]) ** 2).sum(-1)
+        cnt += ((d2 <= rc * rc) & (d2 > 0)).sum()
+    return cnt / len(r)
 
 
 def lattice_points(R):
@@ -618,6 +642,15 @@ def cpu_baseline_periodic(sysm, prm, threads, timeout=600):
         exe = os.path.join(ROOT, "oracle", "_ref", "bd_box")
         if not os.path.exists(exe):
             return None
+    # the reference's start-up moves the centre of mass to the origin and stops unless every
+    # bead then lies within half a box of it (trans.c:75-97); a box-filling suspension has
+    # beads up to |centre of mass| beyond that, so the run takes a box just large enough
+    xyz = np.asarray(sysm.coord, float)[:, :3]
+    Lr = float(2.0 * np.max(np.abs(xyz - xyz.mean(axis=0))) * (1.0 + 1e-6))
+    Lr = max(Lr, float(prm["xbox"]))
+    # its bucket list stops with "Wrong cell" at the second step on this box
+    # (potentials/bucket.c:127), so the forces take its brute-force pair loop
+    prm = dict(prm, xbox=Lr, ybox=Lr, zbox=Lr, nb_list="brute")
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads))
     times = {}
     with tempfile.TemporaryDirectory() as tmp:
@@ -640,7 +673,8 @@ def cpu_baseline_periodic(sysm, prm, threads, timeout=600):
     dt = times[2] - times[1]
     return {"value": 1.0 / dt, "unit": "steps/s", "cores": threads, "kind": "reference",
             "sample": f"1 step of the N={sysm.N} suspension, reference semantics (its Beenakker "
-                      f"split alpha = sqrt(pi)/L, R = 2; on-the-fly TEA), "
+                      f"split alpha = sqrt(pi)/L, R = 2; on-the-fly TEA; brute-force pairs), box "
+                      f"{Lr:.1f} A so that its centring keeps every bead inside, "
                       f"{os.path.basename(exe)}: wall(bdsteps=2) - wall(bdsteps=1) = {dt:.2f} s"}
 
 
@@ -659,14 +693,14 @@ def bench_periodic(args):
     sysm, prm, P, L = periodic_workload(N, args.seed)
     converged = args.ewald == "converged"
     scale = 1.25
-    Rr, Rk = ewald_split_for(scale) if converged else (P.ewald_real, P.ewald_real)
+    Rr, Rk, rc = ewald_split_for(scale) if converged else (P.ewald_real, P.ewald_real, 0.0)
     log(f"periodic workload: N={N} box={L:.1f} A, {steps} timed steps, Ewald split "
-        f"{'alpha = %.2f sqrt(pi)/L, radii %d/%d' % (scale, Rr, Rk) if converged else 'reference'}")
+        f"{'alpha = %.2f sqrt(pi)/L, radii %d/%d, cutoff %.2f L' % (scale, Rr, Rk, rc) if converged else 'reference'}")
     out = {}
     for name, prec in (("mixed", MIXED), ("fp64", FP64)):
         s = BDSystem(sysm, P, precision=prec)
         if converged:
-            s.set_ewald_split(scale, Rr, Rk)
+            s.set_ewald_split(scale, Rr, Rk, r_cut=rc * L)
         s.compute_hydro(warm)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -682,7 +716,8 @@ def bench_periodic(args):
         s.close()
     # roofline of the pair sums (HIP events on the launch stream): FP32-equivalent VALU
     # operations of every unordered pair's block (and the self blocks) per step
-    n_real = lattice_points(Rr) - 1 if converged else lattice_points(Rr)
+    # real-space images per pair: those within the cutoff (a sample's mean), or the sphere
+    n_real = images_within(sysm.coord[:, :3], L, Rr, rc) if converged else lattice_points(Rr)
     n_recip = lattice_points(Rk) - 1
     pairs = N * (N + 1) // 2
     ops = pairs * (n_real * OPS_REAL_IMAGE + n_recip * OPS_RECIP_VECTOR)
@@ -693,7 +728,7 @@ def bench_periodic(args):
         roof = {"kernel": "pair_sums_kernel<float, beenakker, tea>", "bound": "valu",
                 "achieved": ach, "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / FP32_VALU_PEAK_TFLOPS, "traffic": None,
-                "basis": f"{pairs} pair blocks per step x ({n_real} real-space images x "
+                "basis": f"{pairs} pair blocks per step x ({n_real:.1f} real-space images x "
                          f"{OPS_REAL_IMAGE} + {n_recip} reciprocal vectors x {OPS_RECIP_VECTOR}) "
                          f"FP32-equivalent VALU operations (FP64 counted twice) / the pair-sum "
                          f"phase's time per step"}
@@ -708,8 +743,8 @@ def bench_periodic(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"N={N} spheres a=25 A, phi=0.2, periodic cube L={L:.1f} A, "
                                "WCA wall sigma 50 A eps 1, Beenakker-Ewald, TEA, dt 1 ps",
-                   "ewald": (f"alpha = {scale} sqrt(pi)/L, real/reciprocal radii {Rr}/{Rk} "
-                             "(1e-6 truncation)") if converged else
+                   "ewald": (f"alpha = {scale} sqrt(pi)/L, real/reciprocal radii {Rr}/{Rk}, "
+                             f"real-space cutoff {rc:.3f} L (1e-6 truncation)") if converged else
                             "reference split: alpha = sqrt(pi)/L, radius 2",
                    "N": N, "box": L, "parallelism": "1 GPU"},
         "fp64_steps_per_s": out["fp64"][0],

@@ -177,6 +177,12 @@ int bdh_tea_coefficients(bdh_system* sys, double* beta, double* eps);
  * returns to the reference's tensor.  Periodic systems with ewald_method beenakker. */
 int bdh_set_ewald_split(bdh_system* sys, double alpha_scale, int R_real, int R_recip);
 
+/* Extension of the same split (no counterpart in the reference): real-space images farther
+ * than r_cut (A) from the pair are dropped, the distance bound from which the radii of a
+ * split for a target truncation error are derived; 0 keeps every image of the sphere.  After
+ * bdh_set_ewald_split with alpha_scale > 0 (alpha_scale 0 clears it). */
+int bdh_set_ewald_cutoff(bdh_system* sys, double r_cut);
+
 /* A flow field: the velocity gradient tensor G[9] (row, column), the .prm keyword
  * vel_grad_tensor (src/diff_alg/diff_decl.c:72, parsed at src/input.c:1110-1125); NULL
  * removes it.  Every move then adds dt G r as compute_hydro_ermak does

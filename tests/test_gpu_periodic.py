@@ -114,23 +114,27 @@ def test_suspension_8192_mixed_vs_fp64():
 def ewald_split_for(scale: float, tol: float = 1e-6):
     """Real and reciprocal sphere radii for the Beenakker split alpha = scale sqrt(pi)/L at a
     truncation error of about tol, image sums centred on minimum-image separations
-    (|r| <= sqrt(3)/2 L): an omitted image n (|n|^2 > R_real^2) lies at least
-    (sqrt(R_real^2 + 1) - sqrt(3)/2) L from the pair, where the image terms fall as
-    exp(-alpha^2 r^2); an omitted reciprocal vector carries exp(-k^2 / 4 alpha^2) =
-    exp(-pi |n|^2 / scale^2) with |n|^2 >= R_recip^2 + 1.  Both exponents are taken 3 above
-    ln(1/tol) for the polynomial prefactors."""
+    (|r| <= sqrt(3)/2 L), and the real-space cutoff in units of L: an image farther than
+    r_c = sqrt(x) / scale L from the pair carries exp(-alpha^2 r^2) < exp(-pi x); every image
+    within r_c has |n| <= r_c / L + sqrt(3)/2, which fixes R_real; an omitted reciprocal vector
+    carries exp(-k^2 / 4 alpha^2) = exp(-pi |n|^2 / scale^2) with |n|^2 >= R_recip^2 + 1.
+    The reciprocal exponent is taken 3 above ln(1/tol) for the polynomial prefactors of one
+    term, the real-space one 6 above, since a cutoff drops the many images just beyond r_c at
+    once (measured: 3 above leaves 1e-6 to 4e-6 of max |D|, 6 above ~1.5e-7)."""
     x2 = (math.log(1.0 / tol) + 3.0) / math.pi
-    d = math.sqrt(x2) / scale + math.sqrt(3.0) / 2.0
+    rc = math.sqrt((math.log(1.0 / tol) + 6.0) / math.pi) / scale
+    d = rc + math.sqrt(3.0) / 2.0
     Rr = math.ceil(math.sqrt(max(0.0, d * d - 1.0)))
     Rk = math.ceil(math.sqrt(max(0.0, x2 * scale * scale - 1.0)))
-    return Rr, Rk
+    return Rr, Rk, rc
 
 
 def test_beenakker_alpha_invariance():
     """The Beenakker tensor does not depend on the splitting parameter once both sums have
     converged: D at alpha = (0.8, 1, 1.25) sqrt(pi)/L, with radii chosen for 1e-6 and image
-    sums around minimum-image separations, agrees with a fully converged tensor (radius 5)
-    to 1e-6 of its largest entry; a split with too few images does not.  The reference's
+    sums around minimum-image separations, with and without the real-space cutoff the radii
+    are derived from, agrees with a fully converged tensor (radius 5) to 1e-6 of its largest
+    entry; a split with too few images, or a cutoff well inside the bound, does not.  The reference's
     own split (alpha = sqrt(pi)/L, one sphere of radius 2 around the wrapped separation)
     is not that tensor: it forms the reciprocal wave vectors in box units against alpha and
     the radii in A (diff_tensor.c:372-376), so its reciprocal terms underflow and only the
@@ -143,14 +147,18 @@ def test_beenakker_alpha_invariance():
     ref = s.compute_D()
     scale = np.max(np.abs(ref))
     for a in (0.8, 1.0, 1.25):
-        Rr, Rk = ewald_split_for(a)
+        Rr, Rk, rc = ewald_split_for(a)
         assert Rr <= 5 and Rk <= 5
-        s.set_ewald_split(a, Rr, Rk)
-        D = s.compute_D()
-        err = np.max(np.abs(D - ref)) / scale
-        print(f"alpha {a} radii {Rr}/{Rk}: {err:.2e}", flush=True)
-        assert err < 1e-6, (a, Rr, Rk, err)
+        for cut in (0.0, rc * P.box):
+            s.set_ewald_split(a, Rr, Rk, r_cut=cut)
+            D = s.compute_D()
+            err = np.max(np.abs(D - ref)) / scale
+            print(f"alpha {a} radii {Rr}/{Rk} cutoff {cut:.0f} A: {err:.2e}", flush=True)
+            assert err < 1e-6, (a, Rr, Rk, cut, err)
     s.set_ewald_split(0.8, 1, 1)
+    assert np.max(np.abs(s.compute_D() - ref)) / scale > 1e-4
+    # a cutoff well inside the bound drops images that matter
+    s.set_ewald_split(1.25, 3, 3, r_cut=0.8 * P.box)
     assert np.max(np.abs(s.compute_D() - ref)) / scale > 1e-4
     s.set_ewald_split(0.0)
     err = np.max(np.abs(s.compute_D() - ref)) / scale
