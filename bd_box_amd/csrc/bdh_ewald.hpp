@@ -5,9 +5,8 @@
 // lattices share the sphere h^2 + f^2 + g^2 <= R^2 (ewald_real == ewald_recip,
 // src/input.c:402-409), around the pair's separation as the wrapped positions give it (no
 // minimum image).  An extension beyond the reference (bdh_set_ewald_split) scales alpha,
-// gives the real and reciprocal sums spheres of their own radii -- the loops run over the
-// larger sphere, real images beyond R_real are skipped and reciprocal vectors beyond
-// R_recip carry a zero factor in the table -- and centres the image sum on the minimum-image
+// gives the real and reciprocal sums spheres of their own radii (and the real-space images
+// a distance cutoff, bdh_set_ewald_cutoff) and centres the image sum on the minimum-image
 // separation, so the omitted images of every pair lie at least (sqrt(R_real^2 + 1) -
 // sqrt(3)/2) L away, and forms the wave vectors k = 2 pi (h, f, g) / L of the box (the
 // reference's k = 2 pi (h, f, g) is in box units against alpha and sigma in A, so its
@@ -19,7 +18,9 @@
 //   G_k = 6 pi (1 + (k^2/4a^2)(1 + k^2/2a^2)) exp(-k^2/4a^2) / k^2,  Rk = k^2 / 3,
 //   k = 2 pi (h, f, g)  (reference),  k = 2 pi (h, f, g) / L  (extension)
 // so a pair's reciprocal sum is sum_k (1 - sigma^2 Rk) G_k cos(2 pi v.hfg) (I - e_k e_k^T),
-// with v = r / L.  The reciprocal and self sums run in FP64 in both precisions; the
+// with v = r / L, over the half of the sphere with (h, f, g) > 0 lexicographically and 2 G_k
+// in the table (k and -k give the same term); the real-space images and the reciprocal
+// vectors have loops of their own.  The reciprocal and self sums run in FP64 in both precisions; the
 // real-space image sum runs in T.
 #pragma once
 
@@ -39,6 +40,8 @@ struct EwaldTables {
     int RR_real;             // real-space images with h^2 + f^2 + g^2 <= RR_real
     int min_image;           // image sum around the minimum-image separation (extension)
     double rc2;              // extension: real-space images within this squared distance only
+    int RR_recip;            // Beenakker: reciprocal half sphere |(h, f, g)|^2 <= RR_recip
+    int nrecip;              // Beenakker: vectors of that half sphere in c_recip (0: none)
     int method;              // EwaldMethod
     double box, inv_L, alpha, alpha2, alpha3, inv_alpha2, preQ;
     double self_a, self_b;   // Beenakker self term: -6 alpha/sqrt(pi), 40 alpha^3/(3 sqrt(pi))
@@ -164,34 +167,19 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
     const double r2 = rx * rx + ry * ry + rz * rz;
     const double s2 = (ai + aj) * (ai + aj);
     if (!self && r2 < s2) overlap_correction(rx, ry, rz, r2, ai, aj, preDii, preDij, out);
-    const double vx = rx * E.inv_L, vy = ry * E.inv_L, vz = rz * E.inv_L;
     const T al = (T)E.alpha, al2 = (T)E.alpha2, al3 = (T)E.alpha3, sg = (T)sigma_sq;
     const T inv_sqrtpi = (T)(1.0 / SQRTPI);
-    // Reciprocal phases cos 2 pi v.(h, f, g) = Re(e^{i h tx} e^{i f ty} e^{i g tz}): one
-    // sincos per axis, the exponentials stepped along the lattice loops, which visit the
-    // vectors of the sphere in the table's order (h, then f, then g ascending).
-    double sx, cx, sy, cy, sz, cz;
-    sincos(2.0 * PI_F * vx, &sx, &cx);
-    sincos(2.0 * PI_F * vy, &sy, &cy);
-    sincos(2.0 * PI_F * vz, &sz, &cz);
-    const int R = E.R, RR = R * R;
-    double hr = 1.0, hi = 0.0;
-    for (int k = 0; k < R; ++k) cmul_conj_step(hr, hi, cx, sx);
-    int q = 0;
-    for (int h = -R; h <= R; ++h) {
-        const int nf = isqrt_small(RR - h * h);
-        double fr = 1.0, fi = 0.0;
-        for (int k = 0; k < nf; ++k) cmul_conj_step(fr, fi, cy, sy);
+    // real space: images r + L (h, f, g) of the sphere h^2 + f^2 + g^2 <= RR_real (h, then f,
+    // then g ascending), within sqrt(rc2) of the pair
+    const int RRr = E.RR_real, Rr = isqrt_small(RRr);
+    for (int h = -Rr; h <= Rr; ++h) {
+        const int nf = isqrt_small(RRr - h * h);
         for (int f = -nf; f <= nf; ++f) {
-            const int ng = isqrt_small(RR - h * h - f * f);
-            const double hfr = hr * fr - hi * fi, hfi = hr * fi + hi * fr;
-            double gr = 1.0, gi = 0.0;
-            for (int k = 0; k < ng; ++k) cmul_conj_step(gr, gi, cz, sz);
-            for (int g = -ng; g <= ng; ++g, ++q) {
-                // real space image r + L (h, f, g)
+            const int ng = isqrt_small(RRr - h * h - f * f);
+            for (int g = -ng; g <= ng; ++g) {
                 const T x = (T)(rx + E.box * h), y = (T)(ry + E.box * f), z = (T)(rz + E.box * g);
                 const T rr2 = x * x + y * y + z * z;
-                if (rr2 != (T)0 && h * h + f * f + g * g <= E.RR_real && rr2 <= (T)E.rc2) {
+                if (rr2 != (T)0 && rr2 <= (T)E.rc2) {
                     const InvPowers<T> ip = inv_powers(rr2);
                     const T rn = ip.r, inv_r = ip.inv_r, inv_r2 = ip.inv_r2, inv_r3 = ip.inv_r3;
                     const T ex = x * inv_r, ey = y * inv_r, ez = z * inv_r;
@@ -211,8 +199,37 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
                     O[4] += (double)(sc2 * ex * ez);
                     O[5] += (double)(sc2 * ey * ez);
                 }
-                // reciprocal space (FP64)
-                if (h != 0 || f != 0 || g != 0) {
+            }
+        }
+    }
+    // reciprocal space (FP64): a vector and its negative carry the same term (the phase's
+    // cosine is even), so the loops visit the half sphere (h, f, g) > 0 lexicographically,
+    // |.|^2 <= RR_recip, in the table's order, with the factor 2 in G.  The phases
+    // cos 2 pi v.(h, f, g) = Re(e^{i h tx} e^{i f ty} e^{i g tz}) come from one sincos per
+    // axis, the exponentials stepped along the loops.  None when every G is zero (the
+    // reference's split, whose G underflow).
+    if (E.nrecip > 0) {
+        const double vx = rx * E.inv_L, vy = ry * E.inv_L, vz = rz * E.inv_L;
+        double sx, cx, sy, cy, sz, cz;
+        sincos(2.0 * PI_F * vx, &sx, &cx);
+        sincos(2.0 * PI_F * vy, &sy, &cy);
+        sincos(2.0 * PI_F * vz, &sz, &cz);
+        const int RRk = E.RR_recip, Rk = isqrt_small(RRk);
+        double hr = 1.0, hi = 0.0;
+        int q = 0;
+        for (int h = 0; h <= Rk; ++h) {
+            const int nf = isqrt_small(RRk - h * h);
+            const int f0 = h == 0 ? 0 : -nf;
+            double fr = 1.0, fi = 0.0;
+            for (int k = 0; k < -f0; ++k) cmul_conj_step(fr, fi, cy, sy);
+            for (int f = f0; f <= nf; ++f) {
+                const int ng = isqrt_small(RRk - h * h - f * f);
+                const int g0 = (h == 0 && f == 0) ? 1 : -ng;
+                const double hfr = hr * fr - hi * fi, hfi = hr * fi + hi * fr;
+                double gr = 1.0, gi = 0.0;
+                if (g0 > 0) cmul_step(gr, gi, cz, sz);
+                for (int k = 0; k < -g0; ++k) cmul_conj_step(gr, gi, cz, sz);
+                for (int g = g0; g <= ng; ++g, ++q) {
                     const double4 gk = c_recip[q];
                     const double ekz = c_recip_z[q];
                     const double sc = (1.0 - sigma_sq * gk.y) * gk.x * (hfr * gr - hfi * gi);
@@ -222,12 +239,12 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
                     Qr[3] -= sc * gk.z * gk.w;
                     Qr[4] -= sc * gk.z * ekz;
                     Qr[5] -= sc * gk.w * ekz;
+                    cmul_step(gr, gi, cz, sz);
                 }
-                cmul_step(gr, gi, cz, sz);
+                cmul_step(fr, fi, cy, sy);
             }
-            cmul_step(fr, fi, cy, sy);
+            cmul_step(hr, hi, cx, sx);
         }
-        cmul_step(hr, hi, cx, sx);
     }
     out.xx += preDii * (O[0] + E.preQ * Qr[0]);
     out.yy += preDii * (O[1] + E.preQ * Qr[1]);

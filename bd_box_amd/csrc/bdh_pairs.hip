@@ -69,36 +69,54 @@ int ewald_upload(double box, int R, int method, double alpha_smith, double alpha
     const float pif = 3.14159265358979323846f;
     const float mpp = -pif * pif;
     const double m_pi_alpha_sq = (double)mpp / t.alpha2;
-    const int rr = R * R;
-    for (int h = -R; h <= R; ++h) {
-        const int nf = (int)std::sqrt((double)(rr - h * h));
-        for (int f = -nf; f <= nf; ++f) {
-            const int ng = (int)std::sqrt((double)(rr - h * h - f * f));
-            for (int g = -ng; g <= ng; ++g) {
-                ++nv;
-                const double s = (double)(h * h + f * f + g * g);
-                if (s == 0.0) {
-                    rec.push_back(make_double4(0, 0, 0, 0));
-                    recz.push_back(0.0);
-                    recw.push_back(0.0);
-                    continue;
-                }
-                const double nrm = std::sqrt(s);
-                if (method == EW_BEENAKKER) {
+    if (method == EW_BEENAKKER) {
+        // the half sphere (h, f, g) > 0 lexicographically, |.|^2 <= R_recip^2, h then f then g
+        // ascending (beenakker_block's loop order), each vector standing for itself and its
+        // negative (2 G)
+        const int rk = R_recip * R_recip;
+        bool any = false;
+        for (int h = 0; h * h <= rk; ++h) {
+            const int nf = (int)std::sqrt((double)(rk - h * h));
+            for (int f = (h == 0 ? 0 : -nf); f <= nf; ++f) {
+                const int ng = (int)std::sqrt((double)(rk - h * h - f * f));
+                for (int g = (h == 0 && f == 0 ? 1 : -ng); g <= ng; ++g) {
+                    ++nv;
+                    const double s = (double)(h * h + f * f + g * g);
+                    const double nrm = std::sqrt(s);
                     // The reference forms k = 2 pi (h, f, g) in box units (diff_tensor.c:372-376)
                     // against alpha and sigma in A, so its reciprocal terms underflow to zero for
                     // any box of more than a few A; the extension takes k = 2 pi (h, f, g) / L,
                     // the wave vectors of the box, as Beenakker's sum has them.
                     const double k2 = ext ? s * 4.0 * M_PI * M_PI * t.inv_L * t.inv_L : s * 4.0 * PI_F * PI_F;
                     const double six_pi = ext ? 6.0 * M_PI : 6.0 * PI_F;
-                    double G = (1.0 + (t.inv_alpha2 * k2 / 4.0) * (1.0 + t.inv_alpha2 * k2 / 2.0)) *
-                               (six_pi * std::exp(-t.inv_alpha2 * k2 / 4.0) / k2);
-                    if (s > (double)(R_recip * R_recip)) G = 0.0;
-                    rec.push_back(make_double4(G, k2 / 3.0, h / nrm, f / nrm));
+                    const double G = (1.0 + (t.inv_alpha2 * k2 / 4.0) * (1.0 + t.inv_alpha2 * k2 / 2.0)) *
+                                     (six_pi * std::exp(-t.inv_alpha2 * k2 / 4.0) / k2);
+                    any = any || G != 0.0;
+                    rec.push_back(make_double4(2.0 * G, k2 / 3.0, h / nrm, f / nrm));
                     recz.push_back(g / nrm);
                     recw.push_back(0.0);
-                } else {
+                }
+            }
+        }
+        t.RR_recip = rk;
+        t.nrecip = any ? nv : 0;
+    } else {
+        const int rr = R * R;
+        for (int h = -R; h <= R; ++h) {
+            const int nf = (int)std::sqrt((double)(rr - h * h));
+            for (int f = -nf; f <= nf; ++f) {
+                const int ng = (int)std::sqrt((double)(rr - h * h - f * f));
+                for (int g = -ng; g <= ng; ++g) {
+                    ++nv;
+                    const double s = (double)(h * h + f * f + g * g);
+                    if (s == 0.0) {
+                        rec.push_back(make_double4(0, 0, 0, 0));
+                        recz.push_back(0.0);
+                        recw.push_back(0.0);
+                        continue;
+                    }
                     // diff_tensor.c:199-225; e = (h, f, g) / |.| as get_ev2_norms forms it
+                    const double nrm = std::sqrt(s);
                     const double inv = 1.0 / nrm;
                     const double ex = h * inv, ey = f * inv, ez = g * inv;
                     const double G1 = 2.0 * std::exp(m_pi_alpha_sq * s) / ((double)pif * s);
@@ -117,9 +135,11 @@ int ewald_upload(double box, int R, int method, double alpha_smith, double alpha
     t.rc2 = r_cut > 0.0 ? r_cut * r_cut : 1e300;
     BDH_CHECK(t.nv <= EW_MAXV, "Ewald table overflow");
     BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ew), &t, sizeof(t)));
-    BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip), rec.data(), sizeof(double4) * rec.size()));
-    BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip_z), recz.data(), sizeof(double) * recz.size()));
-    BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip_w), recw.data(), sizeof(double) * recw.size()));
+    if (!rec.empty()) {
+        BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip), rec.data(), sizeof(double4) * rec.size()));
+        BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip_z), recz.data(), sizeof(double) * recz.size()));
+        BDH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_recip_w), recw.data(), sizeof(double) * recw.size()));
+    }
     g_uploaded_method = method;
     return t.nv;
 }

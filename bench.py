@@ -718,7 +718,9 @@ def bench_periodic(args):
     # operations of every unordered pair's block (and the self blocks) per step
     # real-space images per pair: those within the cutoff (a sample's mean), or the sphere
     n_real = images_within(sysm.coord[:, :3], L, Rr, rc) if converged else lattice_points(Rr)
-    n_recip = lattice_points(Rk) - 1
+    # reciprocal vectors evaluated per pair: half the sphere (k and -k carry the same term);
+    # none for the reference's split, whose reciprocal factors underflow to zero
+    n_recip = (lattice_points(Rk) - 1) // 2 if converged else 0
     pairs = N * (N + 1) // 2
     ops = pairs * (n_real * OPS_REAL_IMAGE + n_recip * OPS_RECIP_VECTOR)
     t_pairs = out["mixed"][1].get("matvec", 0.0) * 1e-3
