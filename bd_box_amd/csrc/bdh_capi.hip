@@ -43,6 +43,43 @@ __global__ void diag_mobility_kernel(const double4* __restrict__ pos, const doub
     if (Rs) Rs[r] = xi[r] * sqrt(dii * 2.0 * dt);
 }
 
+// The TEA pair sums of a large periodic system over beads in a space-filling-curve order
+// perm: pos_s[i] = pos[perm[i]] (pos2 alike), F_s and X_s the 3-vectors alike.  A 64-bead tile
+// is then a compact region, and a wavefront evaluates few real-space images beyond those its
+// pairs need.
+__global__ void pair_order_gather_kernel(const int* __restrict__ perm, int N, const double4* __restrict__ pos,
+                                         const double4* __restrict__ pos2, const double* __restrict__ F,
+                                         const double* __restrict__ X, double4* __restrict__ pos_s,
+                                         double4* __restrict__ pos2_s, double* __restrict__ F_s,
+                                         double* __restrict__ X_s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int j = perm[i];
+    pos_s[i] = pos[j];
+    if (pos2) pos2_s[i] = pos2[j];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        F_s[3 * i + k] = F[3 * j + k];
+        X_s[3 * i + k] = X[3 * j + k];
+    }
+}
+
+// a[perm[i]] = a_s[i] and, with b, b[perm[i]] = b_s[i] (3-vectors)
+__global__ void pair_order_scatter_kernel(const int* __restrict__ perm, int N, const double* __restrict__ a_s,
+                                          double* __restrict__ a, const double* __restrict__ b_s,
+                                          double* __restrict__ b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int j = perm[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        a[3 * j + k] = a_s[3 * i + k];
+        if (b) b[3 * j + k] = b_s[3 * i + k];
+    }
+}
+
 // out = (a + b) / 2, the IG-T deterministic move Rf0 (diff_equation.c:322-325)
 __global__ void mean2_kernel(const double* __restrict__ a, const double* __restrict__ b, int n,
                              double* __restrict__ out)
@@ -171,6 +208,10 @@ struct bdh_system {
     double G[9] = {};
     double* FS = nullptr;           // the flow terms of the last predictor (IG-T corrector)
     double4* pos_star = nullptr;    // IG-T with TEA noise: the predictor's positions r*
+    int* perm = nullptr;            // periodic TEA: the pair sums' bead order (set_pair_order)
+    double4 *pos_s = nullptr, *pos2_s = nullptr;
+    double *F_s = nullptr, *X_s = nullptr, *Rf_s = nullptr, *Rs_s = nullptr;
+    long long order_step = 0;       // steps_done when perm was last computed
 };
 
 namespace {
@@ -367,6 +408,35 @@ int matvec_dev(bdh_system* s, const double* xi = nullptr, bool tea = false, doub
 // pos2: the sums of the mean tensor of the positions and pos2 (the IG-T corrector).
 int tea_parts(bdh_system* s, const double* X, double dt, double* Rf_out, const double4* pos2 = nullptr)
 {
+    if (s->perm) {
+        // the same sums over the beads in the pair order, the results scattered back
+        const int N = s->N;
+        const dim3 g(ceil_div(N, 256)), b(256);
+        hipLaunchKernelGGL(pair_order_gather_kernel, g, b, 0, s->st, (const int*)s->perm, N, (const double4*)s->pos,
+                           pos2, (const double*)s->F, X, s->pos_s, s->pos2_s, s->F_s, s->X_s);
+        BDH_LAUNCH_CHECK();
+        if (ensure_ewald(s)) return -1;
+        const double4* p2 = pos2 ? s->pos2_s : nullptr;
+        const int rc = s->mixed
+            ? launch_pair_sums<float>(s->pos_s, s->F_s, s->X_s, N, s->preDii, s->preDij, dt, s->ewald, true,
+                                      s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st, p2)
+            : launch_pair_sums<double>(s->pos_s, s->F_s, s->X_s, N, s->preDii, s->preDij, dt, s->ewald, true,
+                                       s->msplits, s->Pf, s->Px, s->Pc, s->Pe, s->st, p2);
+        if (rc) return -1;
+        ReduceJobs jobs{};
+        int nj = 0;
+        if (Rf_out) jobs.job[nj++] = ReduceJob{s->Pf, s->Rf_s, s->n, s->msplits, 0, dt / (kB * s->cfg.T)};
+        jobs.job[nj++] = ReduceJob{s->Px, s->Feff, s->n, s->msplits, 0, 1.0};
+        jobs.job[nj++] = ReduceJob{s->Pc, s->Cs, s->n, s->msplits, 0, 1.0};
+        if (launch_reduce_partials(jobs, nj, s->n, s->st)) return -1;
+        if (launch_tea_close(s->pos_s, s->X_s, s->Feff, s->Cs, s->Pe, pair_eparts(N), N, s->preDii, dt, s->bt,
+                             s->Rs_s, s->st))
+            return -1;
+        hipLaunchKernelGGL(pair_order_scatter_kernel, g, b, 0, s->st, (const int*)s->perm, N,
+                           (const double*)s->Rs_s, s->Rs, (const double*)(Rf_out ? s->Rf_s : nullptr), Rf_out);
+        BDH_LAUNCH_CHECK();
+        return 0;
+    }
     if (matvec_dev(s, X, true, dt, pos2)) return -1;
     ReduceJobs jobs{};
     int nj = 0;
@@ -783,7 +853,8 @@ void bdh_destroy(bdh_system* s)
     dfree(s->conn_off); dfree(s->conn); dfree(s->bond_off); dfree(s->bond_p); dfree(s->bond_parm);
     dfree(s->F); dfree(s->E); dfree(s->PF); dfree(s->PE);
     dfree(s->A); dfree(s->Lbuf); dfree(s->xi); dfree(s->Pf); dfree(s->Pr); dfree(s->Rf); dfree(s->Rs);
-    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->A2); dfree(s->Rf0); dfree(s->Rfs); dfree(s->FS); dfree(s->pos_star); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->gwork); dfree(s->flags);
+    dfree(s->noise); dfree(s->RsB); dfree(s->pos_prev); dfree(s->F_prev); dfree(s->A2); dfree(s->Rf0); dfree(s->Rfs); dfree(s->FS); dfree(s->pos_star); dfree(s->perm); dfree(s->pos_s); dfree(s->pos2_s); dfree(s->F_s); dfree(s->X_s);
+    dfree(s->Rf_s); dfree(s->Rs_s); dfree(s->rng); dfree(s->rng_off); dfree(s->rng_cnt); dfree(s->gwork); dfree(s->flags);
     dfree(s->Px); dfree(s->Pc); dfree(s->Pe); dfree(s->Feff); dfree(s->Cs); dfree(s->bt);
     dfree(s->cl.grid); dfree(s->cl.key); dfree(s->cl.count); dfree(s->cl.fill);
     dfree(s->cl.start); dfree(s->cl.sorted);
@@ -1009,6 +1080,70 @@ static int create_impl(const bdh_config* cfg, int N, const double* coord, const 
     return check_flags(s);
 }
 
+// The pair sums' bead order of a periodic TEA system with at least PAIR_ORDER_MIN beads: the
+// Morton order of the positions (wrapped into the box, 1024 cells per axis), ties by index.
+// Set at creation and renewed once PAIR_ORDER_EVERY steps have passed since (at the end of a
+// bdh_step call, or with positions from bdh_set_coords), as the beads diffuse away from it.  It
+// affects only which beads share a tile, not the sums (beyond their rounding).
+constexpr int PAIR_ORDER_MIN = 256;
+constexpr long long PAIR_ORDER_EVERY = 4096;
+
+static uint32_t spread_bits10(uint32_t v)
+{
+    v &= 0x3FF;
+    v = (v | (v << 16)) & 0x030000FF;
+    v = (v | (v << 8)) & 0x0300F00F;
+    v = (v | (v << 4)) & 0x030C30C3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+static int set_pair_order(bdh_system* s, const double* coord)
+{
+    if (!(s->tea && s->ewald && s->N >= PAIR_ORDER_MIN)) return 0;
+    const int N = s->N, n = s->n;
+    if (!s->perm) {
+        if (dmalloc(&s->perm, sizeof(int) * N) || dmalloc(&s->pos_s, sizeof(double4) * N) ||
+            dmalloc(&s->F_s, sizeof(double) * n) || dmalloc(&s->X_s, sizeof(double) * n) ||
+            dmalloc(&s->Rf_s, sizeof(double) * n) || dmalloc(&s->Rs_s, sizeof(double) * n))
+            return -1;
+        if (dmalloc(&s->pos2_s, sizeof(double4) * N)) return -1;
+    }
+    const double L = s->cfg.box;
+    std::vector<std::pair<uint32_t, int>> key(N);
+    for (int i = 0; i < N; ++i) {
+        uint32_t q[3];
+        for (int k = 0; k < 3; ++k) {
+            double u = coord[4 * i + k] / L + 0.5;
+            u -= std::floor(u);
+            q[k] = (uint32_t)std::min(1023.0, std::max(0.0, std::floor(u * 1024.0)));
+        }
+        key[i] = {spread_bits10(q[0]) | (spread_bits10(q[1]) << 1) | (spread_bits10(q[2]) << 2), i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> perm(N);
+    for (int i = 0; i < N; ++i) perm[i] = key[i].second;
+    BDH_HIP(hipMemcpyAsync(s->perm, perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, s->st));
+    BDH_HIP(hipStreamSynchronize(s->st));
+    s->order_step = s->steps_done;
+    return 0;
+}
+
+// Renews the pair order when it is PAIR_ORDER_EVERY steps old (or with force), from the host
+// positions coord (N x 4) or, without them, from the device's.
+static int refresh_pair_order(bdh_system* s, const double* coord, bool force)
+{
+    if (!s->perm || (!force && s->steps_done - s->order_step < PAIR_ORDER_EVERY)) return 0;
+    std::vector<double> h;
+    if (!coord) {
+        h.resize(4 * (size_t)s->N);
+        BDH_HIP(hipMemcpyAsync(h.data(), s->pos, sizeof(double4) * s->N, hipMemcpyDeviceToHost, s->st));
+        BDH_HIP(hipStreamSynchronize(s->st));
+        coord = h.data();
+    }
+    return set_pair_order(s, coord);
+}
+
 int bdh_create(const bdh_config* cfg, int N, const double* coord, const double* Q,
                const double* LJ, int nbonds, const int* bonds, const double* bond_parms,
                bdh_system** out)
@@ -1017,6 +1152,7 @@ int bdh_create(const bdh_config* cfg, int N, const double* coord, const double* 
     *out = nullptr;
     bdh_system* s = new bdh_system();
     int rc = create_impl(cfg, N, coord, Q, LJ, nbonds, bonds, bond_parms, s);
+    if (!rc) rc = set_pair_order(s, coord);
     if (rc) {
         std::string keep = g_err;
         bdh_destroy(s);
@@ -1043,6 +1179,7 @@ int bdh_step(bdh_system* s, int steps, const double* xi)
     }
     if (run_steps(s, steps, xi != nullptr)) return -1;
     BDH_HIP(hipStreamSynchronize(s->st));
+    if (refresh_pair_order(s, nullptr, false)) return -1;
     return check_flags(s);
 }
 
@@ -1398,6 +1535,7 @@ int bdh_set_coords(bdh_system* s, const double* coord)
 {
     BDH_CHECK(s && coord, "bdh_set_coords: bad arguments");
     BDH_HIP(hipMemcpyAsync(s->pos, coord, sizeof(double) * 4 * s->N, hipMemcpyHostToDevice, s->st));
+    if (refresh_pair_order(s, coord, s->steps_done == 0)) return -1;
     if (compute_forces_dev(s)) return -1;
     BDH_HIP(hipStreamSynchronize(s->st));
     return 0;

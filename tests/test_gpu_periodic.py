@@ -89,6 +89,33 @@ def test_suspension_384_vs_oracle():
         s.close()
 
 
+def test_suspension_384_pair_order_renewed():
+    """The pair sums run over the beads in a Morton order of their positions, renewed at the
+    end of a step call once 4096 steps have passed: after 4100 steps with the device's own
+    Gaussians (the order renewed), one more step with given Gaussians matches the FP64 oracle
+    started from the device's positions, FP64 <= 1e-10 relative displacement."""
+    import dataclasses
+    from bd_box_amd import system as S
+    from oracle import pyoracle as po
+    sysm, P = _suspension(384, seed=5)
+    s = S.BDSystem(sysm, P, precision=S.FP64)
+    print("suspension N=384 4100 steps", flush=True)
+    s.compute_hydro(4100)
+    c = s.coords()
+    xi = np.random.default_rng(4100).normal(size=3 * sysm.N)
+    ref = po.Sim(dataclasses.replace(sysm, coord=c.copy()), po.params_from(P), seed=P.rand_seed)
+    ref.run(1, xi)
+    s.compute_hydro(1, xi)
+    L = P.box
+    dd = (s.coords()[:, :3] - c[:, :3]) - (ref.coords()[:, :3] - c[:, :3])
+    dd -= L * np.floor(dd / L + 0.5)
+    dr = ref.coords()[:, :3] - c[:, :3]
+    dr -= L * np.floor(dr / L + 0.5)
+    err = np.linalg.norm(dd) / np.linalg.norm(dr)
+    s.close()
+    assert err < 1e-10, err
+
+
 def test_suspension_8192_mixed_vs_fp64():
     from bd_box_amd import system as S
     sysm, P = _suspension(8192, seed=1)
