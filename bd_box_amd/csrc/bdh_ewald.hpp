@@ -170,7 +170,8 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
     const T al = (T)E.alpha, al2 = (T)E.alpha2, al3 = (T)E.alpha3, sg = (T)sigma_sq;
     const T inv_sqrtpi = (T)(1.0 / SQRTPI);
     // real space: images r + L (h, f, g) of the sphere h^2 + f^2 + g^2 <= RR_real (h, then f,
-    // then g ascending), within sqrt(rc2) of the pair
+    // then g ascending), within sqrt(rc2) of the pair, summed in T and added to O once
+    T Ot[6] = {0, 0, 0, 0, 0, 0};
     const int RRr = E.RR_real, Rr = isqrt_small(RRr);
     for (int h = -Rr; h <= Rr; ++h) {
         const int nf = isqrt_small(RRr - h * h);
@@ -192,16 +193,18 @@ __device__ __forceinline__ Blk6 beenakker_block(double rx, double ry, double rz,
                     T sc2 = (T)-3 * al * sg * inv_r2 - (T)2 * al3 * sg + (T)1.5 * al + (T)16 * al2 * al3 * sg * rr2 -
                             (T)3 * al3 * rr2 - (T)4 * al3 * al2 * al2 * sg * rr2 * rr2;
                     sc2 = sc2 * eg + ec * ((T)0.75 * inv_r - (T)1.5 * sg * inv_r3);
-                    O[0] += (double)(sc + sc2 * ex * ex);
-                    O[1] += (double)(sc + sc2 * ey * ey);
-                    O[2] += (double)(sc + sc2 * ez * ez);
-                    O[3] += (double)(sc2 * ex * ey);
-                    O[4] += (double)(sc2 * ex * ez);
-                    O[5] += (double)(sc2 * ey * ez);
+                    Ot[0] += sc + sc2 * ex * ex;
+                    Ot[1] += sc + sc2 * ey * ey;
+                    Ot[2] += sc + sc2 * ez * ez;
+                    Ot[3] += sc2 * ex * ey;
+                    Ot[4] += sc2 * ex * ez;
+                    Ot[5] += sc2 * ey * ez;
                 }
             }
         }
     }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) O[k] += (double)Ot[k];
     // reciprocal space (FP64): a vector and its negative carry the same term (the phase's
     // cosine is even), so the loops visit the half sphere (h, f, g) > 0 lexicographically,
     // |.|^2 <= RR_recip, in the table's order, with the factor 2 in G.  The phases
