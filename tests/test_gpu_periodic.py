@@ -25,12 +25,12 @@ from bd_box_amd import workloads as wl
 pytestmark = pytest.mark.gpu
 
 
-def _suspension(N, seed):
+def _suspension(N, seed, **over):
     sysm, L = wl.spheres(N, seed=seed, radius=25.0, phi=0.2)
     prm = wl.chain_params(bc="periodic", xbox=L, ybox=L, zbox=L, hydro="geyer",
                           ewald_method="beenakker", ewald_real=2, ewald_recip=2, elec="no",
                           alpha_lj=4.0, cutoff_lj=-1, lj_6_term="yes", dt=1.0,
-                          nb_list="bucket_list", rand_seed=20240 + seed)
+                          nb_list="bucket_list", rand_seed=20240 + seed, **over)
     return sysm, wl.parse_params(prm)
 
 
@@ -114,6 +114,32 @@ def test_suspension_384_pair_order_renewed():
     err = np.linalg.norm(dd) / np.linalg.norm(dr)
     s.close()
     assert err < 1e-10, err
+
+
+def test_suspension_384_igt_vs_oracle():
+    """IG-T with TEA noise on the N = 384 suspension (the pair sums in the Morton order, the
+    corrector's over the mean tensor of r0 and r*): 3 steps from the device's own stream
+    against the oracle's, FP64 <= 1e-10 relative displacement, and the same stream state."""
+    from bd_box_amd import system as S
+    from oracle import pyoracle as po
+    sysm, P = _suspension(384, seed=7, algorithm="igt_const")
+    ref = po.Sim(sysm, po.params_from(P), seed=P.rand_seed, algorithm=P.algorithm,
+                 move_attempts=P.move_attempts)
+    s = S.BDSystem(sysm, P, precision=S.FP64)
+    x0 = sysm.coord[:, :3]
+    L = P.box
+    print("suspension N=384 IG-T", flush=True)
+    for k in range(3):
+        ref.run(1)
+        s.compute_hydro(1)
+        dd = s.coords()[:, :3] - ref.coords()[:, :3]
+        dd -= L * np.floor(dd / L + 0.5)
+        dr = ref.coords()[:, :3] - x0
+        dr -= L * np.floor(dr / L + 0.5)
+        err = np.linalg.norm(dd) / np.linalg.norm(dr)
+        assert err < 1e-10, (k, err)
+    assert s.rng_state() == ref.rng_state()
+    s.close()
 
 
 def test_suspension_8192_mixed_vs_fp64():
